@@ -1,0 +1,126 @@
+/*
+ * apirl_oracle.h -- CPU restatement of APIRL's Siddon projector / backprojector
+ * and MLEM / OSEM update, used ONLY as test infrastructure (the checker the HIP
+ * path is compared with, and the "port" CPU baseline).  The product never links
+ * it.  Every function follows the reference file:line named beside it, including
+ * its floating-point operation order and its quirks (dropped exit voxel,
+ * zero-length entry segments, upper-/lower-only bounds checks).
+ *
+ * Conventions shared by all functions:
+ *   image  : float[nz][ny][nx], index z*nx*ny + y*nx + x        (data/src/Images.cpp:34-56)
+ *   2D sino: float[nang][nr]                                      (data/inc/Sinogram2D.h:214)
+ *   3D sino: float[nsino][nang][nr], sinograms segment-major      (Sinogram3D.cpp:396-464)
+ *   angles : degrees, r : mm (grids built by orc_sino_grid)
+ *
+ * Precision: ORC_F32 functions reproduce the reference's float arithmetic bit for
+ * bit; the *_f64 variants run the same rules in double and serve as the FP64
+ * oracle that errors are reported against.
+ */
+#ifndef APIRL_ORACLE_H
+#define APIRL_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_image {
+  int nx, ny, nz;
+  float dx, dy, dz;
+} orc_image;
+
+/* One Siddon segment as the reference's SiddonSegment (recon/inc/Siddon.h:37-43). */
+typedef struct orc_seg {
+  int i, j, k;
+  float w;
+} orc_seg;
+
+/* Sinogram grids: ang[i] = i * (180/nang), r[j] = dr/2 + j*dr - rfov, dr = 2*rfov/nr
+ * (data/src/Sinogram2DinCylindrical3Dpet.cpp:46-60, Sinogram2D.cpp:190-198). */
+void orc_sino_grid(int nang, int nr, float rfov, float* ang_deg, float* r_mm);
+
+/* LOR endpoints of bin (ang, r) on a cylinder of radius rscanner
+ * (Sinogram2DinCylindrical3Dpet.cpp:249-266).  Returns 0 when r > rfov (2D rule);
+ * pass rfov = +inf for the 3D form (:216-229), which never rejects. */
+int orc_lor_points(float ang_deg, float r, float rscanner, float rfov, float* p1x, float* p1y,
+                   float* p2x, float* p2y);
+
+/* Ring-pair enumeration of a cylindrical 3D sinogram (Sinogram3D.cpp:358-367,
+ * 396-464) and the mean axial coordinates the projector uses
+ * (SiddonProjector.cpp:349-361,394-395).  Writes, per sinogram, the final P1.Z and
+ * P2.Z of its LORs into za/zb (nsino_total entries) and returns nsino_total, or -1
+ * if a segment's count does not match nsino[seg]. */
+int orc_ring_table(int nrings, float zfov, int nseg, const int* nsino, const int* minrd,
+                   const int* maxrd, float* za, float* zb);
+
+/* Siddon traversals (recon/src/Siddon.cpp:358-643 for 2D, :13-344 for 3D).
+ * Emit up to cap segments into out; return the reference's list length. */
+int orc_siddon2d(float p0x, float p0y, float vx, float vy, const orc_image* im, orc_seg* out,
+                 int cap);
+int orc_siddon3d(float p0x, float p0y, float p0z, float vx, float vy, float vz,
+                 const orc_image* im, orc_seg* out, int cap);
+
+/* Projector, 2D (SiddonProjector.cpp:146-193) and 3D (:322-441).
+ * 2D leaves bins whose LOR is rejected untouched; 3D overwrites every bin. */
+void orc_project2d(const orc_image* im, const float* img, int nang, const float* ang_deg, int nr,
+                   const float* r_mm, float rfov, float rscanner, float* sino);
+void orc_project3d(const orc_image* im, const float* img, int nang, const float* ang_deg, int nr,
+                   const float* r_mm, float rscanner, int nsino, const float* za,
+                   const float* zb, float* sino);
+
+/* Backprojectors accumulate into img in the reference's single-thread order
+ * (SiddonProjector.cpp:34-87 for 2D, :444-568 for 3D). */
+void orc_backproject2d(const orc_image* im, const float* sino, int nang, const float* ang_deg,
+                       int nr, const float* r_mm, float rfov, float rscanner, float* img);
+void orc_backproject3d(const orc_image* im, const float* sino, int nang, const float* ang_deg,
+                       int nr, const float* r_mm, float rscanner, int nsino, const float* za,
+                       const float* zb, float* img);
+/* DivideAndBackproject 3D (SiddonProjector.cpp:571-704), single-thread order. */
+void orc_divide_backproject3d(const orc_image* im, const float* meas, const float* est,
+                              int nang, const float* ang_deg, int nr, const float* r_mm,
+                              float rscanner, int nsino, const float* za, const float* zb,
+                              float* img);
+
+/* Bin-wise sinogram operations (Sinogram3D.cpp:830-870, Sinogram2D.cpp:234-266). */
+void orc_inverse_divide(float* est, const float* meas, long n);
+void orc_add(float* est, const float* add, long n);
+/* Likelihood sum y*log(yhat) - yhat (Sinogram2D.cpp:565-581 uses yhat>0,
+ * Sinogram3D.cpp:584-607 uses yhat!=0); positive_only selects the 2D rule. */
+float orc_likelihood(const float* est, const float* meas, long n, int positive_only);
+
+/* Image helpers (Images.cpp:383-431, recon/src/Mlem.cpp:39-42). */
+int orc_force_positive(float* img, long n);
+float orc_min_nonzero(const float* img, long n);
+float orc_max(const float* img, long n);
+float orc_mlem_threshold(const float* sens, long n);
+/* EM update x := cond ? x*b/s : 0, cond = (s > thr) if strict else (s >= thr)
+ * (Mlem2d.cpp:171-185, MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237). */
+void orc_em_update(float* x, const float* b, const float* s, long n, float thr, int strict);
+
+/* Angle subset s of S: angles {s + S*k}, count floor(P/S) + (P mod S > s)
+ * (Sinogram3DCylindricalPet.cpp:164-197).  Writes the subset's full-angle indices. */
+int orc_subset_angles(int nang, int nsubsets, int subset, int* idx);
+
+/* Drivers.  Each follows the reference loop exactly; outputs the final image,
+ * the sensitivity image(s) and the likelihood trace. */
+int orc_mlem2d(const orc_image* im, int nang, int nr, float rscanner, const float* meas,
+               const float* init, int iters, float* img, float* sens, float* lik);
+int orc_mlem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
+               const float* za, const float* zb, const float* meas, const float* init,
+               int iters, float* img, float* sens, float* lik);
+int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
+               const float* za, const float* zb, const float* meas, const float* init,
+               int nsubsets, int iters, float* img, float* sens_all, float* lik);
+
+/* FP64 oracle: exact chord-length system matrix with the same visited-voxel
+ * rules, evaluated in double (used to report FP64 error, not for parity). */
+void orc_project3d_f64(const orc_image* im, const double* img, int nang, const float* ang_deg,
+                       int nr, const float* r_mm, float rscanner, int nsino, const float* za,
+                       const float* zb, double* sino);
+void orc_backproject3d_f64(const orc_image* im, const double* sino, int nang,
+                           const float* ang_deg, int nr, const float* r_mm, float rscanner,
+                           int nsino, const float* za, const float* zb, double* img);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

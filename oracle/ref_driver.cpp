@@ -1,0 +1,537 @@
+// Test-infrastructure driver around the reference's own compiled CPU Siddon path
+// (oracle/_ref/libapirl_ref.so, built by oracle/ref.mk from the reference's
+// sources).  It is a checker and a CPU baseline only: nothing in the product
+// links it.  It constructs the reference's Image / Sinogram objects the way its
+// command-line tools do and calls the reference's SiddonProjector, Mlem2d,
+// MlemSinogram3d and OsemSinogram3d, then writes the results as raw float32.
+//
+// Usage:  ref_driver <command> key=value ...   (all arrays raw little-endian f32)
+//   siddon2d  nx dx nproj nr rfov rscanner out          per-bin ray lists
+//   siddon3d  <geom3d> out                              per-bin ray lists
+//   proj2d    nx dx nproj nr rfov rscanner img out
+//   back2d    nx dx nproj nr rfov rscanner sino out
+//   mlem2d    nx dx nproj nr rscanner iters meas init outdir
+//   proj3d    <geom3d> img out
+//   back3d    <geom3d> sino out
+//   osem3d    <geom3d> subsets iters meas init outdir
+//   mlem3d    <geom3d> iters meas init outdir
+//   rays3d    <geom3d> img start stride out             sampled reference rays
+//   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
+// The reference's 2D projector checks only lower bounds and its 3D projector only
+// upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
+// read outside the pixel array (undefined behaviour; it segfaults when the array is
+// page-aligned).  For the calls this driver makes directly, the pixel array is moved
+// into a buffer with zeroed guard zones, so such a read returns 0.  Access to the
+// private pointer is the only reason for the macro below.
+#define private public
+#include <Images.h>
+#undef private
+#include <Sinogram2DinCylindrical3Dpet.h>
+#include <Sinogram3DCylindricalPet.h>
+#include <SiddonProjector.h>
+#include <Siddon.h>
+#include <Mlem2d.h>
+#include <MlemSinogram3d.h>
+#include <OsemSinogram3d.h>
+#include <omp.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace {
+
+std::map<std::string, std::string> g_args;
+
+const char* arg(const char* k) {
+  auto it = g_args.find(k);
+  if (it == g_args.end()) {
+    fprintf(stderr, "ref_driver: missing argument %s\n", k);
+    exit(2);
+  }
+  return it->second.c_str();
+}
+int iarg(const char* k) { return atoi(arg(k)); }
+float farg(const char* k) { return strtof(arg(k), nullptr); }
+
+std::vector<float> read_f32(const char* path, size_t n) {
+  std::vector<float> v(n);
+  FILE* f = fopen(path, "rb");
+  if (!f || fread(v.data(), 4, n, f) != n) {
+    fprintf(stderr, "ref_driver: cannot read %zu floats from %s\n", n, path);
+    exit(3);
+  }
+  fclose(f);
+  return v;
+}
+
+void write_f32(const char* path, const float* p, size_t n) {
+  FILE* f = fopen(path, "wb");
+  if (!f || fwrite(p, 4, n, f) != n) {
+    fprintf(stderr, "ref_driver: cannot write %s\n", path);
+    exit(4);
+  }
+  fclose(f);
+}
+
+double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+SizeImage size2d() {
+  SizeImage s;
+  s.nDimensions = 2;
+  s.nPixelsX = s.nPixelsY = iarg("nx");
+  s.nPixelsZ = 1;
+  s.sizePixelX_mm = s.sizePixelY_mm = farg("dx");
+  s.sizePixelZ_mm = 1.0f;
+  return s;
+}
+
+SizeImage size3d() {
+  SizeImage s;
+  s.nDimensions = 3;
+  s.nPixelsX = iarg("nx");
+  s.nPixelsY = iarg("ny");
+  s.nPixelsZ = iarg("nz");
+  s.sizePixelX_mm = s.sizePixelY_mm = farg("dx");
+  s.sizePixelZ_mm = farg("dz");
+  return s;
+}
+
+// Segment table for a span / maximum ring difference, ordered 0, +1, -1, +2, -2 ...
+// (the order the .par / interfile segment lists use for cylindrical scanners).
+struct SegTable {
+  std::vector<int> nsino, minrd, maxrd;
+};
+
+SegTable seg_table(int rings, int span, int mrd) {
+  SegTable t;
+  int h = (span - 1) / 2;
+  t.minrd.push_back(-h);
+  t.maxrd.push_back(h);
+  t.nsino.push_back(2 * rings - 1);
+  for (int lo = h + 1; lo <= mrd; lo += span) {
+    int hi = lo + span - 1;
+    if (hi > mrd) hi = mrd;
+    int n = 0;  // ring sums a+b reachable with lo <= a-b <= hi, 0 <= a,b < rings
+    for (int sum = 0; sum <= 2 * rings - 2; sum++)
+      for (int b = 0; b < rings; b++) {
+        int a = sum - b;
+        if (a >= 0 && a < rings && a - b >= lo && a - b <= hi) { n++; break; }
+      }
+    t.minrd.push_back(lo);  t.maxrd.push_back(hi);  t.nsino.push_back(n);
+    t.minrd.push_back(-hi); t.maxrd.push_back(-lo); t.nsino.push_back(n);
+  }
+  return t;
+}
+
+// Builds a cylindrical 3D sinogram with the ring grid the reference's interfile
+// reader computes (Sinogram3D.cpp:358-367) and its ring-pair lists.
+Sinogram3DCylindricalPet* make_sino3d() {
+  int rings = iarg("rings");
+  float zfov = farg("zfov");
+  SegTable t = seg_table(rings, iarg("span"), iarg("mrd"));
+  // The reference's Sinogram3D(float, float) base constructor leaves ptrAngValues_deg
+  // unset and the parametric constructor then realloc()s it
+  // (Sinogram3DCylindricalPet.cpp:55); constructing on zeroed storage makes that a
+  // realloc(NULL) instead of undefined behaviour.
+  void* mem = calloc(1, sizeof(Sinogram3DCylindricalPet));
+  Sinogram3DCylindricalPet* s = new (mem) Sinogram3DCylindricalPet(
+      iarg("nproj"), iarg("nr"), rings, farg("rfov"), zfov, farg("rscanner"), (int)t.nsino.size(),
+      t.nsino.data(), t.minrd.data(), t.maxrd.data());
+  float* z = s->getAxialPtr();
+  float zIncrement = (float)zfov / rings;
+  for (int i = 0; i < rings; i++) z[i] = zIncrement / 2 + i * zIncrement;
+  if (!s->initRingConfig(t.nsino.data())) {
+    fprintf(stderr, "ref_driver: ring configuration mismatch\n");
+    exit(5);
+  }
+  return s;
+}
+
+size_t bins3d(Sinogram3D* s) {
+  size_t n = 0;
+  for (int i = 0; i < s->getNumSegments(); i++)
+    n += (size_t)s->getSegment(i)->getNumSinograms() * s->getNumProj() * s->getNumR();
+  return n;
+}
+
+void sino3d_from(Sinogram3D* s, const float* src) {
+  size_t off = 0;
+  for (int i = 0; i < s->getNumSegments(); i++)
+    for (int j = 0; j < s->getSegment(i)->getNumSinograms(); j++) {
+      Sinogram2D* s2 = s->getSegment(i)->getSinogram2D(j);
+      size_t n = (size_t)s2->getNumProj() * s2->getNumR();
+      memcpy(s2->getSinogramPtr(), src + off, n * 4);
+      off += n;
+    }
+}
+
+std::vector<float> sino3d_to(Sinogram3D* s) {
+  std::vector<float> v(bins3d(s));
+  size_t off = 0;
+  for (int i = 0; i < s->getNumSegments(); i++)
+    for (int j = 0; j < s->getSegment(i)->getNumSinograms(); j++) {
+      Sinogram2D* s2 = s->getSegment(i)->getSinogram2D(j);
+      size_t n = (size_t)s2->getNumProj() * s2->getNumR();
+      memcpy(v.data() + off, s2->getSinogramPtr(), n * 4);
+      off += n;
+    }
+  return v;
+}
+
+// Exposes the protected results of the reference's EM drivers.
+struct Mlem2dProbe : Mlem2d {
+  using Mlem2d::Mlem2d;
+  float* lik() { return likelihoodValues; }
+  Image* sens() { return sensitivityImage; }
+};
+struct Mlem3dProbe : MlemSinogram3d {
+  using MlemSinogram3d::MlemSinogram3d;
+  float* lik() { return likelihoodValues; }
+  Image* sens() { return sensitivityImage; }
+};
+struct OsemProbe : OsemSinogram3d {
+  using OsemSinogram3d::OsemSinogram3d;
+  float* lik() { return likelihoodValues; }
+  Image* sens(int s) { return sensitivityImages[s]; }
+};
+
+
+// Swaps an Image's pixel array for a copy inside zero guard zones of one slice plus
+// one row on each side; restores the original array (owned by Image) on destruction.
+struct GuardedPixels {
+  Image* img;
+  float* orig;
+  float* buf;
+  size_t guard, n;
+  explicit GuardedPixels(Image* im) : img(im) {
+    SizeImage s = im->getSize();
+    n = (size_t)im->getPixelCount();
+    guard = (size_t)s.nPixelsX * s.nPixelsY + s.nPixelsX;
+    buf = (float*)calloc(n + 2 * guard, sizeof(float));
+    orig = im->pixels;
+    memcpy(buf + guard, orig, n * 4);
+    im->pixels = buf + guard;
+  }
+  ~GuardedPixels() {
+    memcpy(orig, img->pixels, n * 4);
+    img->pixels = orig;
+    free(buf);
+  }
+};
+
+void report_time(const char* what, double t0) {
+  printf("{\"what\": \"%s\", \"seconds\": %.6f, \"threads\": %d}\n", what, now_s() - t0,
+         omp_get_max_threads());
+  fflush(stdout);
+}
+
+void write_lists(const char* path, std::vector<int>& counts, std::vector<SiddonSegment>& segs) {
+  FILE* f = fopen(path, "wb");
+  int nb = (int)counts.size();
+  fwrite(&nb, 4, 1, f);
+  fwrite(counts.data(), 4, counts.size(), f);
+  for (auto& s : segs) {
+    fwrite(&s.IndexX, 4, 1, f);
+    fwrite(&s.IndexY, 4, 1, f);
+    fwrite(&s.IndexZ, 4, 1, f);
+    fwrite(&s.Segment, 4, 1, f);
+  }
+  fclose(f);
+}
+
+int cmd_siddon2d() {
+  Image img(size2d());
+  Sinogram2DinCylindrical3Dpet sino(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
+  std::vector<int> counts;
+  std::vector<SiddonSegment> segs;
+  for (int i = 0; i < sino.getNumProj(); i++)
+    for (int j = 0; j < sino.getNumR(); j++) {
+      Point2D p1, p2;
+      float geom;
+      int len = 0;
+      if (sino.getPointsFromLor(i, j, &p1, &p2, &geom)) {
+        Line2D lor;
+        lor.P0 = p1;
+        lor.Vx = p2.X - p1.X;
+        lor.Vy = p2.Y - p1.Y;
+        SiddonSegment* w = nullptr;
+        Siddon(lor, &img, &w, &len, 1);
+        for (int l = 0; l < len; l++) { w[l].IndexZ = 0; segs.push_back(w[l]); }
+        if (len > 0) free(w);
+      } else {
+        len = -1;
+      }
+      counts.push_back(len);
+    }
+  write_lists(arg("out"), counts, segs);
+  return 0;
+}
+
+int cmd_siddon3d() {
+  Image img(size3d());
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  float w = s->getAxialFoV_mm() / s->getNumRings();
+  std::vector<int> counts;
+  std::vector<SiddonSegment> segs;
+  for (int i = 0; i < s->getNumSegments(); i++)
+    for (int j = 0; j < s->getSegment(i)->getNumSinograms(); j++) {
+      Sinogram2DinCylindrical3Dpet* s2 = s->getSegment(i)->getSinogram2D(j);
+      float z1 = (s2->getAxialValue1FromList(0) + s2->getAxialValue1FromList(s2->getNumZ() - 1)) / 2;
+      float z2 = (s2->getAxialValue2FromList(0) + s2->getAxialValue2FromList(s2->getNumZ() - 1)) / 2;
+      for (int k = 0; k < s2->getNumProj(); k++)
+        for (int l = 0; l < s2->getNumR(); l++) {
+          Point3D p1, p2;
+          float geom;
+          s2->getPointsFromLor(k, l, 0, &p1, &p2, &geom);
+          p1.Z = z1 - w / 2 + w / 2 + 0 * w;
+          p2.Z = z2 - w / 2 + w / 2 + 0 * w;
+          Line3D lor;
+          lor.P0 = p1;
+          lor.Vx = p2.X - p1.X;
+          lor.Vy = p2.Y - p1.Y;
+          lor.Vz = p2.Z - p1.Z;
+          SiddonSegment* wl = nullptr;
+          int len = 0;
+          Siddon(lor, &img, &wl, &len, 1);
+          for (int n = 0; n < len; n++) segs.push_back(wl[n]);
+          if (len > 0) free(wl);
+          counts.push_back(len);
+        }
+    }
+  write_lists(arg("out"), counts, segs);
+  return 0;
+}
+
+int cmd_proj2d() {
+  Image img(size2d());
+  size_t npix = (size_t)img.getPixelCount();
+  std::vector<float> x = read_f32(arg("img"), npix);
+  memcpy(img.getPixelsPtr(), x.data(), npix * 4);
+  Sinogram2DinCylindrical3Dpet sino(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
+  SiddonProjector p;
+  double t0;
+  {
+  GuardedPixels guarded(&img);
+  t0 = now_s();
+  p.Project(&img, &sino);
+  report_time("proj2d", t0);
+  }
+  write_f32(arg("out"), sino.getSinogramPtr(), (size_t)sino.getNumProj() * sino.getNumR());
+  return 0;
+}
+
+int cmd_back2d() {
+  Image img(size2d());
+  Sinogram2DinCylindrical3Dpet sino(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
+  size_t nb = (size_t)sino.getNumProj() * sino.getNumR();
+  std::vector<float> y = read_f32(arg("sino"), nb);
+  memcpy(sino.getSinogramPtr(), y.data(), nb * 4);
+  SiddonProjector p;
+  double t0;
+  {
+  GuardedPixels guarded(&img);
+  t0 = now_s();
+  p.Backproject(&sino, &img);
+  report_time("back2d", t0);
+  }
+  write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
+  return 0;
+}
+
+int cmd_mlem2d() {
+  Image init(size2d());
+  size_t npix = (size_t)init.getPixelCount();
+  std::vector<float> x0 = read_f32(arg("init"), npix);
+  memcpy(init.getPixelsPtr(), x0.data(), npix * 4);
+  // The sinogram is built with the scanner radius as FOV; Mlem2d resets the radial
+  // grid to the image FOV (Mlem2d.cpp:12) exactly as it does for the MLEM tool.
+  Sinogram2DinCylindrical3Dpet sino(iarg("nproj"), iarg("nr"), farg("rscanner"), farg("rscanner"));
+  size_t nb = (size_t)sino.getNumProj() * sino.getNumR();
+  std::vector<float> y = read_f32(arg("meas"), nb);
+  memcpy(sino.getSinogramPtr(), y.data(), nb * 4);
+  SiddonProjector p;
+  std::string dir = arg("outdir");
+  int iters = iarg("iters");
+  Mlem2dProbe mlem(&sino, &init, dir, dir + "/mlem2d", iters, 0, false, false, (Projector*)&p, (Projector*)&p);
+  double t0 = now_s();
+  if (!mlem.Reconstruct()) return 6;
+  report_time("mlem2d", t0);
+  write_f32((dir + "/final.f32").c_str(), mlem.getReconstructedImage()->getPixelsPtr(), npix);
+  write_f32((dir + "/sens.f32").c_str(), mlem.sens()->getPixelsPtr(), npix);
+  write_f32((dir + "/lik.f32").c_str(), mlem.lik(), iters + 1);
+  return 0;
+}
+
+int cmd_proj3d() {
+  Image img(size3d());
+  size_t npix = (size_t)img.getPixelCount();
+  std::vector<float> x = read_f32(arg("img"), npix);
+  memcpy(img.getPixelsPtr(), x.data(), npix * 4);
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  SiddonProjector p;
+  double t0;
+  {
+  GuardedPixels guarded(&img);
+  t0 = now_s();
+  p.Project(&img, s);
+  report_time("proj3d", t0);
+  }
+  std::vector<float> out = sino3d_to(s);
+  write_f32(arg("out"), out.data(), out.size());
+  return 0;
+}
+
+int cmd_back3d() {
+  Image img(size3d());
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  std::vector<float> y = read_f32(arg("sino"), bins3d(s));
+  sino3d_from(s, y.data());
+  SiddonProjector p;
+  double t0;
+  {
+  GuardedPixels guarded(&img);
+  t0 = now_s();
+  p.Backproject(s, &img);
+  report_time("back3d", t0);
+  }
+  write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
+  return 0;
+}
+
+int cmd_osem3d() {
+  Image init(size3d());
+  size_t npix = (size_t)init.getPixelCount();
+  std::vector<float> x0 = read_f32(arg("init"), npix);
+  memcpy(init.getPixelsPtr(), x0.data(), npix * 4);
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  std::vector<float> y = read_f32(arg("meas"), bins3d(s));
+  sino3d_from(s, y.data());
+  SiddonProjector p;
+  std::string dir = arg("outdir");
+  int iters = iarg("iters"), subsets = iarg("subsets");
+  OsemProbe osem(s, &init, dir, dir + "/osem3d", iters, 0, false, false, (Projector*)&p, (Projector*)&p, subsets);
+  double t0 = now_s();
+  if (!osem.Reconstruct()) return 6;
+  report_time("osem3d", t0);
+  write_f32((dir + "/final.f32").c_str(), osem.getReconstructedImage()->getPixelsPtr(), npix);
+  std::vector<float> sens(npix * subsets);
+  for (int k = 0; k < subsets; k++) memcpy(sens.data() + k * npix, osem.sens(k)->getPixelsPtr(), npix * 4);
+  write_f32((dir + "/sens.f32").c_str(), sens.data(), sens.size());
+  write_f32((dir + "/lik.f32").c_str(), osem.lik(), iters * subsets + 1);
+  return 0;
+}
+
+int cmd_mlem3d() {
+  Image init(size3d());
+  size_t npix = (size_t)init.getPixelCount();
+  std::vector<float> x0 = read_f32(arg("init"), npix);
+  memcpy(init.getPixelsPtr(), x0.data(), npix * 4);
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  std::vector<float> y = read_f32(arg("meas"), bins3d(s));
+  sino3d_from(s, y.data());
+  SiddonProjector p;
+  std::string dir = arg("outdir");
+  int iters = iarg("iters");
+  Mlem3dProbe mlem(s, &init, dir, dir + "/mlem3d", iters, 0, false, false, (Projector*)&p, (Projector*)&p);
+  double t0 = now_s();
+  if (!mlem.Reconstruct()) return 6;
+  report_time("mlem3d", t0);
+  write_f32((dir + "/final.f32").c_str(), mlem.getReconstructedImage()->getPixelsPtr(), npix);
+  write_f32((dir + "/sens.f32").c_str(), mlem.sens()->getPixelsPtr(), npix);
+  write_f32((dir + "/lik.f32").c_str(), mlem.lik(), iters + 1);
+  return 0;
+}
+
+
+// Reference Siddon on a sample of rays of a 3D geometry: bins b = start + k*stride.
+// For each sampled bin writes (bin, list length, sum of weights, projection of img
+// accumulated with SiddonProjector::Project's rule, SiddonProjector.cpp:413-423).
+int cmd_rays3d() {
+  Image img(size3d());
+  size_t npix = (size_t)img.getPixelCount();
+  std::vector<float> x = read_f32(arg("img"), npix);
+  memcpy(img.getPixelsPtr(), x.data(), npix * 4);
+  SizeImage sz = img.getSize();
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  float w = s->getAxialFoV_mm() / s->getNumRings();
+  long start = atol(arg("start")), stride = atol(arg("stride"));
+  std::vector<float> out;
+  long bin = 0, next = start;
+  for (int i = 0; i < s->getNumSegments(); i++)
+    for (int j = 0; j < s->getSegment(i)->getNumSinograms(); j++) {
+      Sinogram2DinCylindrical3Dpet* s2 = s->getSegment(i)->getSinogram2D(j);
+      float z1 = (s2->getAxialValue1FromList(0) + s2->getAxialValue1FromList(s2->getNumZ() - 1)) / 2;
+      float z2 = (s2->getAxialValue2FromList(0) + s2->getAxialValue2FromList(s2->getNumZ() - 1)) / 2;
+      for (int k = 0; k < s2->getNumProj(); k++)
+        for (int l = 0; l < s2->getNumR(); l++, bin++) {
+          if (bin != next) continue;
+          next += stride;
+          Point3D p1, p2;
+          float geom;
+          s2->getPointsFromLor(k, l, 0, &p1, &p2, &geom);
+          p1.Z = z1 - w / 2 + w / 2 + 0 * w;
+          p2.Z = z2 - w / 2 + w / 2 + 0 * w;
+          Line3D lor;
+          lor.P0 = p1;
+          lor.Vx = p2.X - p1.X;
+          lor.Vy = p2.Y - p1.Y;
+          lor.Vz = p2.Z - p1.Z;
+          SiddonSegment* wl = nullptr;
+          int len = 0;
+          Siddon(lor, &img, &wl, &len, 1);
+          float sw = 0, acc = 0;
+          for (int n = 0; n < len; n++) {
+            sw += wl[n].Segment;
+            if (wl[n].IndexX < sz.nPixelsX && wl[n].IndexY < sz.nPixelsY && wl[n].IndexZ < sz.nPixelsZ) {
+              long idx = (long)wl[n].IndexZ * sz.nPixelsX * sz.nPixelsY + (long)wl[n].IndexY * sz.nPixelsX + wl[n].IndexX;
+              float pix = (idx >= 0 && idx < (long)npix) ? x[idx] : 0.0f;
+              acc += wl[n].Segment * geom * pix;
+            }
+          }
+          if (len > 0) free(wl);
+          float rec[4];
+          memcpy(&rec[0], &bin, 4);  // low 32 bits of the bin index
+          rec[1] = (float)len;
+          rec[2] = sw;
+          rec[3] = acc;
+          out.insert(out.end(), rec, rec + 4);
+        }
+    }
+  write_f32(arg("out"), out.data(), out.size());
+  return 0;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    fprintf(stderr, "usage: ref_driver <command> key=value ...\n");
+    return 1;
+  }
+  for (int i = 2; i < argc; i++) {
+    const char* eq = strchr(argv[i], '=');
+    if (!eq) continue;
+    g_args[std::string(argv[i], eq - argv[i])] = eq + 1;
+  }
+  std::string c = argv[1];
+  if (c == "siddon2d") return cmd_siddon2d();
+  if (c == "siddon3d") return cmd_siddon3d();
+  if (c == "proj2d") return cmd_proj2d();
+  if (c == "back2d") return cmd_back2d();
+  if (c == "mlem2d") return cmd_mlem2d();
+  if (c == "proj3d") return cmd_proj3d();
+  if (c == "back3d") return cmd_back3d();
+  if (c == "osem3d") return cmd_osem3d();
+  if (c == "mlem3d") return cmd_mlem3d();
+  if (c == "rays3d") return cmd_rays3d();
+  fprintf(stderr, "ref_driver: unknown command %s\n", c.c_str());
+  return 1;
+}

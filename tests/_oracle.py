@@ -1,0 +1,251 @@
+"""ctypes binding of the oracle (oracle/liboracle.so): the CPU restatement of the
+reference's Siddon path.  Test infrastructure only -- imported by tests/, by
+``__graft_entry__.smoke()`` and by the CPU-baseline leg of ``bench.py``, always as
+the checker, never as the thing measured or shipped."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+_f = ctypes.c_float
+_i = ctypes.c_int
+_l = ctypes.c_long
+_P = ctypes.c_void_p
+
+
+class OrcImage(ctypes.Structure):
+    _fields_ = [("nx", _i), ("ny", _i), ("nz", _i), ("dx", _f), ("dy", _f), ("dz", _f)]
+
+
+class OrcSeg(ctypes.Structure):
+    _fields_ = [("i", _i), ("j", _i), ("k", _i), ("w", _f)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+        L = ctypes.CDLL(LIB_PATH)
+        P = _P
+        sig = {
+            "orc_sino_grid": (None, [_i, _i, _f, P, P]),
+            "orc_lor_points": (_i, [_f, _f, _f, _f, P, P, P, P]),
+            "orc_ring_table": (_i, [_i, _f, _i, P, P, P, P, P]),
+            "orc_siddon2d": (_i, [_f, _f, _f, _f, P, P, _i]),
+            "orc_siddon3d": (_i, [_f, _f, _f, _f, _f, _f, P, P, _i]),
+            "orc_project2d": (None, [P, P, _i, P, _i, P, _f, _f, P]),
+            "orc_project3d": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
+            "orc_backproject2d": (None, [P, P, _i, P, _i, P, _f, _f, P]),
+            "orc_backproject3d": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
+            "orc_divide_backproject3d": (None, [P, P, P, _i, P, _i, P, _f, _i, P, P, P]),
+            "orc_inverse_divide": (None, [P, P, _l]),
+            "orc_likelihood": (_f, [P, P, _l, _i]),
+            "orc_mlem_threshold": (_f, [P, _l]),
+            "orc_min_nonzero": (_f, [P, _l]),
+            "orc_em_update": (None, [P, P, P, _l, _f, _i]),
+            "orc_subset_angles": (_i, [_i, _i, _i, P]),
+            "orc_mlem2d": (_i, [P, _i, _i, _f, P, P, _i, P, P, P]),
+            "orc_mlem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, P, P, P]),
+            "orc_osem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P, P, P]),
+            "orc_project3d_f64": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
+            "orc_backproject3d_f64": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(_P)
+
+
+def f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def image_desc(nx, ny, nz, dx, dz=1.0):
+    return OrcImage(nx, ny, nz, dx, dx, dz)
+
+
+def sino_grid(nang, nr, rfov):
+    ang = np.zeros(nang, np.float32)
+    r = np.zeros(nr, np.float32)
+    lib().orc_sino_grid(nang, nr, rfov, _p(ang), _p(r))
+    return ang, r
+
+
+def seg_table(rings, span, mrd):
+    """Segment table ordered 0, +1, -1, +2, -2 ... (min/max ring difference per segment)."""
+    h = (span - 1) // 2
+    nsino, mn, mx = [2 * rings - 1], [-h], [h]
+    lo = h + 1
+    while lo <= mrd:
+        hi = min(lo + span - 1, mrd)
+        n = sum(1 for s in range(2 * rings - 1)
+                if any(0 <= s - b < rings and lo <= s - 2 * b <= hi for b in range(rings)))
+        nsino += [n, n]
+        mn += [lo, -hi]
+        mx += [hi, -lo]
+        lo += span
+    return (np.array(nsino, np.int32), np.array(mn, np.int32), np.array(mx, np.int32))
+
+
+def ring_table(rings, zfov, span, mrd):
+    nsino, mn, mx = seg_table(rings, span, mrd)
+    tot = int(nsino.sum())
+    za = np.zeros(tot, np.float32)
+    zb = np.zeros(tot, np.float32)
+    n = lib().orc_ring_table(rings, zfov, len(nsino), _p(nsino), _p(mn), _p(mx), _p(za), _p(zb))
+    assert n == tot, (n, tot)
+    return za, zb
+
+
+def siddon2d(p0, v, im, cap=4096):
+    out = (OrcSeg * cap)()
+    n = lib().orc_siddon2d(p0[0], p0[1], v[0], v[1], ctypes.byref(im), out, cap)
+    return [(out[m].i, out[m].j, out[m].w) for m in range(min(n, cap))]
+
+
+def siddon3d(p0, v, im, cap=8192):
+    out = (OrcSeg * cap)()
+    n = lib().orc_siddon3d(p0[0], p0[1], p0[2], v[0], v[1], v[2], ctypes.byref(im), out, cap)
+    return [(out[m].i, out[m].j, out[m].k, out[m].w) for m in range(min(n, cap))]
+
+
+def lor_points(ang, r, rscanner, rfov=float("inf")):
+    a, b, c, d = _f(), _f(), _f(), _f()
+    ok = lib().orc_lor_points(ang, r, rscanner, rfov, ctypes.byref(a), ctypes.byref(b),
+                              ctypes.byref(c), ctypes.byref(d))
+    return ok, (a.value, b.value), (c.value, d.value)
+
+
+def project2d(im, img, ang, r, rfov, rscanner, sino=None):
+    img = f32(img)
+    if sino is None:
+        sino = np.zeros((len(ang), len(r)), np.float32)
+    lib().orc_project2d(ctypes.byref(im), _p(img), len(ang), _p(f32(ang)), len(r), _p(f32(r)),
+                        rfov, rscanner, _p(sino))
+    return sino
+
+
+def backproject2d(im, sino, ang, r, rfov, rscanner, img=None):
+    sino = f32(sino)
+    if img is None:
+        img = np.zeros((im.ny, im.nx), np.float32)
+    lib().orc_backproject2d(ctypes.byref(im), _p(sino), len(ang), _p(f32(ang)), len(r),
+                            _p(f32(r)), rfov, rscanner, _p(img))
+    return img
+
+
+def project3d(im, img, ang, r, rscanner, za, zb):
+    img = f32(img)
+    sino = np.zeros((len(za), len(ang), len(r)), np.float32)
+    lib().orc_project3d(ctypes.byref(im), _p(img), len(ang), _p(f32(ang)), len(r), _p(f32(r)),
+                        rscanner, len(za), _p(f32(za)), _p(f32(zb)), _p(sino))
+    return sino
+
+
+def backproject3d(im, sino, ang, r, rscanner, za, zb, img=None):
+    sino = f32(sino)
+    if img is None:
+        img = np.zeros((im.nz, im.ny, im.nx), np.float32)
+    lib().orc_backproject3d(ctypes.byref(im), _p(sino), len(ang), _p(f32(ang)), len(r),
+                            _p(f32(r)), rscanner, len(za), _p(f32(za)), _p(f32(zb)), _p(img))
+    return img
+
+
+def divide_backproject3d(im, meas, est, ang, r, rscanner, za, zb, img=None):
+    meas, est = f32(meas), f32(est)
+    if img is None:
+        img = np.zeros((im.nz, im.ny, im.nx), np.float32)
+    lib().orc_divide_backproject3d(ctypes.byref(im), _p(meas), _p(est), len(ang), _p(f32(ang)),
+                                   len(r), _p(f32(r)), rscanner, len(za), _p(f32(za)),
+                                   _p(f32(zb)), _p(img))
+    return img
+
+
+def project3d_f64(im, img, ang, r, rscanner, za, zb):
+    img = np.ascontiguousarray(img, np.float64)
+    sino = np.zeros((len(za), len(ang), len(r)), np.float64)
+    lib().orc_project3d_f64(ctypes.byref(im), _p(img), len(ang), _p(f32(ang)), len(r), _p(f32(r)),
+                            rscanner, len(za), _p(f32(za)), _p(f32(zb)), _p(sino))
+    return sino
+
+
+def backproject3d_f64(im, sino, ang, r, rscanner, za, zb):
+    sino = np.ascontiguousarray(sino, np.float64)
+    img = np.zeros((im.nz, im.ny, im.nx), np.float64)
+    lib().orc_backproject3d_f64(ctypes.byref(im), _p(sino), len(ang), _p(f32(ang)), len(r),
+                                _p(f32(r)), rscanner, len(za), _p(f32(za)), _p(f32(zb)), _p(img))
+    return img
+
+
+def inverse_divide(est, meas):
+    est = f32(est).copy()
+    lib().orc_inverse_divide(_p(est), _p(f32(meas)), est.size)
+    return est
+
+
+def likelihood(est, meas, positive_only):
+    return lib().orc_likelihood(_p(f32(est)), _p(f32(meas)), f32(est).size, int(positive_only))
+
+
+def mlem_threshold(sens):
+    return lib().orc_mlem_threshold(_p(f32(sens)), f32(sens).size)
+
+
+def em_update(x, b, s, thr, strict):
+    x = f32(x).copy()
+    lib().orc_em_update(_p(x), _p(f32(b)), _p(f32(s)), x.size, thr, int(strict))
+    return x
+
+
+def subset_angles(nang, nsub, s):
+    idx = np.zeros(nang, np.int32)
+    n = lib().orc_subset_angles(nang, nsub, s, _p(idx))
+    return idx[:n].copy()
+
+
+def mlem2d(im, nang, nr, rscanner, meas, init, iters):
+    npix = im.nx * im.ny
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix, np.float32)
+    lik = np.zeros(iters + 1, np.float32)
+    lib().orc_mlem2d(ctypes.byref(im), nang, nr, rscanner, _p(f32(meas)), _p(f32(init)), iters,
+                     _p(img), _p(sens), _p(lik))
+    return img.reshape(im.ny, im.nx), sens.reshape(im.ny, im.nx), lik
+
+
+def mlem3d(im, nang, nr, rfov, rscanner, za, zb, meas, init, iters):
+    npix = im.nx * im.ny * im.nz
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix, np.float32)
+    lik = np.zeros(iters + 1, np.float32)
+    lib().orc_mlem3d(ctypes.byref(im), nang, nr, rfov, rscanner, len(za), _p(f32(za)), _p(f32(zb)),
+                     _p(f32(meas)), _p(f32(init)), iters, _p(img), _p(sens), _p(lik))
+    shp = (im.nz, im.ny, im.nx)
+    return img.reshape(shp), sens.reshape(shp), lik
+
+
+def osem3d(im, nang, nr, rfov, rscanner, za, zb, meas, init, nsub, iters):
+    npix = im.nx * im.ny * im.nz
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix * nsub, np.float32)
+    lik = np.zeros(iters * nsub + 1, np.float32)
+    lib().orc_osem3d(ctypes.byref(im), nang, nr, rfov, rscanner, len(za), _p(f32(za)),
+                     _p(f32(zb)), _p(f32(meas)), _p(f32(init)), nsub, iters, _p(img), _p(sens),
+                     _p(lik))
+    shp = (im.nz, im.ny, im.nx)
+    return img.reshape(shp), sens.reshape((nsub,) + shp), lik

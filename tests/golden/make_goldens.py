@@ -1,0 +1,181 @@
+"""Writes the golden fixtures of tests/golden/ by running the reference's OWN CPU
+Siddon path (oracle/_ref/ref_driver, built by oracle/ref.mk from the reference's
+sources in this container) with OMP_NUM_THREADS=1, so that its float atomics run
+in a fixed order (recon/src/SiddonProjector.cpp:73-74,547-548).
+
+Each fixture is an .npz holding the inputs, the geometry and the reference's
+outputs; nothing of the reference's source text is stored.  Run from the repo
+root:  python tests/golden/make_goldens.py
+"""
+import json
+import os
+import platform
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+from apirl_amd import phantoms  # noqa: E402
+
+DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+TMP = tempfile.mkdtemp(prefix="apirl_golden_")
+
+
+def run(cmd, threads=1, **kw):
+    args = [DRIVER, cmd] + [f"{k}={v}" for k, v in kw.items()]
+    env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+    out = subprocess.run(args, env=env, check=True, capture_output=True, text=True)
+    return out.stdout
+
+
+def tmpf(name):
+    return os.path.join(TMP, name)
+
+
+def read(name, dtype=np.float32):
+    return np.fromfile(tmpf(name), dtype)
+
+
+def read_lists(path):
+    raw = np.fromfile(path, np.int32)
+    nb = raw[0]
+    counts = raw[1:1 + nb]
+    segs = raw[1 + nb:].reshape(-1, 4)
+    return counts, segs[:, :3].copy(), segs[:, 3].copy().view(np.float32)
+
+
+def meta():
+    return json.dumps({"glibc": platform.libc_ver()[1], "omp_threads": 1,
+                       "generator": "tests/golden/make_goldens.py"})
+
+
+def save(name, **arrays):
+    path = os.path.join(HERE, name)
+    np.savez_compressed(path, meta=np.array(meta()), **arrays)
+    print(f"wrote {name}: {os.path.getsize(path) / 1024:.0f} KiB")
+
+
+def geom3d_args(g):
+    keys = ["nx", "ny", "nz", "dx", "dz", "nproj", "nr", "rings", "rfov", "zfov", "rscanner",
+            "span", "mrd"]
+    return {k: g[k] for k in keys}
+
+
+def siddon_lists():
+    g2 = dict(nx=16, dx=2.0, nproj=24, nr=20, rfov=16.0, rscanner=40.0)
+    run("siddon2d", **g2, out=tmpf("l2.bin"))
+    c, idx, w = read_lists(tmpf("l2.bin"))
+    save("siddon2d_lists.npz", geom=np.array(json.dumps(g2)), counts=c, idx=idx, w=w)
+    g3 = dict(nx=16, ny=16, nz=9, dx=2.0, dz=2.0, nproj=12, nr=16, rings=5, rfov=16.0,
+              zfov=18.0, rscanner=40.0, span=3, mrd=3)
+    run("siddon3d", **geom3d_args(g3), out=tmpf("l3.bin"))
+    c, idx, w = read_lists(tmpf("l3.bin"))
+    save("siddon3d_lists.npz", geom=np.array(json.dumps(g3)), counts=c, idx=idx, w=w)
+
+
+def c1():
+    """2D MLEM workload: 128x128 @ 2 mm, 192 angles x 192 r, scanner radius 400 mm.
+    Mlem2d resets the radial grid to the image FOV (128 mm), recon/src/Mlem2d.cpp:12."""
+    g = dict(nx=128, dx=2.0, nproj=192, nr=192, rfov=128.0, rscanner=400.0)
+    img = phantoms.disc2d(128, 2.0)
+    img.tofile(tmpf("c1_img.f32"))
+    run("proj2d", **g, img=tmpf("c1_img.f32"), out=tmpf("c1_proj.f32"))
+    proj = read("c1_proj.f32").reshape(192, 192)
+    meas = phantoms.poisson_counts(proj, 1e6, np.random.default_rng(0))
+    meas.tofile(tmpf("c1_meas.f32"))
+    run("back2d", **g, sino=tmpf("c1_meas.f32"), out=tmpf("c1_back.f32"))
+    back = read("c1_back.f32").reshape(128, 128)
+    save("c1_project_backproject.npz", geom=np.array(json.dumps(g)), img=img, proj=proj,
+         meas=meas.astype(np.uint16), back=back)
+    init = np.ones((128, 128), np.float32)
+    init.tofile(tmpf("c1_init.f32"))
+    iters = 60
+    run("mlem2d", nx=128, dx=2.0, nproj=192, nr=192, rscanner=400.0, iters=iters,
+        meas=tmpf("c1_meas.f32"), init=tmpf("c1_init.f32"), outdir=TMP)
+    save("c1_mlem2d.npz", geom=np.array(json.dumps(dict(g, iters=iters))),
+         meas=meas.astype(np.uint16), final=read("final.f32").reshape(128, 128),
+         sens=read("sens.f32").reshape(128, 128), lik=read("lik.f32"))
+
+
+MID = dict(nx=64, ny=64, nz=17, dx=2.0445, dz=2.027, nproj=42, nr=86, rings=13,
+           rfov=65.424, zfov=34.459, rscanner=400.0, span=5, mrd=8)
+
+
+def mid3d():
+    """3D geometry with the 256x256x81 workload's voxel and bin pitch, scaled down."""
+    g = MID
+    img = phantoms.cylinder3d(64, 17, 2.0445, 2.027, radius=40.0)
+    img[8, 30:34, 36:40] = 5.0
+    img.tofile(tmpf("m_img.f32"))
+    run("proj3d", **geom3d_args(g), img=tmpf("m_img.f32"), out=tmpf("m_proj.f32"))
+    proj = read("m_proj.f32")
+    meas = phantoms.poisson_counts(proj, 2e6, np.random.default_rng(1))
+    meas.tofile(tmpf("m_meas.f32"))
+    run("back3d", **geom3d_args(g), sino=tmpf("m_meas.f32"), out=tmpf("m_back.f32"))
+    back = read("m_back.f32").reshape(17, 64, 64)
+    save("mid3d_project_backproject.npz", geom=np.array(json.dumps(g)), img=img, proj=proj,
+         meas=meas.astype(np.uint16), back=back)
+
+
+# OSEM runs the reference's own EM driver, whose images it allocates itself, so its
+# geometry keeps every ray above slice 0 (ring z >= 1.5 dz): then no entry index of
+# -1 can address memory before the image (SiddonProjector.cpp:417).
+MID_OSEM = dict(nx=64, ny=64, nz=32, dx=2.0445, dz=1.0, nproj=42, nr=86, rings=10,
+                rfov=65.424, zfov=32.0, rscanner=400.0, span=3, mrd=5)
+
+
+def mid3d_osem():
+    g = MID_OSEM
+    img = phantoms.cylinder3d(64, 32, 2.0445, 1.0, radius=40.0)
+    img[12:16, 30:34, 36:40] = 5.0
+    img.tofile(tmpf("o_img.f32"))
+    run("proj3d", **geom3d_args(g), img=tmpf("o_img.f32"), out=tmpf("o_proj.f32"))
+    proj = read("o_proj.f32")
+    meas = phantoms.poisson_counts(proj, 2e6, np.random.default_rng(2))
+    meas.tofile(tmpf("o_meas.f32"))
+    init = np.ones((32, 64, 64), np.float32)
+    init.tofile(tmpf("o_init.f32"))
+    run("osem3d", **geom3d_args(g), subsets=7, iters=2, meas=tmpf("o_meas.f32"),
+        init=tmpf("o_init.f32"), outdir=TMP)
+    sens = read("sens.f32").reshape(7, 32, 64, 64)
+    save("mid3d_osem.npz", geom=np.array(json.dumps(dict(g, subsets=7, iters=2))),
+         meas=meas.astype(np.uint16), final=read("final.f32").reshape(32, 64, 64),
+         sens0=sens[0], sens6=sens[6], sens_sum=sens.sum(axis=(1, 2, 3), dtype=np.float64),
+         lik=read("lik.f32"))
+
+
+C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
+          rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
+
+
+def c2_rays():
+    """Reference Siddon on a sample of the 256x256x81 / 344x252x837 workload's rays."""
+    g = C2
+    img, pts = phantoms.cylinder_points3d(256, 81, 2.0445, 2.027)
+    img.tofile(tmpf("c2_img.f32"))
+    start, stride = 17, 4099
+    run("rays3d", threads=1, **geom3d_args(g), img=tmpf("c2_img.f32"), start=start,
+        stride=stride, out=tmpf("c2_rays.f32"))
+    rec = read("c2_rays.f32").reshape(-1, 4)
+    save("c2_rays.npz", geom=np.array(json.dumps(dict(g, start=start, stride=stride))),
+         points=np.array(pts, np.int32), count=rec[:, 1].astype(np.int32), wsum=rec[:, 2],
+         proj=rec[:, 3])
+
+
+if __name__ == "__main__":
+    if not os.path.exists(DRIVER):
+        subprocess.check_call(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT)
+    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2"]
+    if "lists" in which:
+        siddon_lists()
+    if "c1" in which:
+        c1()
+    if "mid3d" in which:
+        mid3d()
+        mid3d_osem()
+    if "c2" in which:
+        c2_rays()
