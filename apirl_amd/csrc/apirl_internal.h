@@ -1,0 +1,119 @@
+// Internal types shared by the host side (capi.cpp, geometry.cpp) and the gfx950
+// kernels (siddon_kernels.hip).  Not part of the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "apirl_hip.h"
+
+namespace apirl {
+
+// Per-LOR (angle, r) part of the Siddon setup that depends only on the transaxial
+// geometry: recon/src/Siddon.cpp:83-145 (square-FOV alpha clip and clamp),
+// :148-152/:494-498 (entry/exit points), :177-181/:509-512 (entry indices),
+// :253-286/:583-606 (alpha increments and first crossings) and the xy part of the
+// ray length (:248-250/:579-580).  Computed on the host in float with the reference's
+// operation order and uploaded once; 64 bytes so one lane reads one line.
+struct alignas(16) RayXY {
+  float p0x, p0y;  // LOR.P0 (P1 of the bin)
+  float vx, vy;    // LOR.Vx, LOR.Vy = P2 - P1
+  float amin, amax;
+  float ax, ay;    // first x / y crossing (2D: negative values clamped to amin)
+  float axu, ayu;  // |dx / Vx|, |dy / Vy|
+  float lxy;       // ((P0.X+Vx)-P0.X)^2 + ((P0.Y+Vy)-P0.Y)^2
+  int i0, j0;      // entry voxel indices (may be -1 or n)
+  int nxy;         // |imax - imin| + |jmax - jmin|
+  int flags;       // see RAY_* below
+  int pad;
+};
+static_assert(sizeof(RayXY) == 64, "RayXY must be one 64-byte line");
+
+enum : int {
+  RAY_VALID = 1,     // 2D: r <= rfov (Sinogram2DinCylindrical3Dpet.cpp:256-260)
+  RAY_REJECTED = 2,  // 2D: both ends outside on one side (Siddon.cpp:515-519)
+};
+__host__ __device__ inline int ray_iinc(int flags) { return ((flags >> 2) & 3) - 1; }
+__host__ __device__ inline int ray_jinc(int flags) { return ((flags >> 4) & 3) - 1; }
+inline int ray_flags(int iinc, int jinc, int valid, int rejected) {
+  return (valid ? RAY_VALID : 0) | (rejected ? RAY_REJECTED : 0) | ((iinc + 1) << 2) |
+         ((jinc + 1) << 4);
+}
+
+// Geometry constants passed by value to every kernel.
+struct Geo {
+  int nx, ny, nz;
+  float dx, dy, dz;
+  float R;     // (dx * nx) / 2, Images.cpp:55
+  float offz;  // dz / 2, Siddon.cpp:162
+  int nr, nang, nsino;
+  int dims;
+};
+
+}  // namespace apirl
+
+struct apirl_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // scratch owned by the context (likelihood partials, reductions)
+  double* d_red = nullptr;
+  size_t red_cap = 0;
+  float* d_fred = nullptr;
+  unsigned long long* d_ured = nullptr;
+};
+
+struct apirl_plan {
+  apirl_ctx* ctx = nullptr;
+  apirl_image_desc img{};
+  apirl_sino_desc sino{};   // pointers inside are not kept
+  apirl::Geo geo{};
+  apirl::RayXY* d_rays = nullptr;  // [nang][nr]
+  float2* d_zab = nullptr;         // [nsino] (P1.Z, P2.Z), 3D only
+  std::vector<float> ang, r;       // host copies of the grids
+  std::vector<float2> zab;
+};
+
+namespace apirl {
+
+// geometry.cpp
+void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
+                     const std::vector<float>& ang, const std::vector<float>& r,
+                     std::vector<RayXY>& out);
+void build_z_table(const apirl_sino_desc& sino, std::vector<float2>& zab);
+int subset_num_angles(int nang, int nsubsets, int subset);
+
+// siddon_kernels.hip: launchers (enqueue on stream; return hipError_t)
+struct FusedRatio {
+  const float* meas = nullptr;      // full sinogram
+  const float* additive = nullptr;  // full sinogram or null
+  float* ratio = nullptr;           // subset sinogram
+  double* lik_partials = nullptr;   // one per block, or null
+  int lik_positive_only = 0;
+};
+hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, int subset,
+                          int nsubsets, int precision, const FusedRatio* fused, int* nblocks_out,
+                          hipStream_t st);
+hipError_t launch_backproject(const apirl_plan* p, const float* sino, const float* est,
+                              float* img, int subset, int nsubsets, int kind, int mode,
+                              hipStream_t st);
+hipError_t launch_em_update(float* x, const float* b, const float* s, size_t n, float thr,
+                            int strict, hipStream_t st);
+hipError_t launch_force_positive(float* x, size_t n, hipStream_t st);
+hipError_t launch_sum_partials(const double* partials, int n, double* out, hipStream_t st);
+// Likelihood sum y log(x) - x over x > 0 (positive_only) or x != 0, deterministic:
+// block partials into `partials`, then a fixed-order sum into *out.
+hipError_t launch_likelihood(const float* x, const float* y, long n, int positive_only,
+                             double* partials, double* out, hipStream_t st);
+hipError_t launch_min_nz_max(const float* x, size_t n, float* d_out2, hipStream_t st);
+hipError_t launch_count_intersections(const apirl_plan* p, int subset, int nsubsets,
+                                      unsigned long long* d_out, hipStream_t st);
+
+// Backprojection kinds
+enum { BP_PLAIN = 0, BP_ONES = 1, BP_DIVIDE = 2, BP_MULT = 3 };
+
+}  // namespace apirl

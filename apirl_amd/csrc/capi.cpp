@@ -1,0 +1,455 @@
+// C-ABI of the HIP Siddon path: contexts, geometry plans, the Projector entry points
+// and the device-resident MLEM / OSEM drivers.
+#include <cfloat>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "apirl_internal.h"
+
+using apirl::Geo;
+
+namespace {
+
+apirl_status fail(apirl_ctx* ctx, apirl_status code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+apirl_status hip_fail(apirl_ctx* ctx, hipError_t e, const char* where) {
+  std::string m = std::string(where) + ": " + hipGetErrorString(e);
+  return fail(ctx, APIRL_ERR_HIP, m);
+}
+
+#define APIRL_HIP(ctx, call)                             \
+  do {                                                   \
+    hipError_t e_ = (call);                              \
+    if (e_ != hipSuccess) return hip_fail(ctx, e_, #call); \
+  } while (0)
+
+// Makes the plan's device current for the duration of a call.
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+apirl_status ensure_red(apirl_ctx* ctx, size_t n) {
+  if (ctx->red_cap >= n) return APIRL_OK;
+  if (ctx->d_red) (void)hipFree(ctx->d_red);
+  ctx->d_red = nullptr;
+  ctx->red_cap = 0;
+  size_t cap = n < 1024 ? 1024 : n;
+  APIRL_HIP(ctx, hipMalloc(&ctx->d_red, (cap + 8) * sizeof(double)));
+  ctx->red_cap = cap;
+  return APIRL_OK;
+}
+
+bool check_subset(const apirl_plan* p, int subset, int nsubsets) {
+  return nsubsets >= 1 && subset >= 0 && subset < nsubsets && nsubsets <= p->geo.nang;
+}
+
+}  // namespace
+
+extern "C" {
+
+apirl_status apirl_ctx_create(int device, apirl_ctx** out) {
+  if (!out) return APIRL_ERR_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return APIRL_ERR_HIP;
+  if (device < 0 || device >= n) return APIRL_ERR_INVALID;
+  apirl_ctx* c = new apirl_ctx();
+  c->device = device;
+  DeviceGuard g(device);
+  if (hipMalloc(&c->d_fred, 64 * sizeof(float)) != hipSuccess ||
+      hipMalloc(&c->d_ured, 8 * sizeof(unsigned long long)) != hipSuccess) {
+    delete c;
+    return APIRL_ERR_NOMEM;
+  }
+  *out = c;
+  return APIRL_OK;
+}
+
+apirl_status apirl_ctx_destroy(apirl_ctx* ctx) {
+  if (!ctx) return APIRL_OK;
+  DeviceGuard g(ctx->device);
+  if (ctx->d_red) (void)hipFree(ctx->d_red);
+  if (ctx->d_fred) (void)hipFree(ctx->d_fred);
+  if (ctx->d_ured) (void)hipFree(ctx->d_ured);
+  delete ctx;
+  return APIRL_OK;
+}
+
+const char* apirl_last_error(const apirl_ctx* ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+apirl_status apirl_ctx_set_stream(apirl_ctx* ctx, void* stream) {
+  if (!ctx) return APIRL_ERR_INVALID;
+  ctx->stream = (hipStream_t)stream;
+  return APIRL_OK;
+}
+
+apirl_status apirl_ctx_synchronize(apirl_ctx* ctx) {
+  if (!ctx) return APIRL_ERR_INVALID;
+  DeviceGuard g(ctx->device);
+  APIRL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_malloc(apirl_ctx* ctx, void** dptr, size_t bytes) {
+  if (!ctx || !dptr) return APIRL_ERR_INVALID;
+  DeviceGuard g(ctx->device);
+  if (hipMalloc(dptr, bytes ? bytes : 4) != hipSuccess)
+    return fail(ctx, APIRL_ERR_NOMEM, "hipMalloc failed");
+  return APIRL_OK;
+}
+
+apirl_status apirl_free(apirl_ctx* ctx, void* dptr) {
+  if (!ctx) return APIRL_ERR_INVALID;
+  DeviceGuard g(ctx->device);
+  APIRL_HIP(ctx, hipFree(dptr));
+  return APIRL_OK;
+}
+
+apirl_status apirl_memcpy_h2d(apirl_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return APIRL_ERR_INVALID;
+  DeviceGuard g(ctx->device);
+  APIRL_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  APIRL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_memcpy_d2h(apirl_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return APIRL_ERR_INVALID;
+  DeviceGuard g(ctx->device);
+  APIRL_HIP(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  APIRL_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_memset(apirl_ctx* ctx, void* dptr, int value, size_t bytes) {
+  if (!ctx) return APIRL_ERR_INVALID;
+  DeviceGuard g(ctx->device);
+  APIRL_HIP(ctx, hipMemsetAsync(dptr, value, bytes, ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
+                               const apirl_sino_desc* sino, apirl_plan** out) {
+  if (!ctx || !image || !sino || !out) return APIRL_ERR_INVALID;
+  *out = nullptr;
+  const apirl_image_desc& im = *image;
+  const apirl_sino_desc& sd = *sino;
+  if (im.nx <= 0 || im.ny <= 0 || im.nz <= 0 || !(im.dx > 0) || !(im.dy > 0) || !(im.dz > 0))
+    return fail(ctx, APIRL_ERR_INVALID, "invalid image size");
+  if (sd.dims != 2 && sd.dims != 3) return fail(ctx, APIRL_ERR_INVALID, "dims must be 2 or 3");
+  if (sd.nang <= 0 || sd.nr <= 0 || sd.nsino <= 0)
+    return fail(ctx, APIRL_ERR_INVALID, "invalid sinogram size");
+  if (sd.dims == 2 && (sd.nsino != 1 || im.nz != 1))
+    return fail(ctx, APIRL_ERR_INVALID, "2D plans take one sinogram and a one-slice image");
+  if (sd.dims == 3 && (!sd.z1_mean_mm || !sd.z2_mean_mm || !(sd.ring_width_mm > 0)))
+    return fail(ctx, APIRL_ERR_INVALID, "3D plans need ring coordinates and ring width");
+  if (!(sd.rscanner_mm > 0)) return fail(ctx, APIRL_ERR_INVALID, "invalid scanner radius");
+  if ((double)im.nx * im.ny * im.nz > 2147483647.0 ||
+      (double)sd.nsino * sd.nang * sd.nr > 9.0e15)
+    return fail(ctx, APIRL_ERR_INVALID, "problem too large");
+  DeviceGuard g(ctx->device);
+  apirl_plan* p = new apirl_plan();
+  p->ctx = ctx;
+  p->img = im;
+  p->sino = sd;
+  p->sino.ang_deg = p->sino.r_mm = p->sino.z1_mean_mm = p->sino.z2_mean_mm = nullptr;
+  p->ang.resize(sd.nang);
+  p->r.resize(sd.nr);
+  {
+    std::vector<float> ga(sd.nang), gr(sd.nr);
+    apirl_sino_grid(sd.nang, sd.nr, sd.rfov_mm, ga.data(), gr.data());
+    for (int i = 0; i < sd.nang; i++) p->ang[i] = sd.ang_deg ? sd.ang_deg[i] : ga[i];
+    for (int j = 0; j < sd.nr; j++) p->r[j] = sd.r_mm ? sd.r_mm[j] : gr[j];
+  }
+  std::vector<apirl::RayXY> rays;
+  apirl::build_ray_table(im, sd, p->ang, p->r, rays);
+  Geo& geo = p->geo;
+  geo.nx = im.nx; geo.ny = im.ny; geo.nz = im.nz;
+  geo.dx = im.dx; geo.dy = im.dy; geo.dz = im.dz;
+  geo.R = (im.dx * im.nx) / 2;
+  geo.offz = im.dz / 2;
+  geo.nr = sd.nr; geo.nang = sd.nang; geo.nsino = sd.nsino;
+  geo.dims = sd.dims;
+  if (hipMalloc(&p->d_rays, rays.size() * sizeof(apirl::RayXY)) != hipSuccess) {
+    delete p;
+    return fail(ctx, APIRL_ERR_NOMEM, "cannot allocate the LOR table");
+  }
+  if (hipMemcpy(p->d_rays, rays.data(), rays.size() * sizeof(apirl::RayXY),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(p->d_rays);
+    delete p;
+    return fail(ctx, APIRL_ERR_HIP, "cannot upload the LOR table");
+  }
+  if (sd.dims == 3) {
+    apirl::build_z_table(sd, p->zab);
+    if (hipMalloc(&p->d_zab, p->zab.size() * sizeof(float2)) != hipSuccess ||
+        hipMemcpy(p->d_zab, p->zab.data(), p->zab.size() * sizeof(float2),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(p->d_rays);
+      if (p->d_zab) (void)hipFree(p->d_zab);
+      delete p;
+      return fail(ctx, APIRL_ERR_NOMEM, "cannot upload the ring table");
+    }
+  }
+  *out = p;
+  return APIRL_OK;
+}
+
+apirl_status apirl_plan_destroy(apirl_plan* p) {
+  if (!p) return APIRL_OK;
+  DeviceGuard g(p->ctx->device);
+  if (p->d_rays) (void)hipFree(p->d_rays);
+  if (p->d_zab) (void)hipFree(p->d_zab);
+  delete p;
+  return APIRL_OK;
+}
+
+size_t apirl_plan_subset_bins(const apirl_plan* p, int subset, int nsubsets) {
+  if (!p || !check_subset(p, subset, nsubsets)) return 0;
+  return (size_t)p->geo.nsino * apirl::subset_num_angles(p->geo.nang, nsubsets, subset) *
+         p->geo.nr;
+}
+
+size_t apirl_plan_image_voxels(const apirl_plan* p) {
+  return p ? (size_t)p->geo.nx * p->geo.ny * p->geo.nz : 0;
+}
+
+apirl_status apirl_plan_intersections(apirl_plan* p, int subset, int nsubsets,
+                                      unsigned long long* out) {
+  if (!p || !out || !check_subset(p, subset, nsubsets)) return APIRL_ERR_INVALID;
+  apirl_ctx* c = p->ctx;
+  DeviceGuard g(c->device);
+  APIRL_HIP(c, hipMemsetAsync(c->d_ured, 0, sizeof(unsigned long long), c->stream));
+  APIRL_HIP(c, apirl::launch_count_intersections(p, subset, nsubsets, c->d_ured, c->stream));
+  APIRL_HIP(c, hipMemcpyAsync(out, c->d_ured, sizeof(unsigned long long),
+                              hipMemcpyDeviceToHost, c->stream));
+  APIRL_HIP(c, hipStreamSynchronize(c->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_project(apirl_plan* p, const float* d_img, float* d_sino, int subset,
+                           int nsubsets, int precision) {
+  if (!p || !d_img || !d_sino) return APIRL_ERR_INVALID;
+  if (!check_subset(p, subset, nsubsets))
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
+  if (precision < 0 || precision > 2) return fail(p->ctx, APIRL_ERR_INVALID, "invalid precision");
+  DeviceGuard g(p->ctx->device);
+  APIRL_HIP(p->ctx, apirl::launch_project(p, d_img, d_sino, subset, nsubsets, precision,
+                                          nullptr, nullptr, p->ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_backproject(apirl_plan* p, const float* d_sino, float* d_img, int subset,
+                               int nsubsets, int mode) {
+  if (!p || !d_img || !d_sino) return APIRL_ERR_INVALID;
+  if (!check_subset(p, subset, nsubsets))
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
+  if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
+  DeviceGuard g(p->ctx->device);
+  APIRL_HIP(p->ctx, apirl::launch_backproject(p, d_sino, nullptr, d_img, subset, nsubsets,
+                                              apirl::BP_PLAIN, mode, p->ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const float* d_est,
+                                      float* d_img, int subset, int nsubsets, int mode) {
+  if (!p || !d_meas || !d_est || !d_img) return APIRL_ERR_INVALID;
+  if (!check_subset(p, subset, nsubsets))
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
+  if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
+  DeviceGuard g(p->ctx->device);
+  APIRL_HIP(p->ctx, apirl::launch_backproject(p, d_meas, d_est, d_img, subset, nsubsets,
+                                              apirl::BP_DIVIDE, mode, p->ctx->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float* d_meas,
+                                 const float* d_additive, float* d_ratio, double* d_lik,
+                                 int subset, int nsubsets, int lik_positive_only,
+                                 int precision) {
+  if (!p || !d_img || !d_meas || !d_ratio) return APIRL_ERR_INVALID;
+  if (!check_subset(p, subset, nsubsets))
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
+  apirl_ctx* c = p->ctx;
+  DeviceGuard g(c->device);
+  const long nrays = (long)apirl_plan_subset_bins(p, subset, nsubsets);
+  const int nblocks = (int)((nrays + 255) / 256);
+  apirl::FusedRatio fr;
+  fr.meas = d_meas;
+  fr.additive = d_additive;
+  fr.ratio = d_ratio;
+  fr.lik_positive_only = lik_positive_only;
+  if (d_lik) {
+    apirl_status st = ensure_red(c, (size_t)nblocks);
+    if (st != APIRL_OK) return st;
+    fr.lik_partials = c->d_red;
+  }
+  APIRL_HIP(c, apirl::launch_project(p, d_img, nullptr, subset, nsubsets, precision, &fr,
+                                     nullptr, c->stream));
+  if (d_lik) APIRL_HIP(c, apirl::launch_sum_partials(c->d_red, nblocks, d_lik, c->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_em_update(apirl_ctx* c, float* d_x, const float* d_b, const float* d_s,
+                             size_t n, float thr, int strict) {
+  if (!c || !d_x || !d_b || !d_s) return APIRL_ERR_INVALID;
+  DeviceGuard g(c->device);
+  APIRL_HIP(c, apirl::launch_em_update(d_x, d_b, d_s, n, thr, strict, c->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_sensitivity(apirl_plan* p, const float* d_mult, float* d_sens, int subset,
+                               int nsubsets, int mode, int force_positive) {
+  if (!p || !d_sens) return APIRL_ERR_INVALID;
+  if (!check_subset(p, subset, nsubsets))
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
+  apirl_ctx* c = p->ctx;
+  DeviceGuard g(c->device);
+  const size_t nvox = apirl_plan_image_voxels(p);
+  APIRL_HIP(c, hipMemsetAsync(d_sens, 0, nvox * sizeof(float), c->stream));
+  APIRL_HIP(c, apirl::launch_backproject(p, d_mult, nullptr, d_sens, subset, nsubsets,
+                                         d_mult ? apirl::BP_MULT : apirl::BP_ONES, mode,
+                                         c->stream));
+  if (force_positive) APIRL_HIP(c, apirl::launch_force_positive(d_sens, nvox, c->stream));
+  return APIRL_OK;
+}
+
+apirl_status apirl_mlem_threshold(apirl_ctx* c, const float* d_sens, size_t n, float* h_thr) {
+  if (!c || !d_sens || !h_thr || n == 0) return APIRL_ERR_INVALID;
+  DeviceGuard g(c->device);
+  float mm[2];
+  APIRL_HIP(c, apirl::launch_min_nz_max(d_sens, n, c->d_fred, c->stream));
+  APIRL_HIP(c, hipMemcpyAsync(mm, c->d_fred, 2 * sizeof(float), hipMemcpyDeviceToHost,
+                              c->stream));
+  APIRL_HIP(c, hipStreamSynchronize(c->stream));
+  // Mlem.cpp:41
+  *h_thr = mm[0] + (mm[1] - mm[0]) * 0.0001f;
+  return APIRL_OK;
+}
+
+apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
+                                  const float* d_meas, const float* d_init,
+                                  const float* d_additive, const float* d_mult, float* d_img,
+                                  double* h_lik, float* d_sens_out) {
+  if (!p || !opt || !d_meas || !d_init || !d_img) return APIRL_ERR_INVALID;
+  apirl_ctx* c = p->ctx;
+  const int S = opt->subsets, T = opt->iterations;
+  if (S < 1 || S > p->geo.nang || T < 0)
+    return fail(c, APIRL_ERR_INVALID, "invalid iterations / subsets");
+  DeviceGuard g(c->device);
+  const bool is2d = p->geo.dims == 2;
+  const size_t nvox = apirl_plan_image_voxels(p);
+  const size_t nfull = apirl_plan_subset_bins(p, 0, 1);
+  size_t nsub_max = 0;
+  for (int s = 0; s < S; s++) {
+    size_t n = apirl_plan_subset_bins(p, s, S);
+    if (n > nsub_max) nsub_max = n;
+  }
+  // Device-resident state: one sensitivity image per subset (OsemSinogram3d.h:76),
+  // the ratio sinogram of the largest subset and the backprojected image.
+  float *d_sens = nullptr, *d_ratio = nullptr, *d_bp = nullptr;
+  double* d_lik = nullptr;
+  auto cleanup = [&]() {
+    if (d_sens && d_sens != d_sens_out) (void)hipFree(d_sens);
+    if (d_ratio) (void)hipFree(d_ratio);
+    if (d_bp) (void)hipFree(d_bp);
+    if (d_lik) (void)hipFree(d_lik);
+  };
+  if (d_sens_out) d_sens = d_sens_out;
+  else if (hipMalloc(&d_sens, (size_t)S * nvox * sizeof(float)) != hipSuccess) d_sens = nullptr;
+  if (!d_sens || hipMalloc(&d_ratio, std::max(nsub_max, nfull) * sizeof(float)) != hipSuccess ||
+      hipMalloc(&d_bp, nvox * sizeof(float)) != hipSuccess ||
+      hipMalloc(&d_lik, ((size_t)T * S + 1) * sizeof(double)) != hipSuccess) {
+    cleanup();
+    return fail(c, APIRL_ERR_NOMEM, "cannot allocate the reconstruction state");
+  }
+  apirl_status st = APIRL_OK;
+#define APIRL_TRY(x)            \
+  do {                          \
+    st = (x);                   \
+    if (st != APIRL_OK) {       \
+      cleanup();                \
+      return st;                \
+    }                           \
+  } while (0)
+  std::vector<float> thr(S, 0.f);
+  for (int s = 0; s < S; s++) {
+    APIRL_TRY(apirl_sensitivity(p, d_mult, d_sens + (size_t)s * nvox, s, S, opt->bp_mode,
+                                opt->osem_rules ? 1 : 0));
+    if (!opt->osem_rules) APIRL_TRY(apirl_mlem_threshold(c, d_sens + (size_t)s * nvox, nvox, &thr[s]));
+  }
+  // MLEM: 2D updates with s > thr (Mlem2d.cpp:175), 3D with s >= thr
+  // (MlemSinogram3d.cpp:231); OSEM: threshold 0 with >= (OsemSinogram3d.cpp:28,221).
+  const int strict = (!opt->osem_rules && is2d) ? 1 : 0;
+  if (hipMemcpyAsync(d_img, d_init, nvox * sizeof(float), hipMemcpyDeviceToDevice, c->stream) !=
+      hipSuccess) {
+    cleanup();
+    return fail(c, APIRL_ERR_HIP, "image copy failed");
+  }
+  for (int t = 0; t < T; t++) {
+    for (int s = 0; s < S; s++) {
+      APIRL_TRY(apirl_project_ratio(p, d_img, d_meas, d_additive, d_ratio,
+                                    d_lik + (size_t)t * S + s, s, S, is2d ? 1 : 0,
+                                    opt->precision));
+      if (hipMemsetAsync(d_bp, 0, nvox * sizeof(float), c->stream) != hipSuccess) {
+        cleanup();
+        return fail(c, APIRL_ERR_HIP, "memset failed");
+      }
+      APIRL_TRY(apirl_backproject(p, d_ratio, d_bp, s, S, opt->bp_mode));
+      APIRL_TRY(apirl_em_update(c, d_img, d_bp, d_sens + (size_t)s * nvox, nvox, thr[s], strict));
+    }
+  }
+  // Final likelihood.  3D drivers project the final image once more over the full
+  // sinogram (MlemSinogram3d.cpp:277, OsemSinogram3d.cpp:291-293).  Mlem2d takes it on
+  // the last estimated sinogram, which by then holds the ratio (Mlem2d.cpp:227-228).
+  if (!is2d) {
+    APIRL_TRY(apirl_project_ratio(p, d_img, d_meas, d_additive, d_ratio,
+                                  d_lik + (size_t)T * S, 0, 1, 0, opt->precision));
+  } else {
+    const size_t n_last = apirl_plan_subset_bins(p, S - 1, S);
+    APIRL_TRY(ensure_red(c, (n_last + 255) / 256));
+    if (T == 0) {
+      if (hipMemsetAsync(d_ratio, 0, n_last * sizeof(float), c->stream) != hipSuccess) {
+        cleanup();
+        return fail(c, APIRL_ERR_HIP, "memset failed");
+      }
+    }
+    hipError_t e = apirl::launch_likelihood(d_ratio, d_meas, (long)n_last, 1, c->d_red,
+                                            d_lik + (size_t)T * S, c->stream);
+    if (e != hipSuccess) {
+      cleanup();
+      return hip_fail(c, e, "likelihood");
+    }
+  }
+  if (h_lik) {
+    if (hipMemcpyAsync(h_lik, d_lik, ((size_t)T * S + 1) * sizeof(double),
+                       hipMemcpyDeviceToHost, c->stream) != hipSuccess) {
+      cleanup();
+      return fail(c, APIRL_ERR_HIP, "likelihood copy failed");
+    }
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    cleanup();
+    return fail(c, APIRL_ERR_HIP, "stream synchronize failed");
+  }
+#undef APIRL_TRY
+  cleanup();
+  return APIRL_OK;
+}
+
+}  // extern "C"

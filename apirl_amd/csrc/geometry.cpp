@@ -1,0 +1,174 @@
+// Host-side geometry: sinogram grids, LOR endpoints, ring tables and the
+// transaxial part of the Siddon setup, evaluated in float with the reference's
+// operation order (this file is compiled with -ffp-contract=off; the x86-64
+// baseline has no FMA either, so every product and sum is rounded as written).
+#include <cfloat>
+#include <cmath>
+#include <cstdlib>
+
+#include "apirl_internal.h"
+
+namespace apirl {
+
+namespace {
+constexpr float kDegToRad = 0.01745329251994329576f;  // utils/inc/Geometry.h:12
+
+inline float min_ref(float a, float b) { return (b < a) ? b : a; }  // std::min
+inline float max_ref(float a, float b) { return (a < b) ? b : a; }  // std::max
+}  // namespace
+
+int subset_num_angles(int nang, int nsubsets, int subset) {
+  // Sinogram3DCylindricalPet.cpp:168-171
+  int n = (int)floorf((float)nang / (float)nsubsets);
+  if ((nang % nsubsets) > subset) n++;
+  return n;
+}
+
+// LOR endpoints of bin (ang, r): Sinogram2DinCylindrical3Dpet.cpp:249-266 (the 3D
+// form :216-229 is the same in x/y and never rejects).  The reference compiles to
+// glibc sincosf/sqrtf on float arguments; the same calls are made here, on the host.
+static bool lor_points(float ang_deg, float r, float rscanner, float rfov, bool check_fov,
+                       float* p1x, float* p1y, float* p2x, float* p2y) {
+  float rad = ang_deg * kDegToRad;
+  float aux = sqrtf(rscanner * rscanner - r * r);
+  if (check_fov && r > rfov) return false;
+  float s, c;
+  sincosf(rad, &s, &c);
+  *p1x = r * c + s * aux;
+  *p1y = r * s - c * aux;
+  *p2x = r * c - s * aux;
+  *p2y = r * s + c * aux;
+  return true;
+}
+
+void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
+                     const std::vector<float>& ang, const std::vector<float>& r,
+                     std::vector<RayXY>& out) {
+  const bool is2d = sino.dims == 2;
+  const float dx = img.dx, dy = img.dy;
+  const float R = (dx * img.nx) / 2;
+  out.resize((size_t)sino.nang * sino.nr);
+  for (int a = 0; a < sino.nang; a++) {
+    for (int j = 0; j < sino.nr; j++) {
+      RayXY& t = out[(size_t)a * sino.nr + j];
+      t = RayXY{};
+      float p1x = 0, p1y = 0, p2x = 0, p2y = 0;
+      bool valid = lor_points(ang[a], r[j], sino.rscanner_mm, sino.rfov_mm, is2d, &p1x, &p1y,
+                              &p2x, &p2y);
+      if (!valid) {
+        t.flags = ray_flags(0, 0, 0, 0);
+        continue;
+      }
+      const float p0x = p1x, p0y = p1y, vx = p2x - p1x, vy = p2y - p1y;
+      // Siddon.cpp:83-145 / :430-491
+      float amin, amax;
+      if (vx == 0) {
+        float a1 = (-R - p0y) / vy, a2 = (R - p0y) / vy;
+        if (a1 < a2) { amin = a1; amax = a2; } else { amin = a2; amax = a1; }
+      } else if (vy == 0) {
+        float a1 = (-R - p0x) / vx, a2 = (R - p0x) / vx;
+        if (a1 < a2) { amin = a1; amax = a2; } else { amin = a2; amax = a1; }
+      } else {
+        float ax1 = (-R - p0x) / vx, ay1 = (-R - p0y) / vy;
+        float ax2 = (R - p0x) / vx, ay2 = (R - p0y) / vy;
+        amin = max_ref(min_ref(ax1, ax2), min_ref(ay1, ay2));
+        amax = min_ref(max_ref(ax1, ax2), max_ref(ay1, ay2));
+      }
+      if (amin < 0 || amin > 1) amin = 0;
+      if (amax > 1 || amax < 0) amax = 1;
+      float x1 = p0x + vx * amin, y1 = p0y + vy * amin;
+      float x2 = p0x + vx * amax, y2 = p0y + vy * amax;
+      int imin = (int)floorf((x1 + R) / dx), jmin = (int)floorf((y1 + R) / dy);
+      int imax = (int)floorf((x2 + R) / dx), jmax = (int)floorf((y2 + R) / dy);
+      bool rejected = false;
+      if (is2d) {  // Siddon.cpp:515-519 (the 3D form has no such test)
+        rejected = (imin < 0 && imax < 0) || (jmin < 0 && jmax < 0) ||
+                   (imin >= img.nx && imax >= img.nx) || (jmin >= img.ny && jmax >= img.ny);
+      }
+      int iinc = vx > 0 ? 1 : (vx < 0 ? -1 : 0);
+      int jinc = vy > 0 ? 1 : (vy < 0 ? -1 : 0);
+      float ax, ay;
+      if (vx > 0) ax = (-R + (imin + iinc) * dx - p0x) / vx;
+      else if (vx < 0) ax = (-R + imin * dx - p0x) / vx;
+      else ax = FLT_MAX;
+      if (vy > 0) ay = (-R + (jmin + jinc) * dy - p0y) / vy;
+      else if (vy < 0) ay = (-R + jmin * dy - p0y) / vy;
+      else ay = FLT_MAX;
+      if (is2d) {  // the 2D form clamps a negative first crossing (:596-606); 3D does not
+        if (ax < 0) ax = amin;
+        if (ay < 0) ay = amin;
+      }
+      float ex = (p0x + vx) - p0x, ey = (p0y + vy) - p0y;
+      t.p0x = p0x; t.p0y = p0y; t.vx = vx; t.vy = vy;
+      t.amin = amin; t.amax = amax;
+      t.ax = ax; t.ay = ay;
+      t.axu = fabsf(dx / vx);
+      t.ayu = fabsf(dy / vy);
+      t.lxy = ex * ex + ey * ey;
+      t.i0 = imin; t.j0 = jmin;
+      t.nxy = rejected ? 0 : std::abs(imax - imin) + std::abs(jmax - jmin);
+      t.flags = ray_flags(iinc, jinc, 1, rejected);
+    }
+  }
+}
+
+void build_z_table(const apirl_sino_desc& sino, std::vector<float2>& zab) {
+  // SiddonProjector.cpp:335-336,394-395 with one axial sample per bin:
+  // P.Z = zmean - w/2 + dz/2 + 0*dz, dz = w / 1.
+  zab.resize(sino.nsino);
+  const float w = sino.ring_width_mm;
+  const float dzs = w / 1;
+  for (int s = 0; s < sino.nsino; s++) {
+    float z1 = sino.z1_mean_mm[s], z2 = sino.z2_mean_mm[s];
+    zab[s].x = z1 - w / 2 + dzs / 2 + 0 * dzs;
+    zab[s].y = z2 - w / 2 + dzs / 2 + 0 * dzs;
+  }
+}
+
+}  // namespace apirl
+
+extern "C" void apirl_sino_grid(int nang, int nr, float rfov, float* ang_deg, float* r_mm) {
+  // Sinogram2DinCylindrical3Dpet.cpp:46-60 (maxAng_deg is the int 180)
+  float rinc = (2 * rfov) / nr;
+  float phinc = (float)180 / nang;
+  for (int i = 0; i < nang; i++) ang_deg[i] = i * phinc;
+  for (int j = 0; j < nr; j++) r_mm[j] = rinc / 2 + j * rinc - rfov;
+}
+
+extern "C" int apirl_ring_table(int nrings, float zfov, int nseg, const int* nsino,
+                                const int* minrd, const int* maxrd, float* z1m, float* z2m) {
+  // Ring grid Sinogram3D.cpp:358-367; ring-pair lists :396-464; mean of the first and
+  // last pair of each list, SiddonProjector.cpp:357-360.
+  std::vector<float> zr(nrings);
+  float zinc = (float)zfov / nrings;
+  for (int i = 0; i < nrings; i++) zr[i] = zinc / 2 + i * zinc;
+  int total = 0;
+  for (int s = 0; s < nseg; s++) {
+    int count = 0;
+    for (int z1 = 0; z1 < 2 * nrings; z1++) {
+      int n = 0, f1 = 0, f2 = 0, l1 = 0, l2 = 0;
+      int z1a = z1;
+      for (int z2 = 0; z2 < nrings; z2++) {
+        if ((z1a - z2) <= maxrd[s] && (z1a - z2) >= minrd[s] && z1a >= 0 && z2 >= 0 &&
+            z1a < nrings && z2 < nrings) {
+          if (n == 0) { f1 = z1a; f2 = z2; }
+          l1 = z1a; l2 = z2;
+          n++;
+        }
+        z1a--;
+      }
+      if (n > 0) {
+        z1m[total] = (zr[f1] + zr[l1]) / 2;
+        z2m[total] = (zr[f2] + zr[l2]) / 2;
+        total++;
+        count++;
+      }
+    }
+    if (count != nsino[s]) return -1;
+  }
+  return total;
+}
+
+extern "C" int apirl_subset_num_angles(int nang, int nsubsets, int subset) {
+  return apirl::subset_num_angles(nang, nsubsets, subset);
+}

@@ -1,0 +1,515 @@
+// gfx950 kernels of the Siddon projector / backprojector and the EM update.
+//
+// One ray per lane.  Lanes of a wavefront take consecutive radial bins of one
+// (sinogram, angle), so the 64-byte RayXY records, the measured / additive bins and
+// the sinogram writes are contiguous per wavefront, and the 64 rays are parallel
+// neighbours whose voxel gathers hit the same cache lines of the (L2 / MALL
+// resident) image.  The path is gather/scatter bound; no MFMA is involved.
+//
+// Arithmetic follows recon/src/Siddon.cpp in float, operation by operation, and the
+// file is compiled with -ffp-contract=off and correctly rounded division / sqrt, so
+// every voxel index and weight equals the reference's.  Accumulation along a ray
+// is sequential in traversal order, which is the reference's order
+// (SiddonProjector.cpp:176-181, 413-423): the forward projection is bit-exact.
+#include <cfloat>
+#include <cmath>
+
+#include "apirl_internal.h"
+
+namespace apirl {
+
+namespace {
+
+constexpr int kBlock = 256;
+
+// Per-ray traversal state (Siddon.cpp:253-340).
+struct Ray {
+  float ax, ay, az, axu, ayu, azu, ac, L;
+  int i, j, k, iinc, jinc, kinc, n;
+};
+
+// Builds the traversal state of bin (s, angle a, r) from the transaxial table and
+// the sinogram's axial coordinates (Siddon.cpp:150-182, 220, 234-255, 279-286).
+__device__ __forceinline__ bool setup_ray(const Geo& g, const RayXY& t, float za, float zb,
+                                          Ray& ray) {
+  if (!(t.flags & RAY_VALID)) return false;
+  ray.ax = t.ax;
+  ray.ay = t.ay;
+  ray.axu = t.axu;
+  ray.ayu = t.ayu;
+  ray.ac = t.amin;
+  ray.i = t.i0;
+  ray.j = t.j0;
+  ray.iinc = ray_iinc(t.flags);
+  ray.jinc = ray_jinc(t.flags);
+  if (g.dims == 2) {
+    ray.az = FLT_MAX;
+    ray.azu = 0.f;
+    ray.k = 0;
+    ray.kinc = 0;
+    ray.n = t.nxy;
+    ray.L = sqrtf(t.lxy);
+    return true;
+  }
+  const float vz = zb - za;
+  const float z1 = za + vz * t.amin;
+  const float z2 = za + vz * t.amax;
+  const int k0 = (int)floorf((z1 - g.offz) / g.dz);
+  const int k1 = (int)floorf((z2 - g.offz) / g.dz);
+  ray.k = k0;
+  ray.kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
+  ray.n = t.nxy + abs(k1 - k0);
+  const float ez = (za + vz) - za;
+  ray.L = sqrtf(t.lxy + ez * ez);
+  ray.azu = fabsf(g.dz / vz);
+  if (vz > 0) ray.az = (g.offz + (k0 + ray.kinc) * g.dz - za) / vz;
+  else if (vz < 0) ray.az = (g.offz + k0 * g.dz - za) / vz;
+  else ray.az = FLT_MAX;
+  return true;
+}
+
+// One Siddon step: returns the weight of the current voxel (ci, cj, ck) and advances
+// (Siddon.cpp:310-340 for 3D, :615-639 for 2D; the 2D form compares only x and y,
+// which is the 3D rule with az = FLT_MAX).
+__device__ __forceinline__ float step(Ray& r, int& ci, int& cj, int& ck) {
+  ci = r.i;
+  cj = r.j;
+  ck = r.k;
+  const bool bx = (r.ax <= r.ay) && (r.ax <= r.az);
+  const bool by = !bx && (r.ay <= r.az);
+  const float an = bx ? r.ax : (by ? r.ay : r.az);
+  const float w = (an - r.ac) * r.L;
+  r.ac = an;
+  if (bx) {
+    r.i += r.iinc;
+    r.ax += r.axu;
+  } else if (by) {
+    r.j += r.jinc;
+    r.ay += r.ayu;
+  } else {
+    r.k += r.kinc;
+    r.az += r.azu;
+  }
+  return w;
+}
+
+// Decomposes a ray index of a subset sinogram into (sinogram, angle, r).
+__device__ __forceinline__ void ray_coords(const Geo& g, long idx, int nang_sub, int subset,
+                                           int nsubsets, int& s, int& a, int& r, int& k) {
+  r = (int)(idx % g.nr);
+  long t = idx / g.nr;
+  k = (int)(t % nang_sub);
+  s = (int)(t / nang_sub);
+  a = subset + nsubsets * k;
+}
+
+template <int PREC, bool FUSED>
+__global__ __launch_bounds__(kBlock) void project_kernel(
+    Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
+    const float* __restrict__ img, float* __restrict__ sino, int nang_sub, int subset,
+    int nsubsets, long nrays, FusedRatio fr) {
+  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+  double lik = 0.0;
+  if (idx < nrays) {
+    int s, a, rr, k;
+    ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
+    const RayXY t = rays[(long)a * g.nr + rr];
+    float za = 0.f, zb = 0.f;
+    if (g.dims == 3) {
+      const float2 z = zab[s];
+      za = z.x;
+      zb = z.y;
+    }
+    Ray ray;
+    bool valid = setup_ray(g, t, za, zb, ray);
+    float acc = 0.f;
+    if (valid) {
+      if (PREC == APIRL_PREC_REFERENCE) {
+        for (int m = 0; m < ray.n; m++) {
+          int ci, cj, ck;
+          const float w = step(ray, ci, cj, ck);
+          bool ok;
+          long nxy = (long)g.nx * g.ny;
+          if (g.dims == 3) {
+            ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
+            long li = (long)ck * nxy + (long)cj * g.nx + ci;
+            if (ok && li >= 0) acc += w * img[li];
+          } else {
+            ok = (cj >= 0) && (ci >= 0);
+            long li = (long)cj * g.nx + ci;
+            if (ok && li < nxy) acc += 1.0f * w * img[li];
+          }
+        }
+      } else if (PREC == APIRL_PREC_KAHAN) {
+        float c = 0.f;
+        for (int m = 0; m < ray.n; m++) {
+          int ci, cj, ck;
+          const float w = step(ray, ci, cj, ck);
+          long nxy = (long)g.nx * g.ny;
+          bool ok;
+          long li;
+          if (g.dims == 3) {
+            ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
+            li = (long)ck * nxy + (long)cj * g.nx + ci;
+            ok = ok && li >= 0;
+          } else {
+            ok = (cj >= 0) && (ci >= 0);
+            li = (long)cj * g.nx + ci;
+            ok = ok && li < nxy;
+          }
+          if (ok) {
+            const float y = w * img[li] - c;
+            const float tt = acc + y;
+            c = (tt - acc) - y;
+            acc = tt;
+          }
+        }
+      } else {
+        // fp64 alpha stepping: the same traversal rules with every alpha, increment
+        // and weight held in double; accumulation in double.
+        double ax = ray.ax, ay = ray.ay, az = ray.az, ac = ray.ac;
+        double axu = ray.axu, ayu = ray.ayu, azu = ray.azu;
+        const double L = ray.L;
+        if (g.dims == 3) {
+          // recompute the first crossings and increments in double from the LOR
+          const double vx = t.vx, vy = t.vy, vz = (double)zb - za;
+          axu = fabs((double)g.dx / vx);
+          ayu = fabs((double)g.dy / vy);
+          azu = fabs((double)g.dz / vz);
+          const double R = g.R;
+          if (vx > 0) ax = (-R + (double)(ray.i + 1) * g.dx - t.p0x) / vx;
+          else if (vx < 0) ax = (-R + (double)ray.i * g.dx - t.p0x) / vx;
+          if (vy > 0) ay = (-R + (double)(ray.j + 1) * g.dy - t.p0y) / vy;
+          else if (vy < 0) ay = (-R + (double)ray.j * g.dy - t.p0y) / vy;
+          if (vz > 0) az = ((double)g.offz + (double)(ray.k + 1) * g.dz - za) / vz;
+          else if (vz < 0) az = ((double)g.offz + (double)ray.k * g.dz - za) / vz;
+        }
+        double dacc = 0.0;
+        int i = ray.i, j = ray.j, kk = ray.k;
+        const long nxy = (long)g.nx * g.ny;
+        for (int m = 0; m < ray.n; m++) {
+          const int ci = i, cj = j, ck = kk;
+          const bool bx = (ax <= ay) && (ax <= az);
+          const bool by = !bx && (ay <= az);
+          const double an = bx ? ax : (by ? ay : az);
+          const double w = (an - ac) * L;
+          ac = an;
+          if (bx) { i += ray.iinc; ax += axu; }
+          else if (by) { j += ray.jinc; ay += ayu; }
+          else { kk += ray.kinc; az += azu; }
+          bool ok;
+          long li;
+          if (g.dims == 3) {
+            ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
+            li = (long)ck * nxy + (long)cj * g.nx + ci;
+            ok = ok && li >= 0;
+          } else {
+            ok = (cj >= 0) && (ci >= 0);
+            li = (long)cj * g.nx + ci;
+            ok = ok && li < nxy;
+          }
+          if (ok) dacc += w * (double)img[li];
+        }
+        acc = (float)dacc;
+      }
+    }
+    if (!FUSED) {
+      if (valid) sino[idx] = acc;
+    } else {
+      // fused EM epilogue: est (+ additive), likelihood partial, ratio
+      // (Sinogram3D.cpp:584-607, 830-870; Sinogram2D.cpp:234-266, 565-581)
+      const long full = ((long)s * g.nang + a) * g.nr + rr;
+      float est = valid ? acc : 0.f;
+      if (fr.additive) est = est + fr.additive[full];
+      const float y = fr.meas[full];
+      const bool lik_term = fr.lik_positive_only ? (est > 0.f) : (est != 0.f);
+      if (lik_term) lik = (double)(y * logf(est) - est);
+      fr.ratio[idx] = (y != 0.f && est != 0.f) ? y / est : 0.f;
+    }
+  }
+  if (FUSED && fr.lik_partials) {
+    // deterministic block reduction of the likelihood partials (fixed tree)
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = lik;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) fr.lik_partials[blockIdx.x] = red[0];
+  }
+}
+
+// Ray-driven scatter backprojection with fp32 atomics.
+//   BP_PLAIN : term = w * b; 3D skips b == 0 (SiddonProjector.cpp:496) with a full
+//              bounds check (:541-548); 2D keeps every bin (:50-77) and writes
+//              (w * b) * geom (:72).
+//   BP_ONES  : b = 1 (sensitivity of a subset).
+//   BP_MULT  : b = multiplicative sinogram bin (FULL layout).
+//   BP_DIVIDE: 3D: skip meas == 0; term = (w*m)/e if e != 0 else w*m (:607-685) with the
+//              upper-bound-only check of :666; 2D: skip e == 0 or m == 0 (:110);
+//              term = (w*m)/e with the full check (:125-131).
+template <int KIND>
+__global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
+    Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
+    const float* __restrict__ sino, const float* __restrict__ est, float* __restrict__ img,
+    int nang_sub, int subset, int nsubsets, long nrays) {
+  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (idx >= nrays) return;
+  int s, a, rr, k;
+  ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
+  float b = 1.f, e = 0.f;
+  if (KIND == BP_PLAIN) b = sino[idx];
+  if (KIND == BP_MULT) b = sino[((long)s * g.nang + a) * g.nr + rr];
+  if (KIND == BP_DIVIDE) {
+    b = sino[idx];
+    e = est[idx];
+    if (b == 0.f) return;
+    if (g.dims == 2 && e == 0.f) return;
+  }
+  if (g.dims == 3 && (KIND == BP_PLAIN || KIND == BP_MULT) && b == 0.f) return;
+  const RayXY t = rays[(long)a * g.nr + rr];
+  float za = 0.f, zb = 0.f;
+  if (g.dims == 3) {
+    const float2 z = zab[s];
+    za = z.x;
+    zb = z.y;
+  }
+  Ray ray;
+  if (!setup_ray(g, t, za, zb, ray)) return;
+  const long nxy = (long)g.nx * g.ny;
+  const long npix = nxy * g.nz;
+  for (int m = 0; m < ray.n; m++) {
+    int ci, cj, ck;
+    const float w = step(ray, ci, cj, ck);
+    bool ok;
+    float term;
+    if (KIND == BP_DIVIDE && g.dims == 3) {
+      ok = (ck < g.nz) && (cj < g.ny) && (ci < g.nx);
+      term = (e != 0.f) ? w * 1.0f * b / e : w * 1.0f * b;
+    } else {
+      ok = (ck < g.nz) && (cj < g.ny) && (ci < g.nx) && (ck >= 0) && (cj >= 0) && (ci >= 0);
+      if (KIND == BP_DIVIDE) term = 1.0f * w * b / e;
+      else if (g.dims == 2) term = w * b * 1.0f;
+      else term = w * 1.0f * b;
+    }
+    if (ok) {
+      const long li = (long)ck * nxy + (long)cj * g.nx + ci;
+      if (li >= 0 && li < npix) unsafeAtomicAdd(&img[li], term);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void count_kernel(Geo g, const RayXY* __restrict__ rays,
+                                                       const float2* __restrict__ zab,
+                                                       int nang_sub, int subset, int nsubsets,
+                                                       long nrays,
+                                                       unsigned long long* __restrict__ out) {
+  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+  unsigned long long n = 0;
+  if (idx < nrays) {
+    int s, a, rr, k;
+    ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
+    const RayXY t = rays[(long)a * g.nr + rr];
+    float za = 0.f, zb = 0.f;
+    if (g.dims == 3) {
+      za = zab[s].x;
+      zb = zab[s].y;
+    }
+    Ray ray;
+    if (setup_ray(g, t, za, zb, ray)) n = (unsigned long long)ray.n;
+  }
+  __shared__ unsigned long long red[kBlock];
+  red[threadIdx.x] = n;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) atomicAdd(out, red[0]);
+}
+
+__global__ __launch_bounds__(kBlock) void em_update_kernel(float* __restrict__ x,
+                                                           const float* __restrict__ b,
+                                                           const float* __restrict__ s,
+                                                           long n, float thr, int strict) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float sv = s[i];
+  const bool ok = strict ? (sv > thr) : (sv >= thr);
+  x[i] = ok ? x[i] * b[i] / sv : 0.f;
+}
+
+__global__ __launch_bounds__(kBlock) void force_positive_kernel(float* __restrict__ x, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && x[i] < 0.f) x[i] = 0.f;
+}
+
+// Fixed-order sum of block partials (one block): deterministic for a given count.
+__global__ __launch_bounds__(kBlock) void sum_partials_kernel(const double* __restrict__ p,
+                                                              int n, double* __restrict__ out) {
+  __shared__ double red[kBlock];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) acc += p[i];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// Minimum of the non-zero values and maximum (Images.cpp:397-431).  Both are
+// order-independent, so the result is exact.  out[0] = min_nz, out[1] = max; the
+// reference's maximum starts from pixel 0, so a NaN there is kept (out[2] flags it).
+__global__ __launch_bounds__(kBlock) void min_nz_max_kernel(const float* __restrict__ x, long n,
+                                                            float* __restrict__ out) {
+  __shared__ float smin[kBlock], smax[kBlock];
+  float mn = FLT_MAX, mx = -FLT_MAX;
+  for (long i = threadIdx.x; i < n; i += kBlock) {
+    const float v = x[i];
+    if (v != 0.f && v < mn) mn = v;
+    if (v > mx) mx = v;
+  }
+  smin[threadIdx.x] = mn;
+  smax[threadIdx.x] = mx;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      smin[threadIdx.x] = fminf(smin[threadIdx.x], smin[threadIdx.x + off]);
+      smax[threadIdx.x] = fmaxf(smax[threadIdx.x], smax[threadIdx.x + off]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[0] = smin[0];
+    out[1] = (x[0] > smax[0] || x[0] != x[0]) ? x[0] : smax[0];
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void likelihood_kernel(const float* __restrict__ x,
+                                                            const float* __restrict__ y, long n,
+                                                            int positive_only,
+                                                            double* __restrict__ partials) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  double v = 0.0;
+  if (i < n) {
+    const float e = x[i];
+    if (positive_only ? (e > 0.f) : (e != 0.f)) v = (double)(y[i] * logf(e) - e);
+  }
+  __shared__ double red[kBlock];
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
+}
+
+inline unsigned grid_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, int subset,
+                          int nsubsets, int precision, const FusedRatio* fused, int* nblocks_out,
+                          hipStream_t st) {
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long nrays = (long)g.nsino * nang_sub * g.nr;
+  const unsigned grid = grid_for(nrays);
+  if (nblocks_out) *nblocks_out = (int)grid;
+  if (nrays == 0) return hipSuccess;
+  FusedRatio fr = fused ? *fused : FusedRatio{};
+  const bool f = fused != nullptr;
+#define APIRL_LAUNCH_PROJ(PR)                                                              \
+  if (f)                                                                                   \
+    hipLaunchKernelGGL((project_kernel<PR, true>), dim3(grid), dim3(kBlock), 0, st, g,     \
+                       p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr); \
+  else                                                                                     \
+    hipLaunchKernelGGL((project_kernel<PR, false>), dim3(grid), dim3(kBlock), 0, st, g,    \
+                       p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr);
+  switch (precision) {
+    case APIRL_PREC_KAHAN: APIRL_LAUNCH_PROJ(APIRL_PREC_KAHAN); break;
+    case APIRL_PREC_F64_ALPHA: APIRL_LAUNCH_PROJ(APIRL_PREC_F64_ALPHA); break;
+    default: APIRL_LAUNCH_PROJ(APIRL_PREC_REFERENCE); break;
+  }
+#undef APIRL_LAUNCH_PROJ
+  return hipGetLastError();
+}
+
+hipError_t launch_backproject(const apirl_plan* p, const float* sino, const float* est,
+                              float* img, int subset, int nsubsets, int kind, int mode,
+                              hipStream_t st) {
+  if (mode != APIRL_BP_SCATTER) return hipErrorNotSupported;
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long nrays = (long)g.nsino * nang_sub * g.nr;
+  if (nrays == 0) return hipSuccess;
+  const unsigned grid = grid_for(nrays);
+#define APIRL_LAUNCH_BP(K)                                                                  \
+  hipLaunchKernelGGL((backproject_scatter_kernel<K>), dim3(grid), dim3(kBlock), 0, st, g, \
+                     p->d_rays, p->d_zab, sino, est, img, nang_sub, subset, nsubsets, nrays)
+  switch (kind) {
+    case BP_ONES: APIRL_LAUNCH_BP(BP_ONES); break;
+    case BP_MULT: APIRL_LAUNCH_BP(BP_MULT); break;
+    case BP_DIVIDE: APIRL_LAUNCH_BP(BP_DIVIDE); break;
+    default: APIRL_LAUNCH_BP(BP_PLAIN); break;
+  }
+#undef APIRL_LAUNCH_BP
+  return hipGetLastError();
+}
+
+hipError_t launch_count_intersections(const apirl_plan* p, int subset, int nsubsets,
+                                      unsigned long long* d_out, hipStream_t st) {
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long nrays = (long)g.nsino * nang_sub * g.nr;
+  if (nrays == 0) return hipSuccess;
+  hipLaunchKernelGGL(count_kernel, dim3(grid_for(nrays)), dim3(kBlock), 0, st, g, p->d_rays,
+                     p->d_zab, nang_sub, subset, nsubsets, nrays, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_em_update(float* x, const float* b, const float* s, size_t n, float thr,
+                            int strict, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(em_update_kernel, dim3(grid_for((long)n)), dim3(kBlock), 0, st, x, b, s,
+                     (long)n, thr, strict);
+  return hipGetLastError();
+}
+
+hipError_t launch_force_positive(float* x, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(force_positive_kernel, dim3(grid_for((long)n)), dim3(kBlock), 0, st, x,
+                     (long)n);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_partials(const double* partials, int n, double* out, hipStream_t st) {
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, st, partials, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_likelihood(const float* x, const float* y, long n, int positive_only,
+                             double* partials, double* out, hipStream_t st) {
+  if (n <= 0) {
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, st, partials, 0, out);
+    return hipGetLastError();
+  }
+  const unsigned grid = grid_for(n);
+  hipLaunchKernelGGL(likelihood_kernel, dim3(grid), dim3(kBlock), 0, st, x, y, n,
+                     positive_only, partials);
+  hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, st, partials, (int)grid,
+                     out);
+  return hipGetLastError();
+}
+
+hipError_t launch_min_nz_max(const float* x, size_t n, float* d_out2, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(min_nz_max_kernel, dim3(1), dim3(kBlock), 0, st, x, (long)n, d_out2);
+  return hipGetLastError();
+}
+
+}  // namespace apirl

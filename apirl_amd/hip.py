@@ -1,0 +1,350 @@
+"""ctypes binding of the HIP library (apirl_amd/libapirl_hip.so, C-ABI in
+include/apirl_hip.h).
+
+Device buffers are torch tensors on a ROCm device (torch is used only for device
+memory and streams); the compute runs in the library's gfx950 kernels.  There is no
+CPU fallback: if the library or a GPU is missing, every entry point raises."""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libapirl_hip.so")
+
+APIRL_OK = 0
+BP_SCATTER = 0
+BP_GATHER = 1
+PREC_REFERENCE = 0
+PREC_KAHAN = 1
+PREC_F64_ALPHA = 2
+
+_c_int = ctypes.c_int
+_c_float = ctypes.c_float
+_c_size = ctypes.c_size_t
+_P = ctypes.c_void_p
+
+
+class ImageDesc(ctypes.Structure):
+    _fields_ = [("nx", _c_int), ("ny", _c_int), ("nz", _c_int), ("dx", _c_float),
+                ("dy", _c_float), ("dz", _c_float)]
+
+
+class SinoDesc(ctypes.Structure):
+    _fields_ = [("dims", _c_int), ("nang", _c_int), ("nr", _c_int), ("nsino", _c_int),
+                ("rfov_mm", _c_float), ("rscanner_mm", _c_float), ("ring_width_mm", _c_float),
+                ("ang_deg", _P), ("r_mm", _P), ("z1_mean_mm", _P), ("z2_mean_mm", _P)]
+
+
+class EmOptions(ctypes.Structure):
+    _fields_ = [("iterations", _c_int), ("subsets", _c_int), ("bp_mode", _c_int),
+                ("precision", _c_int), ("osem_rules", _c_int)]
+
+
+# Every symbol the header declares, with its ctypes signature.
+SIGNATURES = {
+    "apirl_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_P)]),
+    "apirl_ctx_destroy": (_c_int, [_P]),
+    "apirl_last_error": (ctypes.c_char_p, [_P]),
+    "apirl_ctx_set_stream": (_c_int, [_P, _P]),
+    "apirl_ctx_synchronize": (_c_int, [_P]),
+    "apirl_malloc": (_c_int, [_P, ctypes.POINTER(_P), _c_size]),
+    "apirl_free": (_c_int, [_P, _P]),
+    "apirl_memcpy_h2d": (_c_int, [_P, _P, _P, _c_size]),
+    "apirl_memcpy_d2h": (_c_int, [_P, _P, _P, _c_size]),
+    "apirl_memset": (_c_int, [_P, _P, _c_int, _c_size]),
+    "apirl_plan_create": (_c_int, [_P, ctypes.POINTER(ImageDesc), ctypes.POINTER(SinoDesc),
+                                   ctypes.POINTER(_P)]),
+    "apirl_plan_destroy": (_c_int, [_P]),
+    "apirl_subset_num_angles": (_c_int, [_c_int, _c_int, _c_int]),
+    "apirl_plan_subset_bins": (_c_size, [_P, _c_int, _c_int]),
+    "apirl_plan_image_voxels": (_c_size, [_P]),
+    "apirl_plan_intersections": (_c_int, [_P, _c_int, _c_int,
+                                          ctypes.POINTER(ctypes.c_ulonglong)]),
+    "apirl_project": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int]),
+    "apirl_backproject": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int]),
+    "apirl_divide_backproject": (_c_int, [_P, _P, _P, _P, _c_int, _c_int, _c_int]),
+    "apirl_project_ratio": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
+    "apirl_em_update": (_c_int, [_P, _P, _P, _P, _c_size, _c_float, _c_int]),
+    "apirl_sensitivity": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
+    "apirl_mlem_threshold": (_c_int, [_P, _P, _c_size, ctypes.POINTER(_c_float)]),
+    "apirl_em_reconstruct": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P, _P,
+                                      _P]),
+    "apirl_sino_grid": (None, [_c_int, _c_int, _c_float, _P, _P]),
+    "apirl_ring_table": (_c_int, [_c_int, _c_float, _c_int, _P, _P, _P, _P, _P]),
+}
+
+_lib = None
+
+
+class ApirlError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads libapirl_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ApirlError(f"{LIB_PATH} is missing: build it with `python -m apirl_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def _np_ptr(a):
+    return a.ctypes.data_as(_P) if a is not None else None
+
+
+def sino_grid(nang, nr, rfov):
+    ang = np.zeros(nang, np.float32)
+    r = np.zeros(nr, np.float32)
+    load().apirl_sino_grid(nang, nr, rfov, _np_ptr(ang), _np_ptr(r))
+    return ang, r
+
+
+def segment_table(rings, span, mrd):
+    """Segments ordered 0, +1, -1, +2, -2 ...: (sinograms per segment, min ring
+    difference, max ring difference)."""
+    h = (span - 1) // 2
+    nsino, mn, mx = [2 * rings - 1], [-h], [h]
+    lo = h + 1
+    while lo <= mrd:
+        hi = min(lo + span - 1, mrd)
+        n = sum(1 for s in range(2 * rings - 1)
+                if any(0 <= s - b < rings and lo <= s - 2 * b <= hi for b in range(rings)))
+        nsino += [n, n]
+        mn += [lo, -hi]
+        mx += [hi, -lo]
+        lo += span
+    return (np.array(nsino, np.int32), np.array(mn, np.int32), np.array(mx, np.int32))
+
+
+def ring_table(rings, zfov, span, mrd):
+    """Mean ring coordinates (z1, z2) of every sinogram of a cylindrical 3D sinogram."""
+    nsino, mn, mx = segment_table(rings, span, mrd)
+    tot = int(nsino.sum())
+    z1 = np.zeros(tot, np.float32)
+    z2 = np.zeros(tot, np.float32)
+    n = load().apirl_ring_table(rings, zfov, len(nsino), _np_ptr(nsino), _np_ptr(mn),
+                                _np_ptr(mx), _np_ptr(z1), _np_ptr(z2))
+    if n != tot:
+        raise ApirlError("segment table does not match the ring enumeration")
+    return z1, z2
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise ApirlError("no ROCm GPU visible: the HIP Siddon path needs an MI355X")
+    return torch
+
+
+class Context:
+    """One device; work is enqueued on the null stream (torch's default stream)."""
+
+    def __init__(self, device=0):
+        _torch()
+        self.lib = load()
+        self.h = _P()
+        st = self.lib.apirl_ctx_create(device, ctypes.byref(self.h))
+        if st != APIRL_OK:
+            raise ApirlError(f"apirl_ctx_create failed with status {st}")
+        self.device = device
+
+    def check(self, st, what):
+        if st != APIRL_OK:
+            msg = self.lib.apirl_last_error(self.h).decode()
+            raise ApirlError(f"{what} failed (status {st}): {msg}")
+
+    def synchronize(self):
+        self.check(self.lib.apirl_ctx_synchronize(self.h), "synchronize")
+
+    def close(self):
+        if self.h:
+            self.lib.apirl_ctx_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _dptr(t):
+    return _P(t.data_ptr()) if t is not None else None
+
+
+class Plan:
+    """Geometry plan: image grid + cylindrical sinogram (2D or 3D)."""
+
+    def __init__(self, ctx, nx, ny, nz, dx, dz, nang, nr, rfov, rscanner, dims=3,
+                 z1=None, z2=None, ring_width=0.0, ang=None, r=None):
+        self.ctx = ctx
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.nang, self.nr = nang, nr
+        self.dims = dims
+        self.nsino = 1 if dims == 2 else len(z1)
+        im = ImageDesc(nx, ny, nz, dx, dx, dz)
+        self._keep = []
+
+        def keep(a):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, np.float32)
+            self._keep.append(a)
+            return _np_ptr(a)
+
+        sd = SinoDesc(dims, nang, nr, self.nsino, rfov, rscanner, ring_width, keep(ang), keep(r),
+                      keep(z1), keep(z2))
+        self.h = _P()
+        ctx.check(ctx.lib.apirl_plan_create(ctx.h, ctypes.byref(im), ctypes.byref(sd),
+                                            ctypes.byref(self.h)), "apirl_plan_create")
+
+    @classmethod
+    def for_2d(cls, ctx, nx, dx, nang, nr, rfov, rscanner):
+        return cls(ctx, nx, nx, 1, dx, 1.0, nang, nr, rfov, rscanner, dims=2)
+
+    @classmethod
+    def for_3d(cls, ctx, nx, ny, nz, dx, dz, nang, nr, rfov, rscanner, rings, zfov, span, mrd):
+        z1, z2 = ring_table(rings, zfov, span, mrd)
+        return cls(ctx, nx, ny, nz, dx, dz, nang, nr, rfov, rscanner, dims=3, z1=z1, z2=z2,
+                   ring_width=np.float32(zfov) / np.float32(rings))
+
+    def subset_angles(self, subset, nsubsets):
+        return self.ctx.lib.apirl_subset_num_angles(self.nang, nsubsets, subset)
+
+    def subset_shape(self, subset=0, nsubsets=1):
+        return (self.nsino, self.subset_angles(subset, nsubsets), self.nr)
+
+    def image_shape(self):
+        return (self.nz, self.ny, self.nx)
+
+    def intersections(self, subset=0, nsubsets=1):
+        out = ctypes.c_ulonglong(0)
+        self.ctx.check(self.ctx.lib.apirl_plan_intersections(self.h, subset, nsubsets,
+                                                             ctypes.byref(out)),
+                       "apirl_plan_intersections")
+        return out.value
+
+    # --- Projector entry points on torch device tensors -----------------------
+    def project(self, img, out=None, subset=0, nsubsets=1, precision=PREC_REFERENCE):
+        torch = _torch()
+        if out is None:
+            out = torch.zeros(self.subset_shape(subset, nsubsets), dtype=torch.float32,
+                              device=img.device)
+        self._check_img(img)
+        self._check_sino(out, subset, nsubsets)
+        self.ctx.check(self.ctx.lib.apirl_project(self.h, _dptr(img), _dptr(out), subset,
+                                                  nsubsets, precision), "apirl_project")
+        return out
+
+    def backproject(self, sino, out=None, subset=0, nsubsets=1, mode=BP_SCATTER):
+        torch = _torch()
+        if out is None:
+            out = torch.zeros(self.image_shape(), dtype=torch.float32, device=sino.device)
+        self._check_img(out)
+        self._check_sino(sino, subset, nsubsets)
+        self.ctx.check(self.ctx.lib.apirl_backproject(self.h, _dptr(sino), _dptr(out), subset,
+                                                      nsubsets, mode), "apirl_backproject")
+        return out
+
+    def divide_backproject(self, meas, est, out=None, subset=0, nsubsets=1, mode=BP_SCATTER):
+        torch = _torch()
+        if out is None:
+            out = torch.zeros(self.image_shape(), dtype=torch.float32, device=meas.device)
+        self._check_img(out)
+        self._check_sino(meas, subset, nsubsets)
+        self._check_sino(est, subset, nsubsets)
+        self.ctx.check(self.ctx.lib.apirl_divide_backproject(self.h, _dptr(meas), _dptr(est),
+                                                             _dptr(out), subset, nsubsets, mode),
+                       "apirl_divide_backproject")
+        return out
+
+    def project_ratio(self, img, meas, additive=None, subset=0, nsubsets=1, lik_positive_only=0,
+                      precision=PREC_REFERENCE, ratio=None, want_lik=True):
+        torch = _torch()
+        if ratio is None:
+            ratio = torch.empty(self.subset_shape(subset, nsubsets), dtype=torch.float32,
+                                device=img.device)
+        lik = torch.zeros(1, dtype=torch.float64, device=img.device) if want_lik else None
+        self._check_img(img)
+        self._check_sino(meas, 0, 1)
+        self._check_sino(ratio, subset, nsubsets)
+        if additive is not None:
+            self._check_sino(additive, 0, 1)
+        self.ctx.check(self.ctx.lib.apirl_project_ratio(
+            self.h, _dptr(img), _dptr(meas), _dptr(additive), _dptr(ratio), _dptr(lik), subset,
+            nsubsets, lik_positive_only, precision), "apirl_project_ratio")
+        return ratio, lik
+
+    def sensitivity(self, out=None, mult=None, subset=0, nsubsets=1, mode=BP_SCATTER,
+                    force_positive=1):
+        torch = _torch()
+        if out is None:
+            out = torch.empty(self.image_shape(), dtype=torch.float32, device="cuda")
+        self._check_img(out)
+        self.ctx.check(self.ctx.lib.apirl_sensitivity(self.h, _dptr(mult), _dptr(out), subset,
+                                                      nsubsets, mode, force_positive),
+                       "apirl_sensitivity")
+        return out
+
+    def reconstruct(self, meas, init, iterations, subsets=1, bp_mode=BP_SCATTER,
+                    precision=PREC_REFERENCE, osem_rules=None, additive=None, mult=None,
+                    out=None, sens_out=None):
+        """Device-resident MLEM (subsets=1) or OSEM; returns (image, likelihood[T*S+1])."""
+        torch = _torch()
+        if osem_rules is None:
+            osem_rules = 1 if subsets > 1 else 0
+        if out is None:
+            out = torch.empty(self.image_shape(), dtype=torch.float32, device=init.device)
+        opt = EmOptions(iterations, subsets, bp_mode, precision, osem_rules)
+        lik = np.zeros(iterations * subsets + 1, np.float64)
+        self._check_img(init)
+        self._check_img(out)
+        self._check_sino(meas, 0, 1)
+        self.ctx.check(self.ctx.lib.apirl_em_reconstruct(
+            self.h, ctypes.byref(opt), _dptr(meas), _dptr(init), _dptr(additive), _dptr(mult),
+            _dptr(out), _np_ptr(lik), _dptr(sens_out)), "apirl_em_reconstruct")
+        return out, lik
+
+    def _check_img(self, t):
+        if t.numel() != self.nx * self.ny * self.nz or not t.is_contiguous() or \
+                t.dtype.__str__() != "torch.float32" or not t.is_cuda:
+            raise ApirlError(f"image tensor must be contiguous float32 on the GPU with "
+                             f"{self.nx * self.ny * self.nz} voxels")
+
+    def _check_sino(self, t, subset, nsubsets):
+        n = int(np.prod(self.subset_shape(subset, nsubsets)))
+        if t.numel() != n or not t.is_contiguous() or t.dtype.__str__() != "torch.float32" or \
+                not t.is_cuda:
+            raise ApirlError(f"sinogram tensor must be contiguous float32 on the GPU with {n} bins")
+
+    def close(self):
+        if self.h:
+            self.ctx.lib.apirl_plan_destroy(self.h)
+            self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def em_update(ctx, x, b, s, thr, strict):
+    ctx.check(ctx.lib.apirl_em_update(ctx.h, _dptr(x), _dptr(b), _dptr(s), x.numel(), thr,
+                                      int(strict)), "apirl_em_update")
+    return x
+
+
+def mlem_threshold(ctx, sens):
+    out = ctypes.c_float(0)
+    ctx.check(ctx.lib.apirl_mlem_threshold(ctx.h, _dptr(sens), sens.numel(), ctypes.byref(out)),
+              "apirl_mlem_threshold")
+    return out.value

@@ -1,0 +1,219 @@
+"""Benchmark: 3D OSEM on one MI355X.
+
+Workload (the reference's 3D OSEM single-job configuration): 256x256x81 float32
+image, 2.0445 mm in-plane / 2.027 mm axial voxels; cylindrical 3D sinogram of
+344 radial bins x 252 angles x 837 sinograms (64 rings, span 11, maximum ring
+difference 60, scanner radius 400 mm); phantom = uniform cylinder r=100 mm filling
+the axial FOV + 6 point sources (value 50) at seeded positions; measured data =
+Poisson(3e8 counts) of the phantom's projection (synthetic, seeded); 21 subsets.
+
+A step is one full OSEM iteration (21 sub-iterations: fused projection + ratio +
+likelihood, backprojection, update) with every input resident in HBM.  The
+sensitivity images are computed once before timing, as the reference does before
+its loop (OsemSinogram3d.cpp:101-117).
+
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--bp scatter|gather]
+                    [--no-cpu-baseline] [--calibrate-copy]
+
+Prints one JSON line on stdout; progress goes to stderr."""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
+          rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
+SUBSETS = 21
+COUNTS = 3e8
+HBM_PEAK_GBS = 8000.0
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--bp", choices=["scatter", "gather"], default="scatter")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a plain 1 GiB device copy first (for PMC byte calibration)")
+    return ap.parse_args()
+
+
+def cpu_baseline(meas_host, threads):
+    """The reference's own CPU path (oracle/_ref/ref_driver, built from its sources)
+    timed on one OSEM sub-iteration of the same workload; iterations/s = 1 / (21 x t)."""
+    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(drv):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        mp = os.path.join(td, "meas.f32")
+        meas_host.tofile(mp)
+        args = [drv, "osem3d_sub"] + [f"{k}={v}" for k, v in C2.items()] + \
+            [f"subsets={SUBSETS}", "subset=0", f"meas={mp}"]
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        try:
+            out = subprocess.run(args, env=env, capture_output=True, text=True, timeout=600,
+                                 check=True).stdout
+        except Exception as e:  # the reference may fail; report it rather than crash
+            progress(f"cpu baseline failed: {e}")
+            return None
+    rec = json.loads(out.splitlines()[0])
+    t = rec["seconds"]
+    return {"value": 1.0 / (SUBSETS * t), "unit": "OSEM iters/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"one OSEM sub-iteration (subset 0 of {SUBSETS}: Project + ratio + "
+                      f"Backproject + update, {t:.2f} s) of the same workload, "
+                      f"OMP_NUM_THREADS={threads}; iterations/s = 1/(21 t)"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(whole jobs or ray shards per GPU are described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from apirl_amd import hip, phantoms
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    if args.calibrate_copy:
+        a = torch.empty(1 << 28, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
+    ctx = hip.Context(0)
+    g = C2
+    progress("building geometry plan")
+    plan = hip.Plan.for_3d(ctx, g["nx"], g["ny"], g["nz"], g["dx"], g["dz"], g["nproj"],
+                           g["nr"], g["rfov"], g["rscanner"], g["rings"], g["zfov"], g["span"],
+                           g["mrd"])
+    img_np, _ = phantoms.cylinder_points3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    truth = plan.project(torch.from_numpy(img_np).to(dev))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    lam = truth * (COUNTS / float(truth.double().sum()))
+    meas = torch.poisson(lam, generator=gen).contiguous()
+    del truth, lam
+    nvox = plan.nx * plan.ny * plan.nz
+    progress("sensitivity images")
+    sens = torch.empty((SUBSETS,) + plan.image_shape(), dtype=torch.float32, device=dev)
+    bp_mode = hip.BP_GATHER if args.bp == "gather" else hip.BP_SCATTER
+    for s in range(SUBSETS):
+        plan.sensitivity(out=sens[s], subset=s, nsubsets=SUBSETS, mode=bp_mode)
+    n_int = [plan.intersections(s, SUBSETS) for s in range(SUBSETS)]
+    n_ray = [int(np.prod(plan.subset_shape(s, SUBSETS))) for s in range(SUBSETS)]
+    ratio = torch.empty(max(n_ray), dtype=torch.float32, device=dev)
+    bp = torch.empty(plan.image_shape(), dtype=torch.float32, device=dev)
+    x = torch.ones(plan.image_shape(), dtype=torch.float32, device=dev)
+    torch.cuda.synchronize()
+    assert torch.cuda.current_stream().cuda_stream == 0, "events must be on the library stream"
+
+    ev = {"fwd": [], "bwd": [], "upd": []}
+
+    def iteration(record):
+        for s in range(SUBSETS):
+            r = ratio[: n_ray[s]].view(plan.subset_shape(s, SUBSETS))
+            if record:
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e2 = torch.cuda.Event(enable_timing=True)
+                e3 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+            plan.project_ratio(x, meas, subset=s, nsubsets=SUBSETS, ratio=r, want_lik=False)
+            if record:
+                e1.record()
+            bp.zero_()
+            plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=bp_mode)
+            if record:
+                e2.record()
+            hip.em_update(ctx, x, bp, sens[s], 0.0, 0)
+            if record:
+                e3.record()
+                ev["fwd"].append((e0, e1, s))
+                ev["bwd"].append((e1, e2, s))
+                ev["upd"].append((e2, e3, s))
+
+    progress(f"warmup {args.warmup} iteration(s)")
+    for _ in range(args.warmup):
+        iteration(False)
+    torch.cuda.synchronize()
+    progress(f"timing {args.steps} iteration(s)")
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        iteration(True)
+        torch.cuda.synchronize()
+        progress(f"iteration {k + 1}/{args.steps} done")
+    t1 = time.perf_counter()
+    torch.cuda.synchronize()
+    wall = t1 - t0
+    it_s = args.steps / wall
+
+    def avg_ms(key):
+        return float(np.mean([a.elapsed_time(b) for a, b, _ in ev[key]]))
+
+    fwd_ms, bwd_ms, upd_ms = avg_ms("fwd"), avg_ms("bwd"), avg_ms("upd")
+    nint_it = float(sum(n_int))
+    nray_it = float(sum(n_ray))
+    # algorithmic bytes per launch (averaged over the 21 subsets):
+    #   forward fused: 4 N_int (voxel reads) + 4 N_ray (meas read) + 4 N_ray (ratio write)
+    #   backprojection: 4 N_int (voxel read-modify-writes counted once) + 4 N_ray + 4 N_vox
+    fwd_bytes = (4 * nint_it + 8 * nray_it) / SUBSETS
+    bwd_bytes = (4 * nint_it + 4 * nray_it) / SUBSETS + 4 * nvox
+    dom = "bwd" if bwd_ms >= fwd_ms else "fwd"
+    dom_bytes, dom_ms = (bwd_bytes, bwd_ms) if dom == "bwd" else (fwd_bytes, fwd_ms)
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    gint = nint_it * 2 * args.steps / wall / 1e9  # forward + backward traversals
+    progress(f"{it_s:.3f} it/s; fwd {fwd_ms:.3f} ms, bwd {bwd_ms:.3f} ms, upd {upd_ms:.3f} ms")
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        progress("cpu baseline (reference CPU path, one sub-iteration)")
+        threads = min(os.cpu_count() or 1, 16)
+        cpu = cpu_baseline(meas.cpu().numpy(), threads)
+
+    line = {
+        "metric": "OSEM iters/s (256x256x81, 344x252x837, 21 subsets)",
+        "value": it_s,
+        "unit": "OSEM iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (cylinder + 6 point sources, Poisson 3e8 counts, seed 0)",
+        "config": {"workload": "3D OSEM single job: 256x256x81 image, 344r x 252ang x 837 "
+                               "sinograms (64 rings, span 11, MRD 60), 21 subsets",
+                   "backprojector": args.bp, "parallelism": "single GPU"},
+        "ray_voxel_gint_per_s": gint,
+        "intersections_per_iteration": nint_it,
+        "kernel_ms": {"forward_fused": fwd_ms, "backproject": bwd_ms, "update": upd_ms},
+        "roofline": {"bound": "hbm", "kernel": "forward_fused" if dom == "fwd" else "backproject",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,185 @@
+/*
+ * apirl_hip.h -- C-ABI of the MI355X (gfx950) Siddon projector / backprojector and
+ * MLEM / OSEM update.
+ *
+ * This is the drop-in boundary for APIRL's Projector API and EM drivers.  Every
+ * entry point takes plain pointers and sizes; device pointers are hipMalloc'd
+ * float arrays in the reference's layouts:
+ *   image     float[nz][ny][nx]            (data/src/Images.cpp:34-56)
+ *   sinogram  float[nsino][nang_sub][nr]   sinograms segment-major, angles of the
+ *                                          selected subset only, r fastest
+ *                                          (data/src/Sinogram3D.cpp:396-464,
+ *                                          Sinogram3DCylindricalPet.cpp:164-197)
+ * Errors: every function returns an apirl_status; 0 is success.  The message of
+ * the last failure on a context is returned by apirl_last_error().  No function
+ * calls exit() (the reference's CUDA path does, reconGPU/inc/helper_cuda.h:32-40).
+ * Threading: contexts and plans carry all state; no globals, so distinct contexts
+ * may be used from distinct threads.  Work is enqueued on the context's stream.
+ */
+#ifndef APIRL_HIP_H
+#define APIRL_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum apirl_status {
+  APIRL_OK = 0,
+  APIRL_ERR_INVALID = 1,     /* bad argument or size */
+  APIRL_ERR_HIP = 2,         /* HIP runtime error (message has the HIP error string) */
+  APIRL_ERR_NOMEM = 3,       /* device allocation failed */
+  APIRL_ERR_UNSUPPORTED = 4  /* combination not implemented (the reference's "return false") */
+} apirl_status;
+
+/* Backprojection forms. */
+enum {
+  APIRL_BP_SCATTER = 0, /* ray-driven, fp32 atomics: fastest, order not fixed */
+  APIRL_BP_GATHER = 1   /* voxel-driven, fixed order: deterministic and equal, bit for bit,
+                           to the reference's single-thread result */
+};
+
+/* Precision of the forward line integral. */
+enum {
+  APIRL_PREC_REFERENCE = 0, /* fp32, the reference's operation order (bit-faithful) */
+  APIRL_PREC_KAHAN = 1,     /* fp32 with Kahan-compensated accumulation along the ray */
+  APIRL_PREC_F64_ALPHA = 2  /* fp64 alpha stepping / weights, compensated fp32 sum */
+};
+
+typedef struct apirl_ctx apirl_ctx;
+typedef struct apirl_plan apirl_plan;
+
+typedef struct apirl_image_desc {
+  int nx, ny, nz;   /* nz = 1 for 2D images */
+  float dx, dy, dz; /* voxel size in mm (Images.h SizeImage) */
+} apirl_image_desc;
+
+/* Cylindrical-scanner sinogram geometry.
+ *   dims = 2: one 2D sinogram (Sinogram2DinCylindrical3Dpet, 2D LOR form,
+ *             Sinogram2DinCylindrical3Dpet.cpp:249-266); z arrays ignored.
+ *   dims = 3: nsino 2D sinograms (Sinogram3DCylindricalPet); LOR z taken from the
+ *             per-sinogram mean ring coordinates (SiddonProjector.cpp:349-361,394-395).
+ * ang_deg / r_mm may be NULL for the reference's uniform grids
+ * (Sinogram2DinCylindrical3Dpet.cpp:46-60). */
+typedef struct apirl_sino_desc {
+  int dims;
+  int nang;            /* projection angles of the full sinogram */
+  int nr;              /* radial bins */
+  int nsino;           /* 2D sinograms (1 when dims = 2) */
+  float rfov_mm;       /* radius of the radial grid */
+  float rscanner_mm;   /* scanner radius */
+  float ring_width_mm; /* axial FOV / rings (3D only) */
+  const float* ang_deg;   /* [nang] or NULL */
+  const float* r_mm;      /* [nr] or NULL */
+  const float* z1_mean_mm; /* [nsino] mean axial coordinate of ring 1 (3D) */
+  const float* z2_mean_mm; /* [nsino] mean axial coordinate of ring 2 (3D) */
+} apirl_sino_desc;
+
+/* ---- context ---------------------------------------------------------------- */
+/* Replaces initCuda()/checkCudaErrors of reconGPU (CuMlemSinogram3d.cu:60-137). */
+apirl_status apirl_ctx_create(int device, apirl_ctx** out);
+apirl_status apirl_ctx_destroy(apirl_ctx* ctx);
+const char* apirl_last_error(const apirl_ctx* ctx);
+/* Work is enqueued on this stream (hipStream_t; NULL = the null stream). */
+apirl_status apirl_ctx_set_stream(apirl_ctx* ctx, void* stream);
+apirl_status apirl_ctx_synchronize(apirl_ctx* ctx);
+/* Device memory helpers for callers without a HIP runtime of their own. */
+apirl_status apirl_malloc(apirl_ctx* ctx, void** dptr, size_t bytes);
+apirl_status apirl_free(apirl_ctx* ctx, void* dptr);
+apirl_status apirl_memcpy_h2d(apirl_ctx* ctx, void* dst, const void* src, size_t bytes);
+apirl_status apirl_memcpy_d2h(apirl_ctx* ctx, void* dst, const void* src, size_t bytes);
+apirl_status apirl_memset(apirl_ctx* ctx, void* dptr, int value, size_t bytes);
+
+/* ---- geometry plan ------------------------------------------------------------ */
+/* Host-side LOR tables (endpoints with the same libm calls as the reference,
+ * Sinogram2DinCylindrical3Dpet.cpp:216-266, and the xy part of the Siddon setup,
+ * recon/src/Siddon.cpp:83-145,177-181,253-286) computed once and kept in HBM.
+ * Replaces CuMlemSinogram3d::InitGpuMemory (CuMlemSinogram3d.cu:216-368). */
+apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
+                               const apirl_sino_desc* sino, apirl_plan** out);
+apirl_status apirl_plan_destroy(apirl_plan* plan);
+/* Number of angles in subset `subset` of `nsubsets` (Sinogram3DCylindricalPet.cpp:168-171). */
+int apirl_subset_num_angles(int nang, int nsubsets, int subset);
+/* Bins of a (subset) sinogram: nsino * subset angles * nr. */
+size_t apirl_plan_subset_bins(const apirl_plan* plan, int subset, int nsubsets);
+size_t apirl_plan_image_voxels(const apirl_plan* plan);
+/* Sum over the subset's rays of the reference's visited-voxel count
+ * |di|+|dj|+|dk| (recon/src/Siddon.cpp:220,521); computed on the device. */
+apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets,
+                                      unsigned long long* out);
+
+/* ---- Projector API (recon/inc/Projector.h:48-85) ------------------------------ */
+/* Project: d_sino[subset bins] := A d_img.  Replaces SiddonProjector::Project
+ * (2D SiddonProjector.cpp:146-193, 3D :322-441).  2D bins whose LOR the reference
+ * rejects (r > rfov) are left untouched, as there. */
+apirl_status apirl_project(apirl_plan* plan, const float* d_img, float* d_sino, int subset,
+                           int nsubsets, int precision);
+/* Backproject: d_img += A^T d_sino (accumulates; the caller zeroes, Mlem2d.cpp:162).
+ * Replaces SiddonProjector::Backproject (2D :34-87, 3D :444-568). */
+apirl_status apirl_backproject(apirl_plan* plan, const float* d_sino, float* d_img, int subset,
+                               int nsubsets, int mode);
+/* DivideAndBackproject: d_img += A^T r with r the reference's ratio rule per segment
+ * (3D SiddonProjector.cpp:571-704; 2D :90-144). */
+apirl_status apirl_divide_backproject(apirl_plan* plan, const float* d_meas, const float* d_est,
+                                      float* d_img, int subset, int nsubsets, int mode);
+
+/* ---- fused EM steps ------------------------------------------------------------ */
+/* One forward step of an EM sub-iteration, fused: est = A x (+ additive), the
+ * likelihood partial sum y log(est) - est over est != 0 (or est > 0 when
+ * lik_positive_only, the 2D rule) and the ratio
+ * ratio = (y != 0 && est != 0) ? y / est : 0 (Sinogram3D.cpp:830-856).
+ * d_meas and d_additive are FULL sinograms (all angles); d_ratio is the subset
+ * sinogram.  d_additive and d_lik may be NULL; *d_lik receives a double. */
+apirl_status apirl_project_ratio(apirl_plan* plan, const float* d_img, const float* d_meas,
+                                 const float* d_additive, float* d_ratio, double* d_lik,
+                                 int subset, int nsubsets, int lik_positive_only,
+                                 int precision);
+/* x := cond ? x * b / s : 0, cond = s > thr (strict) or s >= thr
+ * (Mlem2d.cpp:171-185, MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237). */
+apirl_status apirl_em_update(apirl_ctx* ctx, float* d_x, const float* d_b, const float* d_s,
+                             size_t n, float thr, int strict);
+/* Sensitivity of a subset: d_sens := forcePositive(A^T m) with m = 1 or the subset of
+ * a multiplicative sinogram d_mult (FULL layout, may be NULL)
+ * (OsemSinogram3d.cpp:327-346, Mlem2d.cpp:262-278, Images.cpp:383-395).
+ * force_positive = 0 skips the clamp (the MLEM drivers do not clamp). */
+apirl_status apirl_sensitivity(apirl_plan* plan, const float* d_mult, float* d_sens, int subset,
+                               int nsubsets, int mode, int force_positive);
+/* MLEM update threshold min_nonzero + 1e-4 (max - min_nonzero) (Mlem.cpp:39-42,
+ * Images.cpp:397-431); result written to *h_thr. */
+apirl_status apirl_mlem_threshold(apirl_ctx* ctx, const float* d_sens, size_t n, float* h_thr);
+
+/* ---- EM drivers (recon/src/Mlem2d.cpp, MlemSinogram3d.cpp, OsemSinogram3d.cpp) --- */
+typedef struct apirl_em_options {
+  int iterations;
+  int subsets;         /* 1 = MLEM */
+  int bp_mode;         /* APIRL_BP_SCATTER or APIRL_BP_GATHER */
+  int precision;       /* APIRL_PREC_* for the forward step */
+  int osem_rules;      /* 1: per-subset sensitivity with forcePositive, threshold 0 with >=
+                          (OsemSinogram3d); 0: MLEM rules (threshold min+1e-4(max-min),
+                          strict > for 2D, >= for 3D) */
+} apirl_em_options;
+
+/* Runs the reconstruction with every buffer resident on the device.
+ * d_meas: full measured sinogram; d_init: initial image; d_img: output image;
+ * d_additive / d_mult may be NULL; h_lik (may be NULL) receives
+ * iterations*subsets + 1 likelihood values in the reference's order;
+ * d_sens_out (may be NULL) receives the sensitivity image(s). */
+apirl_status apirl_em_reconstruct(apirl_plan* plan, const apirl_em_options* opt,
+                                  const float* d_meas, const float* d_init,
+                                  const float* d_additive, const float* d_mult, float* d_img,
+                                  double* h_lik, float* d_sens_out);
+
+/* ---- host helpers (no device work) --------------------------------------------- */
+/* Reference grids (Sinogram2DinCylindrical3Dpet.cpp:46-60). */
+void apirl_sino_grid(int nang, int nr, float rfov_mm, float* ang_deg, float* r_mm);
+/* Ring-pair enumeration and mean ring coordinates of a cylindrical 3D sinogram
+ * (Sinogram3D.cpp:358-367,396-464; SiddonProjector.cpp:357-360).  Returns the
+ * number of sinograms, or -1 if a segment count does not match nsino_per_seg. */
+int apirl_ring_table(int nrings, float zfov_mm, int nseg, const int* nsino_per_seg,
+                     const int* min_rd, const int* max_rd, float* z1_mean, float* z2_mean);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

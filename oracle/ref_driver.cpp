@@ -16,6 +16,7 @@
 //   osem3d    <geom3d> subsets iters meas init outdir
 //   mlem3d    <geom3d> iters meas init outdir
 //   rays3d    <geom3d> img start stride out             sampled reference rays
+//   osem3d_sub <geom3d> subsets subset meas             one timed OSEM sub-iteration
 //   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
 // The reference's 2D projector checks only lower bounds and its 3D projector only
 // upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
@@ -34,6 +35,8 @@
 #include <MlemSinogram3d.h>
 #include <OsemSinogram3d.h>
 #include <omp.h>
+
+#include <malloc.h>
 
 #include <chrono>
 #include <cstdio>
@@ -509,9 +512,65 @@ int cmd_rays3d() {
   return 0;
 }
 
+
+// One OSEM sub-iteration of the reference (OsemSinogram3d.cpp:164-237 minus the
+// per-iteration file output): getSubset of the measured sinogram, Project,
+// likelihood, inverseDivideBinToBin, Backproject, pixel update.  Timed as the CPU
+// baseline of the OSEM workload.
+int cmd_osem3d_sub() {
+  Image img(size3d());
+  size_t npix = (size_t)img.getPixelCount();
+  for (size_t i = 0; i < npix; i++) img.getPixelsPtr()[i] = 1.0f;
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  std::vector<float> y = read_f32(arg("meas"), bins3d(s));
+  sino3d_from(s, y.data());
+  int subsets = iarg("subsets"), sub = iarg("subset");
+  SiddonProjector p;
+  Image bp(size3d());
+  Image sens(size3d());
+  {
+    Sinogram3D* ones = s->getSubset(sub, subsets);
+    ones->FillConstant(1);
+    GuardedPixels gs(&sens);
+    p.Backproject(ones, &sens);
+    delete ones;
+  }
+  double t0;
+  float lik;
+  {
+    GuardedPixels gi(&img);
+    GuardedPixels gb(&bp);
+    t0 = now_s();
+    Sinogram3D* in = s->getSubset(sub, subsets);
+    Sinogram3D* est = s->getSubset(sub, subsets);
+    est->FillConstant(0);
+    p.Project(&img, est);
+    lik = est->getLikelihoodValue(in);
+    bp.fillConstant(0);
+    est->inverseDivideBinToBin(in);
+    p.Backproject(est, &bp);
+    float* x = img.getPixelsPtr();
+    float* b = bp.getPixelsPtr();
+    float* sv = sens.getPixelsPtr();
+    for (size_t k = 0; k < npix; k++) x[k] = (sv[k] >= 0) ? x[k] * b[k] / sv[k] : 0;
+    report_time("osem3d_sub", t0);
+    delete in;
+    delete est;
+  }
+  printf("{\"likelihood\": %.9g}\n", lik);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
+  // Keep every allocation on the brk heap: the reference's images are allocated
+  // inside its EM drivers, where the guard of GuardedPixels cannot reach, and an
+  // entry index of -1 at slice 0 reads one row before the array
+  // (SiddonProjector.cpp:417).  With a page-aligned mmap'd array that read faults;
+  // on the heap it reads a neighbouring chunk, and its weight is exactly 0
+  // (the first crossing equals alpha_min there).
+  mallopt(M_MMAP_THRESHOLD, 1 << 30);
   if (argc < 2) {
     fprintf(stderr, "usage: ref_driver <command> key=value ...\n");
     return 1;
@@ -532,6 +591,7 @@ int main(int argc, char** argv) {
   if (c == "osem3d") return cmd_osem3d();
   if (c == "mlem3d") return cmd_mlem3d();
   if (c == "rays3d") return cmd_rays3d();
+  if (c == "osem3d_sub") return cmd_osem3d_sub();
   fprintf(stderr, "ref_driver: unknown command %s\n", c.c_str());
   return 1;
 }
