@@ -52,6 +52,17 @@ struct Geo {
   float offz;  // dz / 2, Siddon.cpp:162
   int nr, nang, nsino;
   int dims;
+  float r0, dr, rmargin;  // uniform radial grid r_j = r0 + j dr (gather candidate search)
+};
+
+// A run of consecutive sinograms with increasing mean axial coordinate (one segment
+// of the reference's enumeration): zmean(first + l) ~ z0 + l h.  The gather
+// backprojector starts its sinogram scan at the index this line predicts for a ray
+// of typical axial slope dlo (the middle sinogram's zb - za); dhi holds 1 / h.
+struct GatherSeg {
+  int first, count;
+  float z0, h, dlo, dhi, margin;
+  int pad;
 };
 
 }  // namespace apirl
@@ -74,6 +85,13 @@ struct apirl_plan {
   apirl::Geo geo{};
   apirl::RayXY* d_rays = nullptr;  // [nang][nr]
   float2* d_zab = nullptr;         // [nsino] (P1.Z, P2.Z), 3D only
+  float* d_replicas = nullptr;     // [8][nvox] per-XCD scatter targets (zero between calls)
+  float2* d_angcs = nullptr;       // [nang] (cos, sin) for the gather candidate search
+  apirl::GatherSeg* d_segs = nullptr;
+  int nseg = 0;
+  bool gather_ok = false;          // uniform radial grid and increasing runs found
+  void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
+  size_t gscratch_bytes = 0;
   std::vector<float> ang, r;       // host copies of the grids
   std::vector<float2> zab;
 };
@@ -112,6 +130,14 @@ hipError_t launch_likelihood(const float* x, const float* y, long n, int positiv
 hipError_t launch_min_nz_max(const float* x, size_t n, float* d_out2, hipStream_t st);
 hipError_t launch_count_intersections(const apirl_plan* p, int subset, int nsubsets,
                                       unsigned long long* d_out, hipStream_t st);
+
+hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, float* img,
+                                     int subset, int nsubsets, int kind, void* scratch,
+                                     size_t scratch_bytes, hipStream_t st);
+size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
+void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
+                         std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,
+                         bool& ok);
 
 // Backprojection kinds
 enum { BP_PLAIN = 0, BP_ONES = 1, BP_DIVIDE = 2, BP_MULT = 3 };

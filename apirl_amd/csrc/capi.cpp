@@ -56,6 +56,36 @@ bool check_subset(const apirl_plan* p, int subset, int nsubsets) {
   return nsubsets >= 1 && subset >= 0 && subset < nsubsets && nsubsets <= p->geo.nang;
 }
 
+// Backprojection of one subset in either form.
+apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est, float* img,
+                             int subset, int nsubsets, int kind, int mode) {
+  apirl_ctx* c = p->ctx;
+  if (mode == APIRL_BP_SCATTER) {
+    APIRL_HIP(c, apirl::launch_backproject(p, sino, est, img, subset, nsubsets, kind, mode,
+                                           c->stream));
+    return APIRL_OK;
+  }
+  if (mode != APIRL_BP_GATHER) return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
+  if (!p->gather_ok) return fail(c, APIRL_ERR_UNSUPPORTED, "gather needs a uniform radial grid");
+  if (kind == apirl::BP_DIVIDE)
+    return fail(c, APIRL_ERR_UNSUPPORTED, "divide-and-backproject has the scatter form only");
+  const size_t need = apirl::gather_scratch_bytes(p, subset, nsubsets);
+  if (need > p->gscratch_bytes) {
+    if (p->d_gscratch) {
+      APIRL_HIP(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(p->d_gscratch);
+    }
+    p->d_gscratch = nullptr;
+    p->gscratch_bytes = 0;
+    if (hipMalloc(&p->d_gscratch, need) != hipSuccess)
+      return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather scratch");
+    p->gscratch_bytes = need;
+  }
+  APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind,
+                                                p->d_gscratch, p->gscratch_bytes, c->stream));
+  return APIRL_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -204,6 +234,40 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       return fail(ctx, APIRL_ERR_NOMEM, "cannot upload the ring table");
     }
   }
+  // Tables of the gather backprojector's candidate search.
+  {
+    std::vector<float2> angcs;
+    std::vector<apirl::GatherSeg> segs;
+    float r0 = 0, dr = 1, rm = 0;
+    bool ok = false;
+    apirl::build_gather_tables(*p, angcs, segs, r0, dr, rm, ok);
+    geo.r0 = r0;
+    geo.dr = dr;
+    geo.rmargin = rm;
+    p->nseg = (int)segs.size();
+    p->gather_ok = ok;
+    if (hipMalloc(&p->d_angcs, angcs.size() * sizeof(float2)) != hipSuccess ||
+        hipMemcpy(p->d_angcs, angcs.data(), angcs.size() * sizeof(float2),
+                  hipMemcpyHostToDevice) != hipSuccess)
+      p->gather_ok = false;
+    if (!segs.empty() &&
+        (hipMalloc(&p->d_segs, segs.size() * sizeof(apirl::GatherSeg)) != hipSuccess ||
+         hipMemcpy(p->d_segs, segs.data(), segs.size() * sizeof(apirl::GatherSeg),
+                   hipMemcpyHostToDevice) != hipSuccess))
+      p->gather_ok = false;
+    (void)hipGetLastError();
+  }
+  // Per-XCD scatter targets for the backprojector: 8 image copies, zeroed here and
+  // re-zeroed by the reduction after every use.
+  {
+    const size_t bytes = (size_t)8 * im.nx * im.ny * im.nz * sizeof(float);
+    if (hipMalloc(&p->d_replicas, bytes) != hipSuccess ||
+        hipMemset(p->d_replicas, 0, bytes) != hipSuccess) {
+      if (p->d_replicas) (void)hipFree(p->d_replicas);
+      p->d_replicas = nullptr;  // fall back to direct atomics on the image
+      (void)hipGetLastError();
+    }
+  }
   *out = p;
   return APIRL_OK;
 }
@@ -213,6 +277,10 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   DeviceGuard g(p->ctx->device);
   if (p->d_rays) (void)hipFree(p->d_rays);
   if (p->d_zab) (void)hipFree(p->d_zab);
+  if (p->d_replicas) (void)hipFree(p->d_replicas);
+  if (p->d_angcs) (void)hipFree(p->d_angcs);
+  if (p->d_segs) (void)hipFree(p->d_segs);
+  if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   delete p;
   return APIRL_OK;
 }
@@ -260,9 +328,7 @@ apirl_status apirl_backproject(apirl_plan* p, const float* d_sino, float* d_img,
   if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
   DeviceGuard g(p->ctx->device);
-  APIRL_HIP(p->ctx, apirl::launch_backproject(p, d_sino, nullptr, d_img, subset, nsubsets,
-                                              apirl::BP_PLAIN, mode, p->ctx->stream));
-  return APIRL_OK;
+  return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN, mode);
 }
 
 apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const float* d_est,
@@ -273,9 +339,7 @@ apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const 
   if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
   DeviceGuard g(p->ctx->device);
-  APIRL_HIP(p->ctx, apirl::launch_backproject(p, d_meas, d_est, d_img, subset, nsubsets,
-                                              apirl::BP_DIVIDE, mode, p->ctx->stream));
-  return APIRL_OK;
+  return backproject_any(p, d_meas, d_est, d_img, subset, nsubsets, apirl::BP_DIVIDE, mode);
 }
 
 apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float* d_meas,
@@ -322,9 +386,9 @@ apirl_status apirl_sensitivity(apirl_plan* p, const float* d_mult, float* d_sens
   DeviceGuard g(c->device);
   const size_t nvox = apirl_plan_image_voxels(p);
   APIRL_HIP(c, hipMemsetAsync(d_sens, 0, nvox * sizeof(float), c->stream));
-  APIRL_HIP(c, apirl::launch_backproject(p, d_mult, nullptr, d_sens, subset, nsubsets,
-                                         d_mult ? apirl::BP_MULT : apirl::BP_ONES, mode,
-                                         c->stream));
+  apirl_status st = backproject_any(p, d_mult, nullptr, d_sens, subset, nsubsets,
+                                    d_mult ? apirl::BP_MULT : apirl::BP_ONES, mode);
+  if (st != APIRL_OK) return st;
   if (force_positive) APIRL_HIP(c, apirl::launch_force_positive(d_sens, nvox, c->stream));
   return APIRL_OK;
 }

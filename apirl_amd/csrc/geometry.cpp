@@ -4,6 +4,7 @@
 // baseline has no FMA either, so every product and sum is rounded as written).
 #include <cfloat>
 #include <cmath>
+#include <algorithm>
 #include <cstdlib>
 
 #include "apirl_internal.h"
@@ -123,6 +124,57 @@ void build_z_table(const apirl_sino_desc& sino, std::vector<float2>& zab) {
     zab[s].x = z1 - w / 2 + dzs / 2 + 0 * dzs;
     zab[s].y = z2 - w / 2 + dzs / 2 + 0 * dzs;
   }
+}
+
+void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
+                         std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,
+                         bool& ok) {
+  const int nang = p.geo.nang, nr = p.geo.nr;
+  angcs.resize(nang);
+  for (int a = 0; a < nang; a++) {
+    const double rad = (double)p.ang[a] * (double)kDegToRad;
+    angcs[a].x = (float)cos(rad);
+    angcs[a].y = (float)sin(rad);
+  }
+  r0 = p.r[0];
+  dr = nr > 1 ? (p.r[nr - 1] - p.r[0]) / (float)(nr - 1) : 1.f;
+  double dev = 0;
+  for (int j = 0; j < nr; j++) dev = std::max(dev, std::fabs((double)p.r[j] - (r0 + (double)j * dr)));
+  // candidate margin: float error of the LOR endpoints (~1e-4 mm) plus grid deviation
+  rmargin = (float)(0.01 + dev);
+  ok = nr >= 1 && dr > 0 && dev < 0.25 * dr;
+  segs.clear();
+  if (p.geo.dims != 3) return;
+  const int ns = p.geo.nsino;
+  int first = 0;
+  for (int s = 1; s <= ns; s++) {
+    const bool split = (s == ns) ||
+                       (0.5f * (p.zab[s].x + p.zab[s].y) <= 0.5f * (p.zab[s - 1].x + p.zab[s - 1].y));
+    if (!split) continue;
+    GatherSeg g{};
+    g.first = first;
+    g.count = s - first;
+    const double z0 = 0.5 * ((double)p.zab[first].x + p.zab[first].y);
+    const double zl = 0.5 * ((double)p.zab[s - 1].x + p.zab[s - 1].y);
+    const double h = g.count > 1 ? (zl - z0) / (g.count - 1) : 1.0;
+    double dlo = 1e30, dhi = -1e30, devz = 0;
+    for (int q = first; q < s; q++) {
+      const double zm = 0.5 * ((double)p.zab[q].x + p.zab[q].y);
+      devz = std::max(devz, std::fabs(zm - (z0 + (q - first) * h)));
+      const double d = (double)p.zab[q].y - p.zab[q].x;
+      dlo = std::min(dlo, d);
+      dhi = std::max(dhi, d);
+    }
+    g.z0 = (float)z0;
+    g.h = (float)h;
+    const int mid = first + g.count / 2;
+    g.dlo = p.zab[mid].y - p.zab[mid].x;  // typical axial slope of the run
+    g.dhi = (float)(1.0 / h);             // 1 / h for the start estimate
+    g.margin = (float)(devz + std::max(std::fabs(dlo), std::fabs(dhi)));
+    segs.push_back(g);
+    first = s;
+  }
+  ok = ok && !segs.empty();
 }
 
 }  // namespace apirl

@@ -21,6 +21,7 @@ namespace apirl {
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kNumXcd = 8;  // MI355X: 8 XCDs of 32 CUs
 
 // Per-ray traversal state (Siddon.cpp:253-340).
 struct Ray {
@@ -249,13 +250,23 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
 //   BP_DIVIDE: 3D: skip meas == 0; term = (w*m)/e if e != 0 else w*m (:607-685) with the
 //              upper-bound-only check of :666; 2D: skip e == 0 or m == 0 (:110);
 //              term = (w*m)/e with the full check (:125-131).
-template <int KIND>
+// Hardware id of the XCD (accelerator complex die) running this wave: s_getreg of
+// HW_REG_XCC_ID (hwreg 20, bits [3:0]) on gfx940+.
+__device__ __forceinline__ int xcc_id() {
+  return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kNumXcd - 1);
+}
+
+template <int KIND, bool REPLICA>
 __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
     Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
     const float* __restrict__ sino, const float* __restrict__ est, float* __restrict__ img,
     int nang_sub, int subset, int nsubsets, long nrays) {
   const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= nrays) return;
+  // With REPLICA, `img` holds one image per XCD and each wave adds into its own die's
+  // copy, so the atomics of a cache line stay in one L2 instead of moving between
+  // dies; reduce_replicas_kernel folds the copies afterwards.
+  if (REPLICA) img += (long)xcc_id() * ((long)g.nx * g.ny * g.nz);
   int s, a, rr, k;
   ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
   float b = 1.f, e = 0.f;
@@ -408,6 +419,22 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(const float* __restr
   if (threadIdx.x == 0) partials[blockIdx.x] = red[0];
 }
 
+// img[v] += rep[0][v] + ... + rep[7][v] (fixed order), and clears the replicas for
+// the next backprojection.
+__global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restrict__ img,
+                                                                 float* __restrict__ rep,
+                                                                 long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  float acc = 0.f;
+#pragma unroll
+  for (int x = 0; x < kNumXcd; x++) {
+    acc += rep[(long)x * n + i];
+    rep[(long)x * n + i] = 0.f;
+  }
+  img[i] += acc;
+}
+
 inline unsigned grid_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -448,9 +475,18 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
   const long nrays = (long)g.nsino * nang_sub * g.nr;
   if (nrays == 0) return hipSuccess;
   const unsigned grid = grid_for(nrays);
-#define APIRL_LAUNCH_BP(K)                                                                  \
-  hipLaunchKernelGGL((backproject_scatter_kernel<K>), dim3(grid), dim3(kBlock), 0, st, g, \
-                     p->d_rays, p->d_zab, sino, est, img, nang_sub, subset, nsubsets, nrays)
+  const long nvox = (long)g.nx * g.ny * g.nz;
+  float* rep = p->d_replicas;
+  float* target = rep ? rep : img;
+#define APIRL_LAUNCH_BP(K)                                                                   \
+  if (rep)                                                                                   \
+    hipLaunchKernelGGL((backproject_scatter_kernel<K, true>), dim3(grid), dim3(kBlock), 0, st, \
+                       g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset, nsubsets,  \
+                       nrays);                                                              \
+  else                                                                                       \
+    hipLaunchKernelGGL((backproject_scatter_kernel<K, false>), dim3(grid), dim3(kBlock), 0,   \
+                       st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,       \
+                       nsubsets, nrays)
   switch (kind) {
     case BP_ONES: APIRL_LAUNCH_BP(BP_ONES); break;
     case BP_MULT: APIRL_LAUNCH_BP(BP_MULT); break;
@@ -458,6 +494,9 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
     default: APIRL_LAUNCH_BP(BP_PLAIN); break;
   }
 #undef APIRL_LAUNCH_BP
+  if (rep)
+    hipLaunchKernelGGL(reduce_replicas_kernel, dim3(grid_for(nvox)), dim3(kBlock), 0, st, img,
+                       rep, nvox);
   return hipGetLastError();
 }
 

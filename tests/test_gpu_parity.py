@@ -263,3 +263,93 @@ def test_c2_full_projection_sampled_bins_bitexact(torch, ctx, golden):
     for s in range(21):
         tot += float(p.project(dev(torch, img), subset=s, nsubsets=21).double().sum())
     assert abs(tot - float(flat.double().sum())) <= 1e-9 * abs(tot)
+
+
+# ------------------------------------------------------------ gather (deterministic)
+
+def test_c1_backproject_gather_bitexact_vs_reference(torch, ctx, golden):
+    log("case: c1 2D gather backprojection")
+    f = golden("c1_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    p = c1_plan(ctx, g)
+    meas = dev(torch, f["meas"].astype(np.float32)).reshape(1, 192, 192)
+    out = p.backproject(meas, mode=hip.BP_GATHER)
+    torch.cuda.synchronize()
+    assert_bitexact(host(out), f["back"])
+
+
+def test_mid3d_backproject_gather_bitexact_vs_reference(torch, ctx, golden):
+    log("case: mid3d gather backprojection")
+    f = golden("mid3d_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    p = plan3d(ctx, g)
+    meas = dev(torch, f["meas"].astype(np.float32)).reshape(p.subset_shape())
+    out = p.backproject(meas, mode=hip.BP_GATHER)
+    out2 = p.backproject(meas, mode=hip.BP_GATHER)
+    torch.cuda.synchronize()
+    assert_bitexact(host(out2), host(out))  # deterministic
+    assert_voxel_close(host(out), f["back"])
+
+
+def test_mid3d_sensitivity_gather_bitexact_vs_reference(torch, ctx, golden):
+    log("case: mid3d gather sensitivity")
+    f = golden("mid3d_osem.npz")
+    g = json.loads(str(f["geom"]))
+    p = plan3d(ctx, g)
+    s0 = p.sensitivity(subset=0, nsubsets=g["subsets"], mode=hip.BP_GATHER)
+    s6 = p.sensitivity(subset=6, nsubsets=g["subsets"], mode=hip.BP_GATHER)
+    torch.cuda.synchronize()
+    assert_voxel_close(host(s0), f["sens0"])
+    assert_voxel_close(host(s6), f["sens6"])
+
+
+def test_c1_mlem2d_gather_bitexact_vs_reference(torch, ctx, golden):
+    log("case: c1 MLEM 2D 60 iterations, gather")
+    f = golden("c1_mlem2d.npz")
+    g = json.loads(str(f["geom"]))
+    p = c1_plan(ctx, g)
+    meas = dev(torch, f["meas"].astype(np.float32)).reshape(1, 192, 192)
+    init = torch.ones((1, 128, 128), dtype=torch.float32, device="cuda")
+    sens = torch.empty((1, 128, 128), dtype=torch.float32, device="cuda")
+    img, lik = p.reconstruct(meas, init, g["iters"], subsets=1, bp_mode=hip.BP_GATHER,
+                             sens_out=sens)
+    torch.cuda.synchronize()
+    assert_bitexact(host(sens), f["sens"])
+    assert_bitexact(host(img), f["final"])
+
+
+def test_mid3d_osem_gather_bitexact_vs_reference(torch, ctx, golden):
+    log("case: mid3d OSEM gather 7x2")
+    f = golden("mid3d_osem.npz")
+    g = json.loads(str(f["geom"]))
+    p = plan3d(ctx, g)
+    meas = dev(torch, f["meas"].astype(np.float32))
+    init = torch.ones(p.image_shape(), dtype=torch.float32, device="cuda")
+    img, lik = p.reconstruct(meas, init, g["iters"], subsets=g["subsets"], bp_mode=hip.BP_GATHER)
+    img2, _ = p.reconstruct(meas, init, g["iters"], subsets=g["subsets"], bp_mode=hip.BP_GATHER)
+    torch.cuda.synchronize()
+    assert_bitexact(host(img2), host(img), nan_ok=True)  # bit-for-bit deterministic
+    out, ref = host(img), f["final"]
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    assert rel_l2(out, ref) <= IMG_RL2
+
+
+def test_c2_subset_gather_vs_oracle(torch, ctx, golden):
+    """256x256x81 / 837 sinograms, subset 0 of 21: gather backprojection of the
+    phantom's projection, bit-exact against the oracle."""
+    log("case: c2 gather backprojection of subset 0/21 vs oracle")
+    f = golden("c2_rays.npz")
+    g = json.loads(str(f["geom"]))
+    p = plan3d(ctx, g)
+    img, _ = phantoms.cylinder_points3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    sino = p.project(dev(torch, img), subset=0, nsubsets=21)
+    out = p.backproject(sino, subset=0, nsubsets=21, mode=hip.BP_GATHER)
+    sc = p.backproject(sino, subset=0, nsubsets=21, mode=hip.BP_SCATTER)
+    torch.cuda.synchronize()
+    im = O.image_desc(g["nx"], g["ny"], g["nz"], g["dx"], g["dz"])
+    ang, r = O.sino_grid(g["nproj"], g["nr"], g["rfov"])
+    za, zb = O.ring_table(g["rings"], g["zfov"], g["span"], g["mrd"])
+    idx = O.subset_angles(g["nproj"], 21, 0)
+    ref = O.backproject3d(im, host(sino), ang[idx], r, g["rscanner"], za, zb)
+    assert_voxel_close(host(out), ref)
+    assert_voxel_close(host(sc), ref)
