@@ -84,21 +84,29 @@ APIRL_HD float alpha_after(float a, float u, int n) {
     } else {
       d = Mi + (fr > 0.5f ? 1u : 0u);
     }
-    // steps k whose exact sum N + (k-1) d + U stays below 2^24 round on the q grid
-    const int64_t L0 = (int64_t)(1u << 24) - (int64_t)N - (int64_t)Mi;
-    int64_t kmax;
-    if (L0 < 1) kmax = 0;
-    else if (d == 0u) kmax = n;
-    else kmax = (L0 - 1) / (int64_t)d + 1;
-    const int k = (int)(kmax < (int64_t)n ? kmax : (int64_t)n);
+    // steps k whose exact sum N + (k-1) d + U stays below 2^24 round on the q grid:
+    // k <= kmax = (L0 - 1) / d + 1 with L0 = 2^24 - N - M (all below 2^25).
+    const int32_t L0 = (int32_t)(1u << 24) - (int32_t)N - (int32_t)Mi;
+    int k;
+    if (L0 < 1) {
+      k = 0;
+    } else if (d == 0u || (uint64_t)(uint32_t)(n - 1) * d <= (uint64_t)(L0 - 1)) {
+      k = n;  // the whole run stays in this binade (common case: no division)
+    } else {
+      // kmax - 1 = floor((L0 - 1) / d): float estimate, corrected to the exact quotient
+      uint32_t q = (uint32_t)((float)(L0 - 1) / (float)d);
+      while (q * d > (uint32_t)(L0 - 1)) q--;
+      while ((q + 1) * d <= (uint32_t)(L0 - 1)) q++;
+      k = (int)q + 1;  // < n here
+    }
     if (k == 0) {
       a = a + u;  // this addition crosses into the next binade
       n--;
       continue;
     }
-    // The last of the k sums was below 2^24 before rounding, so Nn <= 2^24: the
+    // The last of the k sums was below 2^24 before rounding, so N + k d <= 2^24: the
     // conversion to float and the scaling by q are exact.
-    const uint64_t Nn = (uint64_t)N + (uint64_t)k * d;
+    const uint32_t Nn = N + (uint32_t)k * d;
     a = (float)Nn * pow2f(e2);
     n -= k;
   }

@@ -34,21 +34,29 @@ constexpr float kZMargin = 0.05f;
 __device__ __forceinline__ float z_at(float za, float vz, float alpha) { return za + vz * alpha; }
 
 // Per-ray axial part of the Siddon setup for one subset (Siddon.cpp:150-182,
-// 220, 244-255, 279-286): computed once per backprojection by zsetup_kernel.
-struct alignas(16) ZRay {
+// 220, 244-255, 279-286) and the bin value, packed in one 32-byte record so a
+// candidate ray costs one load: computed once per backprojection by zsetup_kernel.
+struct alignas(32) ZRay {
   int k0;       // entry slice
   short k1;     // exit slice
   short kinc;   // sign of Vz
   float azu;    // |dz / Vz|
   float az0;    // first z crossing (FLT_MAX when Vz == 0)
+  float L;      // ray length |P2 - P1| in the reference's operand order
+  float b;      // bin value being backprojected
+  float pad0, pad1;
 };
-static_assert(sizeof(ZRay) == 16, "ZRay is one 16-byte load");
+static_assert(sizeof(ZRay) == 32, "ZRay is one 32-byte load");
 
+// nz_pair[ka * nr + r] != 0 iff some sinogram has a non-zero bin at (angle ka, r):
+// the gather skips (angle, r) pairs whose bins are all zero (zero bins add nothing,
+// SiddonProjector.cpp:496).
 __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __restrict__ rays,
                                                          const float2* __restrict__ zab,
+                                                         const float* __restrict__ sino, int kind,
                                                          int nang_sub, int subset, int nsubsets,
                                                          long nrays, ZRay* __restrict__ out,
-                                                         float* __restrict__ lout) {
+                                                         int* __restrict__ nz_pair) {
   const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (idx >= nrays) return;
   const int rr = (int)(idx % g.nr);
@@ -70,9 +78,23 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
   if (vz > 0) z.az0 = (g.offz + (z.k0 + kinc) * g.dz - za) / vz;
   else if (vz < 0) z.az0 = (g.offz + z.k0 * g.dz - za) / vz;
   else z.az0 = FLT_MAX;
-  out[idx] = z;
   const float ez = (za + vz) - za;
-  lout[idx] = sqrtf(R.lxy + ez * ez);
+  z.L = sqrtf(R.lxy + ez * ez);
+  float b = 1.f;
+  if (kind == BP_PLAIN) b = sino[idx];
+  else if (kind == BP_MULT) b = sino[((long)s * g.nang + a) * g.nr + rr];
+  z.b = b;
+  z.pad0 = z.pad1 = 0.f;
+  out[idx] = z;
+  if (b != 0.f && (R.flags & RAY_VALID)) {
+    int* f = &nz_pair[(long)ka * g.nr + rr];
+    if (*f == 0) atomicOr(f, 1);
+  }
+}
+
+__global__ __launch_bounds__(kGBlock) void clear_int_kernel(int* __restrict__ p, long n) {
+  const long i = (long)blockIdx.x * kGBlock + threadIdx.x;
+  if (i < n) p[i] = 0;
 }
 
 struct GatherArgs {
@@ -83,7 +105,7 @@ struct GatherArgs {
   const GatherSeg* segs;
   int nseg;
   const ZRay* zr;
-  const float* zl;
+  const int* nz_pair;
   const float* sino;  // subset sinogram (PLAIN), full sinogram (MULT) or null (ONES)
   float* img;
   int kind;           // BP_PLAIN, BP_ONES or BP_MULT
@@ -123,6 +145,7 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
     jlo = jlo < 0 ? 0 : jlo;
     jhi = jhi > g.nr - 1 ? g.nr - 1 : jhi;
     for (int rr = jlo; rr <= jhi; rr++) {
+      if (DIMS == 3 && A.nz_pair[(long)ka * g.nr + rr] == 0) continue;
       const RayXY t = A.rays[(long)a * g.nr + rr];
       if (!(t.flags & RAY_VALID)) continue;
       const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
@@ -195,6 +218,7 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
           if (fminf(za_lo, za_hi) > zhi_t) break;
           const long ridx = ((long)s * A.nang_sub + ka) * g.nr + rr;
           const ZRay z = A.zr[ridx];
+          if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
           int nc;
           if (z.kinc == 0) {
             if (k != z.k0) continue;
@@ -214,15 +238,11 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
           if (na > 0 && !(Xp <= Zn)) continue;
           if (nb > 0 && !(Yp <= Zn)) continue;
           if (nc > 0 && !(Zp < Xn && Zp < Yn)) continue;
-          float b = 1.f;
-          if (A.kind == BP_PLAIN) b = A.sino[ridx];
-          else if (A.kind == BP_MULT) b = A.sino[((long)s * g.nang + a) * g.nr + rr];
-          if (b == 0.f) continue;  // SiddonProjector.cpp:496
           float ac = t.amin;
           if (m > 0) ac = fmaxf(fmaxf(Xp, Yp), Zp);
           const float an = fminf(fminf(Xn, Yn), Zn);
-          const float w = (an - ac) * A.zl[ridx];
-          acc += w * 1.0f * b;  // SiddonProjector.cpp:546
+          const float w = (an - ac) * z.L;
+          acc += w * 1.0f * z.b;  // SiddonProjector.cpp:546
         }
       }
     }
@@ -257,16 +277,18 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
   A.subset = subset;
   A.nsubsets = nsubsets;
   A.zr = nullptr;
-  A.zl = nullptr;
+  A.nz_pair = nullptr;
   if (g.dims == 3) {
-    const size_t need = (size_t)nrays * (sizeof(ZRay) + sizeof(float));
+    const size_t need = gather_scratch_bytes(p, subset, nsubsets);
     if (scratch_bytes < need) return hipErrorInvalidValue;
     ZRay* zr = (ZRay*)scratch;
-    float* zl = (float*)((char*)scratch + (size_t)nrays * sizeof(ZRay));
+    int* nzp = (int*)((char*)scratch + (size_t)nrays * sizeof(ZRay));
+    const long npair = (long)nang_sub * g.nr;
+    hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair)), dim3(kGBlock), 0, st, nzp, npair);
     hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
-                       p->d_zab, nang_sub, subset, nsubsets, nrays, zr, zl);
+                       p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp);
     A.zr = zr;
-    A.zl = zl;
+    A.nz_pair = nzp;
     const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
     hipLaunchKernelGGL(gather_kernel<3>, grid, dim3(kGBlock), (size_t)g.nsino * sizeof(float2),
                        st, A);
@@ -280,8 +302,9 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
   const Geo& g = p->geo;
   if (g.dims != 3) return 0;
-  const long nrays = (long)g.nsino * subset_num_angles(g.nang, nsubsets, subset) * g.nr;
-  return (size_t)nrays * (sizeof(ZRay) + sizeof(float));
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long nrays = (long)g.nsino * nang_sub * g.nr;
+  return (size_t)nrays * sizeof(ZRay) + (size_t)nang_sub * g.nr * sizeof(int);
 }
 
 }  // namespace apirl

@@ -256,12 +256,31 @@ __device__ __forceinline__ int xcc_id() {
   return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kNumXcd - 1);
 }
 
-template <int KIND, bool REPLICA>
+// Lane mapping of the scatter backprojector.  Rays of neighbouring radial bins run
+// through the same voxels at the same step (their spacing is below the voxel size),
+// so with r-fast lanes one atomic instruction often carries several lanes aimed at
+// one address, which the L2 serialises.  For 3D the lanes of a wavefront therefore
+// take 64 different sinograms of one (angle, r): their rays lie in different axial
+// planes, the RayXY record is one broadcast load, and the atomics spread out.
+constexpr int kSinoLanes = 64;
+constexpr int kRPerBlock = kBlock / kSinoLanes;
+
+template <int KIND, bool REPLICA, bool SINO_LANES>
 __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
     Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
     const float* __restrict__ sino, const float* __restrict__ est, float* __restrict__ img,
     int nang_sub, int subset, int nsubsets, long nrays) {
-  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+  long idx;
+  if (SINO_LANES) {
+    const int s = blockIdx.x * kSinoLanes + (threadIdx.x % kSinoLanes);
+    const int nrq = (g.nr + kRPerBlock - 1) / kRPerBlock;
+    const int ka = blockIdx.y / nrq;
+    const int rr = (blockIdx.y % nrq) * kRPerBlock + (threadIdx.x / kSinoLanes);
+    if (s >= g.nsino || rr >= g.nr) return;
+    idx = ((long)s * nang_sub + ka) * g.nr + rr;
+  } else {
+    idx = (long)blockIdx.x * kBlock + threadIdx.x;
+  }
   if (idx >= nrays) return;
   // With REPLICA, `img` holds one image per XCD and each wave adds into its own die's
   // copy, so the atomics of a cache line stay in one L2 instead of moving between
@@ -479,14 +498,27 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
   float* rep = p->d_replicas;
   float* target = rep ? rep : img;
 #define APIRL_LAUNCH_BP(K)                                                                   \
-  if (rep)                                                                                   \
-    hipLaunchKernelGGL((backproject_scatter_kernel<K, true>), dim3(grid), dim3(kBlock), 0, st, \
-                       g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset, nsubsets,  \
-                       nrays);                                                              \
-  else                                                                                       \
-    hipLaunchKernelGGL((backproject_scatter_kernel<K, false>), dim3(grid), dim3(kBlock), 0,   \
-                       st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,       \
-                       nsubsets, nrays)
+  if (g.dims == 3) {                                                                         \
+    const dim3 sgrid((g.nsino + kSinoLanes - 1) / kSinoLanes,                                \
+                     nang_sub * ((g.nr + kRPerBlock - 1) / kRPerBlock));                     \
+    if (rep)                                                                                 \
+      hipLaunchKernelGGL((backproject_scatter_kernel<K, true, true>), sgrid, dim3(kBlock), 0, \
+                         st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,      \
+                         nsubsets, nrays);                                                  \
+    else                                                                                     \
+      hipLaunchKernelGGL((backproject_scatter_kernel<K, false, true>), sgrid, dim3(kBlock), 0, \
+                         st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,      \
+                         nsubsets, nrays);                                                  \
+  } else {                                                                                   \
+    if (rep)                                                                                 \
+      hipLaunchKernelGGL((backproject_scatter_kernel<K, true, false>), dim3(grid),            \
+                         dim3(kBlock), 0, st, g, p->d_rays, p->d_zab, sino, est, target,       \
+                         nang_sub, subset, nsubsets, nrays);                                  \
+    else                                                                                     \
+      hipLaunchKernelGGL((backproject_scatter_kernel<K, false, false>), dim3(grid),           \
+                         dim3(kBlock), 0, st, g, p->d_rays, p->d_zab, sino, est, target,       \
+                         nang_sub, subset, nsubsets, nrays);                                  \
+  }
   switch (kind) {
     case BP_ONES: APIRL_LAUNCH_BP(BP_ONES); break;
     case BP_MULT: APIRL_LAUNCH_BP(BP_MULT); break;
