@@ -86,6 +86,7 @@ struct apirl_plan {
   apirl::RayXY* d_rays = nullptr;  // [nang][nr]
   float2* d_zab = nullptr;         // [nsino] (P1.Z, P2.Z), 3D only
   float* d_replicas = nullptr;     // [8][nvox] per-XCD scatter targets (zero between calls)
+  int* d_zorder = nullptr;         // [nsino] sinograms sorted by mean axial coordinate
   float2* d_angcs = nullptr;       // [nang] (cos, sin) for the gather candidate search
   apirl::GatherSeg* d_segs = nullptr;
   int nseg = 0;

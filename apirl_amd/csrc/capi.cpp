@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <vector>
 
 #include "apirl_internal.h"
@@ -234,6 +235,21 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       return fail(ctx, APIRL_ERR_NOMEM, "cannot upload the ring table");
     }
   }
+  // Forward-projection order: sinograms by mean axial coordinate (stable).
+  if (sd.dims == 3) {
+    std::vector<int> order(sd.nsino);
+    for (int q = 0; q < sd.nsino; q++) order[q] = q;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return (p->zab[x].x + p->zab[x].y) < (p->zab[y].x + p->zab[y].y);
+    });
+    if (hipMalloc(&p->d_zorder, order.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(p->d_zorder, order.data(), order.size() * sizeof(int),
+                  hipMemcpyHostToDevice) != hipSuccess) {
+      if (p->d_zorder) (void)hipFree(p->d_zorder);
+      p->d_zorder = nullptr;
+      (void)hipGetLastError();
+    }
+  }
   // Tables of the gather backprojector's candidate search.
   {
     std::vector<float2> angcs;
@@ -279,6 +295,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_zab) (void)hipFree(p->d_zab);
   if (p->d_replicas) (void)hipFree(p->d_replicas);
   if (p->d_angcs) (void)hipFree(p->d_angcs);
+  if (p->d_zorder) (void)hipFree(p->d_zorder);
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   delete p;

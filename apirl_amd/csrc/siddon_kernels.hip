@@ -22,6 +22,8 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kNumXcd = 8;  // MI355X: 8 XCDs of 32 CUs
+constexpr int kUnroll = 4;  // Siddon steps per batch of voxel loads
+// Marks a batch slot without a term: no Siddon weight is NaN-boxed like this value.
 
 // Per-ray traversal state (Siddon.cpp:253-340).
 struct Ray {
@@ -33,40 +35,43 @@ struct Ray {
 // the sinogram's axial coordinates (Siddon.cpp:150-182, 220, 234-255, 279-286).
 __device__ __forceinline__ bool setup_ray(const Geo& g, const RayXY& t, float za, float zb,
                                           Ray& ray) {
-  if (!(t.flags & RAY_VALID)) return false;
+  // Every field is assigned unconditionally at the end from locals, so the compiler
+  // keeps the state in registers (no address-selected stores into a private struct).
+  float az = FLT_MAX, azu = 0.f, L;
+  int k = 0, kinc = 0, n = t.nxy;
+  if (g.dims == 2) {
+    L = sqrtf(t.lxy);
+  } else {
+    const float vz = zb - za;
+    const float z1 = za + vz * t.amin;
+    const float z2 = za + vz * t.amax;
+    const int k0 = (int)floorf((z1 - g.offz) / g.dz);
+    const int k1 = (int)floorf((z2 - g.offz) / g.dz);
+    k = k0;
+    kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
+    n = t.nxy + abs(k1 - k0);
+    const float ez = (za + vz) - za;
+    L = sqrtf(t.lxy + ez * ez);
+    azu = fabsf(g.dz / vz);
+    if (vz > 0) az = (g.offz + (k0 + kinc) * g.dz - za) / vz;
+    else if (vz < 0) az = (g.offz + k0 * g.dz - za) / vz;
+  }
   ray.ax = t.ax;
   ray.ay = t.ay;
+  ray.az = az;
   ray.axu = t.axu;
   ray.ayu = t.ayu;
+  ray.azu = azu;
   ray.ac = t.amin;
+  ray.L = L;
   ray.i = t.i0;
   ray.j = t.j0;
+  ray.k = k;
   ray.iinc = ray_iinc(t.flags);
   ray.jinc = ray_jinc(t.flags);
-  if (g.dims == 2) {
-    ray.az = FLT_MAX;
-    ray.azu = 0.f;
-    ray.k = 0;
-    ray.kinc = 0;
-    ray.n = t.nxy;
-    ray.L = sqrtf(t.lxy);
-    return true;
-  }
-  const float vz = zb - za;
-  const float z1 = za + vz * t.amin;
-  const float z2 = za + vz * t.amax;
-  const int k0 = (int)floorf((z1 - g.offz) / g.dz);
-  const int k1 = (int)floorf((z2 - g.offz) / g.dz);
-  ray.k = k0;
-  ray.kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
-  ray.n = t.nxy + abs(k1 - k0);
-  const float ez = (za + vz) - za;
-  ray.L = sqrtf(t.lxy + ez * ez);
-  ray.azu = fabsf(g.dz / vz);
-  if (vz > 0) ray.az = (g.offz + (k0 + ray.kinc) * g.dz - za) / vz;
-  else if (vz < 0) ray.az = (g.offz + k0 * g.dz - za) / vz;
-  else ray.az = FLT_MAX;
-  return true;
+  ray.kinc = kinc;
+  ray.n = n;
+  return (t.flags & RAY_VALID) != 0;
 }
 
 // One Siddon step: returns the weight of the current voxel (ci, cj, ck) and advances
@@ -108,12 +113,23 @@ template <int PREC, bool FUSED>
 __global__ __launch_bounds__(kBlock) void project_kernel(
     Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
     const float* __restrict__ img, float* __restrict__ sino, int nang_sub, int subset,
-    int nsubsets, long nrays, FusedRatio fr) {
-  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
+    int nsubsets, long nrays, FusedRatio fr, const int* __restrict__ zorder) {
+  // Rays are taken in order of their sinogram's mean axial coordinate (zorder) and
+  // each XCD gets one contiguous share of that order: workgroups are dispatched to the
+  // 8 XCDs round-robin, so block b runs on XCD b % 8, and the remap below gives XCD x
+  // the x-th eighth of the z-sorted rays.  Its 4 MB L2 then serves one axial band of
+  // the image instead of all of it.
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned xcd = b % kNumXcd, per = nb / kNumXcd, rem = nb % kNumXcd;
+  const unsigned lb = xcd * per + (xcd < rem ? xcd : rem) + b / kNumXcd;
+  const long lidx = (long)lb * kBlock + threadIdx.x;
   double lik = 0.0;
-  if (idx < nrays) {
+  long idx = 0;
+  if (lidx < nrays) {
     int s, a, rr, k;
-    ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
+    ray_coords(g, lidx, nang_sub, subset, nsubsets, s, a, rr, k);
+    if (zorder) s = zorder[s];
+    idx = ((long)s * nang_sub + k) * g.nr + rr;
     const RayXY t = rays[(long)a * g.nr + rr];
     float za = 0.f, zb = 0.f;
     if (g.dims == 3) {
@@ -126,19 +142,42 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
     float acc = 0.f;
     if (valid) {
       if (PREC == APIRL_PREC_REFERENCE) {
-        for (int m = 0; m < ray.n; m++) {
-          int ci, cj, ck;
-          const float w = step(ray, ci, cj, ck);
-          bool ok;
-          long nxy = (long)g.nx * g.ny;
-          if (g.dims == 3) {
-            ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
-            long li = (long)ck * nxy + (long)cj * g.nx + ci;
-            if (ok && li >= 0) acc += w * img[li];
-          } else {
-            ok = (cj >= 0) && (ci >= 0);
-            long li = (long)cj * g.nx + ci;
-            if (ok && li < nxy) acc += 1.0f * w * img[li];
+        // Steps are taken kUnroll at a time: the voxel loads of a batch are issued
+        // together (their addresses do not depend on loaded data), then added in
+        // traversal order, so the sum is still the reference's sequential one.
+        const long nxy = (long)g.nx * g.ny;
+        for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
+          float wv[kUnroll], pv[kUnroll];
+          bool tv[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            pv[u] = 0.f;
+            wv[u] = 0.f;
+            tv[u] = false;
+            if (m0 + u < ray.n) {
+              int ci, cj, ck;
+              wv[u] = step(ray, ci, cj, ck);
+              bool ok;
+              long li;
+              if (g.dims == 3) {
+                li = (long)ck * nxy + (long)cj * g.nx + ci;
+                ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz) && li >= 0;
+              } else {
+                li = (long)cj * g.nx + ci;
+                ok = (cj >= 0) && (ci >= 0) && li < nxy;
+              }
+              if (ok) {
+                pv[u] = img[li];
+                tv[u] = true;
+              }
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            if (tv[u]) {
+              if (g.dims == 3) acc += wv[u] * pv[u];
+              else acc += 1.0f * wv[u] * pv[u];
+            }
           }
         }
       } else if (PREC == APIRL_PREC_KAHAN) {
@@ -472,10 +511,12 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
 #define APIRL_LAUNCH_PROJ(PR)                                                              \
   if (f)                                                                                   \
     hipLaunchKernelGGL((project_kernel<PR, true>), dim3(grid), dim3(kBlock), 0, st, g,     \
-                       p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr); \
+                       p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr, \
+                       p->d_zorder);                                                       \
   else                                                                                     \
     hipLaunchKernelGGL((project_kernel<PR, false>), dim3(grid), dim3(kBlock), 0, st, g,    \
-                       p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr);
+                       p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr, \
+                       p->d_zorder);
   switch (precision) {
     case APIRL_PREC_KAHAN: APIRL_LAUNCH_PROJ(APIRL_PREC_KAHAN); break;
     case APIRL_PREC_F64_ALPHA: APIRL_LAUNCH_PROJ(APIRL_PREC_F64_ALPHA); break;

@@ -58,3 +58,13 @@ clean:
 	rm -rf $(OUT)
 
 .PHONY: all clean
+
+# The reference's own EM drivers with the MI355X projector plugged in through its
+# reference-side binding (integration/HipSiddonProjector.cpp): the drop-in check.
+HIPLIB := apirl_amd/libapirl_hip.so
+hip: $(OUT)/ref_hip_driver
+
+$(OUT)/ref_hip_driver: oracle/ref_driver.cpp integration/HipSiddonProjector.cpp integration/HipSiddonProjector.h $(OUT)/libapirl_ref.so $(HIPLIB)
+	$(CXX) $(OPT) -fopenmp -DWITH_HIP $(INC) -Iinclude -Iintegration -w -o $@ oracle/ref_driver.cpp integration/HipSiddonProjector.cpp -L$(OUT) -lapirl_ref -Lapirl_amd -lapirl_hip -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../apirl_amd'
+
+.PHONY: hip

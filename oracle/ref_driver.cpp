@@ -36,6 +36,10 @@
 #include <OsemSinogram3d.h>
 #include <omp.h>
 
+#ifdef WITH_HIP
+#include "HipSiddonProjector.h"
+#endif
+
 #include <malloc.h>
 
 #include <chrono>
@@ -60,6 +64,7 @@ const char* arg(const char* k) {
   return it->second.c_str();
 }
 int iarg(const char* k) { return atoi(arg(k)); }
+bool has_arg(const char* k) { return g_args.count(k) != 0; }
 float farg(const char* k) { return strtof(arg(k), nullptr); }
 
 std::vector<float> read_f32(const char* path, size_t n) {
@@ -236,6 +241,24 @@ void report_time(const char* what, double t0) {
   fflush(stdout);
 }
 
+// projector=siddon (default): the reference's SiddonProjector.
+// projector=hip: the MI355X path through its reference-side binding
+// (integration/HipSiddonProjector.cpp); bp=gather|scatter picks the backprojector.
+Projector* make_projector() {
+  const char* which = has_arg("projector") ? arg("projector") : "siddon";
+  if (strcmp(which, "hip") == 0) {
+#ifdef WITH_HIP
+    int mode = (has_arg("bp") && strcmp(arg("bp"), "scatter") == 0) ? APIRL_BP_SCATTER
+                                                                    : APIRL_BP_GATHER;
+    return (Projector*)new HipSiddonProjector(0, mode);
+#else
+    fprintf(stderr, "ref_driver: built without the HIP projector\n");
+    exit(7);
+#endif
+  }
+  return (Projector*)new SiddonProjector();
+}
+
 void write_lists(const char* path, std::vector<int>& counts, std::vector<SiddonSegment>& segs) {
   FILE* f = fopen(path, "wb");
   int nb = (int)counts.size();
@@ -319,12 +342,12 @@ int cmd_proj2d() {
   std::vector<float> x = read_f32(arg("img"), npix);
   memcpy(img.getPixelsPtr(), x.data(), npix * 4);
   Sinogram2DinCylindrical3Dpet sino(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
-  SiddonProjector p;
+  Projector* pp = make_projector();
   double t0;
   {
   GuardedPixels guarded(&img);
   t0 = now_s();
-  p.Project(&img, &sino);
+  pp->Project(&img, &sino);
   report_time("proj2d", t0);
   }
   write_f32(arg("out"), sino.getSinogramPtr(), (size_t)sino.getNumProj() * sino.getNumR());
@@ -337,12 +360,12 @@ int cmd_back2d() {
   size_t nb = (size_t)sino.getNumProj() * sino.getNumR();
   std::vector<float> y = read_f32(arg("sino"), nb);
   memcpy(sino.getSinogramPtr(), y.data(), nb * 4);
-  SiddonProjector p;
+  Projector* pp = make_projector();
   double t0;
   {
   GuardedPixels guarded(&img);
   t0 = now_s();
-  p.Backproject(&sino, &img);
+  pp->Backproject(&sino, &img);
   report_time("back2d", t0);
   }
   write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
@@ -360,10 +383,10 @@ int cmd_mlem2d() {
   size_t nb = (size_t)sino.getNumProj() * sino.getNumR();
   std::vector<float> y = read_f32(arg("meas"), nb);
   memcpy(sino.getSinogramPtr(), y.data(), nb * 4);
-  SiddonProjector p;
+  Projector* pp = make_projector();
   std::string dir = arg("outdir");
   int iters = iarg("iters");
-  Mlem2dProbe mlem(&sino, &init, dir, dir + "/mlem2d", iters, 0, false, false, (Projector*)&p, (Projector*)&p);
+  Mlem2dProbe mlem(&sino, &init, dir, dir + "/mlem2d", iters, 0, false, false, pp, pp);
   double t0 = now_s();
   if (!mlem.Reconstruct()) return 6;
   report_time("mlem2d", t0);
@@ -379,12 +402,12 @@ int cmd_proj3d() {
   std::vector<float> x = read_f32(arg("img"), npix);
   memcpy(img.getPixelsPtr(), x.data(), npix * 4);
   Sinogram3DCylindricalPet* s = make_sino3d();
-  SiddonProjector p;
+  Projector* pp = make_projector();
   double t0;
   {
   GuardedPixels guarded(&img);
   t0 = now_s();
-  p.Project(&img, s);
+  pp->Project(&img, s);
   report_time("proj3d", t0);
   }
   std::vector<float> out = sino3d_to(s);
@@ -397,12 +420,12 @@ int cmd_back3d() {
   Sinogram3DCylindricalPet* s = make_sino3d();
   std::vector<float> y = read_f32(arg("sino"), bins3d(s));
   sino3d_from(s, y.data());
-  SiddonProjector p;
+  Projector* pp = make_projector();
   double t0;
   {
   GuardedPixels guarded(&img);
   t0 = now_s();
-  p.Backproject(s, &img);
+  pp->Backproject(s, &img);
   report_time("back3d", t0);
   }
   write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
@@ -417,10 +440,10 @@ int cmd_osem3d() {
   Sinogram3DCylindricalPet* s = make_sino3d();
   std::vector<float> y = read_f32(arg("meas"), bins3d(s));
   sino3d_from(s, y.data());
-  SiddonProjector p;
+  Projector* pp = make_projector();
   std::string dir = arg("outdir");
   int iters = iarg("iters"), subsets = iarg("subsets");
-  OsemProbe osem(s, &init, dir, dir + "/osem3d", iters, 0, false, false, (Projector*)&p, (Projector*)&p, subsets);
+  OsemProbe osem(s, &init, dir, dir + "/osem3d", iters, 0, false, false, pp, pp, subsets);
   double t0 = now_s();
   if (!osem.Reconstruct()) return 6;
   report_time("osem3d", t0);
@@ -440,10 +463,10 @@ int cmd_mlem3d() {
   Sinogram3DCylindricalPet* s = make_sino3d();
   std::vector<float> y = read_f32(arg("meas"), bins3d(s));
   sino3d_from(s, y.data());
-  SiddonProjector p;
+  Projector* pp = make_projector();
   std::string dir = arg("outdir");
   int iters = iarg("iters");
-  Mlem3dProbe mlem(s, &init, dir, dir + "/mlem3d", iters, 0, false, false, (Projector*)&p, (Projector*)&p);
+  Mlem3dProbe mlem(s, &init, dir, dir + "/mlem3d", iters, 0, false, false, pp, pp);
   double t0 = now_s();
   if (!mlem.Reconstruct()) return 6;
   report_time("mlem3d", t0);
@@ -525,14 +548,14 @@ int cmd_osem3d_sub() {
   std::vector<float> y = read_f32(arg("meas"), bins3d(s));
   sino3d_from(s, y.data());
   int subsets = iarg("subsets"), sub = iarg("subset");
-  SiddonProjector p;
+  Projector* pp = make_projector();
   Image bp(size3d());
   Image sens(size3d());
   {
     Sinogram3D* ones = s->getSubset(sub, subsets);
     ones->FillConstant(1);
     GuardedPixels gs(&sens);
-    p.Backproject(ones, &sens);
+    pp->Backproject(ones, &sens);
     delete ones;
   }
   double t0;
@@ -544,11 +567,11 @@ int cmd_osem3d_sub() {
     Sinogram3D* in = s->getSubset(sub, subsets);
     Sinogram3D* est = s->getSubset(sub, subsets);
     est->FillConstant(0);
-    p.Project(&img, est);
+    pp->Project(&img, est);
     lik = est->getLikelihoodValue(in);
     bp.fillConstant(0);
     est->inverseDivideBinToBin(in);
-    p.Backproject(est, &bp);
+    pp->Backproject(est, &bp);
     float* x = img.getPixelsPtr();
     float* b = bp.getPixelsPtr();
     float* sv = sens.getPixelsPtr();

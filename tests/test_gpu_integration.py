@@ -1,0 +1,94 @@
+"""Drop-in check: the reference's OWN EM drivers (Mlem2d, OsemSinogram3d, compiled
+from its sources into oracle/_ref/) running with the MI355X projector plugged in
+through its reference-side binding (integration/HipSiddonProjector.cpp), compared
+with the same drivers running the reference's SiddonProjector (the goldens)."""
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRV = os.path.join(ROOT, "oracle", "_ref", "ref_hip_driver")
+
+needs_driver = pytest.mark.skipif(not os.path.exists(DRV),
+                                  reason="oracle/_ref/ref_hip_driver not built "
+                                         "(needs the reference tree at build time)")
+
+
+def run(cmd, **kw):
+    args = [DRV, cmd] + [f"{k}={v}" for k, v in kw.items()]
+    print("case:", " ".join(args[1:3]), flush=True)
+    out = subprocess.run(args, env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True,
+                         text=True, timeout=300)
+    assert out.returncode == 0, out.stderr[-2000:]
+    assert "HipSiddonProjector" not in out.stderr, out.stderr[-2000:]
+    return out.stdout
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).ravel().view(np.int32)
+
+
+GEOM3 = ["nx", "ny", "nz", "dx", "dz", "nproj", "nr", "rings", "rfov", "zfov", "rscanner", "span",
+         "mrd"]
+
+
+@needs_driver
+def test_reference_project3d_with_hip_projector_bitexact(golden):
+    f = golden("mid3d_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    with tempfile.TemporaryDirectory() as td:
+        f["img"].astype(np.float32).tofile(os.path.join(td, "img.f32"))
+        run("proj3d", projector="hip", **{k: g[k] for k in GEOM3},
+            img=os.path.join(td, "img.f32"), out=os.path.join(td, "p.f32"))
+        got = np.fromfile(os.path.join(td, "p.f32"), np.float32)
+    assert np.array_equal(bits(got), bits(f["proj"]))
+
+
+@needs_driver
+def test_reference_mlem2d_with_hip_gather_bitexact(golden):
+    """Mlem2d (recon/src/Mlem2d.cpp) for 60 iterations with HipSiddonProjector as
+    forward and backprojector: the final image equals the reference's bit for bit."""
+    f = golden("c1_mlem2d.npz")
+    g = json.loads(str(f["geom"]))
+    with tempfile.TemporaryDirectory() as td:
+        f["meas"].astype(np.float32).tofile(os.path.join(td, "meas.f32"))
+        np.ones((128, 128), np.float32).tofile(os.path.join(td, "init.f32"))
+        run("mlem2d", projector="hip", bp="gather", nx=g["nx"], dx=g["dx"], nproj=g["nproj"],
+            nr=g["nr"], rscanner=g["rscanner"], iters=g["iters"],
+            meas=os.path.join(td, "meas.f32"), init=os.path.join(td, "init.f32"), outdir=td)
+        final = np.fromfile(os.path.join(td, "final.f32"), np.float32)
+        sens = np.fromfile(os.path.join(td, "sens.f32"), np.float32)
+        lik = np.fromfile(os.path.join(td, "lik.f32"), np.float32)
+    assert np.array_equal(bits(sens), bits(f["sens"]))
+    assert np.array_equal(bits(final), bits(f["final"]))
+    assert np.array_equal(bits(lik), bits(f["lik"]))
+
+
+@needs_driver
+def test_reference_osem3d_with_hip_gather(golden):
+    """OsemSinogram3d (recon/src/OsemSinogram3d.cpp), 7 subsets x 2 iterations, with
+    HipSiddonProjector: projections are bit-exact, backprojection sums run in a fixed
+    but different order, so the image matches within 1e-4 relative L2 (NaN where the
+    reference has NaN)."""
+    f = golden("mid3d_osem.npz")
+    g = json.loads(str(f["geom"]))
+    with tempfile.TemporaryDirectory() as td:
+        f["meas"].astype(np.float32).tofile(os.path.join(td, "meas.f32"))
+        np.ones((g["nz"], g["ny"], g["nx"]), np.float32).tofile(os.path.join(td, "init.f32"))
+        run("osem3d", projector="hip", bp="gather", **{k: g[k] for k in GEOM3},
+            subsets=g["subsets"], iters=g["iters"], meas=os.path.join(td, "meas.f32"),
+            init=os.path.join(td, "init.f32"), outdir=td)
+        final = np.fromfile(os.path.join(td, "final.f32"), np.float32)
+    ref = f["final"].ravel()
+    assert np.array_equal(np.isnan(final), np.isnan(ref))
+    m = np.isfinite(ref)
+    e = np.linalg.norm(final[m].astype(np.float64) - ref[m]) / np.linalg.norm(ref[m].astype(np.float64))
+    print(f"osem3d via reference driver + HIP projector: rel L2 {e:.2e}", flush=True)
+    assert e <= 1e-4
