@@ -36,6 +36,7 @@ static_assert(sizeof(RayXY) == 64, "RayXY must be one 64-byte line");
 enum : int {
   RAY_VALID = 1,     // 2D: r <= rfov (Sinogram2DinCylindrical3Dpet.cpp:256-260)
   RAY_REJECTED = 2,  // 2D: both ends outside on one side (Siddon.cpp:515-519)
+  RAY_XMAJOR = 64,   // |Vx| > |Vy|: neighbouring radial bins are offset along y
 };
 __host__ __device__ inline int ray_iinc(int flags) { return ((flags >> 2) & 3) - 1; }
 __host__ __device__ inline int ray_jinc(int flags) { return ((flags >> 4) & 3) - 1; }
@@ -86,6 +87,7 @@ struct apirl_plan {
   apirl::RayXY* d_rays = nullptr;  // [nang][nr]
   float2* d_zab = nullptr;         // [nsino] (P1.Z, P2.Z), 3D only
   float* d_replicas = nullptr;     // [8][nvox] per-XCD scatter targets (zero between calls)
+  float* d_imgT = nullptr;         // [nz][nx][ny] transposed copy of the projected image
   int* d_zorder = nullptr;         // [nsino] sinograms sorted by mean axial coordinate
   float2* d_angcs = nullptr;       // [nang] (cos, sin) for the gather candidate search
   apirl::GatherSeg* d_segs = nullptr;

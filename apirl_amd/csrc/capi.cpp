@@ -246,6 +246,7 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
         hipMemcpy(p->d_zorder, order.data(), order.size() * sizeof(int),
                   hipMemcpyHostToDevice) != hipSuccess) {
       if (p->d_zorder) (void)hipFree(p->d_zorder);
+  if (p->d_imgT) (void)hipFree(p->d_imgT);
       p->d_zorder = nullptr;
       (void)hipGetLastError();
     }
@@ -273,6 +274,11 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       p->gather_ok = false;
     (void)hipGetLastError();
   }
+  // Transposed image for the forward projector's x-major rays.
+  if (hipMalloc(&p->d_imgT, (size_t)im.nx * im.ny * im.nz * sizeof(float)) != hipSuccess) {
+    p->d_imgT = nullptr;
+    (void)hipGetLastError();
+  }
   // Per-XCD scatter targets for the backprojector: 8 image copies, zeroed here and
   // re-zeroed by the reduction after every use.
   {
@@ -296,6 +302,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_replicas) (void)hipFree(p->d_replicas);
   if (p->d_angcs) (void)hipFree(p->d_angcs);
   if (p->d_zorder) (void)hipFree(p->d_zorder);
+  if (p->d_imgT) (void)hipFree(p->d_imgT);
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   delete p;

@@ -108,7 +108,7 @@ void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
       t.lxy = ex * ex + ey * ey;
       t.i0 = imin; t.j0 = jmin;
       t.nxy = rejected ? 0 : std::abs(imax - imin) + std::abs(jmax - jmin);
-      t.flags = ray_flags(iinc, jinc, 1, rejected);
+      t.flags = ray_flags(iinc, jinc, 1, rejected) | (fabsf(vx) > fabsf(vy) ? RAY_XMAJOR : 0);
     }
   }
 }

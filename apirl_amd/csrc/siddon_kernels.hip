@@ -33,13 +33,14 @@ struct Ray {
 
 // Builds the traversal state of bin (s, angle a, r) from the transaxial table and
 // the sinogram's axial coordinates (Siddon.cpp:150-182, 220, 234-255, 279-286).
+template <int DIMS>
 __device__ __forceinline__ bool setup_ray(const Geo& g, const RayXY& t, float za, float zb,
                                           Ray& ray) {
   // Every field is assigned unconditionally at the end from locals, so the compiler
   // keeps the state in registers (no address-selected stores into a private struct).
   float az = FLT_MAX, azu = 0.f, L;
   int k = 0, kinc = 0, n = t.nxy;
-  if (g.dims == 2) {
+  if (DIMS == 2) {
     L = sqrtf(t.lxy);
   } else {
     const float vz = zb - za;
@@ -99,6 +100,36 @@ __device__ __forceinline__ float step(Ray& r, int& ci, int& cj, int& ck) {
   return w;
 }
 
+// Branch-free form of the same step for the hot loops: the three candidate updates
+// are computed and selected (a taken alpha advances by exactly the reference's
+// addition; an untaken one keeps its value), and the voxel's linear index li is
+// carried along (+-1 in x, +-nx in y, +-nx*ny in z) instead of being recomputed.
+struct RayIdx {
+  int ci, cj, ck, li;  // voxel emitted by the step and its linear index
+};
+
+__device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, int kstep) {
+  const bool bx = (r.ax <= r.ay) && (r.ax <= r.az);
+  const bool by = !bx && (r.ay <= r.az);
+  const bool bz = !bx && !by;
+  const float an = bx ? r.ax : (by ? r.ay : r.az);
+  const float w = (an - r.ac) * r.L;
+  r.ac = an;
+  const float nax = r.ax + r.axu, nay = r.ay + r.ayu, naz = r.az + r.azu;
+  r.ax = bx ? nax : r.ax;
+  r.ay = by ? nay : r.ay;
+  r.az = bz ? naz : r.az;
+  v.ci = r.i;
+  v.cj = r.j;
+  v.ck = r.k;
+  v.li = li;
+  r.i += bx ? r.iinc : 0;
+  r.j += by ? r.jinc : 0;
+  r.k += bz ? r.kinc : 0;
+  li += bx ? r.iinc : (by ? jstep : kstep);
+  return w;
+}
+
 // Decomposes a ray index of a subset sinogram into (sinogram, angle, r).
 __device__ __forceinline__ void ray_coords(const Geo& g, long idx, int nang_sub, int subset,
                                            int nsubsets, int& s, int& a, int& r, int& k) {
@@ -109,11 +140,12 @@ __device__ __forceinline__ void ray_coords(const Geo& g, long idx, int nang_sub,
   a = subset + nsubsets * k;
 }
 
-template <int PREC, bool FUSED>
+template <int PREC, bool FUSED, int DIMS>
 __global__ __launch_bounds__(kBlock) void project_kernel(
     Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
     const float* __restrict__ img, float* __restrict__ sino, int nang_sub, int subset,
-    int nsubsets, long nrays, FusedRatio fr, const int* __restrict__ zorder) {
+    int nsubsets, long nrays, FusedRatio fr, const int* __restrict__ zorder,
+    const float* __restrict__ imgT) {
   // Rays are taken in order of their sinogram's mean axial coordinate (zorder) and
   // each XCD gets one contiguous share of that order: workgroups are dispatched to the
   // 8 XCDs round-robin, so block b runs on XCD b % 8, and the remap below gives XCD x
@@ -132,52 +164,55 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
     idx = ((long)s * nang_sub + k) * g.nr + rr;
     const RayXY t = rays[(long)a * g.nr + rr];
     float za = 0.f, zb = 0.f;
-    if (g.dims == 3) {
+    if (DIMS == 3) {
       const float2 z = zab[s];
       za = z.x;
       zb = z.y;
     }
     Ray ray;
-    bool valid = setup_ray(g, t, za, zb, ray);
+    bool valid = setup_ray<DIMS>(g, t, za, zb, ray);
     float acc = 0.f;
     if (valid) {
       if (PREC == APIRL_PREC_REFERENCE) {
         // Steps are taken kUnroll at a time: the voxel loads of a batch are issued
-        // together (their addresses do not depend on loaded data), then added in
-        // traversal order, so the sum is still the reference's sequential one.
-        const long nxy = (long)g.nx * g.ny;
+        // together (addresses never depend on loaded data; a step outside the image
+        // loads voxel 0 and is not added), then added in traversal order, so the sum
+        // is still the reference's sequential one.
+        // Voxel reads come from a layout where the 64 parallel rays of a wavefront
+        // (neighbouring radial bins) touch neighbouring addresses: the image itself
+        // (x fastest) when they are offset along x, its transpose (y fastest) when
+        // the rays run mostly along x and are offset along y.  Steps whose voxel lies
+        // outside the image keep the reference's linear index into the image.
+        const int nxy = g.nx * g.ny, npix = nxy * g.nz;
+        const bool tr = imgT != nullptr && (t.flags & RAY_XMAJOR);
+        const float* src = tr ? imgT : img;
+        const int jstep = ray.jinc * g.nx, kstep = ray.kinc * nxy;
+        const int tistep = tr ? ray.iinc * g.ny : ray.iinc, tjstep = tr ? ray.jinc : jstep;
+        int li = ray.k * nxy + ray.j * g.nx + ray.i;
+        int lt = ray.k * nxy + (tr ? ray.i * g.ny + ray.j : ray.j * g.nx + ray.i);
         for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
           float wv[kUnroll], pv[kUnroll];
           bool tv[kUnroll];
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
-            pv[u] = 0.f;
-            wv[u] = 0.f;
-            tv[u] = false;
-            if (m0 + u < ray.n) {
-              int ci, cj, ck;
-              wv[u] = step(ray, ci, cj, ck);
-              bool ok;
-              long li;
-              if (g.dims == 3) {
-                li = (long)ck * nxy + (long)cj * g.nx + ci;
-                ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz) && li >= 0;
-              } else {
-                li = (long)cj * g.nx + ci;
-                ok = (cj >= 0) && (ci >= 0) && li < nxy;
-              }
-              if (ok) {
-                pv[u] = img[li];
-                tv[u] = true;
-              }
-            }
+            RayIdx c;
+            const int ltcur = lt;
+            wv[u] = step_bf(ray, c, li, jstep, kstep);
+            lt += (c.ci != ray.i) ? tistep : ((c.cj != ray.j) ? tjstep : kstep);
+            const int lcur = c.li;
+            bool ok;
+            if (DIMS == 3) ok = (c.ci < g.nx) && (c.cj < g.ny) && (c.ck < g.nz) && lcur >= 0;
+            else ok = (c.cj >= 0) && (c.ci >= 0) && lcur < nxy;
+            ok = ok && (m0 + u < ray.n) && lcur < npix;
+            const bool inside = c.ci >= 0 && c.cj >= 0 && c.ci < g.nx && c.cj < g.ny;
+            tv[u] = ok;
+            pv[u] = (inside ? src : img)[!ok ? 0 : (inside ? ltcur : lcur)];
           }
 #pragma unroll
           for (int u = 0; u < kUnroll; u++) {
-            if (tv[u]) {
-              if (g.dims == 3) acc += wv[u] * pv[u];
-              else acc += 1.0f * wv[u] * pv[u];
-            }
+            const float term = (DIMS == 3) ? wv[u] * pv[u] : 1.0f * wv[u] * pv[u];
+            const float nacc = acc + term;
+            acc = tv[u] ? nacc : acc;
           }
         }
       } else if (PREC == APIRL_PREC_KAHAN) {
@@ -188,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
           long nxy = (long)g.nx * g.ny;
           bool ok;
           long li;
-          if (g.dims == 3) {
+          if (DIMS == 3) {
             ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
             li = (long)ck * nxy + (long)cj * g.nx + ci;
             ok = ok && li >= 0;
@@ -210,7 +245,7 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
         double ax = ray.ax, ay = ray.ay, az = ray.az, ac = ray.ac;
         double axu = ray.axu, ayu = ray.ayu, azu = ray.azu;
         const double L = ray.L;
-        if (g.dims == 3) {
+        if (DIMS == 3) {
           // recompute the first crossings and increments in double from the LOR
           const double vx = t.vx, vy = t.vy, vz = (double)zb - za;
           axu = fabs((double)g.dx / vx);
@@ -239,7 +274,7 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
           else { kk += ray.kinc; az += azu; }
           bool ok;
           long li;
-          if (g.dims == 3) {
+          if (DIMS == 3) {
             ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
             li = (long)ck * nxy + (long)cj * g.nx + ci;
             ok = ok && li >= 0;
@@ -295,31 +330,15 @@ __device__ __forceinline__ int xcc_id() {
   return __builtin_amdgcn_s_getreg((3 << 11) | (0 << 6) | 20) & (kNumXcd - 1);
 }
 
-// Lane mapping of the scatter backprojector.  Rays of neighbouring radial bins run
-// through the same voxels at the same step (their spacing is below the voxel size),
-// so with r-fast lanes one atomic instruction often carries several lanes aimed at
-// one address, which the L2 serialises.  For 3D the lanes of a wavefront therefore
-// take 64 different sinograms of one (angle, r): their rays lie in different axial
-// planes, the RayXY record is one broadcast load, and the atomics spread out.
-constexpr int kSinoLanes = 64;
-constexpr int kRPerBlock = kBlock / kSinoLanes;
-
-template <int KIND, bool REPLICA, bool SINO_LANES>
+template <int KIND, bool REPLICA>
 __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
     Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
     const float* __restrict__ sino, const float* __restrict__ est, float* __restrict__ img,
     int nang_sub, int subset, int nsubsets, long nrays) {
-  long idx;
-  if (SINO_LANES) {
-    const int s = blockIdx.x * kSinoLanes + (threadIdx.x % kSinoLanes);
-    const int nrq = (g.nr + kRPerBlock - 1) / kRPerBlock;
-    const int ka = blockIdx.y / nrq;
-    const int rr = (blockIdx.y % nrq) * kRPerBlock + (threadIdx.x / kSinoLanes);
-    if (s >= g.nsino || rr >= g.nr) return;
-    idx = ((long)s * nang_sub + ka) * g.nr + rr;
-  } else {
-    idx = (long)blockIdx.x * kBlock + threadIdx.x;
-  }
+  // Lanes take consecutive radial bins of one (sinogram, angle): the atomics of one
+  // instruction then fall in few cache lines (measured faster than spreading the
+  // lanes over sinograms, which puts every lane on its own line).
+  const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= nrays) return;
   // With REPLICA, `img` holds one image per XCD and each wave adds into its own die's
   // copy, so the atomics of a cache line stay in one L2 instead of moving between
@@ -345,7 +364,8 @@ __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
     zb = z.y;
   }
   Ray ray;
-  if (!setup_ray(g, t, za, zb, ray)) return;
+  const bool valid = g.dims == 3 ? setup_ray<3>(g, t, za, zb, ray) : setup_ray<2>(g, t, za, zb, ray);
+  if (!valid) return;
   const long nxy = (long)g.nx * g.ny;
   const long npix = nxy * g.nz;
   for (int m = 0; m < ray.n; m++) {
@@ -386,7 +406,9 @@ __global__ __launch_bounds__(kBlock) void count_kernel(Geo g, const RayXY* __res
       zb = zab[s].y;
     }
     Ray ray;
-    if (setup_ray(g, t, za, zb, ray)) n = (unsigned long long)ray.n;
+    const bool valid =
+        g.dims == 3 ? setup_ray<3>(g, t, za, zb, ray) : setup_ray<2>(g, t, za, zb, ray);
+    if (valid) n = (unsigned long long)ray.n;
   }
   __shared__ unsigned long long red[kBlock];
   red[threadIdx.x] = n;
@@ -493,6 +515,26 @@ __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restri
   img[i] += acc;
 }
 
+// out[k][i][j] = in[k][j][i] through a 32 x 32 LDS tile (coalesced both ways).
+__global__ __launch_bounds__(kBlock) void transpose_xy_kernel(const float* __restrict__ in,
+                                                              float* __restrict__ out, int nx,
+                                                              int ny) {
+  __shared__ float tile[32][33];
+  const int k = blockIdx.z;
+  const long base = (long)k * nx * ny;
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int x = x0 + tx, y = y0 + r;
+    if (x < nx && y < ny) tile[r][tx] = in[base + (long)y * nx + x];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int y = y0 + tx, x = x0 + r;
+    if (x < nx && y < ny) out[base + (long)x * ny + y] = tile[tx][r];
+  }
+}
+
 inline unsigned grid_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -508,20 +550,33 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
   if (nrays == 0) return hipSuccess;
   FusedRatio fr = fused ? *fused : FusedRatio{};
   const bool f = fused != nullptr;
-#define APIRL_LAUNCH_PROJ(PR)                                                              \
+  const float* imgT = nullptr;
+  if (p->d_imgT && precision == APIRL_PREC_REFERENCE) {
+    const dim3 tg((g.nx + 31) / 32, (g.ny + 31) / 32, g.nz);
+    hipLaunchKernelGGL(transpose_xy_kernel, tg, dim3(kBlock), 0, st, img, p->d_imgT, g.nx, g.ny);
+    imgT = p->d_imgT;
+  }
+#define APIRL_LAUNCH_PROJ3(PR, D)                                                           \
   if (f)                                                                                   \
-    hipLaunchKernelGGL((project_kernel<PR, true>), dim3(grid), dim3(kBlock), 0, st, g,     \
+    hipLaunchKernelGGL((project_kernel<PR, true, D>), dim3(grid), dim3(kBlock), 0, st, g,  \
                        p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr, \
-                       p->d_zorder);                                                       \
+                       p->d_zorder, imgT);                                                 \
   else                                                                                     \
-    hipLaunchKernelGGL((project_kernel<PR, false>), dim3(grid), dim3(kBlock), 0, st, g,    \
+    hipLaunchKernelGGL((project_kernel<PR, false, D>), dim3(grid), dim3(kBlock), 0, st, g, \
                        p->d_rays, p->d_zab, img, sino, nang_sub, subset, nsubsets, nrays, fr, \
-                       p->d_zorder);
+                       p->d_zorder, imgT);
+#define APIRL_LAUNCH_PROJ(PR)            \
+  if (g.dims == 3) {                     \
+    APIRL_LAUNCH_PROJ3(PR, 3)            \
+  } else {                               \
+    APIRL_LAUNCH_PROJ3(PR, 2)            \
+  }
   switch (precision) {
     case APIRL_PREC_KAHAN: APIRL_LAUNCH_PROJ(APIRL_PREC_KAHAN); break;
     case APIRL_PREC_F64_ALPHA: APIRL_LAUNCH_PROJ(APIRL_PREC_F64_ALPHA); break;
     default: APIRL_LAUNCH_PROJ(APIRL_PREC_REFERENCE); break;
   }
+#undef APIRL_LAUNCH_PROJ3
 #undef APIRL_LAUNCH_PROJ
   return hipGetLastError();
 }
@@ -539,27 +594,14 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
   float* rep = p->d_replicas;
   float* target = rep ? rep : img;
 #define APIRL_LAUNCH_BP(K)                                                                   \
-  if (g.dims == 3) {                                                                         \
-    const dim3 sgrid((g.nsino + kSinoLanes - 1) / kSinoLanes,                                \
-                     nang_sub * ((g.nr + kRPerBlock - 1) / kRPerBlock));                     \
-    if (rep)                                                                                 \
-      hipLaunchKernelGGL((backproject_scatter_kernel<K, true, true>), sgrid, dim3(kBlock), 0, \
-                         st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,      \
-                         nsubsets, nrays);                                                  \
-    else                                                                                     \
-      hipLaunchKernelGGL((backproject_scatter_kernel<K, false, true>), sgrid, dim3(kBlock), 0, \
-                         st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,      \
-                         nsubsets, nrays);                                                  \
-  } else {                                                                                   \
-    if (rep)                                                                                 \
-      hipLaunchKernelGGL((backproject_scatter_kernel<K, true, false>), dim3(grid),            \
-                         dim3(kBlock), 0, st, g, p->d_rays, p->d_zab, sino, est, target,       \
-                         nang_sub, subset, nsubsets, nrays);                                  \
-    else                                                                                     \
-      hipLaunchKernelGGL((backproject_scatter_kernel<K, false, false>), dim3(grid),           \
-                         dim3(kBlock), 0, st, g, p->d_rays, p->d_zab, sino, est, target,       \
-                         nang_sub, subset, nsubsets, nrays);                                  \
-  }
+  if (rep)                                                                                   \
+    hipLaunchKernelGGL((backproject_scatter_kernel<K, true>), dim3(grid), dim3(kBlock), 0, st, \
+                       g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset, nsubsets,  \
+                       nrays);                                                              \
+  else                                                                                       \
+    hipLaunchKernelGGL((backproject_scatter_kernel<K, false>), dim3(grid), dim3(kBlock), 0,   \
+                       st, g, p->d_rays, p->d_zab, sino, est, target, nang_sub, subset,       \
+                       nsubsets, nrays);
   switch (kind) {
     case BP_ONES: APIRL_LAUNCH_BP(BP_ONES); break;
     case BP_MULT: APIRL_LAUNCH_BP(BP_MULT); break;
