@@ -79,6 +79,20 @@ def cpu_baseline(meas_host, threads):
                       f"OMP_NUM_THREADS={threads}; iterations/s = 1/(21 t)"}
 
 
+def measured_traffic(entry):
+    """HBM bytes per launch of one entry point (all of its kernels), as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/pmc_traffic_c2.md
+    (lines "traffic <entry> <bytes>"); None when no measurement is committed."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == entry:
+            return float(f[2])
+    return None
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -130,6 +144,7 @@ def main():
     def iteration(record):
         for s in range(SUBSETS):
             r = ratio[: n_ray[s]].view(plan.subset_shape(s, SUBSETS))
+            bp.zero_()
             if record:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
@@ -139,7 +154,6 @@ def main():
             plan.project_ratio(x, meas, subset=s, nsubsets=SUBSETS, ratio=r, want_lik=False)
             if record:
                 e1.record()
-            bp.zero_()
             plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=bp_mode)
             if record:
                 e2.record()
@@ -178,6 +192,7 @@ def main():
     bwd_bytes = (4 * nint_it + 4 * nray_it) / SUBSETS + 4 * nvox
     dom = "bwd" if bwd_ms >= fwd_ms else "fwd"
     dom_bytes, dom_ms = (bwd_bytes, bwd_ms) if dom == "bwd" else (fwd_bytes, fwd_ms)
+    dom_entry = "forward_fused" if dom == "fwd" else f"backproject_{args.bp}"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     gint = nint_it * 2 * args.steps / wall / 1e9  # forward + backward traversals
     progress(f"{it_s:.3f} it/s; fwd {fwd_ms:.3f} ms, bwd {bwd_ms:.3f} ms, upd {upd_ms:.3f} ms")
@@ -207,9 +222,9 @@ def main():
         "ray_voxel_gint_per_s": gint,
         "intersections_per_iteration": nint_it,
         "kernel_ms": {"forward_fused": fwd_ms, "backproject": bwd_ms, "update": upd_ms},
-        "roofline": {"bound": "hbm", "kernel": "forward_fused" if dom == "fwd" else "backproject",
+        "roofline": {"bound": "hbm", "kernel": dom_entry,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_entry)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -278,7 +278,7 @@ def test_c1_backproject_gather_bitexact_vs_reference(torch, ctx, golden):
     assert_bitexact(host(out), f["back"])
 
 
-def test_mid3d_backproject_gather_bitexact_vs_reference(torch, ctx, golden):
+def test_mid3d_backproject_gather_deterministic_and_close_to_reference(torch, ctx, golden):
     log("case: mid3d gather backprojection")
     f = golden("mid3d_project_backproject.npz")
     g = json.loads(str(f["geom"]))
@@ -291,7 +291,7 @@ def test_mid3d_backproject_gather_bitexact_vs_reference(torch, ctx, golden):
     assert_voxel_close(host(out), f["back"])
 
 
-def test_mid3d_sensitivity_gather_bitexact_vs_reference(torch, ctx, golden):
+def test_mid3d_sensitivity_gather_close_to_reference(torch, ctx, golden):
     log("case: mid3d gather sensitivity")
     f = golden("mid3d_osem.npz")
     g = json.loads(str(f["geom"]))
@@ -318,7 +318,7 @@ def test_c1_mlem2d_gather_bitexact_vs_reference(torch, ctx, golden):
     assert_bitexact(host(img), f["final"])
 
 
-def test_mid3d_osem_gather_bitexact_vs_reference(torch, ctx, golden):
+def test_mid3d_osem_gather_deterministic_and_close_to_reference(torch, ctx, golden):
     log("case: mid3d OSEM gather 7x2")
     f = golden("mid3d_osem.npz")
     g = json.loads(str(f["geom"]))
@@ -335,8 +335,8 @@ def test_mid3d_osem_gather_bitexact_vs_reference(torch, ctx, golden):
 
 
 def test_c2_subset_gather_vs_oracle(torch, ctx, golden):
-    """256x256x81 / 837 sinograms, subset 0 of 21: gather backprojection of the
-    phantom's projection, bit-exact against the oracle."""
+    """256x256x81 / 837 sinograms, subset 0 of 21: gather and scatter backprojection
+    of the phantom's projection, each voxel within 1e-5 relative of the oracle."""
     log("case: c2 gather backprojection of subset 0/21 vs oracle")
     f = golden("c2_rays.npz")
     g = json.loads(str(f["geom"]))
