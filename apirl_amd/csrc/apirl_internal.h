@@ -93,6 +93,8 @@ struct apirl_plan {
   apirl::GatherSeg* d_segs = nullptr;
   int nseg = 0;
   bool gather_ok = false;          // uniform radial grid and increasing runs found
+  int gather_zlanes = 1;           // 3D gather voxel mapping (1: 16x16 slice tiles; Z>1:
+                                   // Z slices x 64/Z columns per wavefront)
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
   size_t gscratch_bytes = 0;
   std::vector<float> ang, r;       // host copies of the grids
@@ -135,8 +137,8 @@ hipError_t launch_count_intersections(const apirl_plan* p, int subset, int nsubs
                                       unsigned long long* d_out, hipStream_t st);
 
 hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, float* img,
-                                     int subset, int nsubsets, int kind, void* scratch,
-                                     size_t scratch_bytes, hipStream_t st);
+                                     int subset, int nsubsets, int kind, int precision,
+                                     void* scratch, size_t scratch_bytes, hipStream_t st);
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
                          std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,

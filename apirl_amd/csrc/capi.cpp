@@ -59,9 +59,15 @@ bool check_subset(const apirl_plan* p, int subset, int nsubsets) {
 
 // Backprojection of one subset in either form.
 apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est, float* img,
-                             int subset, int nsubsets, int kind, int mode) {
+                             int subset, int nsubsets, int kind, int mode,
+                             int precision = APIRL_PREC_REFERENCE) {
   apirl_ctx* c = p->ctx;
+  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_ALPHA)
+    return fail(c, APIRL_ERR_INVALID, "invalid precision");
   if (mode == APIRL_BP_SCATTER) {
+    if (precision != APIRL_PREC_REFERENCE)
+      return fail(c, APIRL_ERR_UNSUPPORTED,
+                  "the scatter backprojector has the reference precision only");
     APIRL_HIP(c, apirl::launch_backproject(p, sino, est, img, subset, nsubsets, kind, mode,
                                            c->stream));
     return APIRL_OK;
@@ -82,7 +88,7 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
       return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather scratch");
     p->gscratch_bytes = need;
   }
-  APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind,
+  APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream));
   return APIRL_OK;
 }
@@ -332,6 +338,22 @@ apirl_status apirl_plan_intersections(apirl_plan* p, int subset, int nsubsets,
   return APIRL_OK;
 }
 
+apirl_status apirl_plan_get_tuning(const apirl_plan* p, apirl_tuning* out) {
+  if (!p || !out) return APIRL_ERR_INVALID;
+  out->gather_zlanes = p->gather_zlanes;
+  return APIRL_OK;
+}
+
+apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
+  if (!p || !t) return APIRL_ERR_INVALID;
+  if (t->gather_zlanes) {
+    if (t->gather_zlanes < 1 || t->gather_zlanes > 64)
+      return fail(p->ctx, APIRL_ERR_INVALID, "gather_zlanes must be in 1..64");
+    p->gather_zlanes = t->gather_zlanes;
+  }
+  return APIRL_OK;
+}
+
 apirl_status apirl_project(apirl_plan* p, const float* d_img, float* d_sino, int subset,
                            int nsubsets, int precision) {
   if (!p || !d_img || !d_sino) return APIRL_ERR_INVALID;
@@ -345,14 +367,15 @@ apirl_status apirl_project(apirl_plan* p, const float* d_img, float* d_sino, int
 }
 
 apirl_status apirl_backproject(apirl_plan* p, const float* d_sino, float* d_img, int subset,
-                               int nsubsets, int mode) {
+                               int nsubsets, int mode, int precision) {
   if (!p || !d_img || !d_sino) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
   if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
   DeviceGuard g(p->ctx->device);
-  return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN, mode);
+  return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN, mode,
+                         precision);
 }
 
 apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const float* d_est,
@@ -401,8 +424,9 @@ apirl_status apirl_em_update(apirl_ctx* c, float* d_x, const float* d_b, const f
   return APIRL_OK;
 }
 
-apirl_status apirl_sensitivity(apirl_plan* p, const float* d_mult, float* d_sens, int subset,
-                               int nsubsets, int mode, int force_positive) {
+namespace {
+apirl_status sensitivity_impl(apirl_plan* p, const float* d_mult, float* d_sens, int subset,
+                              int nsubsets, int mode, int force_positive, int precision) {
   if (!p || !d_sens) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
@@ -411,10 +435,17 @@ apirl_status apirl_sensitivity(apirl_plan* p, const float* d_mult, float* d_sens
   const size_t nvox = apirl_plan_image_voxels(p);
   APIRL_HIP(c, hipMemsetAsync(d_sens, 0, nvox * sizeof(float), c->stream));
   apirl_status st = backproject_any(p, d_mult, nullptr, d_sens, subset, nsubsets,
-                                    d_mult ? apirl::BP_MULT : apirl::BP_ONES, mode);
+                                    d_mult ? apirl::BP_MULT : apirl::BP_ONES, mode, precision);
   if (st != APIRL_OK) return st;
   if (force_positive) APIRL_HIP(c, apirl::launch_force_positive(d_sens, nvox, c->stream));
   return APIRL_OK;
+}
+}  // namespace
+
+apirl_status apirl_sensitivity(apirl_plan* p, const float* d_mult, float* d_sens, int subset,
+                               int nsubsets, int mode, int force_positive) {
+  return sensitivity_impl(p, d_mult, d_sens, subset, nsubsets, mode, force_positive,
+                          APIRL_PREC_REFERENCE);
 }
 
 apirl_status apirl_mlem_threshold(apirl_ctx* c, const float* d_sens, size_t n, float* h_thr) {
@@ -475,10 +506,13 @@ apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
       return st;                \
     }                           \
   } while (0)
+  // The precision mode applies to the forward projector and, for the gather form,
+  // to the backprojector; the scatter form always uses the reference's arithmetic.
+  const int bp_prec = opt->bp_mode == APIRL_BP_GATHER ? opt->precision : APIRL_PREC_REFERENCE;
   std::vector<float> thr(S, 0.f);
   for (int s = 0; s < S; s++) {
-    APIRL_TRY(apirl_sensitivity(p, d_mult, d_sens + (size_t)s * nvox, s, S, opt->bp_mode,
-                                opt->osem_rules ? 1 : 0));
+    APIRL_TRY(sensitivity_impl(p, d_mult, d_sens + (size_t)s * nvox, s, S, opt->bp_mode,
+                               opt->osem_rules ? 1 : 0, bp_prec));
     if (!opt->osem_rules) APIRL_TRY(apirl_mlem_threshold(c, d_sens + (size_t)s * nvox, nvox, &thr[s]));
   }
   // MLEM: 2D updates with s > thr (Mlem2d.cpp:175), 3D with s >= thr
@@ -498,7 +532,7 @@ apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
         cleanup();
         return fail(c, APIRL_ERR_HIP, "memset failed");
       }
-      APIRL_TRY(apirl_backproject(p, d_ratio, d_bp, s, S, opt->bp_mode));
+      APIRL_TRY(apirl_backproject(p, d_ratio, d_bp, s, S, opt->bp_mode, bp_prec));
       APIRL_TRY(apirl_em_update(c, d_img, d_bp, d_sens + (size_t)s * nvox, nvox, thr[s], strict));
     }
   }

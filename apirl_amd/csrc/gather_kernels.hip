@@ -21,6 +21,7 @@
 
 #include "apirl_internal.h"
 #include "alpha_exact.h"
+#include "siddon_f64.h"
 
 namespace apirl {
 
@@ -85,7 +86,7 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
   else if (kind == BP_MULT) b = sino[((long)s * g.nang + a) * g.nr + rr];
   z.b = b;
   z.pad0 = z.pad1 = 0.f;
-  out[idx] = z;
+  out[((long)ka * g.nr + rr) * g.nsino + s] = z;
   if (b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
     if (*f == 0) atomicOr(f, 1);
@@ -110,13 +111,110 @@ struct GatherArgs {
   float* img;
   int kind;           // BP_PLAIN, BP_ONES or BP_MULT
   int nang_sub, subset, nsubsets;
+  int zlanes;         // voxel mapping (see gather_kernel)
 };
 
-// Voxel tile of one workgroup: 16 x 16 voxels of one slice, so the lanes of a wave
-// (4 rows of 16) share most candidate rays and sinograms.
+// Voxel tile of one workgroup in the transaxial mapping: 16 x 16 voxels of one slice.
 constexpr int kTile = 16;
+constexpr int kWave = 64;
 
-template <int DIMS>
+// Per-voxel accumulator of the gather for each precision mode.
+template <int PREC>
+struct Acc {
+  using alpha_t = float;
+  float s;
+  __device__ void init(float v) { s = v; }
+  __device__ void add(float t) { s = s + t; }
+  __device__ void add_f64(double) {}
+  __device__ float value() const { return s; }
+};
+template <>
+struct Acc<APIRL_PREC_KAHAN> {
+  using alpha_t = float;
+  float s, c;
+  __device__ void init(float v) { s = v; c = 0.f; }
+  __device__ void add(float t) {
+    const float y = t - c;
+    const float u = s + y;
+    c = (u - s) - y;
+    s = u;
+  }
+  __device__ void add_f64(double) {}
+  __device__ float value() const { return s; }
+};
+template <>
+struct Acc<APIRL_PREC_F64_ALPHA> {
+  using alpha_t = double;
+  double s;
+  __device__ void init(float v) { s = v; }
+  __device__ void add(float t) { s += t; }
+  __device__ void add_f64(double t) { s += t; }
+  __device__ float value() const { return (float)s; }
+};
+
+// First x / y crossings and increments of a ray.  REFERENCE and KAHAN use the
+// reference's float values from the ray table (Siddon.cpp:253-286, with the 2D
+// clamp of a negative first crossing to alpha_min, :590-606); F64_ALPHA in 2D
+// recomputes them in double from the float entry voxel (3D F64_ALPHA does the whole
+// setup in double, siddon_f64.h).
+template <int PREC, int DIMS>
+__device__ __forceinline__ void first_crossings(const Geo& g, const RayXY& t, int iinc, int jinc,
+                                                typename Acc<PREC>::alpha_t& ax0,
+                                                typename Acc<PREC>::alpha_t& axu,
+                                                typename Acc<PREC>::alpha_t& ay0,
+                                                typename Acc<PREC>::alpha_t& ayu) {
+  if (PREC != APIRL_PREC_F64_ALPHA) {
+    ax0 = t.ax;
+    axu = t.axu;
+    ay0 = t.ay;
+    ayu = t.ayu;
+  } else {
+    const double R = g.R;
+    const double vx = t.vx, vy = t.vy;
+    ax0 = FLT_MAX;
+    ay0 = FLT_MAX;
+    axu = 0.0;
+    ayu = 0.0;
+    if (iinc != 0) {
+      axu = fabs((double)g.dx / vx);
+      ax0 = (-R + (double)(t.i0 + (iinc > 0 ? 1 : 0)) * g.dx - t.p0x) / vx;
+      if (DIMS == 2 && ax0 < 0.0) ax0 = t.amin;
+    }
+    if (jinc != 0) {
+      ayu = fabs((double)g.dy / vy);
+      ay0 = (-R + (double)(t.j0 + (jinc > 0 ? 1 : 0)) * g.dy - t.p0y) / vy;
+      if (DIMS == 2 && ay0 < 0.0) ay0 = t.amin;
+    }
+  }
+}
+
+// n-th crossing after the first: the exact float accumulator (alpha_exact.h) or,
+// in double, a0 + n u.
+template <int PREC>
+__device__ __forceinline__ typename Acc<PREC>::alpha_t crossing(typename Acc<PREC>::alpha_t a0,
+                                                               typename Acc<PREC>::alpha_t u,
+                                                               int n) {
+  if (PREC == APIRL_PREC_F64_ALPHA) return a0 + (double)n * u;
+  else return alpha_after((float)a0, (float)u, n);
+}
+
+template <int PREC>
+__device__ __forceinline__ void z_crossings(const Geo& g, const ZRay& z, float2 zz,
+                                            typename Acc<PREC>::alpha_t& az0,
+                                            typename Acc<PREC>::alpha_t& azu) {
+  if (PREC != APIRL_PREC_F64_ALPHA) {
+    az0 = z.az0;
+    azu = z.azu;
+  } else {
+    const double za = zz.x, vz = (double)zz.y - zz.x;
+    azu = fabs((double)g.dz / vz);
+    if (vz > 0) az0 = ((double)g.offz + (double)(z.k0 + 1) * g.dz - za) / vz;
+    else if (vz < 0) az0 = ((double)g.offz + (double)z.k0 * g.dz - za) / vz;
+    else az0 = FLT_MAX;
+  }
+}
+
+template <int DIMS, int PREC>
 __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
   extern __shared__ float2 s_zab[];  // (P1.Z, P2.Z) of every sinogram
   const Geo& g = A.g;
@@ -124,17 +222,44 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
     for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
     __syncthreads();
   }
-  const int i = blockIdx.x * kTile + (threadIdx.x % kTile);
-  const int j = blockIdx.y * kTile + (threadIdx.x / kTile);
-  const int k = blockIdx.z;
-  if (i >= g.nx || j >= g.ny) return;
+  // Voxel of this lane.
+  //   zlanes <= 1: a workgroup is a 16 x 16 tile of one slice (grid nx/16, ny/16, nz).
+  //   zlanes = Z > 1: a wavefront holds 64 / Z voxel columns (neighbours along x) times
+  //     Z consecutive slices; every lane of a column shares the column's candidate
+  //     rays, so the transaxial tests agree across those lanes and only the axial
+  //     part differs.  1D grid; workgroups are remapped so that each XCD takes one
+  //     contiguous run of columns (round-robin dispatch puts block b on XCD b % 8).
+  int i, j, k;
+  if (A.zlanes <= 1) {
+    i = blockIdx.x * kTile + (threadIdx.x % kTile);
+    j = blockIdx.y * kTile + (threadIdx.x / kTile);
+    k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+  } else {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned xcd = b % 8, per = nb / 8, rem = nb % 8;
+    const unsigned lb = xcd * per + (xcd < rem ? xcd : rem) + b / 8;
+    const int Z = A.zlanes, C = kWave / Z;
+    const int nch = (g.nz + Z - 1) / Z;
+    const long wave = ((long)lb * kGBlock + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int c = lane / Z, kk = lane % Z;
+    const long q = (wave / nch) * C + c;
+    k = (int)(wave % nch) * Z + kk;
+    if (c >= C || k >= g.nz || q >= (long)g.nx * g.ny) return;
+    i = (int)(q % g.nx);
+    j = (int)(q / g.nx);
+  }
   const long v = ((long)k * g.ny + j) * g.nx + i;
   // voxel centre for the candidate search
   const float xc = -g.R + g.dx * ((float)i + 0.5f);
   const float yc = -g.R + g.dy * ((float)j + 0.5f);
   const float zk0 = g.offz + k * g.dz, zk1 = zk0 + g.dz;
   const float inv_dr = 1.0f / g.dr;
-  float acc = A.img[v];
+  // Accumulator: float in the reference order (REFERENCE), float with a Kahan
+  // compensation term (KAHAN), or double (F64_ALPHA).
+  Acc<PREC> acc;
+  acc.init(A.img[v]);
   for (int ka = 0; ka < A.nang_sub; ka++) {
     const int a = A.subset + A.nsubsets * ka;
     const float2 cs = A.angcs[a];
@@ -149,53 +274,83 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
       const RayXY t = A.rays[(long)a * g.nr + rr];
       if (!(t.flags & RAY_VALID)) continue;
       const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
+      // Entry voxel, transaxial list length and alpha range of the ray: the
+      // reference's float setup from the ray table, or (F64_ALPHA, 3D) the same
+      // setup carried out in double (siddon_f64.h).
+      using R = typename Acc<PREC>::alpha_t;
+      constexpr bool kF64Setup = PREC == APIRL_PREC_F64_ALPHA && DIMS == 3;
+      int ti0 = t.i0, tj0 = t.j0, tnxy = t.nxy;
+      R tamin = t.amin, tamax = t.amax;
+      R ax0, axu, ay0, ayu;
+      SetupXY64 xy;
+      if (kF64Setup) {
+        setup_xy_f64(g, t.p0x, t.p0y, t.vx, t.vy, xy);
+        ti0 = xy.i0;
+        tj0 = xy.j0;
+        tnxy = abs(xy.i1 - xy.i0) + abs(xy.j1 - xy.j0);
+        tamin = (R)xy.amin;
+        tamax = (R)xy.amax;
+        ax0 = (R)xy.ax;
+        axu = (R)xy.axu;
+        ay0 = (R)xy.ay;
+        ayu = (R)xy.ayu;
+      } else {
+        first_crossings<PREC, DIMS>(g, t, iinc, jinc, ax0, axu, ay0, ayu);
+      }
       int na, nb;
       if (iinc == 0) {
-        if (i != t.i0) continue;
+        if (i != ti0) continue;
         na = 0;
       } else {
-        na = (i - t.i0) * iinc;
+        na = (i - ti0) * iinc;
         if (na < 0) continue;
       }
       if (jinc == 0) {
-        if (j != t.j0) continue;
+        if (j != tj0) continue;
         nb = 0;
       } else {
-        nb = (j - t.j0) * jinc;
+        nb = (j - tj0) * jinc;
         if (nb < 0) continue;
       }
-      if (na + nb >= t.nxy + (DIMS == 3 ? g.nz + 1 : 0)) continue;
-      // x / y crossings around the column
-      float Xp = -FLT_MAX, Xn = t.ax, Yp = -FLT_MAX, Yn = t.ay;
+      if (na + nb >= tnxy + (DIMS == 3 ? g.nz + 1 : 0)) continue;
+      // x / y crossings around the column: the float accumulators of the reference
+      // loop reconstructed exactly, or (F64_ALPHA) the crossings in double
+      R Xp = -FLT_MAX, Xn, Yp = -FLT_MAX, Yn;
+      Xn = ax0;
+      Yn = ay0;
       if (na > 0) {
-        Xp = alpha_after(t.ax, t.axu, na - 1);
-        Xn = Xp + t.axu;
+        Xp = crossing<PREC>(ax0, axu, na - 1);
+        Xn = Xp + axu;
       }
       if (nb > 0) {
-        Yp = alpha_after(t.ay, t.ayu, nb - 1);
-        Yn = Yp + t.ayu;
+        Yp = crossing<PREC>(ay0, ayu, nb - 1);
+        Yn = Yp + ayu;
       }
       // taken x before untaken y (x wins ties), taken y before untaken x (strict)
       if (na > 0 && !(Xp <= Yn)) continue;
       if (nb > 0 && !(Yp < Xn)) continue;
       if (DIMS == 2) {
         const int m = na + nb;
-        if (m >= t.nxy) continue;
-        float ac = t.amin;
-        if (m > 0) ac = fmaxf(Xp, Yp);
-        const float an = fminf(Xn, Yn);
-        const float w = (an - ac) * sqrtf(t.lxy);
+        if (m >= tnxy) continue;
+        R ac = tamin;
+        if (m > 0) ac = Xp > Yp ? Xp : Yp;
+        const R an = Xn < Yn ? Xn : Yn;
         float b = 1.f;
         if (A.kind == BP_PLAIN) b = A.sino[(long)ka * g.nr + rr];
         else if (A.kind == BP_MULT) b = A.sino[(long)a * g.nr + rr];
-        acc += w * b * 1.0f;  // SiddonProjector.cpp:72
+        if (PREC == APIRL_PREC_F64_ALPHA) {
+          acc.add_f64((an - ac) * sqrt((double)t.vx * t.vx + (double)t.vy * t.vy) * b);
+        } else {
+          const float w = (float)(an - ac) * sqrtf(t.lxy);
+          acc.add(w * b * 1.0f);  // SiddonProjector.cpp:72
+        }
         continue;
       }
       // alpha interval of the column; the sinograms whose ray crosses the slab
       // [zk0, zk1] inside it form a contiguous run of each segment (the mean ring
       // coordinates grow with the index), found by a short scan from an estimate.
-      const float lo = fmaxf(t.amin, fmaxf(Xp, Yp));
-      const float hi = fminf(t.amax, fminf(Xn, Yn));
+      const float lo = fmaxf((float)tamin, fmaxf((float)Xp, (float)Yp));
+      const float hi = fminf((float)tamax, fminf((float)Xn, (float)Yn));
       const float amid = 0.5f * (lo + hi) - 0.5f;
       const float zlo_t = zk0 - kZMargin, zhi_t = zk1 + kZMargin;
       for (int sg = 0; sg < A.nseg; sg++) {
@@ -216,48 +371,96 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
           const float za_lo = z_at(zz.x, vzz, lo), za_hi = z_at(zz.x, vzz, hi);
           if (fmaxf(za_lo, za_hi) < zlo_t) continue;
           if (fminf(za_lo, za_hi) > zhi_t) break;
-          const long ridx = ((long)s * A.nang_sub + ka) * g.nr + rr;
-          const ZRay z = A.zr[ridx];
+          const ZRay z = A.zr[((long)ka * g.nr + rr) * g.nsino + s];
           if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
+          int zk0 = z.k0, zk1 = z.k1, zkinc = z.kinc;
+          R az0, azu;
+          double Ld = 0.0;
+          if (kF64Setup) {
+            SetupZ64 zs;
+            setup_z_f64(g, xy, zz.x, t.vx, t.vy, (double)(zz.y - zz.x), zs);
+            zk0 = zs.k0;
+            zk1 = zs.k1;
+            zkinc = zs.kinc;
+            az0 = (R)zs.az;
+            azu = (R)zs.azu;
+            Ld = zs.L;
+          } else {
+            z_crossings<PREC>(g, z, zz, az0, azu);
+          }
           int nc;
-          if (z.kinc == 0) {
-            if (k != z.k0) continue;
+          if (zkinc == 0) {
+            if (k != zk0) continue;
             nc = 0;
           } else {
-            nc = (k - z.k0) * z.kinc;
+            nc = (k - zk0) * zkinc;
             if (nc < 0) continue;
           }
           const int m = na + nb + nc;
-          if (m >= t.nxy + abs((int)z.k1 - z.k0)) continue;
-          float Zp = -FLT_MAX, Zn = z.az0;
+          if (m >= tnxy + abs(zk1 - zk0)) continue;
+          R Zp = -FLT_MAX, Zn = az0;
           if (nc > 0) {
-            Zp = alpha_after(z.az0, z.azu, nc - 1);
-            Zn = Zp + z.azu;
+            Zp = crossing<PREC>(az0, azu, nc - 1);
+            Zn = Zp + azu;
           }
           // cross-axis order (x before y before z on ties)
           if (na > 0 && !(Xp <= Zn)) continue;
           if (nb > 0 && !(Yp <= Zn)) continue;
           if (nc > 0 && !(Zp < Xn && Zp < Yn)) continue;
-          float ac = t.amin;
-          if (m > 0) ac = fmaxf(fmaxf(Xp, Yp), Zp);
-          const float an = fminf(fminf(Xn, Yn), Zn);
-          const float w = (an - ac) * z.L;
-          acc += w * 1.0f * z.b;  // SiddonProjector.cpp:546
+          R ac = tamin;
+          if (m > 0) {
+            ac = Xp > Yp ? Xp : Yp;
+            ac = ac > Zp ? ac : Zp;
+          }
+          R an = Xn < Yn ? Xn : Yn;
+          an = an < Zn ? an : Zn;
+          if (PREC == APIRL_PREC_F64_ALPHA) {
+            acc.add_f64((an - ac) * Ld * z.b);
+          } else {
+            const float w = (float)(an - ac) * z.L;
+            acc.add(w * 1.0f * z.b);  // SiddonProjector.cpp:546
+          }
         }
       }
     }
   }
-  A.img[v] = acc;
+  A.img[v] = acc.value();
 }
 
 inline unsigned ggrid(long n) { return (unsigned)((n + kGBlock - 1) / kGBlock); }
 
+template <int PREC>
+void launch_gather3(const GatherArgs& A, int zlanes, hipStream_t st) {
+  const Geo& g = A.g;
+  const size_t lds = (size_t)g.nsino * sizeof(float2);
+  if (zlanes <= 1) {
+    const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
+    hipLaunchKernelGGL((gather_kernel<3, PREC>), grid, dim3(kGBlock), lds, st, A);
+    return;
+  }
+  const long ncol = (long)g.nx * g.ny;
+  const int C = kWave / zlanes;
+  const long nwaves = (ncol + C - 1) / C * ((g.nz + zlanes - 1) / zlanes);
+  const long nblocks = (nwaves * kWave + kGBlock - 1) / kGBlock;
+  hipLaunchKernelGGL((gather_kernel<3, PREC>), dim3((unsigned)nblocks), dim3(kGBlock), lds, st,
+                     A);
+}
+
+template <int PREC>
+void launch_gather2(const GatherArgs& A, hipStream_t st) {
+  const Geo& g = A.g;
+  const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
+  hipLaunchKernelGGL((gather_kernel<2, PREC>), grid, dim3(kGBlock), 0, st, A);
+}
+
 }  // namespace
 
 hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, float* img,
-                                     int subset, int nsubsets, int kind, void* scratch,
-                                     size_t scratch_bytes, hipStream_t st) {
+                                     int subset, int nsubsets, int kind, int precision,
+                                     void* scratch, size_t scratch_bytes, hipStream_t st) {
   if (kind != BP_PLAIN && kind != BP_ONES && kind != BP_MULT) return hipErrorNotSupported;
+  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_ALPHA)
+    return hipErrorInvalidValue;
   const Geo& g = p->geo;
   const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
   const long nrays = (long)g.nsino * nang_sub * g.nr;
@@ -278,6 +481,7 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
   A.nsubsets = nsubsets;
   A.zr = nullptr;
   A.nz_pair = nullptr;
+  A.zlanes = g.dims == 3 ? p->gather_zlanes : 1;
   if (g.dims == 3) {
     const size_t need = gather_scratch_bytes(p, subset, nsubsets);
     if (scratch_bytes < need) return hipErrorInvalidValue;
@@ -289,12 +493,13 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                        p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp);
     A.zr = zr;
     A.nz_pair = nzp;
-    const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
-    hipLaunchKernelGGL(gather_kernel<3>, grid, dim3(kGBlock), (size_t)g.nsino * sizeof(float2),
-                       st, A);
+    if (precision == APIRL_PREC_REFERENCE) launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, st);
+    else if (precision == APIRL_PREC_KAHAN) launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, st);
+    else launch_gather3<APIRL_PREC_F64_ALPHA>(A, A.zlanes, st);
   } else {
-    const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
-    hipLaunchKernelGGL(gather_kernel<2>, grid, dim3(kGBlock), 0, st, A);
+    if (precision == APIRL_PREC_REFERENCE) launch_gather2<APIRL_PREC_REFERENCE>(A, st);
+    else if (precision == APIRL_PREC_KAHAN) launch_gather2<APIRL_PREC_KAHAN>(A, st);
+    else launch_gather2<APIRL_PREC_F64_ALPHA>(A, st);
   }
   return hipGetLastError();
 }

@@ -15,6 +15,7 @@
 #include <cmath>
 
 #include "apirl_internal.h"
+#include "siddon_f64.h"
 
 namespace apirl {
 
@@ -240,50 +241,52 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
           }
         }
       } else {
-        // fp64 alpha stepping: the same traversal rules with every alpha, increment
-        // and weight held in double; accumulation in double.
-        double ax = ray.ax, ay = ray.ay, az = ray.az, ac = ray.ac;
-        double axu = ray.axu, ayu = ray.ayu, azu = ray.azu;
-        const double L = ray.L;
-        if (DIMS == 3) {
-          // recompute the first crossings and increments in double from the LOR
-          const double vx = t.vx, vy = t.vy, vz = (double)zb - za;
-          axu = fabs((double)g.dx / vx);
-          ayu = fabs((double)g.dy / vy);
-          azu = fabs((double)g.dz / vz);
-          const double R = g.R;
-          if (vx > 0) ax = (-R + (double)(ray.i + 1) * g.dx - t.p0x) / vx;
-          else if (vx < 0) ax = (-R + (double)ray.i * g.dx - t.p0x) / vx;
-          if (vy > 0) ay = (-R + (double)(ray.j + 1) * g.dy - t.p0y) / vy;
-          else if (vy < 0) ay = (-R + (double)ray.j * g.dy - t.p0y) / vy;
-          if (vz > 0) az = ((double)g.offz + (double)(ray.k + 1) * g.dz - za) / vz;
-          else if (vz < 0) az = ((double)g.offz + (double)ray.k * g.dz - za) / vz;
-        }
+        // F64_ALPHA.  3D: the whole Siddon setup and traversal in double from the
+        // float LOR (siddon_f64.h), summed in double in list order.  2D: the
+        // reference's float setup (entry voxel, list length, the 2D clamps) with the
+        // crossings, weights and sum in double.
         double dacc = 0.0;
-        int i = ray.i, j = ray.j, kk = ray.k;
         const long nxy = (long)g.nx * g.ny;
-        for (int m = 0; m < ray.n; m++) {
-          const int ci = i, cj = j, ck = kk;
-          const bool bx = (ax <= ay) && (ax <= az);
-          const bool by = !bx && (ay <= az);
-          const double an = bx ? ax : (by ? ay : az);
-          const double w = (an - ac) * L;
-          ac = an;
-          if (bx) { i += ray.iinc; ax += axu; }
-          else if (by) { j += ray.jinc; ay += ayu; }
-          else { kk += ray.kinc; az += azu; }
-          bool ok;
-          long li;
-          if (DIMS == 3) {
-            ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz);
-            li = (long)ck * nxy + (long)cj * g.nx + ci;
-            ok = ok && li >= 0;
-          } else {
-            ok = (cj >= 0) && (ci >= 0);
-            li = (long)cj * g.nx + ci;
-            ok = ok && li < nxy;
+        if (DIMS == 3) {
+          SetupXY64 xy;
+          SetupZ64 zs;
+          const double vz = (double)(zb - za);
+          setup_xy_f64(g, t.p0x, t.p0y, t.vx, t.vy, xy);
+          setup_z_f64(g, xy, za, t.vx, t.vy, vz, zs);
+          const int n = abs(xy.i1 - xy.i0) + abs(xy.j1 - xy.j0) + abs(zs.k1 - zs.k0);
+          double ax = xy.ax, ay = xy.ay, az = zs.az, ac = xy.amin;
+          int i = xy.i0, j = xy.j0, kk = zs.k0;
+          for (int m = 0; m < n; m++) {
+            const int ci = i, cj = j, ck = kk;
+            double w;
+            if (ax <= ay && ax <= az) {
+              w = (ax - ac) * zs.L;
+              i += xy.iinc; ac = ax; ax += xy.axu;
+            } else if (ay <= ax && ay <= az) {
+              w = (ay - ac) * zs.L;
+              j += xy.jinc; ac = ay; ay += xy.ayu;
+            } else {
+              w = (az - ac) * zs.L;
+              kk += zs.kinc; ac = az; az += zs.azu;
+            }
+            if (ci >= 0 && cj >= 0 && ck >= 0 && ci < g.nx && cj < g.ny && ck < g.nz)
+              dacc += w * (double)img[(long)ck * nxy + (long)cj * g.nx + ci];
           }
-          if (ok) dacc += w * (double)img[li];
+        } else {
+          double ax = ray.ax, ay = ray.ay, ac = ray.ac;
+          const double axu = ray.axu, ayu = ray.ayu, L = ray.L;
+          int i = ray.i, j = ray.j;
+          for (int m = 0; m < ray.n; m++) {
+            const int ci = i, cj = j;
+            const bool bx = ax <= ay;
+            const double an = bx ? ax : ay;
+            const double w = (an - ac) * L;
+            ac = an;
+            if (bx) { i += ray.iinc; ax += axu; }
+            else { j += ray.jinc; ay += ayu; }
+            if (cj >= 0 && ci >= 0 && ci < g.nx && cj < g.ny)
+              dacc += w * (double)img[(long)cj * g.nx + ci];
+          }
         }
         acc = (float)dacc;
       }

@@ -41,6 +41,10 @@ class EmOptions(ctypes.Structure):
                 ("precision", _c_int), ("osem_rules", _c_int)]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [("gather_zlanes", _c_int)]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "apirl_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_P)]),
@@ -61,8 +65,10 @@ SIGNATURES = {
     "apirl_plan_image_voxels": (_c_size, [_P]),
     "apirl_plan_intersections": (_c_int, [_P, _c_int, _c_int,
                                           ctypes.POINTER(ctypes.c_ulonglong)]),
+    "apirl_plan_get_tuning": (_c_int, [_P, ctypes.POINTER(Tuning)]),
+    "apirl_plan_set_tuning": (_c_int, [_P, ctypes.POINTER(Tuning)]),
     "apirl_project": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int]),
-    "apirl_backproject": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int]),
+    "apirl_backproject": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_divide_backproject": (_c_int, [_P, _P, _P, _P, _c_int, _c_int, _c_int]),
     "apirl_project_ratio": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_em_update": (_c_int, [_P, _P, _P, _P, _c_size, _c_float, _c_int]),
@@ -232,6 +238,20 @@ class Plan:
                        "apirl_plan_intersections")
         return out.value
 
+    @property
+    def gather_zlanes(self):
+        """3D gather voxel mapping: 1 = 16x16 slice tiles, Z = Z slices per wavefront."""
+        t = Tuning()
+        self.ctx.check(self.ctx.lib.apirl_plan_get_tuning(self.h, ctypes.byref(t)),
+                       "apirl_plan_get_tuning")
+        return t.gather_zlanes
+
+    @gather_zlanes.setter
+    def gather_zlanes(self, n):
+        t = Tuning(gather_zlanes=int(n))
+        self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
+                       "apirl_plan_set_tuning")
+
     # --- Projector entry points on torch device tensors -----------------------
     def project(self, img, out=None, subset=0, nsubsets=1, precision=PREC_REFERENCE):
         torch = _torch()
@@ -244,14 +264,16 @@ class Plan:
                                                   nsubsets, precision), "apirl_project")
         return out
 
-    def backproject(self, sino, out=None, subset=0, nsubsets=1, mode=BP_SCATTER):
+    def backproject(self, sino, out=None, subset=0, nsubsets=1, mode=BP_SCATTER,
+                    precision=PREC_REFERENCE):
         torch = _torch()
         if out is None:
             out = torch.zeros(self.image_shape(), dtype=torch.float32, device=sino.device)
         self._check_img(out)
         self._check_sino(sino, subset, nsubsets)
         self.ctx.check(self.ctx.lib.apirl_backproject(self.h, _dptr(sino), _dptr(out), subset,
-                                                      nsubsets, mode), "apirl_backproject")
+                                                      nsubsets, mode, precision),
+                       "apirl_backproject")
         return out
 
     def divide_backproject(self, meas, est, out=None, subset=0, nsubsets=1, mode=BP_SCATTER):

@@ -109,6 +109,19 @@ size_t apirl_plan_image_voxels(const apirl_plan* plan);
 apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets,
                                       unsigned long long* out);
 
+/* Launch-shape tuning of a plan.  No counterpart in the reference (its CPU projector
+ * has no tiles; reconGPU fixes them at compile time, CuMlemSinogram3d.cu:20-58):
+ *   gather_zlanes: voxel mapping of the 3D gather backprojector.  1 = each workgroup
+ *                  a 16 x 16 tile of one slice; Z in 2..64 = each 64-lane wavefront
+ *                  64/Z neighbouring voxel columns times Z consecutive slices.
+ * Results do not depend on it (the per-voxel summation order is fixed).
+ * Zero in a field leaves it unchanged. */
+typedef struct apirl_tuning {
+  int gather_zlanes;
+} apirl_tuning;
+apirl_status apirl_plan_get_tuning(const apirl_plan* plan, apirl_tuning* out);
+apirl_status apirl_plan_set_tuning(apirl_plan* plan, const apirl_tuning* tuning);
+
 /* ---- Projector API (recon/inc/Projector.h:48-85) ------------------------------ */
 /* Project: d_sino[subset bins] := A d_img.  Replaces SiddonProjector::Project
  * (2D SiddonProjector.cpp:146-193, 3D :322-441).  2D bins whose LOR the reference
@@ -116,9 +129,12 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
 apirl_status apirl_project(apirl_plan* plan, const float* d_img, float* d_sino, int subset,
                            int nsubsets, int precision);
 /* Backproject: d_img += A^T d_sino (accumulates; the caller zeroes, Mlem2d.cpp:162).
- * Replaces SiddonProjector::Backproject (2D :34-87, 3D :444-568). */
+ * Replaces SiddonProjector::Backproject (2D :34-87, 3D :444-568).  precision applies
+ * to the gather form: APIRL_PREC_REFERENCE sums the reference's float weights in
+ * float, APIRL_PREC_KAHAN with a compensation term, APIRL_PREC_F64_ALPHA computes the
+ * crossings and sums in double; the scatter form takes APIRL_PREC_REFERENCE only. */
 apirl_status apirl_backproject(apirl_plan* plan, const float* d_sino, float* d_img, int subset,
-                               int nsubsets, int mode);
+                               int nsubsets, int mode, int precision);
 /* DivideAndBackproject: d_img += A^T r with r the reference's ratio rule per segment
  * (3D SiddonProjector.cpp:571-704; 2D :90-144). */
 apirl_status apirl_divide_backproject(apirl_plan* plan, const float* d_meas, const float* d_est,

@@ -207,7 +207,7 @@ bool HipSiddonProjector::Backproject(Sinogram2D* projection, Image* image) {
     return fail("HipSiddonProjector::Backproject: out of device memory");
   if (apirl_memcpy_h2d(ctx, d_img, image->getPixelsPtr(), nv * 4) != APIRL_OK ||
       apirl_memcpy_h2d(ctx, d_sino, projection->getSinogramPtr(), nb * 4) != APIRL_OK ||
-      apirl_backproject(p, d_sino, d_img, 0, 1, mode) != APIRL_OK ||
+      apirl_backproject(p, d_sino, d_img, 0, 1, mode, APIRL_PREC_REFERENCE) != APIRL_OK ||
       apirl_memcpy_d2h(ctx, image->getPixelsPtr(), d_img, nv * 4) != APIRL_OK)
     return fail("HipSiddonProjector::Backproject");
   return true;
@@ -262,7 +262,7 @@ bool HipSiddonProjector::Backproject(Sinogram3D* projection, Image* image) {
   copySinogramIn(projection, host.data());
   if (apirl_memcpy_h2d(ctx, d_img, image->getPixelsPtr(), nv * 4) != APIRL_OK ||
       apirl_memcpy_h2d(ctx, d_sino, host.data(), nb * 4) != APIRL_OK ||
-      apirl_backproject(p, d_sino, d_img, 0, 1, mode) != APIRL_OK ||
+      apirl_backproject(p, d_sino, d_img, 0, 1, mode, APIRL_PREC_REFERENCE) != APIRL_OK ||
       apirl_memcpy_d2h(ctx, image->getPixelsPtr(), d_img, nv * 4) != APIRL_OK)
     return fail("HipSiddonProjector::Backproject");
   return true;

@@ -528,50 +528,93 @@ int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner
 
 /* ------------------------------------------------------------ FP64 oracle */
 
-/* Exact (double) length of the LOR segment inside voxel box [lo, hi] restricted
- * to alpha in [a0, a1]. */
-static double clip_len(const double p[3], const double v[3], const double lo[3],
-                       const double hi[3], double a0, double a1, double L) {
-  for (int d = 0; d < 3; d++) {
-    if (v[d] == 0) {
-      if (p[d] < lo[d] || p[d] >= hi[d]) return 0;
-      continue;
-    }
-    double t0 = (lo[d] - p[d]) / v[d], t1 = (hi[d] - p[d]) / v[d];
-    if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
-    if (t0 > a0) a0 = t0;
-    if (t1 < a1) a1 = t1;
+/* The FP64 oracle: recon/src/Siddon.cpp:13-344 (orc_siddon3d's statements) carried
+ * out in double from the same float LOR -- alpha range, entry / exit voxels, list
+ * length (with the dropped exit voxel), first crossings, accumulated crossings and
+ * weights.  Away from ties this visits orc_siddon3d's voxels; on a LOR that lies
+ * on a voxel face it decides the side in double. */
+static int siddon3d_d(const orc_image* im, const float* pf, const float* vf, orc_seg* out,
+                      double* wout, int cap) {
+  const double p0x = pf[0], p0y = pf[1], p0z = pf[2], vx = vf[0], vy = vf[1], vz = vf[2];
+  const double dx = im->dx, dy = im->dy, dz = im->dz;
+  const double R = (dx * im->nx) / 2;
+  double amin, amax;
+  if (vx == 0) {
+    double a1 = (-R - p0y) / vy, a2 = (R - p0y) / vy;
+    if (a1 < a2) { amin = a1; amax = a2; } else { amin = a2; amax = a1; }
+  } else if (vy == 0) {
+    double a1 = (-R - p0x) / vx, a2 = (R - p0x) / vx;
+    if (a1 < a2) { amin = a1; amax = a2; } else { amin = a2; amax = a1; }
+  } else {
+    double ax1 = (-R - p0x) / vx, ay1 = (-R - p0y) / vy;
+    double ax2 = (R - p0x) / vx, ay2 = (R - p0y) / vy;
+    double lo1 = ax1 < ax2 ? ax1 : ax2, lo2 = ay1 < ay2 ? ay1 : ay2;
+    double hi1 = ax1 > ax2 ? ax1 : ax2, hi2 = ay1 > ay2 ? ay1 : ay2;
+    amin = lo1 > lo2 ? lo1 : lo2;
+    amax = hi1 < hi2 ? hi1 : hi2;
   }
-  return a1 > a0 ? (a1 - a0) * L : 0;
+  if (amin < 0 || amin > 1) amin = 0;
+  if (amax > 1 || amax < 0) amax = 1;
+  double x1 = p0x + vx * amin, y1 = p0y + vy * amin, z1 = p0z + vz * amin;
+  double x2 = p0x + vx * amax, y2 = p0y + vy * amax, z2 = p0z + vz * amax;
+  double offz = dz / 2;
+  int imin = (int)floor((x1 + R) / dx), jmin = (int)floor((y1 + R) / dy);
+  int kmin = (int)floor((z1 - offz) / dz);
+  int imax = (int)floor((x2 + R) / dx), jmax = (int)floor((y2 + R) / dy);
+  int kmax = (int)floor((z2 - offz) / dz);
+  int n = abs(imax - imin) + abs(jmax - jmin) + abs(kmax - kmin);
+  int iinc = vx > 0 ? 1 : (vx < 0 ? -1 : 0);
+  int jinc = vy > 0 ? 1 : (vy < 0 ? -1 : 0);
+  int kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
+  double L = sqrt(vx * vx + vy * vy + vz * vz);
+  double axu = fabs(dx / vx), ayu = fabs(dy / vy), azu = fabs(dz / vz);
+  double ax, ay, az;
+  if (vx > 0) ax = (-R + (imin + iinc) * dx - p0x) / vx;
+  else if (vx < 0) ax = (-R + imin * dx - p0x) / vx;
+  else ax = DBL_MAX;
+  if (vy > 0) ay = (-R + (jmin + jinc) * dy - p0y) / vy;
+  else if (vy < 0) ay = (-R + jmin * dy - p0y) / vy;
+  else ay = DBL_MAX;
+  if (vz > 0) az = (offz + (kmin + kinc) * dz - p0z) / vz;
+  else if (vz < 0) az = (offz + kmin * dz - p0z) / vz;
+  else az = DBL_MAX;
+  double ac = amin;
+  int i = imin, j = jmin, k = kmin;
+  for (int m = 0; m < n; m++) {
+    double w;
+    int ci = i, cj = j, ck = k;
+    if (ax <= ay && ax <= az) {
+      w = (ax - ac) * L;
+      i += iinc; ac = ax; ax += axu;
+    } else if (ay <= ax && ay <= az) {
+      w = (ay - ac) * L;
+      j += jinc; ac = ay; ay += ayu;
+    } else {
+      w = (az - ac) * L;
+      k += kinc; ac = az; az += azu;
+    }
+    if (m < cap) { out[m].i = ci; out[m].j = cj; out[m].k = ck; out[m].w = (float)w; wout[m] = w; }
+  }
+  return n;
 }
 
-/* The FP64 oracle keeps the reference's visiting rules (the voxel list of the
- * float traversal, so the dropped exit voxel and the entry clamps are the same)
- * and replaces each float weight by the exact chord length in double. */
 static int f64_weights(const orc_image* im, const float* p, const float* v, orc_seg* segs,
                        double* wexact) {
-  int n = orc_siddon3d(p[0], p[1], p[2], v[0], v[1], v[2], im, segs, SEG_CAP);
-  if (n > SEG_CAP) n = SEG_CAP;
-  double pd[3] = {p[0], p[1], p[2]}, vd[3] = {v[0], v[1], v[2]};
-  double R = (double)im->dx * im->nx / 2.0;
-  double L = sqrt(vd[0] * vd[0] + vd[1] * vd[1] + vd[2] * vd[2]);
-  /* alpha range of the square FOV, clamped as the reference does */
-  double a0 = -INFINITY, a1 = INFINITY;
-  for (int d = 0; d < 2; d++) {
-    if (vd[d] == 0) continue;
-    double t0 = (-R - pd[d]) / vd[d], t1 = (R - pd[d]) / vd[d];
-    if (t0 > t1) { double tt = t0; t0 = t1; t1 = tt; }
-    if (t0 > a0) a0 = t0;
-    if (t1 < a1) a1 = t1;
+  int n = siddon3d_d(im, p, v, segs, wexact, SEG_CAP);
+  return n > SEG_CAP ? SEG_CAP : n;
+}
+
+int orc_siddon3d_f64(const float* p, const float* v, const orc_image* im, orc_seg* segs,
+                     double* w, int cap) {
+  orc_seg* all = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  double* wa = (double*)malloc(sizeof(double) * SEG_CAP);
+  int n = f64_weights(im, p, v, all, wa);
+  for (int l = 0; l < n && l < cap; l++) {
+    segs[l] = all[l];
+    w[l] = wa[l];
   }
-  if (a0 < 0 || a0 > 1) a0 = 0;
-  if (a1 > 1 || a1 < 0) a1 = 1;
-  for (int l = 0; l < n; l++) {
-    double lo[3] = {-R + segs[l].i * (double)im->dx, -R + segs[l].j * (double)im->dy,
-                    im->dz / 2.0 + segs[l].k * (double)im->dz};
-    double hi[3] = {lo[0] + im->dx, lo[1] + im->dy, lo[2] + im->dz};
-    wexact[l] = clip_len(pd, vd, lo, hi, a0, a1, L);
-  }
+  free(all);
+  free(wa);
   return n;
 }
 
@@ -617,4 +660,72 @@ void orc_backproject3d_f64(const orc_image* im, const double* sino, int nang,
             img[(long)segs[l].k * nxy + (long)segs[l].j * im->nx + segs[l].i] += w[l] * b;
       }
   free(segs); free(w);
+}
+
+void orc_project3d_bins_f64(const orc_image* im, const double* img, const float* ang_deg,
+                            const float* r_mm, float rscanner, const float* za, const float* zb,
+                            long nbins, const int* s_idx, const int* a_idx, const int* r_idx,
+                            double* out) {
+  orc_seg* segs = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  double* w = (double*)malloc(sizeof(double) * SEG_CAP);
+  long nxy = (long)im->nx * im->ny;
+  for (long q = 0; q < nbins; q++) {
+    float p[3], v[3];
+    lor3d(ang_deg, r_mm, rscanner, za, zb, s_idx[q], a_idx[q], r_idx[q], p, v);
+    int n = f64_weights(im, p, v, segs, w);
+    double acc = 0;
+    for (int l = 0; l < n; l++)
+      if (segs[l].i >= 0 && segs[l].j >= 0 && segs[l].k >= 0 && segs[l].i < im->nx &&
+          segs[l].j < im->ny && segs[l].k < im->nz)
+        acc += w[l] * img[(long)segs[l].k * nxy + (long)segs[l].j * im->nx + segs[l].i];
+    out[q] = acc;
+  }
+  free(segs);
+  free(w);
+}
+
+int orc_osem3d_f64(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
+                   const float* za, const float* zb, const float* meas, const double* init,
+                   int nsubsets, int iters, double* img) {
+  /* orc_osem3d's loop (OsemSinogram3d.cpp:24-346) with the FP64 oracle's system
+   * matrix and every image, sinogram and ratio held in double. */
+  long npix = (long)im->nx * im->ny * im->nz, nb = (long)nsino * nang * nr;
+  float* ang = (float*)malloc(sizeof(float) * nang);
+  float* r = (float*)malloc(sizeof(float) * nr);
+  orc_sino_grid(nang, nr, rfov, ang, r);
+  int* idx = (int*)malloc(sizeof(int) * nang);
+  float* sang = (float*)malloc(sizeof(float) * nang);
+  double* ysub = (double*)malloc(sizeof(double) * nb);
+  double* est = (double*)malloc(sizeof(double) * nb);
+  double* bp = (double*)malloc(sizeof(double) * npix);
+  double* sens = (double*)malloc(sizeof(double) * npix * nsubsets);
+  for (int s = 0; s < nsubsets; s++) {
+    int na = orc_subset_angles(nang, nsubsets, s, idx);
+    for (int k = 0; k < na; k++) sang[k] = ang[idx[k]];
+    long nbs = (long)nsino * na * nr;
+    for (long i = 0; i < nbs; i++) ysub[i] = 1;
+    double* sn = sens + (long)s * npix;
+    memset(sn, 0, sizeof(double) * npix);
+    orc_backproject3d_f64(im, ysub, na, sang, nr, r, rscanner, nsino, za, zb, sn);
+  }
+  memcpy(img, init, sizeof(double) * npix);
+  for (int t = 0; t < iters; t++)
+    for (int s = 0; s < nsubsets; s++) {
+      int na = orc_subset_angles(nang, nsubsets, s, idx);
+      for (int k = 0; k < na; k++) sang[k] = ang[idx[k]];
+      long nbs = (long)nsino * na * nr;
+      for (int q = 0; q < nsino; q++)
+        for (int k = 0; k < na; k++)
+          for (int j = 0; j < nr; j++)
+            ysub[((long)q * na + k) * nr + j] = meas[((long)q * nang + idx[k]) * nr + j];
+      orc_project3d_f64(im, img, na, sang, nr, r, rscanner, nsino, za, zb, est);
+      for (long i = 0; i < nbs; i++)
+        est[i] = (ysub[i] != 0 && est[i] != 0) ? ysub[i] / est[i] : 0;
+      memset(bp, 0, sizeof(double) * npix);
+      orc_backproject3d_f64(im, est, na, sang, nr, r, rscanner, nsino, za, zb, bp);
+      const double* sn = sens + (long)s * npix;
+      for (long i = 0; i < npix; i++) img[i] = (sn[i] >= 0) ? img[i] * bp[i] / sn[i] : 0;
+    }
+  free(ang); free(r); free(idx); free(sang); free(ysub); free(est); free(bp); free(sens);
+  return 0;
 }

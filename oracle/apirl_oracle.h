@@ -111,14 +111,30 @@ int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner
                const float* za, const float* zb, const float* meas, const float* init,
                int nsubsets, int iters, float* img, float* sens_all, float* lik);
 
-/* FP64 oracle: exact chord-length system matrix with the same visited-voxel
- * rules, evaluated in double (used to report FP64 error, not for parity). */
+/* FP64 oracle: the reference's 3D Siddon (recon/src/Siddon.cpp:13-344) carried out
+ * in double from the same float LORs, with double accumulation (used to report the
+ * FP64 error of every result, not for parity). */
+/* The FP64 oracle's list for one LOR (P1 = p, P2 = p + v): voxels and weights of the
+ * double traversal; returns the list length. */
+int orc_siddon3d_f64(const float* p, const float* v, const orc_image* im, orc_seg* segs,
+                     double* w, int cap);
 void orc_project3d_f64(const orc_image* im, const double* img, int nang, const float* ang_deg,
                        int nr, const float* r_mm, float rscanner, int nsino, const float* za,
                        const float* zb, double* sino);
 void orc_backproject3d_f64(const orc_image* im, const double* sino, int nang,
                            const float* ang_deg, int nr, const float* r_mm, float rscanner,
                            int nsino, const float* za, const float* zb, double* img);
+
+/* FP64 oracle projection of selected bins (s_idx, a_idx, r_idx index the full
+ * sinogram's sinogram, angle and radial bin). */
+void orc_project3d_bins_f64(const orc_image* im, const double* img, const float* ang_deg,
+                            const float* r_mm, float rscanner, const float* za, const float* zb,
+                            long nbins, const int* s_idx, const int* a_idx, const int* r_idx,
+                            double* out);
+/* OSEM of orc_osem3d with the FP64 oracle's system matrix, all in double. */
+int orc_osem3d_f64(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
+                   const float* za, const float* zb, const float* meas, const double* init,
+                   int nsubsets, int iters, double* img);
 
 #ifdef __cplusplus
 }

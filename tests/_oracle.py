@@ -56,6 +56,9 @@ def lib():
             "orc_mlem2d": (_i, [P, _i, _i, _f, P, P, _i, P, P, P]),
             "orc_mlem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, P, P, P]),
             "orc_osem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P, P, P]),
+            "orc_siddon3d_f64": (_i, [P, P, P, P, P, _i]),
+            "orc_project3d_bins_f64": (None, [P, P, P, P, _f, P, P, _l, P, P, P, P]),
+            "orc_osem3d_f64": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P]),
             "orc_project3d_f64": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_backproject3d_f64": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
         }
@@ -124,6 +127,16 @@ def siddon3d(p0, v, im, cap=8192):
     return [(out[m].i, out[m].j, out[m].k, out[m].w) for m in range(min(n, cap))]
 
 
+def siddon3d_f64(p0, v, im, cap=8192):
+    """FP64 oracle list of one LOR: [(i, j, k, exact chord length in double)]."""
+    out = (OrcSeg * cap)()
+    w = np.zeros(cap, np.float64)
+    p = f32(p0)
+    vv = f32(v)
+    n = lib().orc_siddon3d_f64(_p(p), _p(vv), ctypes.byref(im), out, _p(w), cap)
+    return [(out[m].i, out[m].j, out[m].k, float(w[m])) for m in range(min(n, cap))]
+
+
 def lor_points(ang, r, rscanner, rfov=float("inf")):
     a, b, c, d = _f(), _f(), _f(), _f()
     ok = lib().orc_lor_points(ang, r, rscanner, rfov, ctypes.byref(a), ctypes.byref(b),
@@ -189,6 +202,25 @@ def backproject3d_f64(im, sino, ang, r, rscanner, za, zb):
     img = np.zeros((im.nz, im.ny, im.nx), np.float64)
     lib().orc_backproject3d_f64(ctypes.byref(im), _p(sino), len(ang), _p(f32(ang)), len(r),
                                 _p(f32(r)), rscanner, len(za), _p(f32(za)), _p(f32(zb)), _p(img))
+    return img
+
+
+def project3d_bins_f64(im, img, ang, r, rscanner, za, zb, s_idx, a_idx, r_idx):
+    """FP64 oracle line integrals of the bins (s_idx, a_idx, r_idx) of the full sinogram."""
+    img = np.ascontiguousarray(img, np.float64)
+    si, ai, ri = (np.ascontiguousarray(x, np.int32) for x in (s_idx, a_idx, r_idx))
+    out = np.zeros(len(si), np.float64)
+    lib().orc_project3d_bins_f64(ctypes.byref(im), _p(img), _p(f32(ang)), _p(f32(r)), rscanner,
+                                 _p(f32(za)), _p(f32(zb)), len(si), _p(si), _p(ai), _p(ri),
+                                 _p(out))
+    return out
+
+
+def osem3d_f64(im, nang, nr, rfov, rscanner, za, zb, meas, init, nsubsets, iters):
+    img = np.zeros((im.nz, im.ny, im.nx), np.float64)
+    lib().orc_osem3d_f64(ctypes.byref(im), nang, nr, rfov, rscanner, len(za), _p(f32(za)),
+                         _p(f32(zb)), _p(f32(meas)), _p(np.ascontiguousarray(init, np.float64)),
+                         nsubsets, iters, _p(img))
     return img
 
 

@@ -1,0 +1,123 @@
+"""Precision modes of the HIP path against the FP64 oracle: the reference's 3D
+Siddon carried out in double from the same float LORs, decisions included (entry /
+exit voxels by floor(), list length |di|+|dj|+|dk|), summed in double; pinned in
+test_oracle_f64.py.
+
+Modes: APIRL_PREC_REFERENCE (the reference's float arithmetic), APIRL_PREC_KAHAN
+(same weights, compensated sums), APIRL_PREC_F64_ALPHA (the whole Siddon setup,
+crossings, weights and sums in double).  The float modes differ from the FP64 oracle
+mostly through discrete decisions, not summation: LORs leave the square FOV exactly
+on its faces and some lie on voxel faces (at 0 and 90 degrees, and the central
+direct plane at z = H/2 with the reference's dz/2 axial offset), so float and double
+floor() put the exit voxel or a whole row on different sides.  F64_ALPHA makes the
+double decisions.  Workload: the hot-spheres phantom (cylinder with 6 spheres of
+diameter 10-37 mm at 4:1 contrast) at 1e9 counts.  Per projection on the
+256x256x81 / 344x252x837 geometry (sampled bins of one 12-angle subset), per
+backprojection and per final image (10 OSEM iterations x 21 subsets) on a 64x64x17
+image with a 122 x 84 x 73 sinogram, whose FP64 OSEM the oracle finishes in
+seconds.  Its radial FOV covers the square image at every angle and each subset has
+4 angles: with fewer, some voxels get a zero sensitivity in one subset while other
+subsets still reach them, and the reference's 0/0 update (OsemSinogram3d.cpp:
+217-237) spreads NaN over the whole image.  Bars: F64_ALPHA within 1e-7 relative L2
+of the oracle for line integrals and backprojections (its double sums are rounded
+once to float) and 1e-4 for the image (stored in float between sub-iterations); the
+float modes' errors are printed."""
+import numpy as np
+import pytest
+
+import _oracle as O
+from apirl_amd import hip, phantoms
+from test_gpu_parity import ctx, dev, host, log, plan3d, torch  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
+          rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
+MID = dict(nx=64, ny=64, nz=17, dx=2.0445, dz=2.027, nproj=84, nr=122, rings=13, rfov=93.0,
+           zfov=34.459, rscanner=400.0, span=5, mrd=8)
+MODES = {"reference": hip.PREC_REFERENCE, "kahan": hip.PREC_KAHAN, "f64_alpha": hip.PREC_F64_ALPHA}
+
+
+def errors(a, ref):
+    a = np.asarray(a, np.float64).ravel()
+    ref = np.asarray(ref, np.float64).ravel()
+    m = np.isfinite(a) & np.isfinite(ref)
+    d = a[m] - ref[m]
+    return np.linalg.norm(d) / np.linalg.norm(ref[m]), np.abs(d).max()
+
+
+def geometry(g):
+    im = O.image_desc(g["nx"], g["ny"], g["nz"], g["dx"], g["dz"])
+    ang, r = O.sino_grid(g["nproj"], g["nr"], g["rfov"])
+    za, zb = O.ring_table(g["rings"], g["zfov"], g["span"], g["mrd"])
+    return im, ang, r, za, zb
+
+
+def mid_phantom():
+    g = MID
+    return phantoms.cylinder_spheres3d(g["nx"], g["nz"], g["dx"], g["dz"], radius=60.0,
+                                       ring_radius=30.0)
+
+
+def test_projection_modes_vs_f64_oracle_c2(torch, ctx):
+    log("case: precision of the 256x256x81 projection, subset 0 of 21")
+    g = C2
+    p = plan3d(ctx, g)
+    img = phantoms.cylinder_spheres3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    x = dev(torch, img)
+    out = {k: host(p.project(x, subset=0, nsubsets=21, precision=m)) for k, m in MODES.items()}
+    torch.cuda.synchronize()
+    im, ang, r, za, zb = geometry(g)
+    rng = np.random.default_rng(0)
+    n = 1500
+    s = rng.integers(0, len(za), n)
+    ka = rng.integers(0, out["reference"].shape[1], n)
+    q = rng.integers(0, g["nr"], n)
+    ref = O.project3d_bins_f64(im, img, ang, r, g["rscanner"], za, zb, s, ka * 21, q)
+    errs = {k: errors(v[s, ka, q], ref) for k, v in out.items()}
+    for k, (rl2, mx) in errs.items():
+        log(f"  projection {k:10s}: rel L2 {rl2:.3e}, max abs {mx:.3e}")
+    assert errs["f64_alpha"][0] <= 1e-7  # the double sums rounded once to float
+    assert errs["reference"][0] <= 5e-2 and errs["kahan"][0] <= 5e-2
+
+
+def test_backprojection_modes_vs_f64_oracle(torch, ctx):
+    log("case: precision of the 64x64x17 gather backprojection")
+    g = MID
+    p = plan3d(ctx, g)
+    im, ang, r, za, zb = geometry(g)
+    sino = O.project3d(im, mid_phantom(), ang, r, g["rscanner"], za, zb)
+    ref = O.backproject3d_f64(im, sino, ang, r, g["rscanner"], za, zb)
+    s = dev(torch, sino)
+    errs = {}
+    for k, m in MODES.items():
+        a = host(p.backproject(s, mode=hip.BP_GATHER, precision=m))
+        b = host(p.backproject(s, mode=hip.BP_GATHER, precision=m))
+        assert np.array_equal(a.view(np.int32), b.view(np.int32))  # deterministic
+        errs[k] = errors(a, ref)
+        log(f"  backprojection {k:10s}: rel L2 {errs[k][0]:.3e}, max abs {errs[k][1]:.3e}")
+    assert errs["f64_alpha"][0] <= 1e-7  # the double sums rounded once to float
+    assert errs["reference"][0] <= 5e-2 and errs["kahan"][0] <= 5e-2
+
+
+def test_osem_modes_vs_f64_oracle(torch, ctx):
+    log("case: precision of 10 OSEM iterations x 21 subsets, 64x64x17")
+    g = MID
+    p = plan3d(ctx, g)
+    im, ang, r, za, zb = geometry(g)
+    truth = O.project3d(im, mid_phantom(), ang, r, g["rscanner"], za, zb)
+    rng = np.random.default_rng(0)
+    meas = phantoms.poisson_counts(truth, 1e9, rng)
+    init = np.ones((g["nz"], g["ny"], g["nx"]), np.float32)
+    ref = O.osem3d_f64(im, g["nproj"], g["nr"], g["rfov"], g["rscanner"], za, zb, meas, init,
+                       21, 10)
+    m = dev(torch, meas)
+    x0 = dev(torch, init)
+    for k, mode in MODES.items():
+        img, _ = p.reconstruct(m, x0, 10, subsets=21, bp_mode=hip.BP_GATHER, precision=mode)
+        a = host(img)
+        assert np.array_equal(np.isnan(a), np.isnan(ref))
+        rl2, mx = errors(a, ref)
+        log(f"  OSEM image {k:10s}: rel L2 {rl2:.3e}, max abs {mx:.3e}")
+        if k == "f64_alpha":
+            assert rl2 <= 1e-4
