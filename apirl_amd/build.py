@@ -27,7 +27,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 SOURCES = [("siddon_kernels.hip", True), ("gather_kernels.hip", True), ("geometry.cpp", False),
-           ("capi.cpp", False)]
+           ("capi.cpp", False), ("batch.cpp", False)]
 
 
 def build(verbose=False):

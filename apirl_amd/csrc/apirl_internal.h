@@ -147,4 +147,106 @@ void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
 // Backprojection kinds
 enum { BP_PLAIN = 0, BP_ONES = 1, BP_DIVIDE = 2, BP_MULT = 3 };
 
+// ---- batched jobs (batch.cpp, grouped kernels) --------------------------------
+// A round runs one OSEM / MLEM sub-iteration of every active job with one launch
+// per phase; each launch walks a job table in device memory.  Every job owns a
+// contiguous run of "virtual blocks" (first .. next job's first) of that phase;
+// the launches are persistent (fixed grid) and each workgroup takes a contiguous
+// share of the virtual blocks, so the launch shapes never change and one captured
+// graph serves every round.
+
+// Per-ray axial part of the Siddon setup and the bin value of one subset bin, used
+// by the gather backprojector (one 32-byte load per candidate ray).
+struct alignas(32) ZRay {
+  int k0;       // entry slice
+  short k1;     // exit slice
+  short kinc;   // sign of Vz
+  float azu;    // |dz / Vz|
+  float az0;    // first z crossing (FLT_MAX when Vz == 0)
+  float L;      // ray length |P2 - P1| in the reference's operand order
+  float b;      // bin value being backprojected
+  float pad0, pad1;
+};
+static_assert(sizeof(ZRay) == 32, "ZRay is one 32-byte load");
+
+struct GatherArgs {
+  Geo g;
+  const RayXY* rays;
+  const float2* zab;
+  const float2* angcs;  // (cos, sin) of every angle of the full sinogram
+  const GatherSeg* segs;
+  int nseg;
+  const ZRay* zr;
+  const int* nz_pair;
+  const float* sino;  // subset sinogram (PLAIN), full sinogram (MULT) or null (ONES)
+  float* img;
+  int kind;           // BP_PLAIN, BP_ONES or BP_MULT
+  int nang_sub, subset, nsubsets;
+  int zlanes;         // voxel mapping (see gather_kernel)
+};
+
+struct BatchHeader {
+  long total[8];  // virtual blocks of each phase (BatchPhase)
+  int count[8];   // jobs in each phase's table
+};
+enum BatchPhase { PH_CLEAR = 0, PH_PROJ2, PH_PROJ3, PH_ZSETUP, PH_GATHER2, PH_GATHER3, PH_UPDATE };
+
+struct ClearJob {  // zero n floats (or ints) at p
+  float* p;
+  long n;
+  long first;
+};
+struct ProjJob {  // fused projection + ratio of one subset (likelihood not taken)
+  Geo g;
+  const RayXY* rays;
+  const float2* zab;
+  const int* zorder;
+  const float* img;
+  FusedRatio fr;
+  int nang_sub, subset, nsubsets, pad;
+  long nrays;
+  long first;
+};
+struct ZsJob {  // axial records of one subset for the gather
+  Geo g;
+  const RayXY* rays;
+  const float2* zab;
+  const float* sino;
+  ZRay* out;
+  int* nz_pair;
+  int kind, nang_sub, subset, nsubsets;
+  long nrays;
+  long first;
+};
+struct GJob {  // gather backprojection of one subset (16 x 16 slice tiles)
+  GatherArgs A;
+  int ntx, nty;
+  long first;
+};
+struct UpdJob {  // x := (s cmp thr) ? x b / s : 0
+  float* x;
+  const float* b;
+  const float* s;
+  long n;
+  float thr;
+  int strict;
+  long first;
+};
+constexpr int kGroupBlock = 256;  // threads per workgroup of the grouped launches
+inline long group_blocks(long n) { return (n + kGroupBlock - 1) / kGroupBlock; }
+inline long gather_tile_blocks(const Geo& g) {
+  return (long)((g.nx + 15) / 16) * ((g.ny + 15) / 16) * g.nz;
+}
+
+hipError_t launch_group_clear(const BatchHeader* hdr, const ClearJob* jobs, int grid,
+                              hipStream_t st);
+hipError_t launch_group_project(const BatchHeader* hdr, const ProjJob* jobs2,
+                                const ProjJob* jobs3, int grid, hipStream_t st);
+hipError_t launch_group_update(const BatchHeader* hdr, const UpdJob* jobs, int grid,
+                               hipStream_t st);
+hipError_t launch_group_gather(const BatchHeader* hdr, const ZsJob* zs, const GJob* g2,
+                               const GJob* g3, int grid, int max_nsino, hipStream_t st);
+void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, const float* sino,
+                     float* img, void* scratch, GatherArgs& A, ZRay*& zr, int*& nz_pair);
+
 }  // namespace apirl

@@ -34,32 +34,19 @@ constexpr float kZMargin = 0.05f;
 
 __device__ __forceinline__ float z_at(float za, float vz, float alpha) { return za + vz * alpha; }
 
-// Per-ray axial part of the Siddon setup for one subset (Siddon.cpp:150-182,
-// 220, 244-255, 279-286) and the bin value, packed in one 32-byte record so a
-// candidate ray costs one load: computed once per backprojection by zsetup_kernel.
-struct alignas(32) ZRay {
-  int k0;       // entry slice
-  short k1;     // exit slice
-  short kinc;   // sign of Vz
-  float azu;    // |dz / Vz|
-  float az0;    // first z crossing (FLT_MAX when Vz == 0)
-  float L;      // ray length |P2 - P1| in the reference's operand order
-  float b;      // bin value being backprojected
-  float pad0, pad1;
-};
-static_assert(sizeof(ZRay) == 32, "ZRay is one 32-byte load");
+// ZRay (apirl_internal.h): per-ray axial part of the Siddon setup for one subset
+// (Siddon.cpp:150-182, 220, 244-255, 279-286) and the bin value, computed once per
+// backprojection by zsetup_kernel.
 
 // nz_pair[ka * nr + r] != 0 iff some sinogram has a non-zero bin at (angle ka, r):
 // the gather skips (angle, r) pairs whose bins are all zero (zero bins add nothing,
 // SiddonProjector.cpp:496).
-__global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __restrict__ rays,
-                                                         const float2* __restrict__ zab,
-                                                         const float* __restrict__ sino, int kind,
-                                                         int nang_sub, int subset, int nsubsets,
-                                                         long nrays, ZRay* __restrict__ out,
-                                                         int* __restrict__ nz_pair) {
-  const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
-  if (idx >= nrays) return;
+__device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict__ rays,
+                                           const float2* __restrict__ zab,
+                                           const float* __restrict__ sino, int kind,
+                                           int nang_sub, int subset, int nsubsets,
+                                           ZRay* __restrict__ out, int* __restrict__ nz_pair,
+                                           long idx) {
   const int rr = (int)(idx % g.nr);
   const long t = idx / g.nr;
   const int ka = (int)(t % nang_sub);
@@ -93,26 +80,23 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
   }
 }
 
+__global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __restrict__ rays,
+                                                         const float2* __restrict__ zab,
+                                                         const float* __restrict__ sino, int kind,
+                                                         int nang_sub, int subset, int nsubsets,
+                                                         long nrays, ZRay* __restrict__ out,
+                                                         int* __restrict__ nz_pair) {
+  const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
+  if (idx < nrays)
+    zsetup_one(g, rays, zab, sino, kind, nang_sub, subset, nsubsets, out, nz_pair, idx);
+}
+
 __global__ __launch_bounds__(kGBlock) void clear_int_kernel(int* __restrict__ p, long n) {
   const long i = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (i < n) p[i] = 0;
 }
 
-struct GatherArgs {
-  Geo g;
-  const RayXY* rays;
-  const float2* zab;
-  const float2* angcs;  // (cos, sin) of every angle of the full sinogram
-  const GatherSeg* segs;
-  int nseg;
-  const ZRay* zr;
-  const int* nz_pair;
-  const float* sino;  // subset sinogram (PLAIN), full sinogram (MULT) or null (ONES)
-  float* img;
-  int kind;           // BP_PLAIN, BP_ONES or BP_MULT
-  int nang_sub, subset, nsubsets;
-  int zlanes;         // voxel mapping (see gather_kernel)
-};
+
 
 // Voxel tile of one workgroup in the transaxial mapping: 16 x 16 voxels of one slice.
 constexpr int kTile = 16;
@@ -214,42 +198,12 @@ __device__ __forceinline__ void z_crossings(const Geo& g, const ZRay& z, float2 
   }
 }
 
+// Gather of one voxel (i, j, k): adds its terms to A.img[v] in the fixed order
+// (angle, radial bin, segment, sinogram).  s_zab is the axial table in LDS (3D).
 template <int DIMS, int PREC>
-__global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
-  extern __shared__ float2 s_zab[];  // (P1.Z, P2.Z) of every sinogram
+__device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* s_zab, int i,
+                                             int j, int k) {
   const Geo& g = A.g;
-  if (DIMS == 3) {
-    for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
-    __syncthreads();
-  }
-  // Voxel of this lane.
-  //   zlanes <= 1: a workgroup is a 16 x 16 tile of one slice (grid nx/16, ny/16, nz).
-  //   zlanes = Z > 1: a wavefront holds 64 / Z voxel columns (neighbours along x) times
-  //     Z consecutive slices; every lane of a column shares the column's candidate
-  //     rays, so the transaxial tests agree across those lanes and only the axial
-  //     part differs.  1D grid; workgroups are remapped so that each XCD takes one
-  //     contiguous run of columns (round-robin dispatch puts block b on XCD b % 8).
-  int i, j, k;
-  if (A.zlanes <= 1) {
-    i = blockIdx.x * kTile + (threadIdx.x % kTile);
-    j = blockIdx.y * kTile + (threadIdx.x / kTile);
-    k = blockIdx.z;
-    if (i >= g.nx || j >= g.ny) return;
-  } else {
-    const unsigned nb = gridDim.x, b = blockIdx.x;
-    const unsigned xcd = b % 8, per = nb / 8, rem = nb % 8;
-    const unsigned lb = xcd * per + (xcd < rem ? xcd : rem) + b / 8;
-    const int Z = A.zlanes, C = kWave / Z;
-    const int nch = (g.nz + Z - 1) / Z;
-    const long wave = ((long)lb * kGBlock + threadIdx.x) / kWave;
-    const int lane = threadIdx.x % kWave;
-    const int c = lane / Z, kk = lane % Z;
-    const long q = (wave / nch) * C + c;
-    k = (int)(wave % nch) * Z + kk;
-    if (c >= C || k >= g.nz || q >= (long)g.nx * g.ny) return;
-    i = (int)(q % g.nx);
-    j = (int)(q / g.nx);
-  }
   const long v = ((long)k * g.ny + j) * g.nx + i;
   // voxel centre for the candidate search
   const float xc = -g.R + g.dx * ((float)i + 0.5f);
@@ -427,6 +381,99 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
   A.img[v] = acc.value();
 }
 
+template <int DIMS, int PREC>
+__global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
+  extern __shared__ float2 s_zab[];  // (P1.Z, P2.Z) of every sinogram
+  const Geo& g = A.g;
+  if (DIMS == 3) {
+    for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
+    __syncthreads();
+  }
+  // Voxel of this lane.
+  //   zlanes <= 1: a workgroup is a 16 x 16 tile of one slice (grid nx/16, ny/16, nz).
+  //   zlanes = Z > 1: a wavefront holds 64 / Z voxel columns (neighbours along x) times
+  //     Z consecutive slices; every lane of a column shares the column's candidate
+  //     rays, so the transaxial tests agree across those lanes and only the axial
+  //     part differs.  1D grid; workgroups are remapped so that each XCD takes one
+  //     contiguous run of columns (round-robin dispatch puts block b on XCD b % 8).
+  int i, j, k;
+  if (A.zlanes <= 1) {
+    i = blockIdx.x * kTile + (threadIdx.x % kTile);
+    j = blockIdx.y * kTile + (threadIdx.x / kTile);
+    k = blockIdx.z;
+    if (i >= g.nx || j >= g.ny) return;
+  } else {
+    const unsigned nb = gridDim.x, b = blockIdx.x;
+    const unsigned xcd = b % 8, per = nb / 8, rem = nb % 8;
+    const unsigned lb = xcd * per + (xcd < rem ? xcd : rem) + b / 8;
+    const int Z = A.zlanes, C = kWave / Z;
+    const int nch = (g.nz + Z - 1) / Z;
+    const long wave = ((long)lb * kGBlock + threadIdx.x) / kWave;
+    const int lane = threadIdx.x % kWave;
+    const int c = lane / Z, kk = lane % Z;
+    const long q = (wave / nch) * C + c;
+    k = (int)(wave % nch) * Z + kk;
+    if (c >= C || k >= g.nz || q >= (long)g.nx * g.ny) return;
+    i = (int)(q % g.nx);
+    j = (int)(q / g.nx);
+  }
+  gather_voxel<DIMS, PREC>(A, s_zab, i, j, k);
+}
+
+// ---- grouped (multi-job) launches --------------------------------------------
+#include "group_common.h"
+
+__global__ __launch_bounds__(kGBlock) void zsetup_group_kernel(const BatchHeader* __restrict__ hdr,
+                                                               const ZsJob* __restrict__ jobs) {
+  const int n = hdr->count[PH_ZSETUP];
+  long lo, hi;
+  vb_range(hdr->total[PH_ZSETUP], lo, hi);
+  if (lo >= hi) return;
+  int jj = find_job(jobs, n, lo);
+  for (long vb = lo; vb < hi; vb++) {
+    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
+    const ZsJob& J = jobs[jj];
+    const long idx = (vb - J.first) * kGBlock + threadIdx.x;
+    if (idx < J.nrays)
+      zsetup_one(J.g, J.rays, J.zab, J.sino, J.kind, J.nang_sub, J.subset, J.nsubsets, J.out,
+                 J.nz_pair, idx);
+  }
+}
+
+// Each virtual block is one 16 x 16 tile of one slice of one job (tile-major within
+// a slice, slices in order), as gather_kernel's grid; 3D jobs reload the axial table
+// into LDS when the workgroup moves to the next job.
+template <int DIMS>
+__global__ __launch_bounds__(kGBlock) void gather_group_kernel(const BatchHeader* __restrict__ hdr,
+                                                               const GJob* __restrict__ jobs) {
+  extern __shared__ float2 s_zab[];
+  const int ph = DIMS == 2 ? PH_GATHER2 : PH_GATHER3;
+  const int n = hdr->count[ph];
+  long lo, hi;
+  vb_range(hdr->total[ph], lo, hi);
+  if (lo >= hi) return;
+  int jj = find_job(jobs, n, lo);
+  int loaded = -1;
+  for (long vb = lo; vb < hi; vb++) {
+    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
+    const GJob& J = jobs[jj];
+    if (DIMS == 3 && loaded != jj) {
+      __syncthreads();
+      for (int q = threadIdx.x; q < J.A.g.nsino; q += kGBlock) s_zab[q] = J.A.zab[q];
+      __syncthreads();
+      loaded = jj;
+    }
+    const long lb = vb - J.first;
+    const int bx = (int)(lb % J.ntx);
+    const int by = (int)((lb / J.ntx) % J.nty);
+    const int k = (int)(lb / ((long)J.ntx * J.nty));
+    const int i = bx * kTile + (threadIdx.x % kTile);
+    const int j = by * kTile + (threadIdx.x / kTile);
+    if (i < J.A.g.nx && j < J.A.g.ny)
+      gather_voxel<DIMS, APIRL_PREC_REFERENCE>(J.A, s_zab, i, j, k);
+  }
+}
+
 inline unsigned ggrid(long n) { return (unsigned)((n + kGBlock - 1) / kGBlock); }
 
 template <int PREC>
@@ -501,6 +548,43 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     else if (precision == APIRL_PREC_KAHAN) launch_gather2<APIRL_PREC_KAHAN>(A, st);
     else launch_gather2<APIRL_PREC_F64_ALPHA>(A, st);
   }
+  return hipGetLastError();
+}
+
+void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, const float* sino,
+                     float* img, void* scratch, GatherArgs& A, ZRay*& zr, int*& nz_pair) {
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long nrays = (long)g.nsino * nang_sub * g.nr;
+  A.g = g;
+  A.rays = p->d_rays;
+  A.zab = p->d_zab;
+  A.angcs = p->d_angcs;
+  A.segs = p->d_segs;
+  A.nseg = p->nseg;
+  A.sino = sino;
+  A.img = img;
+  A.kind = kind;
+  A.nang_sub = nang_sub;
+  A.subset = subset;
+  A.nsubsets = nsubsets;
+  A.zlanes = 1;
+  zr = nullptr;
+  nz_pair = nullptr;
+  if (g.dims == 3 && scratch) {
+    zr = (ZRay*)scratch;
+    nz_pair = (int*)((char*)scratch + (size_t)nrays * sizeof(ZRay));
+  }
+  A.zr = zr;
+  A.nz_pair = nz_pair;
+}
+
+hipError_t launch_group_gather(const BatchHeader* hdr, const ZsJob* zs, const GJob* g2,
+                               const GJob* g3, int grid, int max_nsino, hipStream_t st) {
+  hipLaunchKernelGGL(zsetup_group_kernel, dim3(grid), dim3(kGBlock), 0, st, hdr, zs);
+  hipLaunchKernelGGL(gather_group_kernel<2>, dim3(grid), dim3(kGBlock), 0, st, hdr, g2);
+  hipLaunchKernelGGL(gather_group_kernel<3>, dim3(grid), dim3(kGBlock),
+                     (size_t)max_nsino * sizeof(float2), st, hdr, g3);
   return hipGetLastError();
 }
 

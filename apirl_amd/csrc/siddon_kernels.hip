@@ -141,21 +141,14 @@ __device__ __forceinline__ void ray_coords(const Geo& g, long idx, int nang_sub,
   a = subset + nsubsets * k;
 }
 
+// One ray of a (subset) projection: lidx is the ray's position in the z-sorted
+// order of the subset's rays.  Returns this lane's likelihood term (fused mode).
 template <int PREC, bool FUSED, int DIMS>
-__global__ __launch_bounds__(kBlock) void project_kernel(
-    Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
+__device__ __forceinline__ double project_ray(
+    const Geo& g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
     const float* __restrict__ img, float* __restrict__ sino, int nang_sub, int subset,
-    int nsubsets, long nrays, FusedRatio fr, const int* __restrict__ zorder,
-    const float* __restrict__ imgT) {
-  // Rays are taken in order of their sinogram's mean axial coordinate (zorder) and
-  // each XCD gets one contiguous share of that order: workgroups are dispatched to the
-  // 8 XCDs round-robin, so block b runs on XCD b % 8, and the remap below gives XCD x
-  // the x-th eighth of the z-sorted rays.  Its 4 MB L2 then serves one axial band of
-  // the image instead of all of it.
-  const unsigned nb = gridDim.x, b = blockIdx.x;
-  const unsigned xcd = b % kNumXcd, per = nb / kNumXcd, rem = nb % kNumXcd;
-  const unsigned lb = xcd * per + (xcd < rem ? xcd : rem) + b / kNumXcd;
-  const long lidx = (long)lb * kBlock + threadIdx.x;
+    int nsubsets, long nrays, const FusedRatio& fr, const int* __restrict__ zorder,
+    const float* __restrict__ imgT, long lidx) {
   double lik = 0.0;
   long idx = 0;
   if (lidx < nrays) {
@@ -305,6 +298,26 @@ __global__ __launch_bounds__(kBlock) void project_kernel(
       fr.ratio[idx] = (y != 0.f && est != 0.f) ? y / est : 0.f;
     }
   }
+  return lik;
+}
+
+template <int PREC, bool FUSED, int DIMS>
+__global__ __launch_bounds__(kBlock) void project_kernel(
+    Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
+    const float* __restrict__ img, float* __restrict__ sino, int nang_sub, int subset,
+    int nsubsets, long nrays, FusedRatio fr, const int* __restrict__ zorder,
+    const float* __restrict__ imgT) {
+  // Rays are taken in order of their sinogram's mean axial coordinate (zorder) and
+  // each XCD gets one contiguous share of that order: workgroups are dispatched to the
+  // 8 XCDs round-robin, so block b runs on XCD b % 8, and the remap below gives XCD x
+  // the x-th eighth of the z-sorted rays.  Its 4 MB L2 then serves one axial band of
+  // the image instead of all of it.
+  const unsigned nb = gridDim.x, b = blockIdx.x;
+  const unsigned xcd = b % kNumXcd, per = nb / kNumXcd, rem = nb % kNumXcd;
+  const unsigned lb = xcd * per + (xcd < rem ? xcd : rem) + b / kNumXcd;
+  const long lidx = (long)lb * kBlock + threadIdx.x;
+  const double lik = project_ray<PREC, FUSED, DIMS>(g, rays, zab, img, sino, nang_sub, subset,
+                                                    nsubsets, nrays, fr, zorder, imgT, lidx);
   if (FUSED && fr.lik_partials) {
     // deterministic block reduction of the likelihood partials (fixed tree)
     __shared__ double red[kBlock];
@@ -538,6 +551,64 @@ __global__ __launch_bounds__(kBlock) void transpose_xy_kernel(const float* __res
   }
 }
 
+// ---- grouped (multi-job) launches --------------------------------------------
+static_assert(kGroupBlock == kBlock, "grouped launches use the projector's block size");
+
+// Grouped-launch helpers (also used by gather_kernels.hip through group_common.h).
+#include "group_common.h"
+
+__global__ __launch_bounds__(kBlock) void clear_group_kernel(const BatchHeader* __restrict__ hdr,
+                                                             const ClearJob* __restrict__ jobs) {
+  const int n = hdr->count[PH_CLEAR];
+  long lo, hi;
+  vb_range(hdr->total[PH_CLEAR], lo, hi);
+  if (lo >= hi) return;
+  int jj = find_job(jobs, n, lo);
+  for (long vb = lo; vb < hi; vb++) {
+    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
+    const long i = (vb - jobs[jj].first) * kBlock + threadIdx.x;
+    if (i < jobs[jj].n) jobs[jj].p[i] = 0.f;
+  }
+}
+
+template <int DIMS>
+__global__ __launch_bounds__(kBlock) void project_group_kernel(const BatchHeader* __restrict__ hdr,
+                                                               const ProjJob* __restrict__ jobs) {
+  const int ph = DIMS == 2 ? PH_PROJ2 : PH_PROJ3;
+  const int n = hdr->count[ph];
+  long lo, hi;
+  vb_range(hdr->total[ph], lo, hi);
+  if (lo >= hi) return;
+  int jj = find_job(jobs, n, lo);
+  for (long vb = lo; vb < hi; vb++) {
+    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
+    const ProjJob& J = jobs[jj];
+    const long lidx = (vb - J.first) * kBlock + threadIdx.x;
+    (void)project_ray<APIRL_PREC_REFERENCE, true, DIMS>(J.g, J.rays, J.zab, J.img, nullptr,
+                                                        J.nang_sub, J.subset, J.nsubsets,
+                                                        J.nrays, J.fr, J.zorder, nullptr, lidx);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void update_group_kernel(const BatchHeader* __restrict__ hdr,
+                                                              const UpdJob* __restrict__ jobs) {
+  const int n = hdr->count[PH_UPDATE];
+  long lo, hi;
+  vb_range(hdr->total[PH_UPDATE], lo, hi);
+  if (lo >= hi) return;
+  int jj = find_job(jobs, n, lo);
+  for (long vb = lo; vb < hi; vb++) {
+    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
+    const UpdJob& J = jobs[jj];
+    const long i = (vb - J.first) * kBlock + threadIdx.x;
+    if (i < J.n) {
+      const float sv = J.s[i];
+      const bool ok = J.strict ? (sv > J.thr) : (sv >= J.thr);
+      J.x[i] = ok ? J.x[i] * J.b[i] / sv : 0.f;
+    }
+  }
+}
+
 inline unsigned grid_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -666,6 +737,25 @@ hipError_t launch_likelihood(const float* x, const float* y, long n, int positiv
 hipError_t launch_min_nz_max(const float* x, size_t n, float* d_out2, hipStream_t st) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(min_nz_max_kernel, dim3(1), dim3(kBlock), 0, st, x, (long)n, d_out2);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_clear(const BatchHeader* hdr, const ClearJob* jobs, int grid,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(clear_group_kernel, dim3(grid), dim3(kBlock), 0, st, hdr, jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_project(const BatchHeader* hdr, const ProjJob* jobs2,
+                                const ProjJob* jobs3, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(project_group_kernel<2>, dim3(grid), dim3(kBlock), 0, st, hdr, jobs2);
+  hipLaunchKernelGGL(project_group_kernel<3>, dim3(grid), dim3(kBlock), 0, st, hdr, jobs3);
+  return hipGetLastError();
+}
+
+hipError_t launch_group_update(const BatchHeader* hdr, const UpdJob* jobs, int grid,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(update_group_kernel, dim3(grid), dim3(kBlock), 0, st, hdr, jobs);
   return hipGetLastError();
 }
 

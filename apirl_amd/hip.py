@@ -45,6 +45,12 @@ class Tuning(ctypes.Structure):
     _fields_ = [("gather_zlanes", _c_int)]
 
 
+class JobDesc(ctypes.Structure):
+    _fields_ = [("plan", _P), ("iterations", _c_int), ("subsets", _c_int), ("osem_rules", _c_int),
+                ("d_meas", _P), ("d_init", _P), ("d_additive", _P), ("d_mult", _P),
+                ("d_out", _P)]
+
+
 # Every symbol the header declares, with its ctypes signature.
 SIGNATURES = {
     "apirl_ctx_create": (_c_int, [_c_int, ctypes.POINTER(_P)]),
@@ -76,6 +82,16 @@ SIGNATURES = {
     "apirl_mlem_threshold": (_c_int, [_P, _P, _c_size, ctypes.POINTER(_c_float)]),
     "apirl_em_reconstruct": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P, _P,
                                       _P]),
+    "apirl_batch_create": (_c_int, [_P, _c_int, _c_int, ctypes.POINTER(_P)]),
+    "apirl_batch_destroy": (_c_int, [_P]),
+    "apirl_batch_last_error": (ctypes.c_char_p, [_P]),
+    "apirl_batch_job_bytes": (_c_size, [_P, _c_int]),
+    "apirl_batch_set_launch": (_c_int, [_P, _c_int, _c_int]),
+    "apirl_batch_submit": (_c_int, [_P, ctypes.POINTER(JobDesc), ctypes.POINTER(_c_int)]),
+    "apirl_batch_step": (_c_int, [_P, ctypes.POINTER(_c_int)]),
+    "apirl_batch_job_done": (_c_int, [_P, _c_int, ctypes.POINTER(_c_int)]),
+    "apirl_batch_synchronize": (_c_int, [_P]),
+    "apirl_batch_release": (_c_int, [_P, _c_int]),
     "apirl_sino_grid": (None, [_c_int, _c_int, _c_float, _P, _P]),
     "apirl_ring_table": (_c_int, [_c_int, _c_float, _c_int, _P, _P, _P, _P, _P]),
 }
@@ -351,6 +367,83 @@ class Plan:
         if self.h:
             self.ctx.lib.apirl_plan_destroy(self.h)
             self.h = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Continuous batching of OSEM / MLEM jobs (apirl_batch_*): submit jobs at any
+    time, call step() to run one sub-iteration of every active job, poll done()."""
+
+    def __init__(self, ctx, max_jobs=64, max_sinograms=1024):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        self.h = _P()
+        ctx.check(self.lib.apirl_batch_create(ctx.h, max_jobs, max_sinograms,
+                                              ctypes.byref(self.h)), "apirl_batch_create")
+        self._keep = {}
+
+    def _check(self, st, what):
+        if st != APIRL_OK:
+            msg = self.lib.apirl_batch_last_error(self.h).decode()
+            raise ApirlError(f"{what} failed (status {st}): {msg}")
+
+    @staticmethod
+    def job_bytes(plan, subsets):
+        return plan.ctx.lib.apirl_batch_job_bytes(plan.h, subsets)
+
+    def set_launch(self, grid=0, use_graph=True):
+        self._check(self.lib.apirl_batch_set_launch(self.h, grid, 1 if use_graph else 0),
+                    "apirl_batch_set_launch")
+
+    def submit(self, plan, meas, init, out, iterations, subsets=1, osem_rules=None,
+               additive=None, mult=None):
+        if osem_rules is None:
+            osem_rules = 1 if subsets > 1 else 0
+        plan._check_img(init)
+        plan._check_img(out)
+        plan._check_sino(meas, 0, 1)
+        d = JobDesc(plan.h, iterations, subsets, osem_rules, _dptr(meas), _dptr(init),
+                    _dptr(additive), _dptr(mult), _dptr(out))
+        jid = _c_int(-1)
+        self._check(self.lib.apirl_batch_submit(self.h, ctypes.byref(d), ctypes.byref(jid)),
+                    "apirl_batch_submit")
+        self._keep[jid.value] = (plan, meas, init, out, additive, mult)
+        return jid.value
+
+    def step(self):
+        n = _c_int(0)
+        self._check(self.lib.apirl_batch_step(self.h, ctypes.byref(n)), "apirl_batch_step")
+        return n.value
+
+    def done(self, job_id):
+        d = _c_int(0)
+        self._check(self.lib.apirl_batch_job_done(self.h, job_id, ctypes.byref(d)),
+                    "apirl_batch_job_done")
+        return bool(d.value)
+
+    def release(self, job_id):
+        self._check(self.lib.apirl_batch_release(self.h, job_id), "apirl_batch_release")
+        self._keep.pop(job_id, None)
+
+    def synchronize(self):
+        self._check(self.lib.apirl_batch_synchronize(self.h), "apirl_batch_synchronize")
+
+    def run(self):
+        """Steps until every submitted job has been scheduled, then waits."""
+        while self.step() > 0:
+            pass
+        self.synchronize()
+
+    def close(self):
+        if self.h:
+            self.lib.apirl_batch_destroy(self.h)
+            self.h = _P()
+            self._keep.clear()
 
     def __del__(self):
         try:

@@ -186,6 +186,47 @@ apirl_status apirl_em_reconstruct(apirl_plan* plan, const apirl_em_options* opt,
                                   const float* d_additive, const float* d_mult, float* d_img,
                                   double* h_lik, float* d_sens_out);
 
+/* ---- Batched jobs (continuous batching) ------------------------------------------
+ * No counterpart in the reference, whose EM drivers run one job at a time
+ * (OsemSinogram3d.cpp:36-325, Mlem2d.cpp:60-260).  A batch advances many OSEM / MLEM
+ * jobs of different geometries together: each apirl_batch_step enqueues one round,
+ * one sub-iteration of every active job, as seven grouped launches (clear, fused
+ * projection + ratio, axial setup, gather backprojection, update) replayed from a
+ * captured hipGraph.  Jobs may be submitted between steps while others are running.
+ * A job's image has the same bits as apirl_em_reconstruct with APIRL_BP_GATHER and
+ * APIRL_PREC_REFERENCE gives it alone; the likelihood is not evaluated.  The batch
+ * runs on its own stream; apirl_batch_submit orders the job after the work already
+ * enqueued on the context's stream, and the image is final once apirl_batch_job_done
+ * reports it. */
+typedef struct apirl_batch apirl_batch;
+typedef struct apirl_job_desc {
+  apirl_plan* plan;
+  int iterations, subsets;
+  int osem_rules;           /* 1: OSEM update rules; 0: MLEM thresholds per subset */
+  const float* d_meas;      /* full sinogram */
+  const float* d_init;      /* initial image */
+  const float* d_additive;  /* additive sinogram (full) or NULL */
+  const float* d_mult;      /* multiplicative sinogram (sensitivity only) or NULL */
+  float* d_out;             /* image, updated in place every sub-iteration */
+} apirl_job_desc;
+/* max_sinograms bounds the 3D jobs' sinogram count (LDS of the gather launch). */
+apirl_status apirl_batch_create(apirl_ctx* ctx, int max_jobs, int max_sinograms,
+                                apirl_batch** out);
+apirl_status apirl_batch_destroy(apirl_batch* batch);
+const char* apirl_batch_last_error(const apirl_batch* batch);
+/* Device memory a job of this plan holds while admitted (sensitivities, ratio,
+ * backprojection, gather records), for admission by HBM footprint. */
+size_t apirl_batch_job_bytes(const apirl_plan* plan, int subsets);
+/* Workgroups per grouped launch (0 keeps the current value) and graph replay on/off. */
+apirl_status apirl_batch_set_launch(apirl_batch* batch, int grid, int use_graph);
+apirl_status apirl_batch_submit(apirl_batch* batch, const apirl_job_desc* job, int* job_id);
+/* Enqueues one round; *active = jobs still to be scheduled after it. */
+apirl_status apirl_batch_step(apirl_batch* batch, int* active);
+apirl_status apirl_batch_job_done(apirl_batch* batch, int job_id, int* done);
+apirl_status apirl_batch_synchronize(apirl_batch* batch);
+/* Frees a finished job's device state for reuse by later jobs. */
+apirl_status apirl_batch_release(apirl_batch* batch, int job_id);
+
 /* ---- host helpers (no device work) --------------------------------------------- */
 /* Reference grids (Sinogram2DinCylindrical3Dpet.cpp:46-60). */
 void apirl_sino_grid(int nang, int nr, float rfov_mm, float* ang_deg, float* r_mm);

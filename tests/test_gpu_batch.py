@@ -1,0 +1,124 @@
+"""Batched jobs (apirl_batch_*): every job's image equals, bit for bit, the image the
+single-job driver (apirl_em_reconstruct, gather backprojector) gives it alone --
+whether the jobs start together, join while others run (continuous batching), or
+run with the round graph replayed or launched directly.  NaN voxels (the top
+slice, never visited by any LOR) must coincide."""
+import numpy as np
+import pytest
+
+from apirl_amd import hip, phantoms
+from test_gpu_parity import ctx, dev, host, log, torch  # noqa: F401
+
+pytestmark = pytest.mark.gpu
+
+GEOMS = {
+    "2d_small": dict(dims=2, nx=64, dx=4.0, nproj=48, nr=64, rfov=128.0, rscanner=400.0),
+    "2d_c1": dict(dims=2, nx=128, dx=2.0, nproj=192, nr=192, rfov=128.0, rscanner=400.0),
+    "3d_mid": dict(dims=3, nx=64, ny=64, nz=17, dx=2.0445, dz=2.027, nproj=42, nr=86, rings=13,
+                   rfov=65.424, zfov=34.459, rscanner=400.0, span=5, mrd=8),
+    "3d_tiny": dict(dims=3, nx=32, ny=32, nz=9, dx=4.0, dz=4.0, nproj=24, nr=40, rings=5,
+                    rfov=64.0, zfov=36.0, rscanner=300.0, span=3, mrd=2),
+}
+
+
+def make_plan(ctx, g):
+    if g["dims"] == 2:
+        return hip.Plan.for_2d(ctx, g["nx"], g["dx"], g["nproj"], g["nr"], g["rfov"],
+                               g["rscanner"])
+    return hip.Plan.for_3d(ctx, g["nx"], g["ny"], g["nz"], g["dx"], g["dz"], g["nproj"], g["nr"],
+                           g["rfov"], g["rscanner"], g["rings"], g["zfov"], g["span"], g["mrd"])
+
+
+def make_job(torch, plan, g, seed, iterations, subsets):
+    rng = np.random.default_rng(seed)
+    nz = 1 if g["dims"] == 2 else g["nz"]
+    img = phantoms.ellipsoids3d(g["nx"], nz, g["dx"], g.get("dz", 1.0), rng)
+    truth = plan.project(dev(torch, img.reshape(plan.image_shape())))
+    t = host(truth)
+    meas = phantoms.poisson_counts(t, float(10 ** rng.uniform(5, 7)), rng).reshape(t.shape)
+    init = torch.ones(plan.image_shape(), dtype=torch.float32, device="cuda")
+    return dict(meas=dev(torch, meas), init=init, iterations=iterations, subsets=subsets)
+
+
+def alone(plan, job):
+    img, _ = plan.reconstruct(job["meas"], job["init"], job["iterations"],
+                              subsets=job["subsets"], bp_mode=hip.BP_GATHER)
+    return host(img)
+
+
+def same_bits(a, b):
+    a = np.ascontiguousarray(a, np.float32).ravel()
+    b = np.ascontiguousarray(b, np.float32).ravel()
+    assert np.array_equal(np.isnan(a), np.isnan(b))
+    m = ~np.isnan(a)
+    d = np.flatnonzero(a[m].view(np.int32) != b[m].view(np.int32))
+    assert d.size == 0, f"{d.size} voxels differ"
+
+
+@pytest.fixture(scope="module")
+def mix(torch, ctx):
+    plans = {k: make_plan(ctx, g) for k, g in GEOMS.items()}
+    spec = [("2d_small", 1, 4, 1), ("2d_c1", 2, 3, 1), ("2d_small", 3, 2, 6), ("3d_mid", 4, 2, 7),
+            ("3d_tiny", 5, 3, 4), ("3d_mid", 6, 1, 3), ("2d_c1", 7, 2, 8), ("3d_tiny", 8, 2, 1)]
+    jobs = []
+    for name, seed, it, sub in spec:
+        jobs.append((name, make_job(torch, plans[name], GEOMS[name], seed, it, sub)))
+    ref = [alone(plans[n], j) for n, j in jobs]
+    torch.cuda.synchronize()
+    return plans, jobs, ref
+
+
+def run_batch(torch, ctx, plans, jobs, late=(), use_graph=True, grid=0):
+    b = hip.Batch(ctx, max_jobs=16, max_sinograms=1024)
+    b.set_launch(grid, use_graph)
+    outs, ids = {}, {}
+    for k, (name, j) in enumerate(jobs):
+        if k in late:
+            continue
+        outs[k] = torch.empty(plans[name].image_shape(), dtype=torch.float32, device="cuda")
+        ids[k] = b.submit(plans[name], j["meas"], j["init"], outs[k], j["iterations"], j["subsets"])
+    rounds = 0
+    while True:
+        n = b.step()
+        rounds += 1
+        if rounds == 3:
+            for k in late:  # admitted while the others are running
+                name, j = jobs[k]
+                outs[k] = torch.empty(plans[name].image_shape(), dtype=torch.float32,
+                                      device="cuda")
+                ids[k] = b.submit(plans[name], j["meas"], j["init"], outs[k], j["iterations"],
+                                  j["subsets"])
+            continue
+        if n == 0:
+            break
+    b.synchronize()
+    assert all(b.done(ids[k]) for k in ids)
+    res = {k: host(outs[k]) for k in outs}
+    for k in ids:
+        b.release(ids[k])
+    b.close()
+    return res
+
+
+def test_batch_equals_each_job_alone(torch, ctx, mix):
+    log("case: batched mix of 8 jobs (2D and 3D, 1-8 subsets) vs each alone")
+    plans, jobs, ref = mix
+    res = run_batch(torch, ctx, plans, jobs)
+    for k in range(len(jobs)):
+        same_bits(res[k], ref[k])
+
+
+def test_batch_admission_while_running(torch, ctx, mix):
+    log("case: jobs admitted after 3 rounds of the others")
+    plans, jobs, ref = mix
+    res = run_batch(torch, ctx, plans, jobs, late=(2, 5, 6))
+    for k in range(len(jobs)):
+        same_bits(res[k], ref[k])
+
+
+def test_batch_graph_and_direct_launches_agree(torch, ctx, mix):
+    log("case: direct launches, small persistent grid")
+    plans, jobs, ref = mix
+    res = run_batch(torch, ctx, plans, jobs, use_graph=False, grid=96)
+    for k in range(len(jobs)):
+        same_bits(res[k], ref[k])
