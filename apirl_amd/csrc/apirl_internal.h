@@ -151,9 +151,9 @@ enum { BP_PLAIN = 0, BP_ONES = 1, BP_DIVIDE = 2, BP_MULT = 3 };
 // A round runs one OSEM / MLEM sub-iteration of every active job with one launch
 // per phase; each launch walks a job table in device memory.  Every job owns a
 // contiguous run of "virtual blocks" (first .. next job's first) of that phase;
-// the launches are persistent (fixed grid) and each workgroup takes a contiguous
-// share of the virtual blocks, so the launch shapes never change and one captured
-// graph serves every round.
+// the launches are persistent (fixed grid) and their workgroups take virtual blocks
+// from a per-phase counter (dynamic balance: tiles differ ~10x in cost), so the
+// launch shapes never change and one captured graph serves every round.
 
 // Per-ray axial part of the Siddon setup and the bin value of one subset bin, used
 // by the gather backprojector (one 32-byte load per candidate ray).
@@ -186,8 +186,9 @@ struct GatherArgs {
 };
 
 struct BatchHeader {
-  long total[8];  // virtual blocks of each phase (BatchPhase)
-  int count[8];   // jobs in each phase's table
+  long total[8];                // virtual blocks of each phase (BatchPhase)
+  int count[8];                 // jobs in each phase's table
+  unsigned long long next[8];   // work counters (uploaded as 0 with every round's table)
 };
 enum BatchPhase { PH_CLEAR = 0, PH_PROJ2, PH_PROJ3, PH_ZSETUP, PH_GATHER2, PH_GATHER3, PH_UPDATE };
 
@@ -238,13 +239,13 @@ inline long gather_tile_blocks(const Geo& g) {
   return (long)((g.nx + 15) / 16) * ((g.ny + 15) / 16) * g.nz;
 }
 
-hipError_t launch_group_clear(const BatchHeader* hdr, const ClearJob* jobs, int grid,
+hipError_t launch_group_clear(BatchHeader* hdr, const ClearJob* jobs, int grid,
                               hipStream_t st);
-hipError_t launch_group_project(const BatchHeader* hdr, const ProjJob* jobs2,
+hipError_t launch_group_project(BatchHeader* hdr, const ProjJob* jobs2,
                                 const ProjJob* jobs3, int grid, hipStream_t st);
-hipError_t launch_group_update(const BatchHeader* hdr, const UpdJob* jobs, int grid,
+hipError_t launch_group_update(BatchHeader* hdr, const UpdJob* jobs, int grid,
                                hipStream_t st);
-hipError_t launch_group_gather(const BatchHeader* hdr, const ZsJob* zs, const GJob* g2,
+hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2,
                                const GJob* g3, int grid, int max_nsino, hipStream_t st);
 void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, const float* sino,
                      float* img, void* scratch, GatherArgs& A, ZRay*& zr, int*& nz_pair);

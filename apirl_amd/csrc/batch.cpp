@@ -112,7 +112,7 @@ size_t max_gather_scratch(const apirl_plan* p, int S) {
 }
 
 hipError_t enqueue_round(apirl_batch* b) {
-  const BatchHeader* hdr = (const BatchHeader*)b->d_tab;
+  BatchHeader* hdr = (BatchHeader*)b->d_tab;
   hipError_t e;
   if ((e = launch_group_clear(hdr, (const ClearJob*)(b->d_tab + b->off_clear), b->grid,
                               b->stream)) != hipSuccess)

@@ -423,54 +423,55 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
 // ---- grouped (multi-job) launches --------------------------------------------
 #include "group_common.h"
 
-__global__ __launch_bounds__(kGBlock) void zsetup_group_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(kGBlock) void zsetup_group_kernel(BatchHeader* __restrict__ hdr,
                                                                const ZsJob* __restrict__ jobs) {
   const int n = hdr->count[PH_ZSETUP];
-  long lo, hi;
-  vb_range(hdr->total[PH_ZSETUP], lo, hi);
-  if (lo >= hi) return;
-  int jj = find_job(jobs, n, lo);
-  for (long vb = lo; vb < hi; vb++) {
-    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
-    const ZsJob& J = jobs[jj];
-    const long idx = (vb - J.first) * kGBlock + threadIdx.x;
-    if (idx < J.nrays)
-      zsetup_one(J.g, J.rays, J.zab, J.sino, J.kind, J.nang_sub, J.subset, J.nsubsets, J.out,
-                 J.nz_pair, idx);
+  const long total = hdr->total[PH_ZSETUP];
+  for (;;) {
+    const long v0 = grab_vb(&hdr->next[PH_ZSETUP]);
+    if (v0 >= total) break;
+    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+      const ZsJob& J = jobs[find_job(jobs, n, vb)];
+      const long idx = (vb - J.first) * kGBlock + threadIdx.x;
+      if (idx < J.nrays)
+        zsetup_one(J.g, J.rays, J.zab, J.sino, J.kind, J.nang_sub, J.subset, J.nsubsets, J.out,
+                   J.nz_pair, idx);
+    }
   }
 }
 
 // Each virtual block is one 16 x 16 tile of one slice of one job (tile-major within
 // a slice, slices in order), as gather_kernel's grid; 3D jobs reload the axial table
-// into LDS when the workgroup moves to the next job.
+// into LDS when the workgroup moves to another job.
 template <int DIMS>
-__global__ __launch_bounds__(kGBlock) void gather_group_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __restrict__ hdr,
                                                                const GJob* __restrict__ jobs) {
   extern __shared__ float2 s_zab[];
   const int ph = DIMS == 2 ? PH_GATHER2 : PH_GATHER3;
   const int n = hdr->count[ph];
-  long lo, hi;
-  vb_range(hdr->total[ph], lo, hi);
-  if (lo >= hi) return;
-  int jj = find_job(jobs, n, lo);
+  const long total = hdr->total[ph];
   int loaded = -1;
-  for (long vb = lo; vb < hi; vb++) {
-    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
-    const GJob& J = jobs[jj];
-    if (DIMS == 3 && loaded != jj) {
-      __syncthreads();
-      for (int q = threadIdx.x; q < J.A.g.nsino; q += kGBlock) s_zab[q] = J.A.zab[q];
-      __syncthreads();
-      loaded = jj;
+  for (;;) {
+    const long v0 = grab_vb(&hdr->next[ph]);
+    if (v0 >= total) break;
+    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+      const int jj = find_job(jobs, n, vb);
+      const GJob& J = jobs[jj];
+      if (DIMS == 3 && loaded != jj) {
+        __syncthreads();
+        for (int q = threadIdx.x; q < J.A.g.nsino; q += kGBlock) s_zab[q] = J.A.zab[q];
+        __syncthreads();
+        loaded = jj;
+      }
+      const long lb = vb - J.first;
+      const int bx = (int)(lb % J.ntx);
+      const int by = (int)((lb / J.ntx) % J.nty);
+      const int k = (int)(lb / ((long)J.ntx * J.nty));
+      const int i = bx * kTile + (threadIdx.x % kTile);
+      const int j = by * kTile + (threadIdx.x / kTile);
+      if (i < J.A.g.nx && j < J.A.g.ny)
+        gather_voxel<DIMS, APIRL_PREC_REFERENCE>(J.A, s_zab, i, j, k);
     }
-    const long lb = vb - J.first;
-    const int bx = (int)(lb % J.ntx);
-    const int by = (int)((lb / J.ntx) % J.nty);
-    const int k = (int)(lb / ((long)J.ntx * J.nty));
-    const int i = bx * kTile + (threadIdx.x % kTile);
-    const int j = by * kTile + (threadIdx.x / kTile);
-    if (i < J.A.g.nx && j < J.A.g.ny)
-      gather_voxel<DIMS, APIRL_PREC_REFERENCE>(J.A, s_zab, i, j, k);
   }
 }
 
@@ -579,7 +580,7 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   A.nz_pair = nz_pair;
 }
 
-hipError_t launch_group_gather(const BatchHeader* hdr, const ZsJob* zs, const GJob* g2,
+hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2,
                                const GJob* g3, int grid, int max_nsino, hipStream_t st) {
   hipLaunchKernelGGL(zsetup_group_kernel, dim3(grid), dim3(kGBlock), 0, st, hdr, zs);
   hipLaunchKernelGGL(gather_group_kernel<2>, dim3(grid), dim3(kGBlock), 0, st, hdr, g2);

@@ -557,54 +557,55 @@ static_assert(kGroupBlock == kBlock, "grouped launches use the projector's block
 // Grouped-launch helpers (also used by gather_kernels.hip through group_common.h).
 #include "group_common.h"
 
-__global__ __launch_bounds__(kBlock) void clear_group_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(kBlock) void clear_group_kernel(BatchHeader* __restrict__ hdr,
                                                              const ClearJob* __restrict__ jobs) {
   const int n = hdr->count[PH_CLEAR];
-  long lo, hi;
-  vb_range(hdr->total[PH_CLEAR], lo, hi);
-  if (lo >= hi) return;
-  int jj = find_job(jobs, n, lo);
-  for (long vb = lo; vb < hi; vb++) {
-    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
-    const long i = (vb - jobs[jj].first) * kBlock + threadIdx.x;
-    if (i < jobs[jj].n) jobs[jj].p[i] = 0.f;
+  const long total = hdr->total[PH_CLEAR];
+  for (;;) {
+    const long v0 = grab_vb(&hdr->next[PH_CLEAR]);
+    if (v0 >= total) break;
+    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+      const ClearJob& J = jobs[find_job(jobs, n, vb)];
+      const long i = (vb - J.first) * kBlock + threadIdx.x;
+      if (i < J.n) J.p[i] = 0.f;
+    }
   }
 }
 
 template <int DIMS>
-__global__ __launch_bounds__(kBlock) void project_group_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(kBlock) void project_group_kernel(BatchHeader* __restrict__ hdr,
                                                                const ProjJob* __restrict__ jobs) {
   const int ph = DIMS == 2 ? PH_PROJ2 : PH_PROJ3;
   const int n = hdr->count[ph];
-  long lo, hi;
-  vb_range(hdr->total[ph], lo, hi);
-  if (lo >= hi) return;
-  int jj = find_job(jobs, n, lo);
-  for (long vb = lo; vb < hi; vb++) {
-    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
-    const ProjJob& J = jobs[jj];
-    const long lidx = (vb - J.first) * kBlock + threadIdx.x;
-    (void)project_ray<APIRL_PREC_REFERENCE, true, DIMS>(J.g, J.rays, J.zab, J.img, nullptr,
-                                                        J.nang_sub, J.subset, J.nsubsets,
-                                                        J.nrays, J.fr, J.zorder, nullptr, lidx);
+  const long total = hdr->total[ph];
+  for (;;) {
+    const long v0 = grab_vb(&hdr->next[ph]);
+    if (v0 >= total) break;
+    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+      const ProjJob& J = jobs[find_job(jobs, n, vb)];
+      const long lidx = (vb - J.first) * kBlock + threadIdx.x;
+      (void)project_ray<APIRL_PREC_REFERENCE, true, DIMS>(J.g, J.rays, J.zab, J.img, nullptr,
+                                                          J.nang_sub, J.subset, J.nsubsets,
+                                                          J.nrays, J.fr, J.zorder, nullptr, lidx);
+    }
   }
 }
 
-__global__ __launch_bounds__(kBlock) void update_group_kernel(const BatchHeader* __restrict__ hdr,
+__global__ __launch_bounds__(kBlock) void update_group_kernel(BatchHeader* __restrict__ hdr,
                                                               const UpdJob* __restrict__ jobs) {
   const int n = hdr->count[PH_UPDATE];
-  long lo, hi;
-  vb_range(hdr->total[PH_UPDATE], lo, hi);
-  if (lo >= hi) return;
-  int jj = find_job(jobs, n, lo);
-  for (long vb = lo; vb < hi; vb++) {
-    while (jj + 1 < n && jobs[jj + 1].first <= vb) jj++;
-    const UpdJob& J = jobs[jj];
-    const long i = (vb - J.first) * kBlock + threadIdx.x;
-    if (i < J.n) {
-      const float sv = J.s[i];
-      const bool ok = J.strict ? (sv > J.thr) : (sv >= J.thr);
-      J.x[i] = ok ? J.x[i] * J.b[i] / sv : 0.f;
+  const long total = hdr->total[PH_UPDATE];
+  for (;;) {
+    const long v0 = grab_vb(&hdr->next[PH_UPDATE]);
+    if (v0 >= total) break;
+    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+      const UpdJob& J = jobs[find_job(jobs, n, vb)];
+      const long i = (vb - J.first) * kBlock + threadIdx.x;
+      if (i < J.n) {
+        const float sv = J.s[i];
+        const bool ok = J.strict ? (sv > J.thr) : (sv >= J.thr);
+        J.x[i] = ok ? J.x[i] * J.b[i] / sv : 0.f;
+      }
     }
   }
 }
@@ -740,20 +741,20 @@ hipError_t launch_min_nz_max(const float* x, size_t n, float* d_out2, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_group_clear(const BatchHeader* hdr, const ClearJob* jobs, int grid,
+hipError_t launch_group_clear(BatchHeader* hdr, const ClearJob* jobs, int grid,
                               hipStream_t st) {
   hipLaunchKernelGGL(clear_group_kernel, dim3(grid), dim3(kBlock), 0, st, hdr, jobs);
   return hipGetLastError();
 }
 
-hipError_t launch_group_project(const BatchHeader* hdr, const ProjJob* jobs2,
+hipError_t launch_group_project(BatchHeader* hdr, const ProjJob* jobs2,
                                 const ProjJob* jobs3, int grid, hipStream_t st) {
   hipLaunchKernelGGL(project_group_kernel<2>, dim3(grid), dim3(kBlock), 0, st, hdr, jobs2);
   hipLaunchKernelGGL(project_group_kernel<3>, dim3(grid), dim3(kBlock), 0, st, hdr, jobs3);
   return hipGetLastError();
 }
 
-hipError_t launch_group_update(const BatchHeader* hdr, const UpdJob* jobs, int grid,
+hipError_t launch_group_update(BatchHeader* hdr, const UpdJob* jobs, int grid,
                                hipStream_t st) {
   hipLaunchKernelGGL(update_group_kernel, dim3(grid), dim3(kBlock), 0, st, hdr, jobs);
   return hipGetLastError();

@@ -1,0 +1,139 @@
+"""Batched throughput and a serving loop over apirl_batch (continuous batching).
+
+* run_sequential: the plain single-job path, one apirl_em_reconstruct per job.
+* run_batched: all jobs submitted to one batch (up to its capacity, later ones as
+  slots free up), rounds stepped until every job is done.
+* autotune: picks the batch's persistent grid (workgroups per grouped launch) and
+  graph replay on or off by timing real rounds of the batch it tunes -- results do
+  not depend on either, so the rounds spent tuning are not wasted.
+* serve: open-loop Poisson arrivals; a job is admitted when the batch has a slot and
+  its device state (apirl_batch_job_bytes) fits the HBM budget, joins the next
+  round, and leaves when done; latency = completion - arrival."""
+import time
+
+import numpy as np
+
+from . import hip, workloads
+
+
+def _sync(torch):
+    torch.cuda.synchronize()
+
+
+def run_sequential(ctx, plans, jobs, torch):
+    t0 = time.perf_counter()
+    for spec, d in jobs:
+        plans[spec["kind"]].reconstruct(d["meas"], d["init"], spec["iterations"],
+                                        subsets=spec["subsets"], bp_mode=hip.BP_GATHER,
+                                        mult=d["mult"], out=d["out"])
+    _sync(torch)
+    return time.perf_counter() - t0
+
+
+def autotune(batch, grids=(512, 1024, 2048, 4096), rounds=2):
+    """Times `rounds` rounds per candidate on the live batch; returns the choice."""
+    best = None
+    for use_graph in (True, False):
+        for grid in grids:
+            batch.set_launch(grid, use_graph)
+            batch.synchronize()
+            t0 = time.perf_counter()
+            active = 1
+            for _ in range(rounds):
+                active = batch.step()
+            batch.synchronize()
+            dt = (time.perf_counter() - t0) / rounds
+            if best is None or dt < best[0]:
+                best = (dt, grid, use_graph)
+            if active == 0:
+                break
+        if active == 0:
+            break
+    batch.set_launch(best[1], best[2])
+    return {"grid": best[1], "graph": best[2], "round_s": best[0]}
+
+
+def run_batched(ctx, plans, jobs, torch, max_jobs=64, tune=True):
+    batch = hip.Batch(ctx, max_jobs=max_jobs, max_sinograms=1024)
+    pending = list(range(len(jobs)))
+    ids = {}
+    t0 = time.perf_counter()
+    choice = None
+
+    def admit():
+        while pending and len(ids) - sum(1 for k in ids if done[k]) < max_jobs:
+            k = pending.pop(0)
+            spec, d = jobs[k]
+            ids[k] = batch.submit(plans[spec["kind"]], d["meas"], d["init"], d["out"],
+                                  spec["iterations"], spec["subsets"], mult=d["mult"])
+            done[k] = False
+
+    done = {}
+    admit()
+    if tune:
+        choice = autotune(batch)
+    while True:
+        active = batch.step()
+        if pending:
+            for k in list(ids):
+                if not done[k] and batch.done(ids[k]):
+                    done[k] = True
+                    batch.release(ids[k])
+            admit()
+        elif active == 0:
+            break
+    batch.synchronize()
+    dt = time.perf_counter() - t0
+    batch.close()
+    return dt, choice
+
+
+def serve(ctx, plans, jobs, rate, torch, max_jobs=64, hbm_budget=200e9, seed=0, grid=0,
+          use_graph=True):
+    """Open-loop arrivals at `rate` jobs/s of `jobs` (specs with materialised inputs).
+    Returns latencies (s) in arrival order and the wall time of the run."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, len(jobs)))
+    batch = hip.Batch(ctx, max_jobs=max_jobs, max_sinograms=1024)
+    batch.set_launch(grid, use_graph)
+    need = [hip.Batch.job_bytes(plans[s["kind"]], s["subsets"]) for s, _ in jobs]
+    if max(need) > hbm_budget:
+        raise ValueError(f"a job needs {max(need)} bytes, above the HBM budget {hbm_budget}")
+    queue, running = [], {}
+    finished = np.full(len(jobs), np.nan)
+    used = 0
+    nxt = 0
+    _sync(torch)
+    t0 = time.perf_counter()
+    while nxt < len(jobs) or queue or running:
+        now = time.perf_counter() - t0
+        while nxt < len(jobs) and arrivals[nxt] <= now:
+            queue.append(nxt)
+            nxt += 1
+        # admission: capacity and HBM footprint, in arrival order
+        while queue and len(running) < max_jobs and used + need[queue[0]] <= hbm_budget:
+            k = queue.pop(0)
+            spec, d = jobs[k]
+            running[k] = batch.submit(plans[spec["kind"]], d["meas"], d["init"], d["out"],
+                                      spec["iterations"], spec["subsets"], mult=d["mult"])
+            used += need[k]
+        if running:
+            batch.step()
+            for k in list(running):
+                if batch.done(running[k]):
+                    finished[k] = time.perf_counter() - t0
+                    batch.release(running.pop(k))
+                    used -= need[k]
+        elif nxt < len(jobs):
+            time.sleep(max(0.0, min(arrivals[nxt] - now, 0.01)))
+    batch.synchronize()
+    wall = time.perf_counter() - t0
+    batch.close()
+    return finished - arrivals, wall
+
+
+def materialise_all(ctx, specs, torch):
+    plans = {k: workloads.make_plan(ctx, k) for k in sorted({s["kind"] for s in specs})}
+    jobs = [(s, workloads.materialise(s, plans[s["kind"]], torch)) for s in specs]
+    _sync(torch)
+    return plans, jobs

@@ -49,6 +49,12 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (for PMC byte calibration)")
+    ap.add_argument("--workload", choices=["osem", "batch", "serve"], default="osem",
+                    help="osem: the single-job 256x256x81 OSEM line (default); batch: "
+                         "64 seeded jobs of uneven size, batched vs one at a time; serve: "
+                         "open-loop arrivals over the batch, p50/p99 latency per rate")
+    ap.add_argument("--jobs", type=int, default=64)
+    ap.add_argument("--rates", default="5,10,20,50,100,200")
     return ap.parse_args()
 
 
@@ -93,12 +99,93 @@ def measured_traffic(entry):
     return None
 
 
+def batch_main(args):
+    """Batched jobs: the 64-job mix of four geometries (and its small-job part) run
+    one job at a time through the single-job driver, then through one batch."""
+    import torch
+
+    from apirl_amd import hip, serving, workloads
+
+    ctx = hip.Context(0)
+    out = {}
+    mixes = (("mix", tuple(workloads.GEOMETRIES)), ("small", workloads.SMALL),
+             ("2d_128", ("2d_128",)))
+    for label, kinds in mixes:
+        specs = [workloads.draw(k, kinds=kinds, attenuation=True) for k in range(args.jobs)]
+        progress(f"{label}: materialising {len(specs)} jobs")
+        plans, jobs = serving.materialise_all(ctx, specs, torch)
+        # warm-up: one small job each way (kernels loaded, pools primed)
+        serving.run_sequential(ctx, plans, jobs[:2], torch)
+        t_seq = serving.run_sequential(ctx, plans, jobs, torch)
+        progress(f"{label}: one at a time {t_seq:.2f} s")
+        ref = [d["out"].clone() for _, d in jobs]
+        t_bat, choice = serving.run_batched(ctx, plans, jobs, torch, max_jobs=args.jobs)
+        progress(f"{label}: batched {t_bat:.2f} s (autotuned {choice})")
+        same = all(torch.equal(torch.nan_to_num(a, nan=-1.0), torch.nan_to_num(d["out"], nan=-1.0))
+                   for a, (_, d) in zip(ref, jobs))
+        kinds_count = {k: sum(1 for s in specs if s["kind"] == k) for k in kinds}
+        out[label] = {"jobs": len(specs), "kinds": kinds_count,
+                      "sequential_jobs_per_s": len(specs) / t_seq,
+                      "batched_jobs_per_s": len(specs) / t_bat,
+                      "speedup": t_seq / t_bat, "autotuned": choice,
+                      "results_bit_identical": bool(same)}
+        del plans, jobs, ref
+        torch.cuda.empty_cache()
+    line = {"metric": "batched OSEM/MLEM jobs/s (64 seeded jobs, 4 geometries, 1-21 subsets, "
+                      "1-10 iterations)",
+            "value": out["mix"]["batched_jobs_per_s"], "unit": "jobs/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded ellipsoids, "
+            "Poisson 1e5-1e8 counts, attenuation sinograms on half the jobs)",
+            "config": {"workload": "batched jobs", "backprojector": "gather"},
+            "vs_single_job_path": out["mix"]["speedup"], "detail": out}
+    print(json.dumps(line), flush=True)
+
+
+def serve_main(args):
+    """Serving: open-loop Poisson arrivals of the job mix, per rate."""
+    import torch
+
+    from apirl_amd import hip, serving, workloads
+
+    ctx = hip.Context(0)
+    rates = [float(r) for r in args.rates.split(",")]
+    res = {}
+    mixes = (("mix", tuple(workloads.GEOMETRIES)), ("small", workloads.SMALL),
+             ("2d_128", ("2d_128",)))
+    for label, kinds in mixes:
+        specs = [workloads.draw(k, kinds=kinds, attenuation=True) for k in range(args.jobs)]
+        progress(f"{label}: materialising {len(specs)} jobs")
+        plans, jobs = serving.materialise_all(ctx, specs, torch)
+        serving.run_sequential(ctx, plans, jobs[:2], torch)
+        res[label] = {}
+        for rate in rates:
+            lat, wall = serving.serve(ctx, plans, jobs, rate, torch, max_jobs=args.jobs)
+            res[label][str(rate)] = {"p50_s": float(np.percentile(lat, 50)),
+                                     "p99_s": float(np.percentile(lat, 99)),
+                                     "jobs_per_s": len(jobs) / wall}
+            progress(f"{label} rate {rate}: p50 {res[label][str(rate)]['p50_s']:.3f} s, "
+                     f"p99 {res[label][str(rate)]['p99_s']:.3f} s, "
+                     f"{res[label][str(rate)]['jobs_per_s']:.2f} jobs/s")
+        del plans, jobs
+        torch.cuda.empty_cache()
+    line = {"metric": "served jobs: p50/p99 latency and jobs/s per arrival rate",
+            "value": res["mix"][str(rates[0])]["jobs_per_s"], "unit": "jobs/s", "n_gpus": 1,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded job mix)",
+            "config": {"workload": "serving, open-loop Poisson arrivals",
+                       "jobs_per_rate": args.jobs}, "detail": res}
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(whole jobs or ray shards per GPU are described in DESIGN.md)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "batch":
+        return batch_main(args)
+    if args.workload == "serve":
+        return serve_main(args)
     import torch
 
     from apirl_amd import hip, phantoms

@@ -122,3 +122,27 @@ def test_batch_graph_and_direct_launches_agree(torch, ctx, mix):
     res = run_batch(torch, ctx, plans, jobs, use_graph=False, grid=96)
     for k in range(len(jobs)):
         same_bits(res[k], ref[k])
+
+
+@pytest.mark.parametrize("budget", ["ample", "one_job"])
+def test_serving_loop_results_equal_alone(torch, ctx, budget):
+    """Open-loop arrivals through serving.serve (admission by HBM footprint, release
+    and reuse of device state): each image equals the job run alone.  With the budget
+    set to one job's footprint the jobs are admitted one after another."""
+    from apirl_amd import serving, workloads
+    log(f"case: serving loop, {budget} HBM budget")
+    specs = [workloads.draw(k, kinds=("2d_128", "3d_small"), attenuation=True, max_iterations=2)
+             for k in range(6)]
+    plans, jobs = serving.materialise_all(ctx, specs, torch)
+    ref = []
+    for spec, d in jobs:
+        img, _ = plans[spec["kind"]].reconstruct(d["meas"], d["init"], spec["iterations"],
+                                                 subsets=spec["subsets"], bp_mode=hip.BP_GATHER,
+                                                 mult=d["mult"])
+        ref.append(host(img))
+    need = max(hip.Batch.job_bytes(plans[s["kind"]], s["subsets"]) for s in specs)
+    lat, wall = serving.serve(ctx, plans, jobs, rate=200.0, torch=torch, max_jobs=8,
+                              hbm_budget=1e12 if budget == "ample" else need)
+    assert np.all(np.isfinite(lat)) and np.all(lat > 0)
+    for k, (_, d) in enumerate(jobs):
+        same_bits(host(d["out"]), ref[k])
