@@ -96,6 +96,8 @@ struct apirl_plan {
   int gather_zlanes = 1;           // 3D gather voxel mapping (1: 16x16 slice tiles; Z>1:
                                    // Z slices x 64/Z columns per wavefront)
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
+  int* d_zsoff = nullptr;          // [nsino + 1] z-crossing table slots (3D gather)
+  int zblock = 0;                  // z-crossing table entries per (angle, radial bin)
   size_t gscratch_bytes = 0;
   std::vector<float> ang, r;       // host copies of the grids
   std::vector<float2> zab;
@@ -140,6 +142,7 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                                      int subset, int nsubsets, int kind, int precision,
                                      void* scratch, size_t scratch_bytes, hipStream_t st);
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
+size_t ztable_bytes(const apirl_plan* p, int nang_sub);
 void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
                          std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,
                          bool& ok);
@@ -165,7 +168,8 @@ struct alignas(32) ZRay {
   float az0;    // first z crossing (FLT_MAX when Vz == 0)
   float L;      // ray length |P2 - P1| in the reference's operand order
   float b;      // bin value being backprojected
-  float pad0, pad1;
+  int zflag;    // 1: the ray's z crossings are not in the crossing table (use alpha_after)
+  int pad;
 };
 static_assert(sizeof(ZRay) == 32, "ZRay is one 32-byte load");
 
@@ -183,6 +187,11 @@ struct GatherArgs {
   int kind;           // BP_PLAIN, BP_ONES or BP_MULT
   int nang_sub, subset, nsubsets;
   int zlanes;         // voxel mapping (see gather_kernel)
+  // z-crossing table: the reference's accumulated z crossings of every subset ray,
+  // ztab[(ka * nr + r) * zblock + zsoff[s] + t] = value after t additions
+  float* ztab;
+  const int* zsoff;   // [nsino + 1] slot offsets of the sinograms within a block
+  int zblock;
 };
 
 struct BatchHeader {
@@ -215,6 +224,9 @@ struct ZsJob {  // axial records of one subset for the gather
   const float* sino;
   ZRay* out;
   int* nz_pair;
+  float* ztab;
+  const int* zsoff;
+  int zblock, pad;
   int kind, nang_sub, subset, nsubsets;
   long nrays;
   long first;
@@ -249,5 +261,9 @@ hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2
                                const GJob* g3, int grid, int max_nsino, hipStream_t st);
 void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, const float* sino,
                      float* img, void* scratch, GatherArgs& A, ZRay*& zr, int*& nz_pair);
+// LDS bytes of the 3D gather launches for nsino sinograms (axial table + slot offsets)
+inline size_t gather_lds_bytes(int nsino) {
+  return (size_t)nsino * sizeof(float2) + (size_t)(nsino + 1) * sizeof(int);
+}
 
 }  // namespace apirl

@@ -403,6 +403,10 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
       z.sino = j->ratio;
       z.out = zr;
       z.nz_pair = nzp;
+      z.ztab = A.ztab;
+      z.zsoff = A.zsoff;
+      z.zblock = A.zblock;
+      z.pad = 0;
       z.kind = BP_PLAIN;
       z.nang_sub = nang_sub;
       z.subset = s;

@@ -280,6 +280,25 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       p->gather_ok = false;
     (void)hipGetLastError();
   }
+  // Slots of the gather's z-crossing table: sinogram s of a (angle, radial bin) block
+  // gets room for every z crossing any of its rays can take, |k1 - k0| <=
+  // |Vz| (amax - amin) / dz + 1 (one more for rounding).
+  if (sd.dims == 3 && p->gather_ok) {
+    float span = 0.f;
+    for (const auto& t : rays) span = std::max(span, t.amax - t.amin);
+    std::vector<int> soff(p->zab.size() + 1, 0);
+    for (size_t s = 0; s < p->zab.size(); s++) {
+      const double vz = std::fabs((double)p->zab[s].y - p->zab[s].x);
+      soff[s + 1] = soff[s] + (int)std::ceil(vz * span / im.dz) + 2;
+    }
+    p->zblock = soff.back();
+    if (hipMalloc(&p->d_zsoff, soff.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(p->d_zsoff, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      p->gather_ok = false;
+      (void)hipGetLastError();
+    }
+  }
   // Transposed image for the forward projector's x-major rays.
   if (hipMalloc(&p->d_imgT, (size_t)im.nx * im.ny * im.nz * sizeof(float)) != hipSuccess) {
     p->d_imgT = nullptr;
@@ -311,6 +330,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_imgT) (void)hipFree(p->d_imgT);
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
+  if (p->d_zsoff) (void)hipFree(p->d_zsoff);
   delete p;
   return APIRL_OK;
 }

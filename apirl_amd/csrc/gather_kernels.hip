@@ -46,6 +46,8 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
                                            const float* __restrict__ sino, int kind,
                                            int nang_sub, int subset, int nsubsets,
                                            ZRay* __restrict__ out, int* __restrict__ nz_pair,
+                                           float* __restrict__ ztab,
+                                           const int* __restrict__ zsoff, int zblock,
                                            long idx) {
   const int rr = (int)(idx % g.nr);
   const long t = idx / g.nr;
@@ -72,7 +74,22 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   if (kind == BP_PLAIN) b = sino[idx];
   else if (kind == BP_MULT) b = sino[((long)s * g.nang + a) * g.nr + rr];
   z.b = b;
-  z.pad0 = z.pad1 = 0.f;
+  // The ray's z crossings as the reference accumulates them (alpha_z += alpha_z_u,
+  // Siddon.cpp:338): entry t is the value after t additions.  A ray with more
+  // crossings than its slot (not expected: the slot bounds |Vz| (amax - amin) / dz)
+  // is flagged and the gather reconstructs its crossings with alpha_after.
+  const int nzc = abs((int)z.k1 - z.k0);
+  const int base = ztab ? zsoff[s] : 0, slot = ztab ? zsoff[s + 1] - base : -1;
+  z.zflag = nzc > slot ? 1 : 0;
+  z.pad = 0;
+  if (!z.zflag && kinc != 0) {
+    float* dst = ztab + ((long)ka * g.nr + rr) * zblock + base;
+    float az = z.az0;
+    for (int q = 0; q < nzc; q++) {
+      dst[q] = az;
+      az = az + z.azu;
+    }
+  }
   out[((long)ka * g.nr + rr) * g.nsino + s] = z;
   if (b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
@@ -85,10 +102,14 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
                                                          const float* __restrict__ sino, int kind,
                                                          int nang_sub, int subset, int nsubsets,
                                                          long nrays, ZRay* __restrict__ out,
-                                                         int* __restrict__ nz_pair) {
+                                                         int* __restrict__ nz_pair,
+                                                         float* __restrict__ ztab,
+                                                         const int* __restrict__ zsoff,
+                                                         int zblock) {
   const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (idx < nrays)
-    zsetup_one(g, rays, zab, sino, kind, nang_sub, subset, nsubsets, out, nz_pair, idx);
+    zsetup_one(g, rays, zab, sino, kind, nang_sub, subset, nsubsets, out, nz_pair, ztab, zsoff,
+               zblock, idx);
 }
 
 __global__ __launch_bounds__(kGBlock) void clear_int_kernel(int* __restrict__ p, long n) {
@@ -204,6 +225,7 @@ template <int DIMS, int PREC>
 __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* s_zab, int i,
                                              int j, int k) {
   const Geo& g = A.g;
+  const int* s_soff = (const int*)(s_zab + g.nsino);  // slot offsets follow the axial table
   const long v = ((long)k * g.ny + j) * g.nx + i;
   // voxel centre for the candidate search
   const float xc = -g.R + g.dx * ((float)i + 0.5f);
@@ -354,7 +376,14 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           if (m >= tnxy + abs(zk1 - zk0)) continue;
           R Zp = -FLT_MAX, Zn = az0;
           if (nc > 0) {
-            Zp = crossing<PREC>(az0, azu, nc - 1);
+            if (PREC == APIRL_PREC_F64_ALPHA) {
+              Zp = crossing<PREC>(az0, azu, nc - 1);
+            } else {
+              // beyond the last z crossing the loop never reaches this slice
+              if (nc > abs((int)z.k1 - z.k0)) continue;
+              Zp = z.zflag ? crossing<PREC>(az0, azu, nc - 1)
+                           : A.ztab[((long)ka * g.nr + rr) * A.zblock + s_soff[s] + nc - 1];
+            }
             Zn = Zp + azu;
           }
           // cross-axis order (x before y before z on ties)
@@ -386,7 +415,9 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
   extern __shared__ float2 s_zab[];  // (P1.Z, P2.Z) of every sinogram
   const Geo& g = A.g;
   if (DIMS == 3) {
+    int* s_soff = (int*)(s_zab + g.nsino);
     for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
+    for (int q = threadIdx.x; q <= g.nsino; q += kGBlock) s_soff[q] = A.zsoff[q];
     __syncthreads();
   }
   // Voxel of this lane.
@@ -435,7 +466,7 @@ __global__ __launch_bounds__(kGBlock) void zsetup_group_kernel(BatchHeader* __re
       const long idx = (vb - J.first) * kGBlock + threadIdx.x;
       if (idx < J.nrays)
         zsetup_one(J.g, J.rays, J.zab, J.sino, J.kind, J.nang_sub, J.subset, J.nsubsets, J.out,
-                   J.nz_pair, idx);
+                   J.nz_pair, J.ztab, J.zsoff, J.zblock, idx);
     }
   }
 }
@@ -459,7 +490,9 @@ __global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __re
       const GJob& J = jobs[jj];
       if (DIMS == 3 && loaded != jj) {
         __syncthreads();
+        int* s_soff = (int*)(s_zab + J.A.g.nsino);
         for (int q = threadIdx.x; q < J.A.g.nsino; q += kGBlock) s_zab[q] = J.A.zab[q];
+        for (int q = threadIdx.x; q <= J.A.g.nsino; q += kGBlock) s_soff[q] = J.A.zsoff[q];
         __syncthreads();
         loaded = jj;
       }
@@ -480,7 +513,7 @@ inline unsigned ggrid(long n) { return (unsigned)((n + kGBlock - 1) / kGBlock); 
 template <int PREC>
 void launch_gather3(const GatherArgs& A, int zlanes, hipStream_t st) {
   const Geo& g = A.g;
-  const size_t lds = (size_t)g.nsino * sizeof(float2);
+  const size_t lds = gather_lds_bytes(g.nsino);
   if (zlanes <= 1) {
     const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
     hipLaunchKernelGGL((gather_kernel<3, PREC>), grid, dim3(kGBlock), lds, st, A);
@@ -514,33 +547,20 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
   const long nrays = (long)g.nsino * nang_sub * g.nr;
   const long nvox = (long)g.nx * g.ny * g.nz;
   if (nrays == 0 || nvox == 0) return hipSuccess;
+  if (g.dims == 3 && scratch_bytes < gather_scratch_bytes(p, subset, nsubsets))
+    return hipErrorInvalidValue;
   GatherArgs A;
-  A.g = g;
-  A.rays = p->d_rays;
-  A.zab = p->d_zab;
-  A.angcs = p->d_angcs;
-  A.segs = p->d_segs;
-  A.nseg = p->nseg;
-  A.sino = sino;
-  A.img = img;
-  A.kind = kind;
-  A.nang_sub = nang_sub;
-  A.subset = subset;
-  A.nsubsets = nsubsets;
-  A.zr = nullptr;
-  A.nz_pair = nullptr;
+  ZRay* zr = nullptr;
+  int* nzp = nullptr;
+  gather_args_for(p, subset, nsubsets, kind, sino, img, g.dims == 3 ? scratch : nullptr, A, zr,
+                  nzp);
   A.zlanes = g.dims == 3 ? p->gather_zlanes : 1;
   if (g.dims == 3) {
-    const size_t need = gather_scratch_bytes(p, subset, nsubsets);
-    if (scratch_bytes < need) return hipErrorInvalidValue;
-    ZRay* zr = (ZRay*)scratch;
-    int* nzp = (int*)((char*)scratch + (size_t)nrays * sizeof(ZRay));
     const long npair = (long)nang_sub * g.nr;
     hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair)), dim3(kGBlock), 0, st, nzp, npair);
     hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
-                       p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp);
-    A.zr = zr;
-    A.nz_pair = nzp;
+                       p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp, A.ztab,
+                       A.zsoff, A.zblock);
     if (precision == APIRL_PREC_REFERENCE) launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, st);
     else if (precision == APIRL_PREC_KAHAN) launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, st);
     else launch_gather3<APIRL_PREC_F64_ALPHA>(A, A.zlanes, st);
@@ -550,6 +570,14 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     else launch_gather2<APIRL_PREC_F64_ALPHA>(A, st);
   }
   return hipGetLastError();
+}
+
+// Bytes of the z-crossing table for a subset of nang_sub angles; 0 (no table: the
+// gather reconstructs z crossings with alpha_after) above 1 GiB, e.g. for MLEM over
+// all angles of a large sinogram.
+size_t ztable_bytes(const apirl_plan* p, int nang_sub) {
+  const size_t bytes = (size_t)nang_sub * p->geo.nr * (size_t)p->zblock * sizeof(float);
+  return (p->d_zsoff && bytes <= ((size_t)1 << 30)) ? bytes : 0;
 }
 
 void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, const float* sino,
@@ -572,9 +600,17 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   A.zlanes = 1;
   zr = nullptr;
   nz_pair = nullptr;
+  A.ztab = nullptr;
+  A.zsoff = p->d_zsoff;
+  A.zblock = p->zblock;
   if (g.dims == 3 && scratch) {
+    // scratch: [ZRay x nrays][non-zero flag per (angle, r)][z-crossing table]
+    const size_t npair = (size_t)nang_sub * g.nr;
     zr = (ZRay*)scratch;
     nz_pair = (int*)((char*)scratch + (size_t)nrays * sizeof(ZRay));
+    if (ztable_bytes(p, nang_sub) > 0)
+      A.ztab = (float*)((char*)scratch + (size_t)nrays * sizeof(ZRay) +
+                        (npair * sizeof(int) + 255) / 256 * 256);
   }
   A.zr = zr;
   A.nz_pair = nz_pair;
@@ -585,7 +621,7 @@ hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2
   hipLaunchKernelGGL(zsetup_group_kernel, dim3(grid), dim3(kGBlock), 0, st, hdr, zs);
   hipLaunchKernelGGL(gather_group_kernel<2>, dim3(grid), dim3(kGBlock), 0, st, hdr, g2);
   hipLaunchKernelGGL(gather_group_kernel<3>, dim3(grid), dim3(kGBlock),
-                     (size_t)max_nsino * sizeof(float2), st, hdr, g3);
+                     gather_lds_bytes(max_nsino), st, hdr, g3);
   return hipGetLastError();
 }
 
@@ -594,7 +630,9 @@ size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
   if (g.dims != 3) return 0;
   const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
   const long nrays = (long)g.nsino * nang_sub * g.nr;
-  return (size_t)nrays * sizeof(ZRay) + (size_t)nang_sub * g.nr * sizeof(int);
+  const size_t npair = (size_t)nang_sub * g.nr;
+  return (size_t)nrays * sizeof(ZRay) + (npair * sizeof(int) + 255) / 256 * 256 +
+         ztable_bytes(p, nang_sub);
 }
 
 }  // namespace apirl
