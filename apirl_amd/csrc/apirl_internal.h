@@ -96,7 +96,8 @@ struct apirl_plan {
   int gather_zlanes = 1;           // 3D gather voxel mapping (1: 16x16 slice tiles; Z>1:
                                    // Z slices x 64/Z columns per wavefront)
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
-  int* d_zsoff = nullptr;          // [nsino + 1] z-crossing table slots (3D gather)
+  int* d_zsoff = nullptr;          // [2 nsino + 1] z-crossing table slots, first slices
+  float* d_xytab = nullptr;        // [nang][nr][nx + ny] entry crossings (gather)
   int zblock = 0;                  // z-crossing table entries per (angle, radial bin)
   size_t gscratch_bytes = 0;
   std::vector<float> ang, r;       // host copies of the grids
@@ -143,6 +144,8 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                                      void* scratch, size_t scratch_bytes, hipStream_t st);
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 size_t ztable_bytes(const apirl_plan* p, int nang_sub);
+void build_xy_crossings(const apirl_image_desc& img, const std::vector<RayXY>& rays,
+                        std::vector<float>& out);
 void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
                          std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,
                          bool& ok);
@@ -168,7 +171,7 @@ struct alignas(32) ZRay {
   float az0;    // first z crossing (FLT_MAX when Vz == 0)
   float L;      // ray length |P2 - P1| in the reference's operand order
   float b;      // bin value being backprojected
-  int zflag;    // 1: the ray's z crossings are not in the crossing table (use alpha_after)
+  int zcount;   // z crossings of the ray held by the crossing table (the rest: alpha_after)
   int pad;
 };
 static_assert(sizeof(ZRay) == 32, "ZRay is one 32-byte load");
@@ -187,11 +190,15 @@ struct GatherArgs {
   int kind;           // BP_PLAIN, BP_ONES or BP_MULT
   int nang_sub, subset, nsubsets;
   int zlanes;         // voxel mapping (see gather_kernel)
-  // z-crossing table: the reference's accumulated z crossings of every subset ray,
-  // ztab[(ka * nr + r) * zblock + zsoff[s] + t] = value after t additions
+  // z-crossing table: for every subset ray, the reference's accumulated z crossing
+  // by which it enters slice k, at ztab[(ka * nr + r) * zblock + zsoff[s] + k - kmin[s]]
   float* ztab;
-  const int* zsoff;   // [nsino + 1] slot offsets of the sinograms within a block
+  const int* zsoff;   // [nsino + 1] slot offsets, then [nsino] first slice kmin per sinogram
   int zblock;
+  // x / y crossing table of the plan: xytab[(a * nr + r) * (nx + ny) + i] is the
+  // accumulated x crossing by which ray (a, r) enters column i, [... + nx + j] the y
+  // crossing by which it enters row j
+  const float* xytab;
 };
 
 struct BatchHeader {
@@ -261,9 +268,10 @@ hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2
                                const GJob* g3, int grid, int max_nsino, hipStream_t st);
 void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, const float* sino,
                      float* img, void* scratch, GatherArgs& A, ZRay*& zr, int*& nz_pair);
-// LDS bytes of the 3D gather launches for nsino sinograms (axial table + slot offsets)
+// LDS bytes of the 3D gather launches for nsino sinograms (axial table, slot offsets
+// and first slices of the z-crossing table)
 inline size_t gather_lds_bytes(int nsino) {
-  return (size_t)nsino * sizeof(float2) + (size_t)(nsino + 1) * sizeof(int);
+  return (size_t)nsino * sizeof(float2) + (size_t)(2 * nsino + 1) * sizeof(int);
 }
 
 }  // namespace apirl

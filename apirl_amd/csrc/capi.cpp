@@ -280,20 +280,36 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       p->gather_ok = false;
     (void)hipGetLastError();
   }
-  // Slots of the gather's z-crossing table: sinogram s of a (angle, radial bin) block
-  // gets room for every z crossing any of its rays can take, |k1 - k0| <=
-  // |Vz| (amax - amin) / dz + 1 (one more for rounding).
+  // Slots of the gather's z-crossing table, indexed by slice: sinogram s of a
+  // (angle, radial bin) block gets one entry per slice its LORs can enter (the slices
+  // between its two ring planes, one more on each side for rounding), holding the
+  // accumulated z crossing at which the ray enters that slice.
   if (sd.dims == 3 && p->gather_ok) {
-    float span = 0.f;
-    for (const auto& t : rays) span = std::max(span, t.amax - t.amin);
-    std::vector<int> soff(p->zab.size() + 1, 0);
-    for (size_t s = 0; s < p->zab.size(); s++) {
-      const double vz = std::fabs((double)p->zab[s].y - p->zab[s].x);
-      soff[s + 1] = soff[s] + (int)std::ceil(vz * span / im.dz) + 2;
+    const int ns = (int)p->zab.size();
+    std::vector<int> tab(2 * ns + 1, 0);  // [soff (ns + 1) | kmin (ns)]
+    for (int s = 0; s < ns; s++) {
+      const double z0 = std::min(p->zab[s].x, p->zab[s].y);
+      const double z1 = std::max(p->zab[s].x, p->zab[s].y);
+      const double offz = im.dz / 2.0;
+      const int kmin = (int)std::floor((z0 - offz) / im.dz) - 1;
+      const int kmax = (int)std::floor((z1 - offz) / im.dz) + 1;
+      tab[s + 1] = tab[s] + (kmax - kmin + 1);
+      tab[ns + 1 + s] = kmin;
     }
-    p->zblock = soff.back();
-    if (hipMalloc(&p->d_zsoff, soff.size() * sizeof(int)) != hipSuccess ||
-        hipMemcpy(p->d_zsoff, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice) !=
+    p->zblock = tab[ns];
+    if (hipMalloc(&p->d_zsoff, tab.size() * sizeof(int)) != hipSuccess ||
+        hipMemcpy(p->d_zsoff, tab.data(), tab.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+      p->gather_ok = false;
+      (void)hipGetLastError();
+    }
+  }
+  // Entry-crossing table of the gather (independent of subset and sinogram).
+  if (p->gather_ok) {
+    std::vector<float> xy;
+    apirl::build_xy_crossings(im, rays, xy);
+    if (hipMalloc(&p->d_xytab, xy.size() * sizeof(float)) != hipSuccess ||
+        hipMemcpy(p->d_xytab, xy.data(), xy.size() * sizeof(float), hipMemcpyHostToDevice) !=
             hipSuccess) {
       p->gather_ok = false;
       (void)hipGetLastError();
@@ -331,6 +347,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
+  if (p->d_xytab) (void)hipFree(p->d_xytab);
   delete p;
   return APIRL_OK;
 }

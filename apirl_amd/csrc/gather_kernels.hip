@@ -78,18 +78,27 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   // Siddon.cpp:338): entry t is the value after t additions.  A ray with more
   // crossings than its slot (not expected: the slot bounds |Vz| (amax - amin) / dz)
   // is flagged and the gather reconstructs its crossings with alpha_after.
+  // Entries are written for the slices k0 + kinc, k0 + 2 kinc, ... as far as the
+  // slot reaches, at most |k1 - k0| + 2 of them (the loop's N steps can include a z
+  // step more than the floor-based count when an x or y step fewer is taken);
+  // zcount records how many, and the gather reconstructs any crossing beyond them
+  // with alpha_after.
   const int nzc = abs((int)z.k1 - z.k0);
-  const int base = ztab ? zsoff[s] : 0, slot = ztab ? zsoff[s + 1] - base : -1;
-  z.zflag = nzc > slot ? 1 : 0;
-  z.pad = 0;
-  if (!z.zflag && kinc != 0) {
-    float* dst = ztab + ((long)ka * g.nr + rr) * zblock + base;
+  int nw = 0;
+  if (ztab && kinc != 0) {
+    const int base = zsoff[s], slot = zsoff[s + 1] - base, kmin = zsoff[g.nsino + 1 + s];
+    float* dst = ztab + ((long)ka * g.nr + rr) * zblock + base - kmin;
     float az = z.az0;
-    for (int q = 0; q < nzc; q++) {
-      dst[q] = az;
+    int kk = z.k0 + kinc;
+    while (nw < nzc + 2 && kk - kmin >= 0 && kk - kmin < slot) {
+      dst[kk] = az;  // entering slice kk: the value after nw additions
       az = az + z.azu;
+      kk += kinc;
+      nw++;
     }
   }
+  z.zcount = nw;
+  z.pad = 0;
   out[((long)ka * g.nr + rr) * g.nsino + s] = z;
   if (b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
@@ -226,6 +235,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
                                              int j, int k) {
   const Geo& g = A.g;
   const int* s_soff = (const int*)(s_zab + g.nsino);  // slot offsets follow the axial table
+  const int* s_kmin = s_soff + g.nsino + 1;           // then the slots' first slices
   const long v = ((long)k * g.ny + j) * g.nx + i;
   // voxel centre for the candidate search
   const float xc = -g.R + g.dx * ((float)i + 0.5f);
@@ -294,13 +304,27 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
       R Xp = -FLT_MAX, Xn, Yp = -FLT_MAX, Yn;
       Xn = ax0;
       Yn = ay0;
-      if (na > 0) {
-        Xp = crossing<PREC>(ax0, axu, na - 1);
-        Xn = Xp + axu;
-      }
-      if (nb > 0) {
-        Yp = crossing<PREC>(ay0, ayu, nb - 1);
-        Yn = Yp + ayu;
+      if (PREC == APIRL_PREC_F64_ALPHA) {
+        if (na > 0) {
+          Xp = crossing<PREC>(ax0, axu, na - 1);
+          Xn = Xp + axu;
+        }
+        if (nb > 0) {
+          Yp = crossing<PREC>(ay0, ayu, nb - 1);
+          Yn = Yp + ayu;
+        }
+      } else {
+        // the accumulated crossings by which the ray enters column i and row j
+        // (plan-level table, written by repeated float addition)
+        const float* xy = A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny);
+        if (na > 0) {
+          Xp = xy[i];
+          Xn = Xp + axu;
+        }
+        if (nb > 0) {
+          Yp = xy[g.nx + j];
+          Yn = Yp + ayu;
+        }
       }
       // taken x before untaken y (x wins ties), taken y before untaken x (strict)
       if (na > 0 && !(Xp <= Yn)) continue;
@@ -348,6 +372,16 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           if (fmaxf(za_lo, za_hi) < zlo_t) continue;
           if (fminf(za_lo, za_hi) > zhi_t) break;
           const ZRay z = A.zr[((long)ka * g.nr + rr) * g.nsino + s];
+          // the table entry for this slice, loaded alongside the record (its address
+          // does not depend on it); used only when the ray enters the slice by a z
+          // crossing of its own, which makes it the entry zsetup wrote
+          float ztv = 0.f;
+          if (PREC != APIRL_PREC_F64_ALPHA && A.ztab) {
+            const int slot = s_soff[s + 1] - s_soff[s];
+            int kk = k - s_kmin[s];
+            kk = kk < 0 ? 0 : (kk >= slot ? slot - 1 : kk);
+            ztv = A.ztab[((long)ka * g.nr + rr) * A.zblock + s_soff[s] + kk];
+          }
           if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
           int zk0 = z.k0, zk1 = z.k1, zkinc = z.kinc;
           R az0, azu;
@@ -379,10 +413,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
             if (PREC == APIRL_PREC_F64_ALPHA) {
               Zp = crossing<PREC>(az0, azu, nc - 1);
             } else {
-              // beyond the last z crossing the loop never reaches this slice
-              if (nc > abs((int)z.k1 - z.k0)) continue;
-              Zp = z.zflag ? crossing<PREC>(az0, azu, nc - 1)
-                           : A.ztab[((long)ka * g.nr + rr) * A.zblock + s_soff[s] + nc - 1];
+              Zp = nc <= z.zcount ? ztv : crossing<PREC>(az0, azu, nc - 1);
             }
             Zn = Zp + azu;
           }
@@ -417,7 +448,8 @@ __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
   if (DIMS == 3) {
     int* s_soff = (int*)(s_zab + g.nsino);
     for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
-    for (int q = threadIdx.x; q <= g.nsino; q += kGBlock) s_soff[q] = A.zsoff[q];
+    if (A.zsoff)
+      for (int q = threadIdx.x; q <= 2 * g.nsino; q += kGBlock) s_soff[q] = A.zsoff[q];
     __syncthreads();
   }
   // Voxel of this lane.
@@ -492,7 +524,8 @@ __global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __re
         __syncthreads();
         int* s_soff = (int*)(s_zab + J.A.g.nsino);
         for (int q = threadIdx.x; q < J.A.g.nsino; q += kGBlock) s_zab[q] = J.A.zab[q];
-        for (int q = threadIdx.x; q <= J.A.g.nsino; q += kGBlock) s_soff[q] = J.A.zsoff[q];
+        if (J.A.zsoff)
+          for (int q = threadIdx.x; q <= 2 * J.A.g.nsino; q += kGBlock) s_soff[q] = J.A.zsoff[q];
         __syncthreads();
         loaded = jj;
       }
@@ -603,6 +636,7 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   A.ztab = nullptr;
   A.zsoff = p->d_zsoff;
   A.zblock = p->zblock;
+  A.xytab = p->d_xytab;
   if (g.dims == 3 && scratch) {
     // scratch: [ZRay x nrays][non-zero flag per (angle, r)][z-crossing table]
     const size_t npair = (size_t)nang_sub * g.nr;

@@ -224,3 +224,38 @@ extern "C" int apirl_ring_table(int nrings, float zfov, int nseg, const int* nsi
 extern "C" int apirl_subset_num_angles(int nang, int nsubsets, int subset) {
   return apirl::subset_num_angles(nang, nsubsets, subset);
 }
+
+namespace apirl {
+
+// Entry crossings of every LOR (Siddon.cpp:322,330 / :626,637: alpha_x += alpha_x_u):
+// for ray (a, r), out[(a nr + r) (nx + ny) + i] = the value after t float additions
+// of axu to ax, where column i = i0 + (t + 1) iinc is the one that crossing enters,
+// for every column of the image in the ray's direction; likewise rows after nx.
+// Float addition here rounds as on the device (IEEE binary32, no contraction).
+void build_xy_crossings(const apirl_image_desc& img, const std::vector<RayXY>& rays,
+                        std::vector<float>& out) {
+  const int nx = img.nx, ny = img.ny;
+  out.assign(rays.size() * (size_t)(nx + ny), 0.f);
+  for (size_t q = 0; q < rays.size(); q++) {
+    const RayXY& t = rays[q];
+    float* row = out.data() + q * (size_t)(nx + ny);
+    const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
+    // walk until past the far side of the image (the entry voxel may lie outside it)
+    if (iinc != 0) {
+      float a = t.ax;
+      for (int i = t.i0 + iinc; iinc > 0 ? i < nx : i >= 0; i += iinc) {
+        if (i >= 0 && i < nx) row[i] = a;
+        a = a + t.axu;
+      }
+    }
+    if (jinc != 0) {
+      float a = t.ay;
+      for (int j = t.j0 + jinc; jinc > 0 ? j < ny : j >= 0; j += jinc) {
+        if (j >= 0 && j < ny) row[nx + j] = a;
+        a = a + t.ayu;
+      }
+    }
+  }
+}
+
+}  // namespace apirl
