@@ -191,7 +191,7 @@ struct GatherArgs {
   int nang_sub, subset, nsubsets;
   int zlanes;         // voxel mapping (see gather_kernel)
   // z-crossing table: for every subset ray, the reference's accumulated z crossing
-  // by which it enters slice k, at ztab[(ka * nr + r) * zblock + zsoff[s] + k - kmin[s]]
+  // by which it enters slice k, at ztab[((zsoff[s] + k - kmin[s]) * nang_sub + ka) * nr + r]
   float* ztab;
   const int* zsoff;   // [nsino + 1] slot offsets, then [nsino] first slice kmin per sinogram
   int zblock;

@@ -87,11 +87,14 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   int nw = 0;
   if (ztab && kinc != 0) {
     const int base = zsoff[s], slot = zsoff[s + 1] - base, kmin = zsoff[g.nsino + 1 + s];
-    float* dst = ztab + ((long)ka * g.nr + rr) * zblock + base - kmin;
+    // entry e of the sinogram's slot lives at ((base + e) * nang_sub + ka) * nr + rr:
+    // the rays of one (sinogram, angle) are adjacent, as the gather's lanes read them
+    const long npair = (long)nang_sub * g.nr;
+    float* dst = ztab + (long)ka * g.nr + rr;
     float az = z.az0;
     int kk = z.k0 + kinc;
     while (nw < nzc + 2 && kk - kmin >= 0 && kk - kmin < slot) {
-      dst[kk] = az;  // entering slice kk: the value after nw additions
+      dst[(long)(base + kk - kmin) * npair] = az;  // entering slice kk: value after nw additions
       az = az + z.azu;
       kk += kinc;
       nw++;
@@ -99,7 +102,7 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   }
   z.zcount = nw;
   z.pad = 0;
-  out[((long)ka * g.nr + rr) * g.nsino + s] = z;
+  out[idx] = z;  // (sinogram, angle, radial bin) order
   if (b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
     if (*f == 0) atomicOr(f, 1);
@@ -371,7 +374,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           const float za_lo = z_at(zz.x, vzz, lo), za_hi = z_at(zz.x, vzz, hi);
           if (fmaxf(za_lo, za_hi) < zlo_t) continue;
           if (fminf(za_lo, za_hi) > zhi_t) break;
-          const ZRay z = A.zr[((long)ka * g.nr + rr) * g.nsino + s];
+          const ZRay z = A.zr[((long)s * A.nang_sub + ka) * g.nr + rr];
           // the table entry for this slice, loaded alongside the record (its address
           // does not depend on it); used only when the ray enters the slice by a z
           // crossing of its own, which makes it the entry zsetup wrote
@@ -380,7 +383,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
             const int slot = s_soff[s + 1] - s_soff[s];
             int kk = k - s_kmin[s];
             kk = kk < 0 ? 0 : (kk >= slot ? slot - 1 : kk);
-            ztv = A.ztab[((long)ka * g.nr + rr) * A.zblock + s_soff[s] + kk];
+            ztv = A.ztab[((long)(s_soff[s] + kk) * A.nang_sub + ka) * g.nr + rr];
           }
           if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
           int zk0 = z.k0, zk1 = z.k1, zkinc = z.kinc;
