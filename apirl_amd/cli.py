@@ -1,0 +1,179 @@
+"""Command-line reconstruction from the reference's .par files, on the GPU path.
+
+    python -m apirl_amd.cli OSEM recon.par
+    python -m apirl_amd.cli MLEM recon.par
+
+The commands mirror cmd/src/OSEM.cpp and cmd/src/MLEM.cpp for the inputs this path
+covers.
+  * Input types: "Sinogram3D" (cylindrical scanner) and "Sinogram2D" (2D MLEM
+    under either command, as in the reference).
+  * Keys, all read with the reference's parser rules (apirl_amd.interfile):
+      "input type", "input file", "initial estimate", "output filename prefix",
+      "number of iterations", "number of subsets" (OSEM);
+      "cylindrical pet radius (in mm)", "radius fov (in mm)" and "axial fov (in mm)"
+      (data/src/readParameters.cpp:281-330);
+      "forwardprojector" and "backprojector" (required, as in the reference);
+      optional "save estimates at iteration intervals", "multiplicative sinogram",
+      "additive sinogram".  "enforce initial positivity condition" is accepted and,
+      as in the reference, has no effect (readParameters.cpp:242-270).
+  * Projectors: "Siddon" and "HipSiddonProjector" select this path; any other name
+    is refused, and so are oversampled Siddon ("siddon number of samples on the
+    detector" other than 1), "sensitivity filename" and "attenuation image
+    filename".
+  * Outputs, named as in OsemSinogram3d.cpp:110-113, 270-286:
+      <prefix>_sensitivity_subset_<s>.h33 (MLEM: <prefix>_sensitivity.h33),
+      <prefix>_iter_<t>.h33 at the save interval, <prefix>_final.h33,
+      <prefix>.log with the likelihood of every sub-iteration.
+The backprojector is the deterministic gather form; the arithmetic is the
+reference's (APIRL_PREC_REFERENCE)."""
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import hip
+from .interfile import (FormatError, ParFile, atof, atoi, read_image, read_sinogram2d,
+                        read_sinogram3d, write_image)
+
+PROJECTORS = ("Siddon", "HipSiddonProjector")
+
+
+def _load_sino(par, key, kind3d, torch):
+    name = par.get(key)
+    if not name:
+        return None
+    if kind3d:
+        s = read_sinogram3d(name)["bins"]
+    else:
+        s = read_sinogram2d(name)
+    return torch.from_numpy(np.ascontiguousarray(s)).cuda()
+
+
+def run(method, par_path, log=print):
+    import torch
+
+    if not par_path.endswith(".par"):
+        raise FormatError("the parameter file must have the .par extension")
+    par = ParFile(par_path, method)
+    itype = par.get("input type", required=True)
+    infile = par.get("input file", required=True)
+    init_name = par.get("initial estimate", required=True)
+    prefix = par.get("output filename prefix", required=True)
+    iters = atoi(par.get("number of iterations", required=True))
+    subsets = atoi(par.get("number of subsets", required=True)) if method == "OSEM" else 1
+    interval = atoi(par.get("save estimates at iteration intervals", "0"))
+    if iters < 0 or subsets < 1 or interval < 0:
+        raise FormatError(f"{par_path}: negative iterations or interval, or no subsets")
+    # both projector names are required (readParameters.cpp:117-158)
+    for key in ("forwardprojector", "backprojector"):
+        name = par.get(key, required=True).strip()
+        if name not in PROJECTORS:
+            raise FormatError(f"{key} '{name}' is not available on this path "
+                              f"(use one of {', '.join(PROJECTORS)})")
+    # one ray per bin (readParameters.cpp:162-203); oversampled Siddon is not built
+    for key in ("siddon number of samples on the detector",
+                "siddon number of axial samples on the detector"):
+        v = par.get(key)
+        if v is not None and atoi(v) != 1:
+            raise FormatError(f"'{key}' = {v.strip()}: only one sample per bin is available")
+    for key in ("sensitivity filename", "attenuation image filename"):
+        if par.get(key) is not None:
+            raise FormatError(f"'{key}' is not available on this path: the sensitivity "
+                              "is computed on the device from the multiplicative sinogram")
+    init, (dx, dy, dz) = read_image(init_name)
+    nz, ny, nx = init.shape
+    ctx = hip.Context(0)
+    if itype == "Sinogram3D":
+        rscanner = atof(par.get("cylindrical pet radius (in mm)", required=True))
+        rfov = atof(par.get("radius fov (in mm)", required=True))
+        zfov = atof(par.get("axial fov (in mm)", required=True))
+        sino = read_sinogram3d(infile)
+        z1, z2 = hip.ring_table_from(sino["rings"], zfov, sino["nsino"], sino["minrd"],
+                                     sino["maxrd"])
+        plan = hip.Plan(ctx, nx, ny, nz, dx, dz, sino["nproj"], sino["nr"], rfov, rscanner,
+                        dims=3, z1=z1, z2=z2,
+                        ring_width=np.float32(zfov) / np.float32(sino["rings"]))
+        meas = torch.from_numpy(sino["bins"]).cuda()
+        is3d = True
+    elif itype == "Sinogram2D":
+        # Mlem2d takes the radial FOV from the image, (dx * nx) / 2 in float
+        # (Mlem2d.cpp:12, Images.cpp:378); the reference leaves the scanner radius of
+        # this input type unset, here it is "cylindrical pet radius (in mm)" or 400 mm
+        rscanner = atof(par.get("cylindrical pet radius (in mm)", "400"))
+        s2 = read_sinogram2d(infile)
+        nproj, nr = s2.shape
+        rfov = float(np.float32(np.float32(dx) * np.float32(nx)) / np.float32(2))
+        plan = hip.Plan.for_2d(ctx, nx, dx, nproj, nr, rfov, rscanner)
+        meas = torch.from_numpy(np.ascontiguousarray(s2)).cuda()
+        is3d = False
+        # both commands run Mlem2d on this input: no subsets, MLEM rules
+        # (OSEM.cpp:360-369, MLEM.cpp:356-366)
+        subsets = 1
+    else:
+        raise FormatError(f"input type '{itype}' is not available on this path")
+    mult = _load_sino(par, "multiplicative sinogram", is3d, torch)
+    add = _load_sino(par, "additive sinogram", is3d, torch)
+    x = torch.from_numpy(np.ascontiguousarray(init)).cuda()
+    osem_rules = 1 if method == "OSEM" and is3d else 0
+    sens = torch.empty((subsets,) + plan.image_shape(), dtype=torch.float32, device="cuda")
+    t0 = time.perf_counter()
+    liks = []
+    done = 0
+    # iteration t (0-based) is saved when t % interval == 0 (OsemSinogram3d.cpp:266-276):
+    # run up to each save point, save, continue; the state between chunks is the image
+    while True:
+        step = iters - done
+        if interval:
+            nxt = 1 if done == 0 else ((done - 1) // interval + 1) * interval + 1
+            step = min(step, nxt - done)
+        img, lik = plan.reconstruct(meas, x, step, subsets=subsets, bp_mode=hip.BP_GATHER,
+                                    osem_rules=osem_rules, additive=add, mult=mult,
+                                    sens_out=sens)
+        liks.extend(lik[:step * subsets].tolist())
+        final_lik = float(lik[step * subsets])
+        x = img
+        done += step
+        if step and interval and (done - 1) % interval == 0:
+            write_image(f"{prefix}_iter_{done - 1}", img.cpu().numpy(), dx, dy, dz)
+        if done >= iters:
+            break
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    s_host = sens.cpu().numpy()
+    if method == "OSEM" and is3d:
+        for s in range(subsets):
+            write_image(f"{prefix}_sensitivity_subset_{s}", s_host[s], dx, dy, dz)
+    else:
+        write_image(f"{prefix}_sensitivity", s_host[0], dx, dy, dz)
+    final = write_image(f"{prefix}_final", x.cpu().numpy(), dx, dy, dz)
+    with open(prefix + ".log", "w") as f:
+        f.write(f"Name := {prefix}\n")
+        f.write(f"Method := {method}\n")
+        f.write(f"Iterations := {iters}\n")
+        f.write(f"Subsets := {subsets}\n")
+        f.write(f"Input := {infile}\n")
+        f.write(f"Reconstruction time (s) := {elapsed:.6f}\n")
+        for k, v in enumerate(liks):
+            f.write(f"Likelihood [{k // subsets}][{k % subsets}] := {v:.9g}\n")
+        f.write(f"Final likelihood := {final_lik:.9g}\n")
+        f.write(f"Final image := {final}\n")
+    log(f"{method}: {iters} iteration(s) x {subsets} subset(s) in {elapsed:.3f} s -> {final}")
+    return final
+
+
+def main(argv=None):
+    argv = sys.argv[1:] if argv is None else argv
+    if len(argv) != 2 or argv[0] not in ("OSEM", "MLEM"):
+        print("usage: python -m apirl_amd.cli OSEM|MLEM parameters.par", file=sys.stderr)
+        return 1
+    try:
+        run(argv[0], os.path.abspath(argv[1]))
+    except (FormatError, hip.ApirlError, KeyError, ValueError) as e:
+        print(f"{argv[0]}: {e}", file=sys.stderr)
+        return 1
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

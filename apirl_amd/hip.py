@@ -149,6 +149,15 @@ def segment_table(rings, span, mrd):
 def ring_table(rings, zfov, span, mrd):
     """Mean ring coordinates (z1, z2) of every sinogram of a cylindrical 3D sinogram."""
     nsino, mn, mx = segment_table(rings, span, mrd)
+    return ring_table_from(rings, zfov, nsino, mn, mx)
+
+
+def ring_table_from(rings, zfov, nsino, mn, mx):
+    """(z1, z2) per sinogram for an explicit segment table (as an Interfile header
+    gives it: sinograms, minimum and maximum ring difference per segment)."""
+    nsino = np.ascontiguousarray(nsino, np.int32)
+    mn = np.ascontiguousarray(mn, np.int32)
+    mx = np.ascontiguousarray(mx, np.int32)
     tot = int(nsino.sum())
     z1 = np.zeros(tot, np.float32)
     z2 = np.zeros(tot, np.float32)

@@ -17,6 +17,9 @@
 //   mlem3d    <geom3d> iters meas init outdir
 //   rays3d    <geom3d> img start stride out             sampled reference rays
 //   osem3d_sub <geom3d> subsets subset meas             one timed OSEM sub-iteration
+//   img_write / img_read / sino3d_write / sino3d_read / sino2d_write / sino2d_read
+//                                                       the on-disk formats
+//   par_read file method key / itf_read header key     one key lookup: "rc<TAB>value"
 //   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
 // The reference's 2D projector checks only lower bounds and its 3D projector only
 // upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
@@ -40,6 +43,8 @@
 #include "HipSiddonProjector.h"
 #endif
 
+#include <Interfile.h>
+#include <ParametersFile.h>
 #include <malloc.h>
 
 #include <chrono>
@@ -586,6 +591,105 @@ int cmd_osem3d_sub() {
 
 }  // namespace
 
+
+// ---- on-disk formats ---------------------------------------------------------
+// img_write: an Image of the given size with pixels from `in`, written by the
+// reference's Image::writeInterfile (medcon) to prefix.h33 / prefix.i33.
+int cmd_img_write() {
+  SizeImage sz = size3d();
+  Image img(sz);
+  std::vector<float> v = read_f32(arg("in"), (size_t)sz.nPixelsX * sz.nPixelsY * sz.nPixelsZ);
+  memcpy(img.getPixelsPtr(), v.data(), v.size() * 4);
+  if (!img.writeInterfile((char*)arg("prefix"))) {
+    fprintf(stderr, "ref_driver: %s\n", img.getError().c_str());
+    return 6;
+  }
+  return 0;
+}
+
+// img_read: Image::readFromInterfile of `header`; prints the size, writes the pixels.
+int cmd_img_read() {
+  Image img;
+  if (!img.readFromInterfile((char*)arg("header"))) {
+    fprintf(stderr, "ref_driver: %s\n", img.getError().c_str());
+    return 6;
+  }
+  SizeImage sz = img.getSize();
+  printf("{\"nx\": %d, \"ny\": %d, \"nz\": %d, \"dx\": %.9g, \"dy\": %.9g, \"dz\": %.9g}\n",
+         sz.nPixelsX, sz.nPixelsY, sz.nPixelsZ, sz.sizePixelX_mm, sz.sizePixelY_mm,
+         sz.sizePixelZ_mm);
+  write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
+  return 0;
+}
+
+// sino3d_write: a cylindrical 3D sinogram (geometry as make_sino3d) with bins from
+// `in`, written by Sinogram3D::writeInterfile to `header`.
+int cmd_sino3d_write() {
+  Sinogram3DCylindricalPet* s = make_sino3d();
+  std::vector<float> v = read_f32(arg("in"), bins3d(s));
+  sino3d_from(s, v.data());
+  if (!s->writeInterfile(arg("header"))) return 6;
+  return 0;
+}
+
+// sino3d_read: the reference's Sinogram3DCylindricalPet(interfile, rfov, zfov, rscanner)
+// constructor; prints the segment table, writes the bins (segment-major).
+int cmd_sino3d_read() {
+  void* mem = calloc(1, sizeof(Sinogram3DCylindricalPet));
+  Sinogram3DCylindricalPet* s = new (mem) Sinogram3DCylindricalPet(
+      (char*)arg("header"), farg("rfov"), farg("zfov"), farg("rscanner"));
+  printf("{\"nproj\": %d, \"nr\": %d, \"rings\": %d, \"nseg\": %d, \"nsino\": [", s->getNumProj(),
+         s->getNumR(), s->getNumRings(), s->getNumSegments());
+  for (int i = 0; i < s->getNumSegments(); i++)
+    printf("%s%d", i ? ", " : "", s->getSegment(i)->getNumSinograms());
+  printf("], \"minrd\": [");
+  for (int i = 0; i < s->getNumSegments(); i++)
+    printf("%s%d", i ? ", " : "", s->getSegment(i)->getMinRingDiff());
+  printf("], \"maxrd\": [");
+  for (int i = 0; i < s->getNumSegments(); i++)
+    printf("%s%d", i ? ", " : "", s->getSegment(i)->getMaxRingDiff());
+  printf("]}\n");
+  std::vector<float> v = sino3d_to(s);
+  write_f32(arg("out"), v.data(), v.size());
+  return 0;
+}
+
+// par_read: parametersFile_read(file, method, key); prints the return code and value.
+int cmd_par_read() {
+  char value[512] = {0}, err[512] = {0};
+  int rc = parametersFile_read((char*)arg("file"), (char*)arg("method"), (char*)arg("key"), value,
+                               err);
+  printf("%d\t%s\n", rc, value);
+  return 0;
+}
+
+// itf_read: interfile_read(header, key); prints the return code and value.
+int cmd_itf_read() {
+  char value[1024] = {0}, err[1024] = {0};
+  int rc = interfile_read((char*)arg("header"), (char*)arg("key"), value, err);
+  printf("%d\t%s\n", rc, value);
+  return 0;
+}
+
+// sino2d_write: a 2D sinogram (nproj x nr) with bins from `in`, written by
+// Sinogram2D::writeInterfile to prefix.h33 / prefix.i33.
+int cmd_sino2d_write() {
+  Sinogram2DinCylindrical3Dpet sino(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
+  std::vector<float> v = read_f32(arg("in"), (size_t)iarg("nproj") * iarg("nr"));
+  memcpy(sino.getSinogramPtr(), v.data(), v.size() * 4);
+  if (!sino.writeInterfile(arg("prefix"))) return 6;
+  return 0;
+}
+
+// sino2d_read: Sinogram2DinCylindrical3Dpet(header, rfov, rscanner); prints the size,
+// writes the bins.
+int cmd_sino2d_read() {
+  Sinogram2DinCylindrical3Dpet sino((char*)arg("header"), farg("rfov"), farg("rscanner"));
+  printf("{\"nproj\": %d, \"nr\": %d}\n", sino.getNumProj(), sino.getNumR());
+  write_f32(arg("out"), sino.getSinogramPtr(), (size_t)sino.getNumProj() * sino.getNumR());
+  return 0;
+}
+
 int main(int argc, char** argv) {
   // Keep every allocation on the brk heap: the reference's images are allocated
   // inside its EM drivers, where the guard of GuardedPixels cannot reach, and an
@@ -615,6 +719,14 @@ int main(int argc, char** argv) {
   if (c == "mlem3d") return cmd_mlem3d();
   if (c == "rays3d") return cmd_rays3d();
   if (c == "osem3d_sub") return cmd_osem3d_sub();
+  if (c == "img_write") return cmd_img_write();
+  if (c == "img_read") return cmd_img_read();
+  if (c == "sino3d_write") return cmd_sino3d_write();
+  if (c == "sino3d_read") return cmd_sino3d_read();
+  if (c == "par_read") return cmd_par_read();
+  if (c == "itf_read") return cmd_itf_read();
+  if (c == "sino2d_write") return cmd_sino2d_write();
+  if (c == "sino2d_read") return cmd_sino2d_read();
   fprintf(stderr, "ref_driver: unknown command %s\n", c.c_str());
   return 1;
 }

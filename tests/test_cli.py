@@ -1,0 +1,81 @@
+"""The command-line drop-in's handling of parameter files, without a GPU: every
+refusal happens before a device is touched.  The keys and their rules follow
+cmd/src/OSEM.cpp:220-300 and data/src/readParameters.cpp."""
+import numpy as np
+import pytest
+
+from apirl_amd import cli, interfile
+
+BASE = """{method} Parameters :=
+input type := Sinogram3D
+input file := sino.h33
+initial estimate := {init}
+output filename prefix := {out}
+number of iterations := 2
+number of subsets := 4
+forwardprojector := Siddon
+backprojector := Siddon
+cylindrical pet radius (in mm) := 300
+radius fov (in mm) := 64
+axial fov (in mm) := 36
+"""
+
+
+@pytest.fixture
+def files(tmp_path):
+    init = interfile.write_image(str(tmp_path / "init"), np.ones((9, 32, 32), np.float32), 4.0,
+                                 4.0, 4.0)
+
+    def make(extra="", drop=(), method="OSEM", name="r.par"):
+        text = BASE.format(method=method, init=init, out=tmp_path / "out")
+        lines = [ln for ln in text.splitlines() if not any(ln.startswith(d) for d in drop)]
+        path = tmp_path / name
+        path.write_text("\n".join(lines) + "\n" + extra)
+        return str(path)
+
+    return make
+
+
+def refused(path, match, method="OSEM"):
+    with pytest.raises(interfile.FormatError, match=match):
+        cli.run(method, path, log=lambda *_: None)
+
+
+def test_extension_required(files):
+    refused(files(name="r.txt"), "extension")
+
+
+def test_first_line_must_name_the_method(files):
+    refused(files(method="MLEM"), "first line")
+
+
+@pytest.mark.parametrize("key", ["input type", "input file", "initial estimate",
+                                 "output filename prefix", "number of iterations",
+                                 "number of subsets", "forwardprojector", "backprojector"])
+def test_required_keys(files, key):
+    refused(files(drop=(key,)), f"'{key}' not found")
+
+
+def test_duplicate_key(files):
+    refused(files(extra="number of iterations := 3\n"), "more than once")
+
+
+@pytest.mark.parametrize("extra,match", [
+    ("siddon number of samples on the detector := 2\n", "one sample per bin"),
+    ("siddon number of axial samples on the detector := 3\n", "one sample per bin"),
+    ("sensitivity filename := sens.h33\n", "not available on this path"),
+    ("attenuation image filename := mu.h33\n", "not available on this path"),
+])
+def test_unsupported_options_refused(files, extra, match):
+    refused(files(extra=extra), match)
+
+
+def test_unknown_projector(files):
+    path = files(drop=("forwardprojector",), extra="forwardprojector := RotationBasedProjector\n")
+    refused(path, "RotationBasedProjector")
+
+
+def test_main_reports_errors(files, capsys):
+    assert cli.main(["OSEM", files(drop=("backprojector",))]) == 1
+    assert "backprojector" in capsys.readouterr().err
+    assert cli.main(["FBP", "x.par"]) == 1

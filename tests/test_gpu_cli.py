@@ -1,0 +1,130 @@
+"""The command-line drop-in end to end: Interfile inputs and a .par file in, the
+reference's output files out, on the GPU path.
+
+  * 2D MLEM (Sinogram2D input, 60 iterations): the final image and the sensitivity
+    equal the reference's Mlem2d output (tests/golden/c1_mlem2d.npz) bit for bit.
+  * 3D OSEM (Sinogram3D input, 7 subsets x 2 iterations, saved every iteration):
+    the final image is within the image bar (1e-4 relative L2) of the reference's
+    OsemSinogram3d output (mid3d_osem.npz), its NaN voxels coincide, and the saved
+    iterations, final image and subset sensitivities equal, bit for bit, what
+    Plan.reconstruct / Plan.sensitivity give for the same inputs.
+  * 3D OSEM with multiplicative and additive sinograms read from Interfile files:
+    bit-identical to Plan.reconstruct with the same corrections."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from apirl_amd import cli, hip, interfile
+from test_gpu_parity import (IMG_RL2, assert_bitexact, assert_voxel_close, c1_plan, ctx,  # noqa: F401
+                             dev, host, log, plan3d, rel_l2, torch)
+
+pytestmark = pytest.mark.gpu
+
+
+def write_par(path, method, **keys):
+    lines = [f"{method} Parameters :="] + [f"{k} := {v}" for k, v in keys.items()]
+    path.write_text("\n".join(lines) + "\n")
+    return str(path)
+
+
+def read(prefix):
+    return interfile.read_image(prefix + ".h33")[0]
+
+
+def test_cli_mlem2d_equals_reference(torch, ctx, golden, tmp_path):
+    log("case: CLI MLEM, Sinogram2D input, c1 geometry, 60 iterations")
+    f = golden("c1_mlem2d.npz")
+    g = json.loads(str(f["geom"]))
+    sino = interfile.write_sinogram2d(str(tmp_path / "sino"), f["meas"].astype(np.float32))
+    init = interfile.write_image(str(tmp_path / "init"), np.ones((1, 128, 128), np.float32),
+                                 g["dx"], g["dx"], 1.0)
+    out = str(tmp_path / "mlem")
+    par = write_par(tmp_path / "mlem.par", "MLEM", **{
+        "input type": "Sinogram2D", "input file": sino, "initial estimate": init,
+        "output filename prefix": out, "number of iterations": g["iters"],
+        "forwardprojector": "Siddon", "backprojector": "Siddon",
+        "cylindrical pet radius (in mm)": g["rscanner"]})
+    assert cli.main(["MLEM", par]) == 0
+    assert_bitexact(read(out + "_final")[0], f["final"])
+    assert_bitexact(read(out + "_sensitivity")[0], f["sens"])
+    with open(out + ".log") as fh:
+        liks = [float(ln.split(":=")[1]) for ln in fh if ln.startswith("Likelihood [")]
+    assert len(liks) == g["iters"]
+    assert np.allclose(liks, f["lik"][:-1].astype(np.float64), rtol=1e-4)
+
+
+def _sino3d_inputs(tmp_path, g, meas):
+    nsino, mn, mx = hip.segment_table(g["rings"], g["span"], g["mrd"])
+    meas = np.asarray(meas, np.float32).reshape(int(nsino.sum()), g["nproj"], g["nr"])
+    sino = interfile.write_sinogram3d(str(tmp_path / "sino"), meas, nsino, mn, mx, g["rings"],
+                                      2.0, 180.0 / g["nproj"])
+    init = interfile.write_image(str(tmp_path / "init"),
+                                 np.ones((g["nz"], g["ny"], g["nx"]), np.float32),
+                                 g["dx"], g["dx"], g["dz"])
+    keys = {"input type": "Sinogram3D", "input file": sino, "initial estimate": init,
+            "forwardprojector": "HipSiddonProjector", "backprojector": "HipSiddonProjector",
+            "cylindrical pet radius (in mm)": g["rscanner"], "radius fov (in mm)": g["rfov"],
+            "axial fov (in mm)": g["zfov"]}
+    return meas, keys
+
+
+def test_cli_osem3d_matches_reference_and_plan(torch, ctx, golden, tmp_path):
+    log("case: CLI OSEM, Sinogram3D input, mid3d geometry, 7 subsets x 2 iterations")
+    f = golden("mid3d_osem.npz")
+    g = json.loads(str(f["geom"]))
+    meas, keys = _sino3d_inputs(tmp_path, g, f["meas"])
+    out = str(tmp_path / "osem")
+    par = write_par(tmp_path / "osem.par", "OSEM", **keys, **{
+        "output filename prefix": out, "number of iterations": g["iters"],
+        "number of subsets": g["subsets"], "save estimates at iteration intervals": 1})
+    assert cli.main(["OSEM", par]) == 0
+    final = read(out + "_final")
+    ref = f["final"]
+    assert np.array_equal(np.isnan(final), np.isnan(ref)), "NaN positions differ"
+    e = rel_l2(final, ref)
+    log(f"CLI OSEM rel L2 vs reference {e:.2e}")
+    assert e <= IMG_RL2
+    assert_voxel_close(read(out + "_sensitivity_subset_0"), f["sens0"])
+
+    p = plan3d(ctx, g)
+    md = dev(torch, meas)
+    init = torch.ones(p.image_shape(), dtype=torch.float32, device="cuda")
+    for t in range(g["iters"]):
+        img, _ = p.reconstruct(md, init, t + 1, subsets=g["subsets"], bp_mode=hip.BP_GATHER)
+        assert_bitexact(read(f"{out}_iter_{t}"), host(img), nan_ok=True)
+    assert_bitexact(final, host(img), nan_ok=True)
+    for s in (0, g["subsets"] - 1):
+        sens = p.sensitivity(subset=s, nsubsets=g["subsets"], mode=hip.BP_GATHER)
+        assert_bitexact(read(f"{out}_sensitivity_subset_{s}"), host(sens))
+    assert not os.path.exists(f"{out}_iter_{g['iters']}.h33")
+
+
+def test_cli_osem3d_with_corrections_equals_plan(torch, ctx, tmp_path):
+    log("case: CLI OSEM with multiplicative and additive sinograms, tiny 3D geometry")
+    g = dict(nx=32, ny=32, nz=9, dx=4.0, dz=4.0, nproj=24, nr=40, rings=5, rfov=64.0, zfov=36.0,
+             rscanner=300.0, span=3, mrd=2)
+    nsino, mn, mx = hip.segment_table(g["rings"], g["span"], g["mrd"])
+    rng = np.random.default_rng(11)
+    shape = (int(nsino.sum()), g["nproj"], g["nr"])
+    meas = rng.poisson(20.0, shape).astype(np.float32)
+    mult = rng.uniform(1.0, 5.0, shape).astype(np.float32)
+    add = rng.uniform(0.0, 2.0, shape).astype(np.float32)
+    meas, keys = _sino3d_inputs(tmp_path, g, meas)
+    interfile.write_sinogram3d(str(tmp_path / "mult"), mult, nsino, mn, mx, g["rings"], 2.0, 7.5)
+    interfile.write_sinogram3d(str(tmp_path / "add"), add, nsino, mn, mx, g["rings"], 2.0, 7.5)
+    out = str(tmp_path / "corr")
+    par = write_par(tmp_path / "corr.par", "OSEM", **keys, **{
+        "output filename prefix": out, "number of iterations": 3, "number of subsets": 4,
+        "multiplicative sinogram": str(tmp_path / "mult.h33"),
+        "additive sinogram": str(tmp_path / "add.h33")})
+    assert cli.main(["OSEM", par]) == 0
+    p = plan3d(ctx, g)
+    init = torch.ones(p.image_shape(), dtype=torch.float32, device="cuda")
+    img, lik = p.reconstruct(dev(torch, meas), init, 3, subsets=4, bp_mode=hip.BP_GATHER,
+                             osem_rules=1, additive=dev(torch, add), mult=dev(torch, mult))
+    assert_bitexact(read(out + "_final"), host(img), nan_ok=True)
+    with open(out + ".log") as fh:
+        liks = [float(ln.split(":=")[1]) for ln in fh if ln.startswith("Likelihood [")]
+    assert np.allclose(liks, lik[:-1], rtol=1e-7, equal_nan=True)
