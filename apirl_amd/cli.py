@@ -5,8 +5,11 @@
 
 The commands mirror cmd/src/OSEM.cpp and cmd/src/MLEM.cpp for the inputs this path
 covers.
-  * Input types: "Sinogram3D" (cylindrical scanner) and "Sinogram2D" (2D MLEM
-    under either command, as in the reference).
+  * Input types: "Sinogram3D" (cylindrical scanner), "Sinogram2D" (2D MLEM under
+    either command, as in the reference) and "Sinograms2D" /
+    "Sinograms2DinCylindrical3Dpet" (slice-by-slice 2D MLEM, Mlem2dMultiple, run as
+    one batch; outputs <prefix>_slice_<i>_{sensitivity,final,iter_<t>}.h33 and
+    <prefix>_volFinal.h33).
   * Keys, all read with the reference's parser rules (apirl_amd.interfile):
       "input type", "input file", "initial estimate", "output filename prefix",
       "number of iterations", "number of subsets" (OSEM);
@@ -32,7 +35,7 @@ import time
 
 import numpy as np
 
-from . import hip
+from . import hip, multislice
 from .interfile import (FormatError, ParFile, atof, atoi, read_image, read_sinogram2d,
                         read_sinogram3d, write_image)
 
@@ -110,6 +113,18 @@ def run(method, par_path, log=print):
         # both commands run Mlem2d on this input: no subsets, MLEM rules
         # (OSEM.cpp:360-369, MLEM.cpp:356-366)
         subsets = 1
+    elif itype in ("Sinograms2D", "Sinograms2DinCylindrical3Dpet"):
+        # MLEM.cpp:429-440 / OSEM.cpp:384-392: Mlem2dMultiple over the slices
+        for key in ("cylindrical pet radius (in mm)", "radius fov (in mm)", "axial fov (in mm)"):
+            par.get(key, required=True)
+        rscanner = atof(par.get("cylindrical pet radius (in mm)"))
+        sinos, _ = read_image(infile)
+        if sinos.shape[0] != nz:
+            raise FormatError(f"{infile}: {sinos.shape[0]} 2D sinograms for an initial "
+                              f"estimate of {nz} slices")
+        plan = multislice.slice_plan(ctx, nx, dx, sinos.shape[1], sinos.shape[2], rscanner)
+        return _run_multislice(par, method, plan, sinos, iters, interval, prefix, infile,
+                               (dx, dy, dz), torch, log)
     else:
         raise FormatError(f"input type '{itype}' is not available on this path")
     mult = _load_sino(par, "multiplicative sinogram", is3d, torch)
@@ -159,6 +174,60 @@ def run(method, par_path, log=print):
         f.write(f"Final likelihood := {final_lik:.9g}\n")
         f.write(f"Final image := {final}\n")
     log(f"{method}: {iters} iteration(s) x {subsets} subset(s) in {elapsed:.3f} s -> {final}")
+    return final
+
+
+def _run_multislice(par, method, plan, sinos, iters, interval, prefix, infile, vox, torch,
+                    log):
+    """Mlem2dMultiple (Mlem2dMultiple.cpp:64-84): each slice's Mlem2d writes its own
+    files under <prefix>_slice_<i>, and the stacked volume goes to <prefix>_volFinal."""
+    dx, dy, dz = vox
+    nz = sinos.shape[0]
+
+    def load(key):
+        name = par.get(key)
+        if not name:
+            return None
+        s, _ = read_image(name)
+        if s.shape != sinos.shape:
+            raise FormatError(f"{name}: shape {s.shape}, the sinograms are {sinos.shape}")
+        return torch.from_numpy(np.ascontiguousarray(s)).cuda()
+
+    mult = load("multiplicative sinogram")
+    add = load("additive sinogram")
+    meas = torch.from_numpy(np.ascontiguousarray(sinos)).cuda()
+    t0 = time.perf_counter()
+    vol, done = None, 0
+    while True:
+        step = iters - done
+        if interval:
+            nxt = 1 if done == 0 else ((done - 1) // interval + 1) * interval + 1
+            step = min(step, nxt - done)
+        vol = multislice.mlem2d_multiple(plan, meas, step, init=vol, mult=mult, additive=add)
+        done += step
+        if step and interval and (done - 1) % interval == 0:
+            host = vol.cpu().numpy()
+            for k in range(nz):
+                write_image(f"{prefix}_slice_{k}_iter_{done - 1}", host[k], dx, dy, dz)
+        if done >= iters:
+            break
+    sens = multislice.sensitivities(plan, nz, mult=mult)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    host, s_host = vol.cpu().numpy(), sens.cpu().numpy()
+    for k in range(nz):
+        write_image(f"{prefix}_slice_{k}_sensitivity", s_host[k], dx, dy, dz)
+        write_image(f"{prefix}_slice_{k}_final", host[k], dx, dy, dz)
+    final = write_image(f"{prefix}_volFinal", host, dx, dy, dz)
+    with open(prefix + ".log", "w") as f:
+        f.write(f"Name := {prefix}\n")
+        f.write(f"Method := {method} (slice by slice)\n")
+        f.write(f"Iterations := {iters}\n")
+        f.write(f"Slices := {nz}\n")
+        f.write(f"Input := {infile}\n")
+        f.write(f"Reconstruction time (s) := {elapsed:.6f}\n")
+        f.write(f"Final image := {final}\n")
+    log(f"{method}: {nz} slices x {iters} iteration(s) in {elapsed:.3f} s -> {final}")
     return final
 
 

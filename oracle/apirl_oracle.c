@@ -261,6 +261,26 @@ void orc_backproject2d(const orc_image* im, const float* sino, int nang, const f
   free(segs);
 }
 
+void orc_divide_backproject2d(const orc_image* im, const float* meas, const float* est,
+                              int nang, const float* ang_deg, int nr, const float* r_mm,
+                              float rfov, float rscanner, float* img) {
+  /* SiddonProjector.cpp:90-144: bins with est == 0 or meas == 0 skipped (:109-110);
+   * term = ((geom * w) * m) / e with geom = 1 (:125); full bounds check (:121-122). */
+  orc_seg* segs = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  for (int a = 0; a < nang; a++)
+    for (int r = 0; r < nr; r++) {
+      float m = meas[(long)a * nr + r], e = est[(long)a * nr + r];
+      if (e == 0 || m == 0) continue;
+      float p1x, p1y, p2x, p2y;
+      if (!orc_lor_points(ang_deg[a], r_mm[r], rscanner, rfov, &p1x, &p1y, &p2x, &p2y)) continue;
+      int n = orc_siddon2d(p1x, p1y, p2x - p1x, p2y - p1y, im, segs, SEG_CAP);
+      for (int l = 0; l < n; l++)
+        if (segs[l].j >= 0 && segs[l].i >= 0 && segs[l].j < im->ny && segs[l].i < im->nx)
+          img[(long)segs[l].j * im->nx + segs[l].i] += 1.0f * segs[l].w * m / e;
+    }
+  free(segs);
+}
+
 /* LOR of 3D bin (sinogram s, angle a, r): xy from the 2D form, z from za/zb
  * (SiddonProjector.cpp:385-402). */
 static void lor3d(const float* ang_deg, const float* r_mm, float rscanner, const float* za,

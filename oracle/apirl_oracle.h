@@ -69,6 +69,9 @@ void orc_project3d(const orc_image* im, const float* img, int nang, const float*
 
 /* Backprojectors accumulate into img in the reference's single-thread order
  * (SiddonProjector.cpp:34-87 for 2D, :444-568 for 3D). */
+void orc_divide_backproject2d(const orc_image* im, const float* meas, const float* est,
+                              int nang, const float* ang_deg, int nr, const float* r_mm,
+                              float rfov, float rscanner, float* img);
 void orc_backproject2d(const orc_image* im, const float* sino, int nang, const float* ang_deg,
                        int nr, const float* r_mm, float rfov, float rscanner, float* img);
 void orc_backproject3d(const orc_image* im, const float* sino, int nang, const float* ang_deg,

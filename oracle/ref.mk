@@ -24,7 +24,7 @@ CFLAGS   := $(OPT) -fopenmp -fPIC -w $(INC)
 # globs every *.cpp; the other projectors are independent translation units and
 # are left out because nothing on the path calls them).
 RECON_SRC := Siddon.cpp SiddonProjector.cpp Projector.cpp Mlem.cpp Mlem2d.cpp \
-             MlemSinogram3d.cpp OsemSinogram3d.cpp
+             Mlem2dMultiple.cpp MlemSinogram3d.cpp OsemSinogram3d.cpp
 DATA_SRC  := $(notdir $(wildcard $(REF)/data/src/*.cpp))
 UTILS_SRC := $(notdir $(wildcard $(REF)/utils/src/*.cpp))
 MEDCON_SRC:= $(notdir $(wildcard $(REF)/data/medcon/*.c))

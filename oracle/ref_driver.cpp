@@ -11,6 +11,9 @@
 //   proj2d    nx dx nproj nr rfov rscanner img out
 //   back2d    nx dx nproj nr rfov rscanner sino out
 //   mlem2d    nx dx nproj nr rscanner iters meas init outdir
+//   divback2d nx dx nproj nr rfov rscanner meas est out
+//   mlem2dmulti nx ny nz dx dz header rfov zfov rscanner iters outdir out
+//   divback3d <geom3d> meas est out
 //   proj3d    <geom3d> img out
 //   back3d    <geom3d> sino out
 //   osem3d    <geom3d> subsets iters meas init outdir
@@ -35,6 +38,7 @@
 #include <SiddonProjector.h>
 #include <Siddon.h>
 #include <Mlem2d.h>
+#include <Mlem2dMultiple.h>
 #include <MlemSinogram3d.h>
 #include <OsemSinogram3d.h>
 #include <omp.h>
@@ -377,6 +381,24 @@ int cmd_back2d() {
   return 0;
 }
 
+// divback2d: Projector::DivideAndBackproject(meas, est, img) on a 2D sinogram.
+int cmd_divback2d() {
+  Image img(size2d());
+  Sinogram2DinCylindrical3Dpet meas(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
+  Sinogram2DinCylindrical3Dpet est(iarg("nproj"), iarg("nr"), farg("rfov"), farg("rscanner"));
+  size_t nb = (size_t)meas.getNumProj() * meas.getNumR();
+  std::vector<float> m = read_f32(arg("meas"), nb), e = read_f32(arg("est"), nb);
+  memcpy(meas.getSinogramPtr(), m.data(), nb * 4);
+  memcpy(est.getSinogramPtr(), e.data(), nb * 4);
+  Projector* pp = make_projector();
+  {
+  GuardedPixels guarded(&img);
+  pp->DivideAndBackproject(&meas, &est, &img);
+  }
+  write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
+  return 0;
+}
+
 int cmd_mlem2d() {
   Image init(size2d());
   size_t npix = (size_t)init.getPixelCount();
@@ -398,6 +420,25 @@ int cmd_mlem2d() {
   write_f32((dir + "/final.f32").c_str(), mlem.getReconstructedImage()->getPixelsPtr(), npix);
   write_f32((dir + "/sens.f32").c_str(), mlem.sens()->getPixelsPtr(), npix);
   write_f32((dir + "/lik.f32").c_str(), mlem.lik(), iters + 1);
+  return 0;
+}
+
+// mlem2dmulti: Mlem2dMultiple (one Mlem2d per slice) on the multi-slice 2D sinogram
+// file `header` read by Sinograms2DinCylindrical3Dpet(header, rfov, zfov, rscanner);
+// the image grid is nx ny nz dx dz.  Writes the reconstructed volume to `out`.
+int cmd_mlem2dmulti() {
+  Sinograms2DinCylindrical3Dpet sinos(std::string(arg("header")), farg("rfov"), farg("zfov"),
+                                      farg("rscanner"));
+  Image init(size3d());
+  init.fillConstant(1);
+  Projector* pp = make_projector();
+  std::string dir = arg("outdir");
+  Mlem2dMultiple mlem(&sinos, &init, dir, dir + "/multi", iarg("iters"), 0, false, false, pp, pp);
+  double t0 = now_s();
+  if (!mlem.Reconstruct()) return 6;
+  report_time("mlem2dmulti", t0);
+  write_f32(arg("out"), mlem.getReconstructedImage()->getPixelsPtr(),
+            (size_t)init.getPixelCount());
   return 0;
 }
 
@@ -432,6 +473,23 @@ int cmd_back3d() {
   t0 = now_s();
   pp->Backproject(s, &img);
   report_time("back3d", t0);
+  }
+  write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
+  return 0;
+}
+
+// divback3d: Projector::DivideAndBackproject(meas, est, img) on a 3D sinogram.
+int cmd_divback3d() {
+  Image img(size3d());
+  Sinogram3DCylindricalPet* m = make_sino3d();
+  Sinogram3DCylindricalPet* e = make_sino3d();
+  std::vector<float> vm = read_f32(arg("meas"), bins3d(m)), ve = read_f32(arg("est"), bins3d(e));
+  sino3d_from(m, vm.data());
+  sino3d_from(e, ve.data());
+  Projector* pp = make_projector();
+  {
+  GuardedPixels guarded(&img);
+  pp->DivideAndBackproject(m, e, &img);
   }
   write_f32(arg("out"), img.getPixelsPtr(), (size_t)img.getPixelCount());
   return 0;
@@ -713,6 +771,9 @@ int main(int argc, char** argv) {
   if (c == "proj2d") return cmd_proj2d();
   if (c == "back2d") return cmd_back2d();
   if (c == "mlem2d") return cmd_mlem2d();
+  if (c == "divback2d") return cmd_divback2d();
+  if (c == "mlem2dmulti") return cmd_mlem2dmulti();
+  if (c == "divback3d") return cmd_divback3d();
   if (c == "proj3d") return cmd_proj3d();
   if (c == "back3d") return cmd_back3d();
   if (c == "osem3d") return cmd_osem3d();

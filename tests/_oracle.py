@@ -45,6 +45,7 @@ def lib():
             "orc_project2d": (None, [P, P, _i, P, _i, P, _f, _f, P]),
             "orc_project3d": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_backproject2d": (None, [P, P, _i, P, _i, P, _f, _f, P]),
+            "orc_divide_backproject2d": (None, [P, P, P, _i, P, _i, P, _f, _f, P]),
             "orc_backproject3d": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_divide_backproject3d": (None, [P, P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_inverse_divide": (None, [P, P, _l]),
@@ -159,6 +160,14 @@ def backproject2d(im, sino, ang, r, rfov, rscanner, img=None):
         img = np.zeros((im.ny, im.nx), np.float32)
     lib().orc_backproject2d(ctypes.byref(im), _p(sino), len(ang), _p(f32(ang)), len(r),
                             _p(f32(r)), rfov, rscanner, _p(img))
+    return img
+
+
+def divide_backproject2d(im, meas, est, ang, r, rfov, rscanner, img=None):
+    if img is None:
+        img = np.zeros((im.ny, im.nx), np.float32)
+    lib().orc_divide_backproject2d(ctypes.byref(im), _p(f32(meas)), _p(f32(est)), len(ang),
+                                   _p(f32(ang)), len(r), _p(f32(r)), rfov, rscanner, _p(img))
     return img
 
 

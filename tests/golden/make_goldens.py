@@ -143,6 +143,59 @@ def mid3d_osem():
          lik=read("lik.f32"))
 
 
+def divide_backproject():
+    """DivideAndBackproject, 2D (C1 geometry) and 3D (the mid-size geometry), with
+    measured and estimated bins that are zero in every combination: the 2D form skips
+    a bin when either is zero, the 3D form only when the measured bin is
+    (SiddonProjector.cpp:109-110, 607, 669-683)."""
+    rng = np.random.default_rng(5)
+    out = {}
+    g2 = dict(nx=128, dx=2.0, nproj=192, nr=192, rfov=128.0, rscanner=400.0)
+    for dims, g, shape, cmd, extra in (
+            (2, g2, (192, 192), "divback2d", g2),
+            (3, MID, (73, 42, 86), "divback3d", geom3d_args(MID))):
+        meas = rng.poisson(3.0, shape).astype(np.float32)
+        est2 = rng.integers(1, 13, shape).astype(np.uint8)  # estimate x 2: 0.5 .. 6.0
+        est2[rng.random(shape) < 0.2] = 0
+        est = est2.astype(np.float32) * np.float32(0.5)
+        meas.tofile(tmpf("dv_meas.f32"))
+        est.tofile(tmpf("dv_est.f32"))
+        run(cmd, **extra, meas=tmpf("dv_meas.f32"), est=tmpf("dv_est.f32"),
+            out=tmpf("dv_out.f32"))
+        img = read("dv_out.f32")
+        out[f"meas{dims}d"] = meas.astype(np.uint16)
+        out[f"est{dims}d_x2"] = est2
+        out[f"img{dims}d"] = img
+    save("divide_backproject.npz", geom2d=np.array(json.dumps(g2)),
+         geom3d=np.array(json.dumps(MID)), **out)
+
+
+def multislice():
+    """Mlem2dMultiple: four 64x64 @ 4 mm slices, 48 angles x 64 bins, scanner radius
+    400 mm, 10 iterations; the multi-slice sinogram file is read by the reference's
+    Sinograms2DinCylindrical3Dpet (medcon) from an Interfile file written here."""
+    from apirl_amd import interfile
+    g = dict(nx=64, dx=4.0, nproj=48, nr=64, nslices=4, rfov=128.0, zfov=16.0, rscanner=400.0,
+             iters=10)
+    rng = np.random.default_rng(9)
+    sinos = []
+    for k in range(g["nslices"]):
+        img = phantoms.ellipsoids3d(g["nx"], 1, g["dx"], 1.0, rng)[0]
+        img.tofile(tmpf("ms_img.f32"))
+        run("proj2d", nx=g["nx"], dx=g["dx"], nproj=g["nproj"], nr=g["nr"], rfov=g["rfov"],
+            rscanner=g["rscanner"], img=tmpf("ms_img.f32"), out=tmpf("ms_proj.f32"))
+        proj = read("ms_proj.f32").reshape(g["nproj"], g["nr"])
+        sinos.append(phantoms.poisson_counts(proj, 1e5, rng).reshape(g["nproj"], g["nr"]))
+    sinos = np.array(sinos, np.float32)
+    hdr = interfile.write_image(tmpf("ms_sinos"), sinos, 1.0, 1.0, 1.0)
+    run("mlem2dmulti", nx=g["nx"], ny=g["nx"], nz=g["nslices"], dx=g["dx"], dz=4.0, header=hdr,
+        rfov=g["rfov"], zfov=g["zfov"], rscanner=g["rscanner"], iters=g["iters"], outdir=TMP,
+        out=tmpf("ms_vol.f32"))
+    vol = read("ms_vol.f32").reshape(g["nslices"], g["nx"], g["nx"])
+    save("mlem2d_multiple.npz", geom=np.array(json.dumps(g)), sinos=sinos.astype(np.uint16),
+         volume=vol)
+
+
 C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
           rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
 
@@ -164,7 +217,7 @@ def c2_rays():
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT)
-    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2"]
+    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice"]
     if "lists" in which:
         siddon_lists()
     if "c1" in which:
@@ -174,3 +227,7 @@ if __name__ == "__main__":
         mid3d_osem()
     if "c2" in which:
         c2_rays()
+    if "divide" in which:
+        divide_backproject()
+    if "multislice" in which:
+        multislice()

@@ -183,6 +183,22 @@ def test_mid3d_divide_backproject_vs_oracle(torch, ctx, golden):
     assert_voxel_close(host(out), ref)
 
 
+@pytest.mark.parametrize("dims", [2, 3])
+def test_divide_backproject_vs_reference(torch, ctx, golden, dims):
+    """DivideAndBackproject against the reference's own output, with measured and
+    estimated bins zero in every combination (2D: C1 geometry, 3D: mid-size)."""
+    log(f"case: {dims}D divide-and-backproject vs reference")
+    f = golden("divide_backproject.npz")
+    g = json.loads(str(f[f"geom{dims}d"]))
+    p = c1_plan(ctx, g) if dims == 2 else plan3d(ctx, g)
+    meas = f[f"meas{dims}d"].astype(np.float32)
+    est = f[f"est{dims}d_x2"].astype(np.float32) * np.float32(0.5)
+    shape = (1,) + meas.shape if dims == 2 else meas.shape
+    out = p.divide_backproject(dev(torch, meas.reshape(shape)), dev(torch, est.reshape(shape)))
+    torch.cuda.synchronize()
+    assert_voxel_close(host(out).ravel(), f[f"img{dims}d"])
+
+
 def test_mid3d_fused_ratio_and_likelihood(torch, ctx, golden):
     log("case: mid3d fused projection + ratio + likelihood")
     f = golden("mid3d_project_backproject.npz")
