@@ -95,6 +95,7 @@ struct apirl_plan {
   bool gather_ok = false;          // uniform radial grid and increasing runs found
   int gather_zlanes = 1;           // 3D gather voxel mapping (1: 16x16 slice tiles; Z>1:
                                    // Z slices x 64/Z columns per wavefront)
+  int gather_zvox = 3;             // 3D gather voxels per lane along z (1..4, tiles only)
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
   int* d_zsoff = nullptr;          // [2 nsino + 1] z-crossing table slots, first slices
   float* d_xytab = nullptr;        // [nang][nr][nx + ny] entry crossings (gather)

@@ -378,6 +378,7 @@ apirl_status apirl_plan_intersections(apirl_plan* p, int subset, int nsubsets,
 apirl_status apirl_plan_get_tuning(const apirl_plan* p, apirl_tuning* out) {
   if (!p || !out) return APIRL_ERR_INVALID;
   out->gather_zlanes = p->gather_zlanes;
+  out->gather_zvox = p->gather_zvox;
   return APIRL_OK;
 }
 
@@ -387,6 +388,11 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     if (t->gather_zlanes < 1 || t->gather_zlanes > 64)
       return fail(p->ctx, APIRL_ERR_INVALID, "gather_zlanes must be in 1..64");
     p->gather_zlanes = t->gather_zlanes;
+  }
+  if (t->gather_zvox) {
+    if (t->gather_zvox < 1 || t->gather_zvox > 4)
+      return fail(p->ctx, APIRL_ERR_INVALID, "gather_zvox must be in 1..4");
+    p->gather_zvox = t->gather_zvox;
   }
   return APIRL_OK;
 }

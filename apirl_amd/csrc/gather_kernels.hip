@@ -444,6 +444,204 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
   A.img[v] = acc.value();
 }
 
+// LDS of the column kernel: the axial table (float2 per sinogram) and, per sinogram,
+// where slice k's crossing-table entry lives: entry index clamp(k + x, y, z) with
+// x = slot start - first slice, [y, z] = the slot's entries (int4, 16-byte aligned).
+__host__ __device__ inline size_t column_lds_off(int nsino) { return ((size_t)nsino * sizeof(float2) + 15) / 16 * 16; }
+inline size_t column_lds_bytes(int nsino) {
+  return column_lds_off(nsino) + (size_t)nsino * sizeof(int4);
+}
+
+// 3D gather of ZV voxels of one column, (i, j, kb) .. (i, j, kb + ZV - 1), by one lane
+// (float precision modes).  Each voxel receives exactly the terms gather_voxel gives
+// it, in the same order (angle, radial bin, segment, sinogram): the voxels share the
+// transaxial part of every candidate ray (its column tests and x / y crossings) and
+// one scan of each segment's sinograms over the union of their slabs.  Inside the
+// scan, each sinogram is tested against each voxel's own slab exactly as
+// gather_voxel tests it.  Record and table addresses are formed from a per-(angle,
+// radial bin) base and 32-bit offsets (a subset's records and table entries number
+// below 2^31).
+template <int PREC, int ZV>
+__device__ __forceinline__ void gather_column(const GatherArgs& A, const float2* s_zab,
+                                              const int4* s_slot, int i, int j, int kb) {
+  static_assert(PREC != APIRL_PREC_F64_ALPHA, "double-precision setup uses gather_voxel");
+  const Geo& g = A.g;
+  const int nk = g.nz - kb < ZV ? g.nz - kb : ZV;  // voxels of this column segment (>= 1)
+  const long nxy = (long)g.nx * g.ny;
+  const long v0 = ((long)kb * g.ny + j) * g.nx + i;
+  const float xc = -g.R + g.dx * ((float)i + 0.5f);
+  const float yc = -g.R + g.dy * ((float)j + 0.5f);
+  const float inv_dr = 1.0f / g.dr;
+  const unsigned npair = (unsigned)A.nang_sub * (unsigned)g.nr;
+  // each voxel's slab and search bounds, as gather_voxel computes them
+  float zlo_t[ZV], zhi_t[ZV];
+  Acc<PREC> acc[ZV];
+#pragma unroll
+  for (int q = 0; q < ZV; q++) {
+    const int k = kb + q;
+    const float zk0 = g.offz + k * g.dz, zk1 = zk0 + g.dz;
+    zlo_t[q] = zk0 - kZMargin;
+    zhi_t[q] = zk1 + kZMargin;
+    acc[q].init(q < nk ? A.img[v0 + q * nxy] : 0.f);
+  }
+  const float zlo_all = zlo_t[0];
+  float zhi_all = zhi_t[0];
+#pragma unroll
+  for (int q = 1; q < ZV; q++)
+    if (q < nk) zhi_all = zhi_t[q];
+  const float zmid_all = 0.5f * (zlo_all + zhi_all);
+  for (int ka = 0; ka < A.nang_sub; ka++) {
+    const int a = A.subset + A.nsubsets * ka;
+    const float2 cs = A.angcs[a];
+    const float sc = xc * cs.x + yc * cs.y;
+    const float hw = 0.5f * g.dx * (fabsf(cs.x) + fabsf(cs.y)) + g.rmargin;
+    int jlo = (int)ceilf((sc - hw - g.r0) * inv_dr);
+    int jhi = (int)floorf((sc + hw - g.r0) * inv_dr);
+    jlo = jlo < 0 ? 0 : jlo;
+    jhi = jhi > g.nr - 1 ? g.nr - 1 : jhi;
+    for (int rr = jlo; rr <= jhi; rr++) {
+      const unsigned pair = (unsigned)ka * (unsigned)g.nr + (unsigned)rr;
+      if (A.nz_pair[pair] == 0) continue;
+      const RayXY t = A.rays[(long)a * g.nr + rr];
+      if (!(t.flags & RAY_VALID)) continue;
+      const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
+      const int tnxy = t.nxy;
+      const float tamin = t.amin, tamax = t.amax;
+      float ax0, axu, ay0, ayu;
+      first_crossings<PREC, 3>(g, t, iinc, jinc, ax0, axu, ay0, ayu);
+      int na, nb;
+      if (iinc == 0) {
+        if (i != t.i0) continue;
+        na = 0;
+      } else {
+        na = (i - t.i0) * iinc;
+        if (na < 0) continue;
+      }
+      if (jinc == 0) {
+        if (j != t.j0) continue;
+        nb = 0;
+      } else {
+        nb = (j - t.j0) * jinc;
+        if (nb < 0) continue;
+      }
+      if (na + nb >= tnxy + g.nz + 1) continue;
+      float Xp = -FLT_MAX, Xn = ax0, Yp = -FLT_MAX, Yn = ay0;
+      const float* xy = A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny);
+      if (na > 0) {
+        Xp = xy[i];
+        Xn = Xp + axu;
+      }
+      if (nb > 0) {
+        Yp = xy[g.nx + j];
+        Yn = Yp + ayu;
+      }
+      if (na > 0 && !(Xp <= Yn)) continue;
+      if (nb > 0 && !(Yp < Xn)) continue;
+      const float lo = fmaxf(tamin, fmaxf(Xp, Yp));
+      const float hi = fminf(tamax, fminf(Xn, Yn));
+      const float amid = 0.5f * (lo + hi) - 0.5f;
+      const ZRay* zr_pair = A.zr + pair;            // record of sinogram s: [s * npair]
+      const float* zt_pair = A.ztab ? A.ztab + pair : nullptr;  // entry e: [e * npair]
+      for (int sg = 0; sg < A.nseg; sg++) {
+        const GatherSeg S = A.segs[sg];
+        int sl = (int)floorf((zmid_all - amid * S.dlo - S.z0) * S.dhi + 0.5f);
+        sl = sl < 0 ? 0 : (sl > S.count - 1 ? S.count - 1 : sl);
+        while (sl > 0) {
+          const float2 z = s_zab[S.first + sl - 1];
+          const float vz = z.y - z.x;
+          if (fmaxf(z.x + vz * lo, z.x + vz * hi) < zlo_all) break;
+          sl--;
+        }
+        for (; sl < S.count; sl++) {
+          const int s = S.first + sl;
+          const float2 zz = s_zab[s];
+          const float vzz = zz.y - zz.x;
+          const float za_lo = z_at(zz.x, vzz, lo), za_hi = z_at(zz.x, vzz, hi);
+          const float zmax = fmaxf(za_lo, za_hi), zmin = fminf(za_lo, za_hi);
+          if (zmax < zlo_all) continue;
+          if (zmin > zhi_all) break;
+          const ZRay z = zr_pair[(unsigned)s * npair];
+          // each slice's table entry, loaded alongside the record (its address does not
+          // depend on it) and used only when the ray enters that slice by a z crossing
+          // of its own, which makes it an entry zsetup wrote
+          float ztv[ZV];
+          if (zt_pair) {
+            const int4 sl4 = s_slot[s];
+#pragma unroll
+            for (int q = 0; q < ZV; q++) {
+              int e = kb + q + sl4.x;
+              e = e < sl4.y ? sl4.y : (e > sl4.z ? sl4.z : e);
+              ztv[q] = zt_pair[(unsigned)e * npair];
+            }
+          } else {
+#pragma unroll
+            for (int q = 0; q < ZV; q++) ztv[q] = 0.f;
+          }
+          if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
+          const float az0 = z.az0, azu = z.azu;
+          const int zk0r = z.k0, zkinc = z.kinc, zn = abs((int)z.k1 - z.k0);
+#pragma unroll
+          for (int q = 0; q < ZV; q++) {
+            if (q >= nk || zmax < zlo_t[q] || zmin > zhi_t[q]) continue;
+            const int k = kb + q;
+            int nc;
+            if (zkinc == 0) {
+              if (k != zk0r) continue;
+              nc = 0;
+            } else {
+              nc = (k - zk0r) * zkinc;
+              if (nc < 0) continue;
+            }
+            const int m = na + nb + nc;
+            if (m >= tnxy + zn) continue;
+            float Zp = -FLT_MAX, Zn = az0;
+            if (nc > 0) {
+              Zp = (zt_pair && nc <= z.zcount) ? ztv[q] : crossing<PREC>(az0, azu, nc - 1);
+              Zn = Zp + azu;
+            }
+            if (na > 0 && !(Xp <= Zn)) continue;
+            if (nb > 0 && !(Yp <= Zn)) continue;
+            if (nc > 0 && !(Zp < Xn && Zp < Yn)) continue;
+            float ac = tamin;
+            if (m > 0) {
+              ac = Xp > Yp ? Xp : Yp;
+              ac = ac > Zp ? ac : Zp;
+            }
+            float an = Xn < Yn ? Xn : Yn;
+            an = an < Zn ? an : Zn;
+            const float w = (an - ac) * z.L;
+            acc[q].add(w * 1.0f * z.b);  // SiddonProjector.cpp:546
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < ZV; q++)
+    if (q < nk) A.img[v0 + q * nxy] = acc[q].value();
+}
+
+template <int PREC, int ZV>
+__global__ __launch_bounds__(kGBlock) void gather_column_kernel(GatherArgs A) {
+  extern __shared__ float2 s_zab[];
+  const Geo& g = A.g;
+  int4* s_slot = (int4*)((char*)s_zab + column_lds_off(g.nsino));
+  for (int q = threadIdx.x; q < g.nsino; q += kGBlock) {
+    s_zab[q] = A.zab[q];
+    if (A.zsoff) {
+      const int so = A.zsoff[q], so1 = A.zsoff[q + 1], km = A.zsoff[g.nsino + 1 + q];
+      s_slot[q] = make_int4(so - km, so, so1 - 1 < so ? so : so1 - 1, 0);
+    }
+  }
+  __syncthreads();
+  // a workgroup is a 16 x 16 tile of columns, ZV slices deep
+  const int i = blockIdx.x * kTile + (threadIdx.x % kTile);
+  const int j = blockIdx.y * kTile + (threadIdx.x / kTile);
+  const int kb = blockIdx.z * ZV;
+  if (i >= g.nx || j >= g.ny || kb >= g.nz) return;
+  gather_column<PREC, ZV>(A, s_zab, s_slot, i, j, kb);
+}
+
 template <int DIMS, int PREC>
 __global__ __launch_bounds__(kGBlock) void gather_kernel(GatherArgs A) {
   extern __shared__ float2 s_zab[];  // (P1.Z, P2.Z) of every sinogram
@@ -546,10 +744,26 @@ __global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __re
 
 inline unsigned ggrid(long n) { return (unsigned)((n + kGBlock - 1) / kGBlock); }
 
+template <int PREC, int ZV>
+void launch_gather_column(const GatherArgs& A, hipStream_t st) {
+  const Geo& g = A.g;
+  const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, (g.nz + ZV - 1) / ZV);
+  hipLaunchKernelGGL((gather_column_kernel<PREC, ZV>), grid, dim3(kGBlock),
+                     column_lds_bytes(g.nsino), st, A);
+}
+
 template <int PREC>
-void launch_gather3(const GatherArgs& A, int zlanes, hipStream_t st) {
+void launch_gather3(const GatherArgs& A, int zlanes, int zvox, hipStream_t st) {
   const Geo& g = A.g;
   const size_t lds = gather_lds_bytes(g.nsino);
+  if (PREC != APIRL_PREC_F64_ALPHA && zlanes <= 1 && zvox >= 1) {
+    constexpr int P = PREC == APIRL_PREC_F64_ALPHA ? APIRL_PREC_REFERENCE : PREC;
+    if (zvox == 1) launch_gather_column<P, 1>(A, st);
+    else if (zvox == 2) launch_gather_column<P, 2>(A, st);
+    else if (zvox == 3) launch_gather_column<P, 3>(A, st);
+    else launch_gather_column<P, 4>(A, st);
+    return;
+  }
   if (zlanes <= 1) {
     const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, g.nz);
     hipLaunchKernelGGL((gather_kernel<3, PREC>), grid, dim3(kGBlock), lds, st, A);
@@ -597,9 +811,11 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
                        p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp, A.ztab,
                        A.zsoff, A.zblock);
-    if (precision == APIRL_PREC_REFERENCE) launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, st);
-    else if (precision == APIRL_PREC_KAHAN) launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, st);
-    else launch_gather3<APIRL_PREC_F64_ALPHA>(A, A.zlanes, st);
+    const int zv = p->gather_zvox;
+    if (precision == APIRL_PREC_REFERENCE)
+      launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, zv, st);
+    else if (precision == APIRL_PREC_KAHAN) launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, zv, st);
+    else launch_gather3<APIRL_PREC_F64_ALPHA>(A, A.zlanes, zv, st);
   } else {
     if (precision == APIRL_PREC_REFERENCE) launch_gather2<APIRL_PREC_REFERENCE>(A, st);
     else if (precision == APIRL_PREC_KAHAN) launch_gather2<APIRL_PREC_KAHAN>(A, st);

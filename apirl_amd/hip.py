@@ -42,7 +42,7 @@ class EmOptions(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("gather_zlanes", _c_int)]
+    _fields_ = [("gather_zlanes", _c_int), ("gather_zvox", _c_int)]
 
 
 class JobDesc(ctypes.Structure):
@@ -274,6 +274,20 @@ class Plan:
     @gather_zlanes.setter
     def gather_zlanes(self, n):
         t = Tuning(gather_zlanes=int(n))
+        self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
+                       "apirl_plan_set_tuning")
+
+    @property
+    def gather_zvox(self):
+        """3D gather voxels per lane along z (1..4)."""
+        t = Tuning()
+        self.ctx.check(self.ctx.lib.apirl_plan_get_tuning(self.h, ctypes.byref(t)),
+                       "apirl_plan_get_tuning")
+        return t.gather_zvox
+
+    @gather_zvox.setter
+    def gather_zvox(self, n):
+        t = Tuning(gather_zvox=int(n))
         self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
                        "apirl_plan_set_tuning")
 

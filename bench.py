@@ -19,7 +19,6 @@ Prints one JSON line on stdout; progress goes to stderr."""
 import argparse
 import json
 import os
-import subprocess
 import sys
 import tempfile
 import time
@@ -58,11 +57,16 @@ def parse():
     return ap.parse_args()
 
 
-def cpu_baseline(meas_host, threads):
+REF_DRIVER = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+
+
+def cpu_baseline(meas_host, threads, launcher):
     """The reference's own CPU path (oracle/_ref/ref_driver, built from its sources)
-    timed on one OSEM sub-iteration of the same workload; iterations/s = 1 / (21 x t)."""
-    drv = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    if not os.path.exists(drv):
+    timed on one OSEM sub-iteration of the same workload; iterations/s = 1 / (21 x t).
+    The driver is started by `launcher` (apirl_amd.spawn), whose helper process was
+    created before this process initialised the GPU."""
+    drv = REF_DRIVER
+    if not os.path.exists(drv) or launcher is None:
         return None
     with tempfile.TemporaryDirectory() as td:
         mp = os.path.join(td, "meas.f32")
@@ -71,10 +75,12 @@ def cpu_baseline(meas_host, threads):
             [f"subsets={SUBSETS}", "subset=0", f"meas={mp}"]
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
         try:
-            out = subprocess.run(args, env=env, capture_output=True, text=True, timeout=600,
-                                 check=True).stdout
+            rc, out, err = launcher.run(args, env=env, timeout=600)
         except Exception as e:  # the reference may fail; report it rather than crash
             progress(f"cpu baseline failed: {e}")
+            return None
+        if rc != 0:
+            progress(f"cpu baseline failed (exit {rc}): {err[-500:]}")
             return None
     rec = json.loads(out.splitlines()[0])
     t = rec["seconds"]
@@ -186,6 +192,10 @@ def main():
         return batch_main(args)
     if args.workload == "serve":
         return serve_main(args)
+    launcher = None
+    if not args.no_cpu_baseline and os.path.exists(REF_DRIVER):
+        from apirl_amd.spawn import Launcher
+        launcher = Launcher().start()  # before the GPU is initialised
     import torch
 
     from apirl_amd import hip, phantoms
@@ -288,7 +298,9 @@ def main():
     if not args.no_cpu_baseline:
         progress("cpu baseline (reference CPU path, one sub-iteration)")
         threads = min(os.cpu_count() or 1, 16)
-        cpu = cpu_baseline(meas.cpu().numpy(), threads)
+        cpu = cpu_baseline(meas.cpu().numpy(), threads, launcher)
+    if launcher is not None:
+        launcher.close()
 
     line = {
         "metric": "OSEM iters/s (256x256x81, 344x252x837, 21 subsets)",

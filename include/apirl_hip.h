@@ -114,10 +114,15 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
  *   gather_zlanes: voxel mapping of the 3D gather backprojector.  1 = each workgroup
  *                  a 16 x 16 tile of one slice; Z in 2..64 = each 64-lane wavefront
  *                  64/Z neighbouring voxel columns times Z consecutive slices.
- * Results do not depend on it (the per-voxel summation order is fixed).
+ *   gather_zvox:   with gather_zlanes = 1, voxels per lane along z (1..4): a lane
+ *                  gathers V consecutive voxels of a column, sharing each candidate
+ *                  ray's transaxial tests and sinogram search among them (float
+ *                  precision modes; APIRL_PREC_F64_ALPHA always uses 1).
+ * Results do not depend on either (the per-voxel summation order is fixed).
  * Zero in a field leaves it unchanged. */
 typedef struct apirl_tuning {
   int gather_zlanes;
+  int gather_zvox;
 } apirl_tuning;
 apirl_status apirl_plan_get_tuning(const apirl_plan* plan, apirl_tuning* out);
 apirl_status apirl_plan_set_tuning(apirl_plan* plan, const apirl_tuning* tuning);

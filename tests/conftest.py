@@ -8,6 +8,22 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 
+from apirl_amd.spawn import Launcher  # noqa: E402
+
+# Starts programs for GPU tests (the reference-side driver with the HIP projector):
+# started before any test touches the GPU, see apirl_amd/spawn.py.
+LAUNCHER = Launcher()
+
+
+@pytest.fixture(scope="session", autouse=True)
+def _launcher(request):
+    if any(item.get_closest_marker("gpu") for item in request.session.items) and \
+            os.path.exists(os.path.join(ROOT, "oracle", "_ref", "ref_hip_driver")):
+        LAUNCHER.start()
+    yield LAUNCHER
+    LAUNCHER.close()
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
     config.addinivalue_line("markers", "slow: long-running CPU test")

@@ -4,7 +4,6 @@ through its reference-side binding (integration/HipSiddonProjector.cpp), compare
 with the same drivers running the reference's SiddonProjector (the goldens)."""
 import json
 import os
-import subprocess
 import sys
 import tempfile
 
@@ -22,13 +21,15 @@ needs_driver = pytest.mark.skipif(not os.path.exists(DRV),
 
 
 def run(cmd, **kw):
+    """Runs the driver through the session's launcher: this process has already used
+    the GPU, so it must not fork and exec the driver itself."""
+    from conftest import LAUNCHER
     args = [DRV, cmd] + [f"{k}={v}" for k, v in kw.items()]
     print("case:", " ".join(args[1:3]), flush=True)
-    out = subprocess.run(args, env=dict(os.environ, OMP_NUM_THREADS="1"), capture_output=True,
-                         text=True, timeout=300)
-    assert out.returncode == 0, out.stderr[-2000:]
-    assert "HipSiddonProjector" not in out.stderr, out.stderr[-2000:]
-    return out.stdout
+    rc, out, err = LAUNCHER.run(args, env=dict(os.environ, OMP_NUM_THREADS="1"), timeout=300)
+    assert rc == 0, err[-2000:]
+    assert "HipSiddonProjector" not in err, err[-2000:]
+    return out
 
 
 def bits(a):

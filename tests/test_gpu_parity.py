@@ -351,6 +351,36 @@ def test_mid3d_osem_gather_deterministic_and_close_to_reference(torch, ctx, gold
     assert rel_l2(out, ref) <= IMG_RL2
 
 
+@pytest.mark.parametrize("which", ["mid3d", "c2"])
+def test_gather_voxel_mappings_bit_identical(torch, ctx, golden, which):
+    """The 3D gather's launch shapes give every voxel the same terms in the same
+    order: the per-voxel kernel (gather_zlanes = 2) and the column kernel with 1-4
+    voxels per lane (gather_zvox) agree bit for bit, in both float precision modes,
+    on a sinogram with zero bins."""
+    log(f"case: gather voxel mappings, {which}")
+    if which == "mid3d":
+        g = json.loads(str(golden("mid3d_project_backproject.npz")["geom"]))
+        nsub = 7
+    else:
+        g = json.loads(str(golden("c2_rays.npz")["geom"]))
+        nsub = 21
+    p = plan3d(ctx, g)
+    rng = np.random.default_rng(17)
+    sino = rng.random(p.subset_shape(1, nsub), dtype=np.float32)
+    sino[rng.random(sino.shape) < 0.3] = 0.0
+    sd = dev(torch, sino)
+    for prec in (hip.PREC_REFERENCE, hip.PREC_KAHAN):
+        p.gather_zlanes = 2
+        ref = host(p.backproject(sd, subset=1, nsubsets=nsub, mode=hip.BP_GATHER, precision=prec))
+        p.gather_zlanes = 1
+        for zv in (1, 2, 3, 4):
+            p.gather_zvox = zv
+            out = p.backproject(sd, subset=1, nsubsets=nsub, mode=hip.BP_GATHER, precision=prec)
+            torch.cuda.synchronize()
+            assert_bitexact(host(out), ref)
+    p.gather_zvox = 3
+
+
 def test_c2_subset_gather_vs_oracle(torch, ctx, golden):
     """256x256x81 / 837 sinograms, subset 0 of 21: gather and scatter backprojection
     of the phantom's projection, each voxel within 1e-5 relative of the oracle."""
