@@ -54,6 +54,7 @@ struct Geo {
   int nr, nang, nsino;
   int dims;
   float r0, dr, rmargin;  // uniform radial grid r_j = r0 + j dr (gather candidate search)
+  int walk24;  // voxel indices fit 24-bit multiplies (nx ny, nx, ny, nz below 2^23)
 };
 
 // A run of consecutive sinograms with increasing mean axial coordinate (one segment

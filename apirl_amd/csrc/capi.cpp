@@ -220,6 +220,10 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
   geo.offz = im.dz / 2;
   geo.nr = sd.nr; geo.nang = sd.nang; geo.nsino = sd.nsino;
   geo.dims = sd.dims;
+  {
+    const long nxy = (long)im.nx * im.ny, lim = 1L << 23;
+    geo.walk24 = (nxy < lim && im.nx < lim && im.ny < lim && im.nz < lim) ? 1 : 0;
+  }
   if (hipMalloc(&p->d_rays, rays.size() * sizeof(apirl::RayXY)) != hipSuccess) {
     delete p;
     return fail(ctx, APIRL_ERR_NOMEM, "cannot allocate the LOR table");

@@ -444,6 +444,23 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
   A.img[v] = acc.value();
 }
 
+// One 32-byte record as two 16-byte loads.
+__device__ __forceinline__ ZRay load_zray(const ZRay* p) {
+  const int4* q = reinterpret_cast<const int4*>(p);
+  const int4 a = q[0], b = q[1];
+  ZRay z;
+  z.k0 = a.x;
+  z.k1 = (short)(a.y & 0xffff);
+  z.kinc = (short)(a.y >> 16);
+  z.azu = __int_as_float(a.z);
+  z.az0 = __int_as_float(a.w);
+  z.L = __int_as_float(b.x);
+  z.b = __int_as_float(b.y);
+  z.zcount = b.z;
+  z.pad = b.w;
+  return z;
+}
+
 // LDS of the column kernel: the axial table (float2 per sinogram) and, per sinogram,
 // where slice k's crossing-table entry lives: entry index clamp(k + x, y, z) with
 // x = slot start - first slice, [y, z] = the slot's entries (int4, 16-byte aligned).
@@ -560,29 +577,33 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
           const float zmax = fmaxf(za_lo, za_hi), zmin = fminf(za_lo, za_hi);
           if (zmax < zlo_all) continue;
           if (zmin > zhi_all) break;
-          const ZRay z = zr_pair[(unsigned)s * npair];
-          // each slice's table entry, loaded alongside the record (its address does not
-          // depend on it) and used only when the ray enters that slice by a z crossing
-          // of its own, which makes it an entry zsetup wrote
+          const ZRay z = load_zray(zr_pair + (unsigned)s * npair);
+          // the table entry of each slice this sinogram can reach, loaded alongside the
+          // record (its address does not depend on it) and used only when the ray
+          // enters that slice by a z crossing of its own, which makes it an entry
+          // zsetup wrote
           float ztv[ZV];
+          bool hit[ZV];
+#pragma unroll
+          for (int q = 0; q < ZV; q++) {
+            hit[q] = q < nk && !(zmax < zlo_t[q]) && !(zmin > zhi_t[q]);
+            ztv[q] = 0.f;
+          }
           if (zt_pair) {
             const int4 sl4 = s_slot[s];
 #pragma unroll
             for (int q = 0; q < ZV; q++) {
               int e = kb + q + sl4.x;
               e = e < sl4.y ? sl4.y : (e > sl4.z ? sl4.z : e);
-              ztv[q] = zt_pair[(unsigned)e * npair];
+              if (hit[q]) ztv[q] = zt_pair[(unsigned)e * npair];
             }
-          } else {
-#pragma unroll
-            for (int q = 0; q < ZV; q++) ztv[q] = 0.f;
           }
           if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
           const float az0 = z.az0, azu = z.azu;
           const int zk0r = z.k0, zkinc = z.kinc, zn = abs((int)z.k1 - z.k0);
 #pragma unroll
           for (int q = 0; q < ZV; q++) {
-            if (q >= nk || zmax < zlo_t[q] || zmin > zhi_t[q]) continue;
+            if (!hit[q]) continue;
             const int k = kb + q;
             int nc;
             if (zkinc == 0) {

@@ -131,6 +131,60 @@ __device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, 
   return w;
 }
 
+// 3D traversal of one ray in the reference's arithmetic, kUnroll steps per batch of
+// voxel loads (addresses never depend on loaded data), terms added in traversal
+// order.  The reference reads voxel (ci, cj, ck) when ci < nx, cj < ny, ck < nz and
+// the linear index li = ck nx ny + cj nx + ci is not negative (SiddonProjector.cpp:
+// 417-423; a negative ci or cj then reads a neighbouring voxel of the array, as
+// there).  li <= nx ny nz - 1 follows from the three upper bounds.  TR: the rays are
+// offset along y, so in-image voxels are read from the transposed copy (y fastest),
+// where the wavefront's 64 parallel rays touch neighbouring addresses; out-of-image
+// reads keep the reference's index into the image.  Indices use 24-bit multiplies
+// (the plan keeps nx ny below 2^23).
+template <bool TR>
+__device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
+                                        const float* __restrict__ imgT) {
+  const int nxy = g.nx * g.ny;
+  float acc = 0.f;
+  for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
+    float wv[kUnroll], pv[kUnroll];
+    bool tv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const int ci = ray.i, cj = ray.j, ck = ray.k;
+      const bool bx = (ray.ax <= ray.ay) && (ray.ax <= ray.az);
+      const bool by = !bx && (ray.ay <= ray.az);
+      const bool bz = !bx && !by;
+      const float an = bx ? ray.ax : (by ? ray.ay : ray.az);
+      wv[u] = (an - ray.ac) * ray.L;
+      ray.ac = an;
+      const float nax = ray.ax + ray.axu, nay = ray.ay + ray.ayu, naz = ray.az + ray.azu;
+      ray.ax = bx ? nax : ray.ax;
+      ray.ay = by ? nay : ray.ay;
+      ray.az = bz ? naz : ray.az;
+      ray.i += bx ? ray.iinc : 0;
+      ray.j += by ? ray.jinc : 0;
+      ray.k += bz ? ray.kinc : 0;
+      const int li = __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
+      const bool ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz) && li >= 0 && (m0 + u < ray.n);
+      int idx = li;
+      const float* base = img;
+      if (TR && ci >= 0 && cj >= 0) {
+        idx = __mul24(ck, nxy) + __mul24(ci, g.ny) + cj;
+        base = imgT;
+      }
+      tv[u] = ok;
+      pv[u] = base[ok ? idx : 0];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const float nacc = acc + wv[u] * pv[u];
+      acc = tv[u] ? nacc : acc;
+    }
+  }
+  return acc;
+}
+
 // Decomposes a ray index of a subset sinogram into (sinogram, angle, r).
 __device__ __forceinline__ void ray_coords(const Geo& g, long idx, int nang_sub, int subset,
                                            int nsubsets, int& s, int& a, int& r, int& k) {
@@ -167,7 +221,10 @@ __device__ __forceinline__ double project_ray(
     bool valid = setup_ray<DIMS>(g, t, za, zb, ray);
     float acc = 0.f;
     if (valid) {
-      if (PREC == APIRL_PREC_REFERENCE) {
+      if (PREC == APIRL_PREC_REFERENCE && DIMS == 3 && g.walk24) {
+        if (imgT != nullptr && (t.flags & RAY_XMAJOR)) acc = walk3d<true>(g, ray, img, imgT);
+        else acc = walk3d<false>(g, ray, img, imgT);
+      } else if (PREC == APIRL_PREC_REFERENCE) {
         // Steps are taken kUnroll at a time: the voxel loads of a batch are issued
         // together (addresses never depend on loaded data; a step outside the image
         // loads voxel 0 and is not added), then added in traversal order, so the sum
