@@ -256,8 +256,11 @@ struct UpdJob {  // x := (s cmp thr) ? x b / s : 0
 };
 constexpr int kGroupBlock = 256;  // threads per workgroup of the grouped launches
 inline long group_blocks(long n) { return (n + kGroupBlock - 1) / kGroupBlock; }
+// Voxels per lane along z of the grouped 3D gather (the column kernel's default).
+constexpr int kGroupZV = 3;
 inline long gather_tile_blocks(const Geo& g) {
-  return (long)((g.nx + 15) / 16) * ((g.ny + 15) / 16) * g.nz;
+  const int nzg = g.dims == 3 ? (g.nz + kGroupZV - 1) / kGroupZV : g.nz;
+  return (long)((g.nx + 15) / 16) * ((g.ny + 15) / 16) * nzg;
 }
 
 hipError_t launch_group_clear(BatchHeader* hdr, const ClearJob* jobs, int grid,

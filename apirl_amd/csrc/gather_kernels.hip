@@ -725,9 +725,9 @@ __global__ __launch_bounds__(kGBlock) void zsetup_group_kernel(BatchHeader* __re
   }
 }
 
-// Each virtual block is one 16 x 16 tile of one slice of one job (tile-major within
-// a slice, slices in order), as gather_kernel's grid; 3D jobs reload the axial table
-// into LDS when the workgroup moves to another job.
+// Each virtual block is one 16 x 16 tile of one job (tile-major), of one slice in 2D
+// and of kGroupZV slices in 3D (the column kernel's shape); 3D jobs reload the axial
+// and crossing-slot tables into LDS when the workgroup moves to another job.
 template <int DIMS>
 __global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __restrict__ hdr,
                                                                const GJob* __restrict__ jobs) {
@@ -742,23 +742,32 @@ __global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __re
     for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
       const int jj = find_job(jobs, n, vb);
       const GJob& J = jobs[jj];
+      const int ns = J.A.g.nsino;
+      int4* s_slot = (int4*)((char*)s_zab + column_lds_off(ns));
       if (DIMS == 3 && loaded != jj) {
         __syncthreads();
-        int* s_soff = (int*)(s_zab + J.A.g.nsino);
-        for (int q = threadIdx.x; q < J.A.g.nsino; q += kGBlock) s_zab[q] = J.A.zab[q];
-        if (J.A.zsoff)
-          for (int q = threadIdx.x; q <= 2 * J.A.g.nsino; q += kGBlock) s_soff[q] = J.A.zsoff[q];
+        for (int q = threadIdx.x; q < ns; q += kGBlock) {
+          s_zab[q] = J.A.zab[q];
+          if (J.A.zsoff) {
+            const int so = J.A.zsoff[q], so1 = J.A.zsoff[q + 1], km = J.A.zsoff[ns + 1 + q];
+            s_slot[q] = make_int4(so - km, so, so1 - 1 < so ? so : so1 - 1, 0);
+          }
+        }
         __syncthreads();
         loaded = jj;
       }
       const long lb = vb - J.first;
       const int bx = (int)(lb % J.ntx);
       const int by = (int)((lb / J.ntx) % J.nty);
-      const int k = (int)(lb / ((long)J.ntx * J.nty));
+      const int kg = (int)(lb / ((long)J.ntx * J.nty));
       const int i = bx * kTile + (threadIdx.x % kTile);
       const int j = by * kTile + (threadIdx.x / kTile);
-      if (i < J.A.g.nx && j < J.A.g.ny)
-        gather_voxel<DIMS, APIRL_PREC_REFERENCE>(J.A, s_zab, i, j, k);
+      if (i < J.A.g.nx && j < J.A.g.ny) {
+        if constexpr (DIMS == 3)
+          gather_column<APIRL_PREC_REFERENCE, kGroupZV>(J.A, s_zab, s_slot, i, j, kg * kGroupZV);
+        else
+          gather_voxel<DIMS, APIRL_PREC_REFERENCE>(J.A, s_zab, i, j, kg);
+      }
     }
   }
 }
@@ -895,7 +904,7 @@ hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2
   hipLaunchKernelGGL(zsetup_group_kernel, dim3(grid), dim3(kGBlock), 0, st, hdr, zs);
   hipLaunchKernelGGL(gather_group_kernel<2>, dim3(grid), dim3(kGBlock), 0, st, hdr, g2);
   hipLaunchKernelGGL(gather_group_kernel<3>, dim3(grid), dim3(kGBlock),
-                     gather_lds_bytes(max_nsino), st, hdr, g3);
+                     column_lds_bytes(max_nsino), st, hdr, g3);
   return hipGetLastError();
 }
 

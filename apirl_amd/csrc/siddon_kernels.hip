@@ -23,7 +23,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kNumXcd = 8;  // MI355X: 8 XCDs of 32 CUs
-constexpr int kUnroll = 4;  // Siddon steps per batch of voxel loads
+constexpr int kUnroll = 8;  // Siddon steps per batch of voxel loads
 // Marks a batch slot without a term: no Siddon weight is NaN-boxed like this value.
 
 // Per-ray traversal state (Siddon.cpp:253-340).
