@@ -1,6 +1,7 @@
 // Batched OSEM / MLEM jobs: many reconstructions of different geometries advance
-// together, one sub-iteration of every active job per round.  A round is seven
-// grouped launches (clear, fused projection + ratio for 2D and 3D jobs, axial setup,
+// together, one sub-iteration of every active job per round.  Jobs of at least
+// kLargeVoxels voxels run their sub-iteration with the single-job kernels after the
+// grouped launches; the others share the round's seven grouped launches (clear, fused projection + ratio for 2D and 3D jobs, axial setup,
 // gather backprojection for 2D and 3D jobs, update) that read per-job descriptors
 // from a table in device memory; the launches are persistent (fixed grid), so the
 // whole round is captured once into a hipGraph and replayed, and jobs join or leave
@@ -41,8 +42,15 @@ struct BJob {
   float* ratio = nullptr;    // largest subset sinogram
   float* bp = nullptr;       // backprojection
   void* zscr = nullptr;      // gather axial records + non-zero flags (3D)
+  size_t zscr_bytes = 0;
+  bool large = false;        // runs its sub-iterations with the single-job kernels
   std::vector<std::pair<void*, size_t>> bufs;
 };
+
+// A 3D job of at least this many voxels fills the GPU with its own launches, so its
+// sub-iterations are enqueued with the single-job kernels (which read the plan's
+// transposed image copy and need no job table) after each grouped round.
+constexpr size_t kLargeVoxels = size_t(1) << 21;
 
 template <class T>
 size_t align_up(size_t n) {
@@ -278,6 +286,8 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   j->ratio = (float*)get(ratio_b);
   j->bp = (float*)get(bp_b);
   if (z_b) j->zscr = get(z_b);
+  j->zscr_bytes = z_b;
+  j->large = !j->is2d && j->nvox >= kLargeVoxels;
   if (!j->sens || !j->ratio || !j->bp || (z_b && !j->zscr) ||
       hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming) != hipSuccess) {
     release_bufs(b, j);
@@ -346,11 +356,20 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
   GJob* g3 = (GJob*)(h + b->off_g3);
   UpdJob* up = (UpdJob*)(h + b->off_upd);
   std::vector<BJob*> finishing;
+  std::vector<std::pair<BJob*, int>> direct;  // large jobs and their subset this round
   for (int id : b->order) {
     BJob* j = b->jobs[id];
     const apirl_plan* p = j->plan;
     const Geo& g = p->geo;
     const int s = j->s, S = j->S;
+    if (j->large) {
+      direct.emplace_back(j, s);
+      if (++j->s == S) {
+        j->s = 0;
+        if (++j->t == j->T) finishing.push_back(j);
+      }
+      continue;
+    }
     const int nang_sub = subset_num_angles(g.nang, S, s);
     const long nrays = (long)g.nsino * nang_sub * g.nr;
     // clear the backprojection (and the gather's non-zero flags)
@@ -441,20 +460,46 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
       if (++j->t == j->T) finishing.push_back(j);
     }
   }
-  if (hipMemcpyAsync(b->d_tab, h, b->tab_bytes, hipMemcpyHostToDevice, b->stream) != hipSuccess)
-    return bfail(b, APIRL_ERR_HIP, "table upload failed");
-  (void)hipEventRecord(b->copy_ev[b->cur], b->stream);
-  b->cur ^= 1;
-  hipError_t e;
-  if (b->use_graph) {
-    if (!b->exec && (e = capture(b)) != hipSuccess)
-      return bfail(b, APIRL_ERR_HIP, std::string("graph capture failed: ") + hipGetErrorString(e));
-    e = hipGraphLaunch(b->exec, b->stream);
-  } else {
-    e = enqueue_round(b);
+  hipError_t e = hipSuccess;
+  if (direct.size() < b->order.size()) {  // some jobs run in the grouped launches
+    if (hipMemcpyAsync(b->d_tab, h, b->tab_bytes, hipMemcpyHostToDevice, b->stream) !=
+        hipSuccess)
+      return bfail(b, APIRL_ERR_HIP, "table upload failed");
+    (void)hipEventRecord(b->copy_ev[b->cur], b->stream);
+    b->cur ^= 1;
+    if (b->use_graph) {
+      if (!b->exec && (e = capture(b)) != hipSuccess)
+        return bfail(b, APIRL_ERR_HIP,
+                     std::string("graph capture failed: ") + hipGetErrorString(e));
+      e = hipGraphLaunch(b->exec, b->stream);
+    } else {
+      e = enqueue_round(b);
+    }
+    if (e != hipSuccess)
+      return bfail(b, APIRL_ERR_HIP, std::string("round launch failed: ") + hipGetErrorString(e));
   }
-  if (e != hipSuccess)
-    return bfail(b, APIRL_ERR_HIP, std::string("round launch failed: ") + hipGetErrorString(e));
+  // large jobs: the sub-iteration apirl_em_reconstruct runs for them (fused projection
+  // and ratio, gather backprojection, update), on the batch stream
+  for (auto& pr : direct) {
+    BJob* j = pr.first;
+    const int s = pr.second;
+    FusedRatio fr;
+    fr.meas = j->meas;
+    fr.additive = j->add;
+    fr.ratio = j->ratio;
+    fr.lik_partials = nullptr;
+    fr.lik_positive_only = 0;
+    if ((e = launch_project(j->plan, j->x, nullptr, s, j->S, APIRL_PREC_REFERENCE, &fr, nullptr,
+                            b->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(j->bp, 0, j->nvox * sizeof(float), b->stream)) != hipSuccess ||
+        (e = launch_backproject_gather(j->plan, j->ratio, j->bp, s, j->S, BP_PLAIN,
+                                       APIRL_PREC_REFERENCE, j->zscr, j->zscr_bytes,
+                                       b->stream)) != hipSuccess ||
+        (e = launch_em_update(j->x, j->bp, j->sens + (size_t)s * j->nvox, j->nvox, j->thr[s],
+                              j->strict, b->stream)) != hipSuccess)
+      return bfail(b, APIRL_ERR_HIP, std::string("large-job launch failed: ") +
+                                         hipGetErrorString(e));
+  }
   for (BJob* j : finishing) {
     (void)hipEventRecord(j->done_ev, b->stream);
     j->done_recorded = true;

@@ -146,3 +146,24 @@ def test_serving_loop_results_equal_alone(torch, ctx, budget):
     assert np.all(np.isfinite(lat)) and np.all(lat > 0)
     for k, (_, d) in enumerate(jobs):
         same_bits(host(d["out"]), ref[k])
+
+
+def test_batch_with_large_job_equals_alone(torch, ctx):
+    """A job of at least 2^21 voxels (256x256x81) runs its sub-iterations with the
+    single-job kernels after each grouped round; it and the small jobs beside it
+    still equal, bit for bit, each job alone."""
+    from apirl_amd import workloads
+    log("case: batch with a 256x256x81 job beside small 2D / 3D jobs")
+    big = workloads.make_plan(ctx, "3d_c2")
+    g = workloads.GEOMETRIES["3d_c2"]
+    plans = {"big": big, "3d_tiny": make_plan(ctx, GEOMS["3d_tiny"]),
+             "2d_small": make_plan(ctx, GEOMS["2d_small"])}
+    jobs = [("big", make_job(torch, big, g, 21, 1, 2)),
+            ("3d_tiny", make_job(torch, plans["3d_tiny"], GEOMS["3d_tiny"], 22, 2, 3)),
+            ("2d_small", make_job(torch, plans["2d_small"], GEOMS["2d_small"], 23, 3, 2))]
+    ref = [alone(plans[n], j) for n, j in jobs]
+    torch.cuda.synchronize()
+    for use_graph in (True, False):
+        res = run_batch(torch, ctx, plans, jobs, use_graph=use_graph)
+        for k in range(len(jobs)):
+            same_bits(res[k], ref[k])
