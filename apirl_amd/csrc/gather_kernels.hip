@@ -142,6 +142,11 @@ struct Acc {
   float s;
   __device__ void init(float v) { s = v; }
   __device__ void add(float t) { s = s + t; }
+  // add(t) when c, else nothing (selected, so no branch)
+  __device__ void add_if(bool c, float t) {
+    const float n = s + t;
+    s = c ? n : s;
+  }
   __device__ void add_f64(double) {}
   __device__ float value() const { return s; }
 };
@@ -155,6 +160,13 @@ struct Acc<APIRL_PREC_KAHAN> {
     const float u = s + y;
     c = (u - s) - y;
     s = u;
+  }
+  __device__ void add_if(bool cond, float t) {
+    const float y = t - c;
+    const float u = s + y;
+    const float nc = (u - s) - y;
+    s = cond ? u : s;
+    c = cond ? nc : c;
   }
   __device__ void add_f64(double) {}
   __device__ float value() const { return s; }
@@ -520,40 +532,24 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
       const unsigned pair = (unsigned)ka * (unsigned)g.nr + (unsigned)rr;
       if (A.nz_pair[pair] == 0) continue;
       const RayXY t = A.rays[(long)a * g.nr + rr];
-      if (!(t.flags & RAY_VALID)) continue;
+      const float* xy = A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny);
+      // entry crossings of column i and row j (table entries exist for every i, j, so
+      // they are loaded before the tests that decide whether they are used)
+      const float xyi = xy[i], xyj = xy[g.nx + j];
       const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
       const int tnxy = t.nxy;
       const float tamin = t.amin, tamax = t.amax;
       float ax0, axu, ay0, ayu;
       first_crossings<PREC, 3>(g, t, iinc, jinc, ax0, axu, ay0, ayu);
-      int na, nb;
-      if (iinc == 0) {
-        if (i != t.i0) continue;
-        na = 0;
-      } else {
-        na = (i - t.i0) * iinc;
-        if (na < 0) continue;
-      }
-      if (jinc == 0) {
-        if (j != t.j0) continue;
-        nb = 0;
-      } else {
-        nb = (j - t.j0) * jinc;
-        if (nb < 0) continue;
-      }
-      if (na + nb >= tnxy + g.nz + 1) continue;
-      float Xp = -FLT_MAX, Xn = ax0, Yp = -FLT_MAX, Yn = ay0;
-      const float* xy = A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny);
-      if (na > 0) {
-        Xp = xy[i];
-        Xn = Xp + axu;
-      }
-      if (nb > 0) {
-        Yp = xy[g.nx + j];
-        Yn = Yp + ayu;
-      }
-      if (na > 0 && !(Xp <= Yn)) continue;
-      if (nb > 0 && !(Yp < Xn)) continue;
+      // the transaxial tests of gather_voxel as one predicate
+      const int na = (i - t.i0) * iinc, nb = (j - t.j0) * jinc;
+      bool okt = (t.flags & RAY_VALID) != 0;
+      okt = okt && (iinc == 0 ? i == t.i0 : na >= 0) && (jinc == 0 ? j == t.j0 : nb >= 0);
+      okt = okt && na + nb < tnxy + g.nz + 1;
+      const float Xp = na > 0 ? xyi : -FLT_MAX, Xn = na > 0 ? xyi + axu : ax0;
+      const float Yp = nb > 0 ? xyj : -FLT_MAX, Yn = nb > 0 ? xyj + ayu : ay0;
+      okt = okt && (na <= 0 || Xp <= Yn) && (nb <= 0 || Yp < Xn);
+      if (!okt) continue;
       const float lo = fmaxf(tamin, fmaxf(Xp, Yp));
       const float hi = fminf(tamax, fminf(Xn, Yn));
       const float amid = 0.5f * (lo + hi) - 0.5f;
@@ -604,34 +600,27 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
 #pragma unroll
           for (int q = 0; q < ZV; q++) {
             if (!hit[q]) continue;
+            // the tests of gather_voxel, evaluated as one predicate (selects, not
+            // branches); the term is added only when all hold
             const int k = kb + q;
-            int nc;
-            if (zkinc == 0) {
-              if (k != zk0r) continue;
-              nc = 0;
-            } else {
-              nc = (k - zk0r) * zkinc;
-              if (nc < 0) continue;
-            }
+            const int nc = (k - zk0r) * zkinc;
+            bool ok = zkinc == 0 ? k == zk0r : nc >= 0;
             const int m = na + nb + nc;
-            if (m >= tnxy + zn) continue;
-            float Zp = -FLT_MAX, Zn = az0;
-            if (nc > 0) {
-              Zp = (zt_pair && nc <= z.zcount) ? ztv[q] : crossing<PREC>(az0, azu, nc - 1);
-              Zn = Zp + azu;
-            }
-            if (na > 0 && !(Xp <= Zn)) continue;
-            if (nb > 0 && !(Yp <= Zn)) continue;
-            if (nc > 0 && !(Zp < Xn && Zp < Yn)) continue;
-            float ac = tamin;
-            if (m > 0) {
-              ac = Xp > Yp ? Xp : Yp;
-              ac = ac > Zp ? ac : Zp;
-            }
+            ok = ok && m < tnxy + zn;
+            const bool hz = nc > 0;
+            float Zt = ztv[q];
+            if (ok && hz && !(zt_pair && nc <= z.zcount)) Zt = crossing<PREC>(az0, azu, nc - 1);
+            const float Zp = hz ? Zt : -FLT_MAX;
+            const float Zn = hz ? Zt + azu : az0;
+            ok = ok && (na <= 0 || Xp <= Zn) && (nb <= 0 || Yp <= Zn) &&
+                 (!hz || (Zp < Xn && Zp < Yn));
+            float ac = Xp > Yp ? Xp : Yp;
+            ac = ac > Zp ? ac : Zp;
+            ac = m > 0 ? ac : tamin;
             float an = Xn < Yn ? Xn : Yn;
             an = an < Zn ? an : Zn;
             const float w = (an - ac) * z.L;
-            acc[q].add(w * 1.0f * z.b);  // SiddonProjector.cpp:546
+            acc[q].add_if(ok, w * 1.0f * z.b);  // SiddonProjector.cpp:546
           }
         }
       }
