@@ -25,8 +25,9 @@ covers.
     filename".
   * Outputs, named as in OsemSinogram3d.cpp:110-113, 270-286:
       <prefix>_sensitivity_subset_<s>.h33 (MLEM: <prefix>_sensitivity.h33),
-      <prefix>_iter_<t>.h33 at the save interval, <prefix>_final.h33,
-      <prefix>.log with the likelihood of every sub-iteration.
+      <prefix>_iter_<t>.h33 at the save interval, <prefix>_final.h33, and
+      <prefix>.log in the reference's log format (parameters, saved images, total
+      time, per sub-iteration times and likelihoods).
 The backprojector is the deterministic gather form; the arithmetic is the
 reference's (APIRL_PREC_REFERENCE)."""
 import os
@@ -133,7 +134,7 @@ def run(method, par_path, log=print):
     osem_rules = 1 if method == "OSEM" and is3d else 0
     sens = torch.empty((subsets,) + plan.image_shape(), dtype=torch.float32, device="cuda")
     t0 = time.perf_counter()
-    liks = []
+    liks, times, saved = [], [], []
     done = 0
     # iteration t (0-based) is saved when t % interval == 0 (OsemSinogram3d.cpp:266-276):
     # run up to each save point, save, continue; the state between chunks is the image
@@ -142,15 +143,18 @@ def run(method, par_path, log=print):
         if interval:
             nxt = 1 if done == 0 else ((done - 1) // interval + 1) * interval + 1
             step = min(step, nxt - done)
+        tm = np.zeros((4, step * subsets), np.float32)
         img, lik = plan.reconstruct(meas, x, step, subsets=subsets, bp_mode=hip.BP_GATHER,
                                     osem_rules=osem_rules, additive=add, mult=mult,
-                                    sens_out=sens)
+                                    sens_out=sens, times_out=tm if step else None)
         liks.extend(lik[:step * subsets].tolist())
+        times.append(tm)
         final_lik = float(lik[step * subsets])
         x = img
         done += step
         if step and interval and (done - 1) % interval == 0:
             write_image(f"{prefix}_iter_{done - 1}", img.cpu().numpy(), dx, dy, dz)
+            saved.append(done - 1)
         if done >= iters:
             break
     torch.cuda.synchronize()
@@ -162,19 +166,45 @@ def run(method, par_path, log=print):
     else:
         write_image(f"{prefix}_sensitivity", s_host[0], dx, dy, dz)
     final = write_image(f"{prefix}_final", x.cpu().numpy(), dx, dy, dz)
-    with open(prefix + ".log", "w") as f:
-        f.write(f"Name := {prefix}\n")
-        f.write(f"Method := {method}\n")
-        f.write(f"Iterations := {iters}\n")
-        f.write(f"Subsets := {subsets}\n")
-        f.write(f"Input := {infile}\n")
-        f.write(f"Reconstruction time (s) := {elapsed:.6f}\n")
-        for k, v in enumerate(liks):
-            f.write(f"Likelihood [{k // subsets}][{k % subsets}] := {v:.9g}\n")
-        f.write(f"Final likelihood := {final_lik:.9g}\n")
-        f.write(f"Final image := {final}\n")
+    head = [("Iterations", str(iters)),
+            ("Input Projections", "3D Sinogram" if is3d else "2D Sinogram"),
+            ("Size of Sinogram2D", f"{meas.shape[-1]}[r] x {meas.shape[-2]}[ang]")]
+    if is3d:
+        head += [("Rings", str(sino["rings"])), ("Segments", str(len(sino["nsino"])))]
+    head += [("Image Dimensions", "3" if nz > 1 else "2"),
+             ("Image Size", f"{nx}[x] x {ny}[y] x {nz}[z]")]
+    write_log(prefix, head, saved, elapsed * 1e3, np.concatenate(times, axis=1),
+              liks + [final_lik])
     log(f"{method}: {iters} iteration(s) x {subsets} subset(s) in {elapsed:.3f} s -> {final}")
     return final
+
+
+def _row(values):
+    """Logger::writeRowOfNumbers: floats as an ostream prints them (6 significant
+    digits), separated by ", " (data/src/Logger.cpp:119-135)."""
+    return ", ".join(format(float(np.float32(v)), "g") for v in values)
+
+
+def write_log(prefix, head, saved, total_ms, times, liks):
+    """<prefix>.log in the reference drivers' format (OsemSinogram3d.cpp:125-150,
+    266-317; Logger.cpp: "label: value" lines, CR LF line ends): the parameters, the
+    saved images, the total time, per sub-iteration times in ms (whole, projection,
+    backprojection, update; measured with HIP events here) and the likelihoods."""
+    lines = ["######## ML-EM Reconstruction #########", f"Name: {prefix}", "Type: ML-EM"]
+    lines += [f"{k}: {v}" for k, v in head]
+    lines.append("_______RECONSTRUCCION_______")
+    lines += [f"Imagen de iteración {t} guardada en: {prefix}_iter_{t}" for t in saved]
+    lines.append(f"Imagen final guardada en: {prefix}_final")
+    lines.append("_______RESULTADOS DE RECONSTRUCCION_______")
+    lines.append(f"Tiempo Total de Reconstrucción:: {total_ms:f}")
+    for title, row in zip(("Tiempos de Reconstrucción por Iteración [mseg]",
+                           "Tiempos de Forwardprojection por Iteración [mseg]",
+                           "Tiempos de Backwardprojection por Iteración [mseg]",
+                           "Tiempos de UpdatePixel por Iteración [mseg]"), times):
+        lines += [title, _row(row)]
+    lines += ["Likelihood por Iteración:", _row(liks)]
+    with open(prefix + ".log", "wb") as f:
+        f.write("".join(line + "\r\n" for line in lines).encode("utf-8"))
 
 
 def _run_multislice(par, method, plan, sinos, iters, interval, prefix, infile, vox, torch,

@@ -512,6 +512,15 @@ apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
                                   const float* d_meas, const float* d_init,
                                   const float* d_additive, const float* d_mult, float* d_img,
                                   double* h_lik, float* d_sens_out) {
+  return apirl_em_reconstruct_timed(p, opt, d_meas, d_init, d_additive, d_mult, d_img, h_lik,
+                                    d_sens_out, nullptr);
+}
+
+apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* opt,
+                                        const float* d_meas, const float* d_init,
+                                        const float* d_additive, const float* d_mult,
+                                        float* d_img, double* h_lik, float* d_sens_out,
+                                        float* h_times) {
   if (!p || !opt || !d_meas || !d_init || !d_img) return APIRL_ERR_INVALID;
   apirl_ctx* c = p->ctx;
   const int S = opt->subsets, T = opt->iterations;
@@ -530,7 +539,11 @@ apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
   // the ratio sinogram of the largest subset and the backprojected image.
   float *d_sens = nullptr, *d_ratio = nullptr, *d_bp = nullptr;
   double* d_lik = nullptr;
+  std::vector<hipEvent_t> ev;  // phase timing events (h_times)
   auto cleanup = [&]() {
+    for (auto& x : ev)
+      if (x) (void)hipEventDestroy(x);
+    ev.clear();
     if (d_sens && d_sens != d_sens_out) (void)hipFree(d_sens);
     if (d_ratio) (void)hipFree(d_ratio);
     if (d_bp) (void)hipFree(d_bp);
@@ -570,19 +583,38 @@ apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
     cleanup();
     return fail(c, APIRL_ERR_HIP, "image copy failed");
   }
+  // Per sub-iteration phase timing (h_times): events before the projection, the
+  // backprojection, the update and after it.
+  if (h_times) {
+    ev.assign((size_t)T * S * 3 + 1, nullptr);
+    for (auto& x : ev)
+      if (hipEventCreate(&x) != hipSuccess) {
+        x = nullptr;
+        cleanup();
+        return fail(c, APIRL_ERR_HIP, "event creation failed");
+      }
+  }
+  auto mark = [&](size_t i) {
+    if (h_times) (void)hipEventRecord(ev[i], c->stream);
+  };
   for (int t = 0; t < T; t++) {
     for (int s = 0; s < S; s++) {
+      const size_t q = ((size_t)t * S + s) * 3;
+      mark(q);
       APIRL_TRY(apirl_project_ratio(p, d_img, d_meas, d_additive, d_ratio,
                                     d_lik + (size_t)t * S + s, s, S, is2d ? 1 : 0,
                                     opt->precision));
+      mark(q + 1);
       if (hipMemsetAsync(d_bp, 0, nvox * sizeof(float), c->stream) != hipSuccess) {
         cleanup();
         return fail(c, APIRL_ERR_HIP, "memset failed");
       }
       APIRL_TRY(apirl_backproject(p, d_ratio, d_bp, s, S, opt->bp_mode, bp_prec));
+      mark(q + 2);
       APIRL_TRY(apirl_em_update(c, d_img, d_bp, d_sens + (size_t)s * nvox, nvox, thr[s], strict));
     }
   }
+  mark((size_t)T * S * 3);
   // Final likelihood.  3D drivers project the final image once more over the full
   // sinogram (MlemSinogram3d.cpp:277, OsemSinogram3d.cpp:291-293).  Mlem2d takes it on
   // the last estimated sinogram, which by then holds the ratio (Mlem2d.cpp:227-228).
@@ -615,6 +647,20 @@ apirl_status apirl_em_reconstruct(apirl_plan* p, const apirl_em_options* opt,
   if (hipStreamSynchronize(c->stream) != hipSuccess) {
     cleanup();
     return fail(c, APIRL_ERR_HIP, "stream synchronize failed");
+  }
+  if (h_times) {
+    // [total | forward | backward | update] x (T S), in ms
+    const size_t n = (size_t)T * S;
+    for (size_t k = 0; k < n; k++) {
+      float f = 0.f, b = 0.f, u = 0.f;
+      (void)hipEventElapsedTime(&f, ev[3 * k], ev[3 * k + 1]);
+      (void)hipEventElapsedTime(&b, ev[3 * k + 1], ev[3 * k + 2]);
+      (void)hipEventElapsedTime(&u, ev[3 * k + 2], ev[3 * k + 3]);
+      h_times[k] = f + b + u;
+      h_times[n + k] = f;
+      h_times[2 * n + k] = b;
+      h_times[3 * n + k] = u;
+    }
   }
 #undef APIRL_TRY
   cleanup();

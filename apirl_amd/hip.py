@@ -82,6 +82,8 @@ SIGNATURES = {
     "apirl_mlem_threshold": (_c_int, [_P, _P, _c_size, ctypes.POINTER(_c_float)]),
     "apirl_em_reconstruct": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P, _P,
                                       _P]),
+    "apirl_em_reconstruct_timed": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P,
+                                            _P, _P, _P]),
     "apirl_batch_create": (_c_int, [_P, _c_int, _c_int, ctypes.POINTER(_P)]),
     "apirl_batch_destroy": (_c_int, [_P]),
     "apirl_batch_last_error": (ctypes.c_char_p, [_P]),
@@ -357,8 +359,10 @@ class Plan:
 
     def reconstruct(self, meas, init, iterations, subsets=1, bp_mode=BP_SCATTER,
                     precision=PREC_REFERENCE, osem_rules=None, additive=None, mult=None,
-                    out=None, sens_out=None):
-        """Device-resident MLEM (subsets=1) or OSEM; returns (image, likelihood[T*S+1])."""
+                    out=None, sens_out=None, times_out=None):
+        """Device-resident MLEM (subsets=1) or OSEM; returns (image, likelihood[T*S+1]).
+        times_out: optional float32 array [4, T*S] that receives each sub-iteration's
+        time in ms: whole, projection + ratio, backprojection, update."""
         torch = _torch()
         if osem_rules is None:
             osem_rules = 1 if subsets > 1 else 0
@@ -369,6 +373,15 @@ class Plan:
         self._check_img(init)
         self._check_img(out)
         self._check_sino(meas, 0, 1)
+        if times_out is not None:
+            if times_out.dtype != np.float32 or times_out.size != 4 * iterations * subsets or \
+                    not times_out.flags.c_contiguous:
+                raise ValueError("times_out must be a contiguous float32 array of 4*T*S")
+            self.ctx.check(self.ctx.lib.apirl_em_reconstruct_timed(
+                self.h, ctypes.byref(opt), _dptr(meas), _dptr(init), _dptr(additive),
+                _dptr(mult), _dptr(out), _np_ptr(lik), _dptr(sens_out), _np_ptr(times_out)),
+                "apirl_em_reconstruct_timed")
+            return out, lik
         self.ctx.check(self.ctx.lib.apirl_em_reconstruct(
             self.h, ctypes.byref(opt), _dptr(meas), _dptr(init), _dptr(additive), _dptr(mult),
             _dptr(out), _np_ptr(lik), _dptr(sens_out)), "apirl_em_reconstruct")

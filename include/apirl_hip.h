@@ -190,6 +190,16 @@ apirl_status apirl_em_reconstruct(apirl_plan* plan, const apirl_em_options* opt,
                                   const float* d_meas, const float* d_init,
                                   const float* d_additive, const float* d_mult, float* d_img,
                                   double* h_lik, float* d_sens_out);
+/* The same, also timing every sub-iteration as the reference's drivers log it
+ * (OsemSinogram3d.cpp:155-240, 303-313; MlemSinogram3d.cpp, Mlem2d.cpp alike):
+ * h_times (may be NULL) receives 4 x iterations*subsets floats in ms, the rows
+ * [whole sub-iteration | projection + ratio | backprojection | update], measured with
+ * HIP events on the context's stream. */
+apirl_status apirl_em_reconstruct_timed(apirl_plan* plan, const apirl_em_options* opt,
+                                        const float* d_meas, const float* d_init,
+                                        const float* d_additive, const float* d_mult,
+                                        float* d_img, double* h_lik, float* d_sens_out,
+                                        float* h_times);
 
 /* ---- Batched jobs (continuous batching) ------------------------------------------
  * No counterpart in the reference, whose EM drivers run one job at a time

@@ -23,6 +23,7 @@
 //   img_write / img_read / sino3d_write / sino3d_read / sino2d_write / sino2d_read
 //                                                       the on-disk formats
 //   par_read file method key / itf_read header key     one key lookup: "rc<TAB>value"
+//   log_row   n in out                                 Logger::writeRowOfNumbers
 //   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
 // The reference's 2D projector checks only lower bounds and its 3D projector only
 // upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
@@ -48,6 +49,7 @@
 #endif
 
 #include <Interfile.h>
+#include <Logger.h>
 #include <ParametersFile.h>
 #include <malloc.h>
 
@@ -748,6 +750,15 @@ int cmd_sino2d_read() {
   return 0;
 }
 
+// log_row: Logger(out).writeRowOfNumbers of the n floats in `in`.
+int cmd_log_row() {
+  const int n = iarg("n");
+  std::vector<float> v = read_f32(arg("in"), (size_t)n);
+  Logger logger(arg("out"));
+  logger.writeRowOfNumbers(v.data(), n);
+  return 0;
+}
+
 int main(int argc, char** argv) {
   // Keep every allocation on the brk heap: the reference's images are allocated
   // inside its EM drivers, where the guard of GuardedPixels cannot reach, and an
@@ -786,6 +797,7 @@ int main(int argc, char** argv) {
   if (c == "sino3d_read") return cmd_sino3d_read();
   if (c == "par_read") return cmd_par_read();
   if (c == "itf_read") return cmd_itf_read();
+  if (c == "log_row") return cmd_log_row();
   if (c == "sino2d_write") return cmd_sino2d_write();
   if (c == "sino2d_read") return cmd_sino2d_read();
   fprintf(stderr, "ref_driver: unknown command %s\n", c.c_str());

@@ -13,6 +13,7 @@ oracle/ref.mk from the reference's sources in this container).
   * itf_*.h33: seeded synthetic Interfile headers for interfile_read.
   * par_expected.json / itf_expected.json: parametersFile_read / interfile_read
     return code and value for every (file, method, key) probed.
+  * log_row.json: Logger::writeRowOfNumbers (the log's number rows) on a few values.
   * ref_image.h33/.i33, ref_sino3d.h33/.i33, ref_sino2d.h33/.i33: written by the
     reference's Image / Sinogram3D / Sinogram2D writers; formats.npz holds the
     arrays they were given and the geometry.
@@ -120,7 +121,7 @@ def main():
         par[name] = {m: {key: lookup("par_read", os.path.join(OUT, name), key, m) for key in PAR_KEYS}
                      for m in ("MLEM", "OSEM")}
     itf = {}
-    for name in sorted(x for x in os.listdir(OUT) if x.startswith("itf_")):
+    for name in sorted(x for x in os.listdir(OUT) if x.startswith("itf_") and x.endswith(".h33")):
         itf[name] = {key: lookup("itf_read", os.path.join(OUT, name), key) for key in ITF_KEYS}
 
     # files written by the reference's writers, with relative names (run in OUT)
@@ -144,6 +145,19 @@ def main():
         os.chdir(cwd)
     for f in ("_img.f32", "_s3.f32", "_s2.f32"):
         os.remove(os.path.join(OUT, f))
+    # Logger::writeRowOfNumbers on values of every printing regime
+    row = np.array([1234567.0, 0.0001, 100.0, -2.5, 3.14159265, 1e-7, 6.02e23, 0.0, -0.0,
+                    123456.75, 1.5e-5, 99999.95, 16777217.0, -3244243.8, 446506.9],
+                   np.float32)
+    row.tofile(os.path.join(OUT, "_row.f32"))
+    driver("log_row", f"n={row.size}", f"in={os.path.join(OUT, '_row.f32')}",
+           f"out={os.path.join(OUT, '_row.log')}")
+    with open(os.path.join(OUT, "_row.log"), "rb") as f:
+        row_text = f.read().decode("latin-1")
+    os.remove(os.path.join(OUT, "_row.f32"))
+    os.remove(os.path.join(OUT, "_row.log"))
+    with open(os.path.join(OUT, "log_row.json"), "w") as f:
+        json.dump({"values": [float(v) for v in row], "text": row_text}, f, indent=0)
     np.savez_compressed(os.path.join(OUT, "formats.npz"), image=img, image_voxel=np.float32([2.5, 2.5, 3.25]),
                         sino3d=sino3, sino3d_geom=np.array(json.dumps(s3)), sino2d=sino2)
     with open(os.path.join(OUT, "par_expected.json"), "w") as f:

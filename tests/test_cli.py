@@ -79,3 +79,34 @@ def test_main_reports_errors(files, capsys):
     assert cli.main(["OSEM", files(drop=("backprojector",))]) == 1
     assert "backprojector" in capsys.readouterr().err
     assert cli.main(["FBP", "x.par"]) == 1
+
+
+def test_log_rows_match_reference_logger():
+    """Number rows of the log as Logger::writeRowOfNumbers prints them (golden from the
+    reference's Logger, tests/golden/formats/log_row.json)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(__file__), "golden", "formats", "log_row.json")
+    with open(path) as f:
+        g = json.load(f)
+    assert cli._row(g["values"]) + "\r\n" == g["text"]
+
+
+def test_log_layout(tmp_path):
+    prefix = str(tmp_path / "osem")
+    times = np.arange(8, dtype=np.float32).reshape(4, 2)
+    cli.write_log(prefix, [("Iterations", "1"), ("Rings", "4")], [0], 12.5, times,
+                  [1.5, 2.5, 3.5])
+    text = open(prefix + ".log", "rb").read().decode("utf-8")
+    lines = text.split("\r\n")
+    assert lines[0] == "######## ML-EM Reconstruction #########"
+    assert lines[1] == f"Name: {prefix}" and lines[2] == "Type: ML-EM"
+    assert lines[3:5] == ["Iterations: 1", "Rings: 4"]
+    assert lines[5] == "_______RECONSTRUCCION_______"
+    assert lines[6] == f"Imagen de iteración 0 guardada en: {prefix}_iter_0"
+    assert lines[7] == f"Imagen final guardada en: {prefix}_final"
+    assert lines[9] == "Tiempo Total de Reconstrucción:: 12.500000"
+    assert lines[10:12] == ["Tiempos de Reconstrucción por Iteración [mseg]", "0, 1"]
+    assert lines[16:18] == ["Tiempos de UpdatePixel por Iteración [mseg]", "6, 7"]
+    assert lines[18:20] == ["Likelihood por Iteración:", "1.5, 2.5, 3.5"]
+    assert text.endswith("\r\n") and "\n" not in text.replace("\r\n", "")

@@ -33,6 +33,16 @@ def read(prefix):
     return interfile.read_image(prefix + ".h33")[0]
 
 
+def log_rows(prefix):
+    """{title: numbers} of the number rows of <prefix>.log (reference log format)."""
+    lines = open(prefix + ".log", "rb").read().decode("utf-8").split("\r\n")
+    rows = {}
+    for k, line in enumerate(lines[:-1]):
+        if line.startswith("Tiempos de") or line.startswith("Likelihood por"):
+            rows[line] = [float(v) for v in lines[k + 1].split(", ")]
+    return rows
+
+
 def test_cli_mlem2d_equals_reference(torch, ctx, golden, tmp_path):
     log("case: CLI MLEM, Sinogram2D input, c1 geometry, 60 iterations")
     f = golden("c1_mlem2d.npz")
@@ -49,10 +59,16 @@ def test_cli_mlem2d_equals_reference(torch, ctx, golden, tmp_path):
     assert cli.main(["MLEM", par]) == 0
     assert_bitexact(read(out + "_final")[0], f["final"])
     assert_bitexact(read(out + "_sensitivity")[0], f["sens"])
-    with open(out + ".log") as fh:
-        liks = [float(ln.split(":=")[1]) for ln in fh if ln.startswith("Likelihood [")]
-    assert len(liks) == g["iters"]
-    assert np.allclose(liks, f["lik"][:-1].astype(np.float64), rtol=1e-4)
+    rows = log_rows(out)
+    liks = rows["Likelihood por Iteración:"]
+    assert len(liks) == g["iters"] + 1
+    assert np.allclose(liks[:-1], f["lik"][:-1].astype(np.float64), rtol=1e-4)
+    whole = np.array(rows["Tiempos de Reconstrucción por Iteración [mseg]"])
+    parts = sum(np.array(rows[k]) for k in ("Tiempos de Forwardprojection por Iteración [mseg]",
+                                            "Tiempos de Backwardprojection por Iteración [mseg]",
+                                            "Tiempos de UpdatePixel por Iteración [mseg]"))
+    assert len(whole) == g["iters"] and np.all(whole > 0)
+    assert np.allclose(whole, parts, rtol=1e-4, atol=1e-3)
 
 
 def _sino3d_inputs(tmp_path, g, meas):
@@ -125,9 +141,9 @@ def test_cli_osem3d_with_corrections_equals_plan(torch, ctx, tmp_path):
     img, lik = p.reconstruct(dev(torch, meas), init, 3, subsets=4, bp_mode=hip.BP_GATHER,
                              osem_rules=1, additive=dev(torch, add), mult=dev(torch, mult))
     assert_bitexact(read(out + "_final"), host(img), nan_ok=True)
-    with open(out + ".log") as fh:
-        liks = [float(ln.split(":=")[1]) for ln in fh if ln.startswith("Likelihood [")]
-    assert np.allclose(liks, lik[:-1], rtol=1e-7, equal_nan=True)
+    liks = log_rows(out)["Likelihood por Iteración:"]
+    assert len(liks) == 3 * 4 + 1
+    assert np.allclose(liks, lik.astype(np.float32), rtol=1e-5, equal_nan=True)
 
 
 def test_mlem2d_multiple_equals_reference(torch, ctx, golden, tmp_path):
