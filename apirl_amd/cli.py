@@ -20,16 +20,16 @@ covers.
       "additive sinogram".  "enforce initial positivity condition" is accepted and,
       as in the reference, has no effect (readParameters.cpp:242-270).
   * Projectors: "Siddon" and "HipSiddonProjector" select this path; any other name
-    is refused, and so are oversampled Siddon ("siddon number of samples on the
-    detector" other than 1), "sensitivity filename" and "attenuation image
-    filename".
+    is refused, and so are "sensitivity filename" and "attenuation image filename".
+    "siddon number of samples on the detector" / "... axial samples ..." trace 3D
+    bins as several LORs (apirl_plan_set_oversampling), backprojected with the
+    scatter form; the backprojector is otherwise the deterministic gather form.
   * Outputs, named as in OsemSinogram3d.cpp:110-113, 270-286:
       <prefix>_sensitivity_subset_<s>.h33 (MLEM: <prefix>_sensitivity.h33),
       <prefix>_iter_<t>.h33 at the save interval, <prefix>_final.h33, and
       <prefix>.log in the reference's log format (parameters, saved images, total
       time, per sub-iteration times and likelihoods).
-The backprojector is the deterministic gather form; the arithmetic is the
-reference's (APIRL_PREC_REFERENCE)."""
+The arithmetic is the reference's (APIRL_PREC_REFERENCE)."""
 import os
 import sys
 import time
@@ -75,12 +75,15 @@ def run(method, par_path, log=print):
         if name not in PROJECTORS:
             raise FormatError(f"{key} '{name}' is not available on this path "
                               f"(use one of {', '.join(PROJECTORS)})")
-    # one ray per bin (readParameters.cpp:162-203); oversampled Siddon is not built
+    # LORs per bin (readParameters.cpp:162-203; absent keys mean one)
+    samples = []
     for key in ("siddon number of samples on the detector",
                 "siddon number of axial samples on the detector"):
         v = par.get(key)
-        if v is not None and atoi(v) != 1:
-            raise FormatError(f"'{key}' = {v.strip()}: only one sample per bin is available")
+        n = 1 if v is None else atoi(v)
+        if n < 1:
+            raise FormatError(f"'{key}' = {v.strip()}: at least one sample per bin")
+        samples.append(n)
     for key in ("sensitivity filename", "attenuation image filename"):
         if par.get(key) is not None:
             raise FormatError(f"'{key}' is not available on this path: the sensitivity "
@@ -98,6 +101,8 @@ def run(method, par_path, log=print):
         plan = hip.Plan(ctx, nx, ny, nz, dx, dz, sino["nproj"], sino["nr"], rfov, rscanner,
                         dims=3, z1=z1, z2=z2,
                         ring_width=np.float32(zfov) / np.float32(sino["rings"]))
+        if samples != [1, 1]:
+            plan.set_oversampling(*samples)
         meas = torch.from_numpy(sino["bins"]).cuda()
         is3d = True
     elif itype == "Sinogram2D":
@@ -132,6 +137,10 @@ def run(method, par_path, log=print):
     add = _load_sino(par, "additive sinogram", is3d, torch)
     x = torch.from_numpy(np.ascontiguousarray(init)).cuda()
     osem_rules = 1 if method == "OSEM" and is3d else 0
+    # the gather backprojector needs one LOR per bin; oversampled 3D bins are
+    # backprojected by the scatter form (2D projectors have no oversampling,
+    # SiddonProjector.cpp:34-193, so there the keys have no effect)
+    bp_mode = hip.BP_SCATTER if (is3d and samples != [1, 1]) else hip.BP_GATHER
     sens = torch.empty((subsets,) + plan.image_shape(), dtype=torch.float32, device="cuda")
     t0 = time.perf_counter()
     liks, times, saved = [], [], []
@@ -144,7 +153,7 @@ def run(method, par_path, log=print):
             nxt = 1 if done == 0 else ((done - 1) // interval + 1) * interval + 1
             step = min(step, nxt - done)
         tm = np.zeros((4, step * subsets), np.float32)
-        img, lik = plan.reconstruct(meas, x, step, subsets=subsets, bp_mode=hip.BP_GATHER,
+        img, lik = plan.reconstruct(meas, x, step, subsets=subsets, bp_mode=bp_mode,
                                     osem_rules=osem_rules, additive=add, mult=mult,
                                     sens_out=sens, times_out=tm if step else None)
         liks.extend(lik[:step * subsets].tolist())

@@ -55,6 +55,7 @@ struct Geo {
   int dims;
   float r0, dr, rmargin;  // uniform radial grid r_j = r0 + j dr (gather candidate search)
   int walk24;  // voxel indices fit 24-bit multiplies (nx ny, nx, ny, nz below 2^23)
+  int ns, nas;  // LORs per bin: transaxial x axial samples (oversampled 3D Siddon)
 };
 
 // A run of consecutive sinograms with increasing mean axial coordinate (one segment
@@ -85,8 +86,8 @@ struct apirl_plan {
   apirl_image_desc img{};
   apirl_sino_desc sino{};   // pointers inside are not kept
   apirl::Geo geo{};
-  apirl::RayXY* d_rays = nullptr;  // [nang][nr]
-  float2* d_zab = nullptr;         // [nsino] (P1.Z, P2.Z), 3D only
+  apirl::RayXY* d_rays = nullptr;  // [nang][nr][ns]
+  float2* d_zab = nullptr;         // [nsino][nas] (P1.Z, P2.Z), 3D only
   float* d_replicas = nullptr;     // [8][nvox] per-XCD scatter targets (zero between calls)
   float* d_imgT = nullptr;         // [nz][nx][ny] transposed copy of the projected image
   int* d_zorder = nullptr;         // [nsino] sinograms sorted by mean axial coordinate
@@ -103,7 +104,9 @@ struct apirl_plan {
   int zblock = 0;                  // z-crossing table entries per (angle, radial bin)
   size_t gscratch_bytes = 0;
   std::vector<float> ang, r;       // host copies of the grids
-  std::vector<float2> zab;
+  std::vector<float2> zab;         // [nsino] one axial sample per bin
+  std::vector<float> z1m, z2m;     // [nsino] mean ring coordinates of each sinogram
+  bool gather_tables = false;      // gather tables built (uniform grid, one LOR per bin)
 };
 
 namespace apirl {
@@ -113,6 +116,11 @@ void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
                      const std::vector<float>& ang, const std::vector<float>& r,
                      std::vector<RayXY>& out);
 void build_z_table(const apirl_sino_desc& sino, std::vector<float2>& zab);
+// Oversampled bins (SiddonProjector.cpp:374-395, Sinogram2DinCylindrical3Dpet.cpp:
+// 283-299): ns radial sub-LORs per bin and nas axial ones per sinogram.
+void build_subsample_r(const std::vector<float>& r, int ns, std::vector<float>& r_sub);
+void build_z_table_os(const std::vector<float>& z1m, const std::vector<float>& z2m,
+                      float ring_width, int nas, std::vector<float2>& zab);
 int subset_num_angles(int nang, int nsubsets, int subset);
 
 // siddon_kernels.hip: launchers (enqueue on stream; return hipError_t)

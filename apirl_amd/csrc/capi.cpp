@@ -58,6 +58,12 @@ bool check_subset(const apirl_plan* p, int subset, int nsubsets) {
 }
 
 // Backprojection of one subset in either form.
+// Oversampled bins are traversed in the reference's arithmetic only (and need the
+// 24-bit index walk).
+bool os_precision_ok(const apirl_plan* p, int precision) {
+  return p->geo.ns * p->geo.nas == 1 || (precision == APIRL_PREC_REFERENCE && p->geo.walk24);
+}
+
 apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est, float* img,
                              int subset, int nsubsets, int kind, int mode,
                              int precision = APIRL_PREC_REFERENCE) {
@@ -73,7 +79,9 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     return APIRL_OK;
   }
   if (mode != APIRL_BP_GATHER) return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
-  if (!p->gather_ok) return fail(c, APIRL_ERR_UNSUPPORTED, "gather needs a uniform radial grid");
+  if (!p->gather_ok)
+    return fail(c, APIRL_ERR_UNSUPPORTED,
+                "gather needs a uniform radial grid and one LOR per bin (no oversampling)");
   if (kind == apirl::BP_DIVIDE)
     return fail(c, APIRL_ERR_UNSUPPORTED, "divide-and-backproject has the scatter form only");
   const size_t need = apirl::gather_scratch_bytes(p, subset, nsubsets);
@@ -220,6 +228,7 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
   geo.offz = im.dz / 2;
   geo.nr = sd.nr; geo.nang = sd.nang; geo.nsino = sd.nsino;
   geo.dims = sd.dims;
+  geo.ns = geo.nas = 1;
   {
     const long nxy = (long)im.nx * im.ny, lim = 1L << 23;
     geo.walk24 = (nxy < lim && im.nx < lim && im.ny < lim && im.nz < lim) ? 1 : 0;
@@ -235,6 +244,8 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
     return fail(ctx, APIRL_ERR_HIP, "cannot upload the LOR table");
   }
   if (sd.dims == 3) {
+    p->z1m.assign(sd.z1_mean_mm, sd.z1_mean_mm + sd.nsino);
+    p->z2m.assign(sd.z2_mean_mm, sd.z2_mean_mm + sd.nsino);
     apirl::build_z_table(sd, p->zab);
     if (hipMalloc(&p->d_zab, p->zab.size() * sizeof(float2)) != hipSuccess ||
         hipMemcpy(p->d_zab, p->zab.data(), p->zab.size() * sizeof(float2),
@@ -335,6 +346,7 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       (void)hipGetLastError();
     }
   }
+  p->gather_tables = p->gather_ok;
   *out = p;
   return APIRL_OK;
 }
@@ -379,6 +391,54 @@ apirl_status apirl_plan_intersections(apirl_plan* p, int subset, int nsubsets,
   return APIRL_OK;
 }
 
+apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
+  if (!p) return APIRL_ERR_INVALID;
+  apirl_ctx* c = p->ctx;
+  if (p->geo.dims != 3)
+    return fail(c, APIRL_ERR_UNSUPPORTED, "oversampled LORs exist for 3D sinograms only");
+  if (ns < 1 || nas < 1 || ns > 16 || nas > 16 || ns * nas > 64)
+    return fail(c, APIRL_ERR_INVALID, "samples per bin must be 1..16 each, 64 in all");
+  if (ns == p->geo.ns && nas == p->geo.nas) return APIRL_OK;
+  DeviceGuard g(c->device);
+  // radial sub-LORs: the bin's own LOR when ns == 1 (getPointsFromLor), the
+  // subsampled r otherwise (getPointsFromOverSampledLor)
+  std::vector<float> r_sub;
+  if (ns == 1) r_sub = p->r;
+  else apirl::build_subsample_r(p->r, ns, r_sub);
+  apirl_sino_desc sd = p->sino;
+  sd.nr = p->sino.nr * ns;
+  std::vector<apirl::RayXY> rays;
+  apirl::build_ray_table(p->img, sd, p->ang, r_sub, rays);
+  std::vector<float2> zab;
+  apirl::build_z_table_os(p->z1m, p->z2m, p->sino.ring_width_mm, nas, zab);
+  apirl::RayXY* d_rays = nullptr;
+  float2* d_zab = nullptr;
+  if (hipMalloc(&d_rays, rays.size() * sizeof(apirl::RayXY)) != hipSuccess ||
+      hipMalloc(&d_zab, zab.size() * sizeof(float2)) != hipSuccess) {
+    if (d_rays) (void)hipFree(d_rays);
+    (void)hipGetLastError();
+    return fail(c, APIRL_ERR_NOMEM, "cannot allocate the oversampled LOR tables");
+  }
+  if (hipMemcpy(d_rays, rays.data(), rays.size() * sizeof(apirl::RayXY),
+                hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d_zab, zab.data(), zab.size() * sizeof(float2), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    (void)hipFree(d_rays);
+    (void)hipFree(d_zab);
+    return fail(c, APIRL_ERR_HIP, "cannot upload the oversampled LOR tables");
+  }
+  APIRL_HIP(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(p->d_rays);
+  (void)hipFree(p->d_zab);
+  p->d_rays = d_rays;
+  p->d_zab = d_zab;
+  p->geo.ns = ns;
+  p->geo.nas = nas;
+  // the gather's candidate search and tables assume one LOR per bin
+  p->gather_ok = p->gather_tables && ns == 1 && nas == 1;
+  return APIRL_OK;
+}
+
 apirl_status apirl_plan_get_tuning(const apirl_plan* p, apirl_tuning* out) {
   if (!p || !out) return APIRL_ERR_INVALID;
   out->gather_zlanes = p->gather_zlanes;
@@ -407,6 +467,8 @@ apirl_status apirl_project(apirl_plan* p, const float* d_img, float* d_sino, int
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
   if (precision < 0 || precision > 2) return fail(p->ctx, APIRL_ERR_INVALID, "invalid precision");
+  if (!os_precision_ok(p, precision))
+    return fail(p->ctx, APIRL_ERR_UNSUPPORTED, "oversampled LORs use the reference precision");
   DeviceGuard g(p->ctx->device);
   APIRL_HIP(p->ctx, apirl::launch_project(p, d_img, d_sino, subset, nsubsets, precision,
                                           nullptr, nullptr, p->ctx->stream));
@@ -444,6 +506,8 @@ apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float*
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
   apirl_ctx* c = p->ctx;
+  if (!os_precision_ok(p, precision))
+    return fail(c, APIRL_ERR_UNSUPPORTED, "oversampled LORs use the reference precision");
   DeviceGuard g(c->device);
   const long nrays = (long)apirl_plan_subset_bins(p, subset, nsubsets);
   const int nblocks = (int)((nrays + 255) / 256);

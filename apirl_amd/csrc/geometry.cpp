@@ -126,6 +126,35 @@ void build_z_table(const apirl_sino_desc& sino, std::vector<float2>& zab) {
   }
 }
 
+void build_subsample_r(const std::vector<float>& r, int ns, std::vector<float>& r_sub) {
+  // r of sub-LOR o of bin j: r - inc/2 + inc/ns * o + inc/(2 ns), inc the bin's width
+  // from its neighbours (Sinogram2D.cpp:172-187), in float as written
+  const int nr = (int)r.size();
+  r_sub.resize((size_t)nr * ns);
+  for (int j = 0; j < nr; j++) {
+    float inc;
+    if (j > 0 && j < nr - 1) inc = (r[j + 1] - r[j - 1]) / 2;
+    else if (j == 0) inc = nr > 1 ? r[1] - r[0] : 0.f;
+    else inc = r[nr - 1] - r[nr - 2];
+    for (int o = 0; o < ns; o++)
+      r_sub[(size_t)j * ns + o] = r[j] - (inc / 2) + inc / ns * o + (inc / (2 * ns));
+  }
+}
+
+void build_z_table_os(const std::vector<float>& z1m, const std::vector<float>& z2m,
+                      float ring_width, int nas, std::vector<float2>& zab) {
+  // P.Z = zmean - w/2 + dz/2 + p dz, dz = w / nas (SiddonProjector.cpp:335-336, 394-395)
+  const size_t ns = z1m.size();
+  zab.resize(ns * nas);
+  const float w = ring_width;
+  const float dzs = w / nas;
+  for (size_t s = 0; s < ns; s++)
+    for (int q = 0; q < nas; q++) {
+      zab[s * nas + q].x = z1m[s] - w / 2 + dzs / 2 + q * dzs;
+      zab[s * nas + q].y = z2m[s] - w / 2 + dzs / 2 + q * dzs;
+    }
+}
+
 void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
                          std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,
                          bool& ok) {

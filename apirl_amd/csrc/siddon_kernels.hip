@@ -143,9 +143,8 @@ __device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, 
 // (the plan keeps nx ny below 2^23).
 template <bool TR>
 __device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
-                                        const float* __restrict__ imgT) {
+                                        const float* __restrict__ imgT, float acc = 0.f) {
   const int nxy = g.nx * g.ny;
-  float acc = 0.f;
   for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
     float wv[kUnroll], pv[kUnroll];
     bool tv[kUnroll];
@@ -210,17 +209,35 @@ __device__ __forceinline__ double project_ray(
     ray_coords(g, lidx, nang_sub, subset, nsubsets, s, a, rr, k);
     if (zorder) s = zorder[s];
     idx = ((long)s * nang_sub + k) * g.nr + rr;
-    const RayXY t = rays[(long)a * g.nr + rr];
+    // tables hold ns (transaxial) x nas (axial) LORs per bin; one each without
+    // oversampling
+    const long rbase = ((long)a * g.nr + rr) * g.ns;
+    const RayXY t = rays[rbase];
     float za = 0.f, zb = 0.f;
     if (DIMS == 3) {
-      const float2 z = zab[s];
+      const float2 z = zab[(long)s * g.nas];
       za = z.x;
       zb = z.y;
     }
     Ray ray;
     bool valid = setup_ray<DIMS>(g, t, za, zb, ray);
     float acc = 0.f;
-    if (valid) {
+    if (DIMS == 3 && PREC == APIRL_PREC_REFERENCE && g.walk24 && g.ns * g.nas > 1) {
+      // oversampled bin: its LORs (o, p) in the reference's order, every term added to
+      // the one bin total (SiddonProjector.cpp:366-425)
+      valid = false;
+      for (int o = 0; o < g.ns; o++) {
+        const RayXY to = rays[rbase + o];
+        for (int q = 0; q < g.nas; q++) {
+          const float2 z = zab[(long)s * g.nas + q];
+          Ray ro;
+          if (!setup_ray<DIMS>(g, to, z.x, z.y, ro)) continue;
+          valid = true;
+          if (imgT != nullptr && (to.flags & RAY_XMAJOR)) acc = walk3d<true>(g, ro, img, imgT, acc);
+          else acc = walk3d<false>(g, ro, img, imgT, acc);
+        }
+      }
+    } else if (valid) {
       if (PREC == APIRL_PREC_REFERENCE && DIMS == 3 && g.walk24) {
         if (imgT != nullptr && (t.flags & RAY_XMAJOR)) acc = walk3d<true>(g, ray, img, imgT);
         else acc = walk3d<false>(g, ray, img, imgT);
@@ -429,18 +446,21 @@ __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
     if (g.dims == 2 && e == 0.f) return;
   }
   if (g.dims == 3 && (KIND == BP_PLAIN || KIND == BP_MULT) && b == 0.f) return;
-  const RayXY t = rays[(long)a * g.nr + rr];
+  const long nxy = (long)g.nx * g.ny;
+  const long npix = nxy * g.nz;
+  // every LOR of the bin (ns x nas when oversampled, SiddonProjector.cpp:491-548)
+  for (int o = 0; o < g.ns; o++)
+  for (int q = 0; q < g.nas; q++) {
+  const RayXY t = rays[((long)a * g.nr + rr) * g.ns + o];
   float za = 0.f, zb = 0.f;
   if (g.dims == 3) {
-    const float2 z = zab[s];
+    const float2 z = zab[(long)s * g.nas + q];
     za = z.x;
     zb = z.y;
   }
   Ray ray;
   const bool valid = g.dims == 3 ? setup_ray<3>(g, t, za, zb, ray) : setup_ray<2>(g, t, za, zb, ray);
-  if (!valid) return;
-  const long nxy = (long)g.nx * g.ny;
-  const long npix = nxy * g.nz;
+  if (!valid) continue;
   for (int m = 0; m < ray.n; m++) {
     int ci, cj, ck;
     const float w = step(ray, ci, cj, ck);
@@ -460,6 +480,7 @@ __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
       if (li >= 0 && li < npix) unsafeAtomicAdd(&img[li], term);
     }
   }
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void count_kernel(Geo g, const RayXY* __restrict__ rays,
@@ -472,16 +493,19 @@ __global__ __launch_bounds__(kBlock) void count_kernel(Geo g, const RayXY* __res
   if (idx < nrays) {
     int s, a, rr, k;
     ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
-    const RayXY t = rays[(long)a * g.nr + rr];
-    float za = 0.f, zb = 0.f;
-    if (g.dims == 3) {
-      za = zab[s].x;
-      zb = zab[s].y;
-    }
-    Ray ray;
-    const bool valid =
-        g.dims == 3 ? setup_ray<3>(g, t, za, zb, ray) : setup_ray<2>(g, t, za, zb, ray);
-    if (valid) n = (unsigned long long)ray.n;
+    for (int o = 0; o < g.ns; o++)
+      for (int q = 0; q < g.nas; q++) {
+        const RayXY t = rays[((long)a * g.nr + rr) * g.ns + o];
+        float za = 0.f, zb = 0.f;
+        if (g.dims == 3) {
+          za = zab[(long)s * g.nas + q].x;
+          zb = zab[(long)s * g.nas + q].y;
+        }
+        Ray ray;
+        const bool valid =
+            g.dims == 3 ? setup_ray<3>(g, t, za, zb, ray) : setup_ray<2>(g, t, za, zb, ray);
+        if (valid) n += (unsigned long long)ray.n;
+      }
   }
   __shared__ unsigned long long red[kBlock];
   red[threadIdx.x] = n;

@@ -82,6 +82,7 @@ SIGNATURES = {
     "apirl_mlem_threshold": (_c_int, [_P, _P, _c_size, ctypes.POINTER(_c_float)]),
     "apirl_em_reconstruct": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P, _P,
                                       _P]),
+    "apirl_plan_set_oversampling": (_c_int, [_P, _c_int, _c_int]),
     "apirl_em_reconstruct_timed": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P,
                                             _P, _P, _P]),
     "apirl_batch_create": (_c_int, [_P, _c_int, _c_int, ctypes.POINTER(_P)]),
@@ -278,6 +279,12 @@ class Plan:
         t = Tuning(gather_zlanes=int(n))
         self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
                        "apirl_plan_set_tuning")
+
+    def set_oversampling(self, ns, nas=1):
+        """Trace every bin as ns x nas LORs (3D; the reference's SiddonProjector with
+        numSamples / numAxialSamples).  Gather and batches need ns = nas = 1."""
+        self.ctx.check(self.ctx.lib.apirl_plan_set_oversampling(self.h, int(ns), int(nas)),
+                       "apirl_plan_set_oversampling")
 
     @property
     def gather_zvox(self):

@@ -124,6 +124,14 @@ typedef struct apirl_tuning {
   int gather_zlanes;
   int gather_zvox;
 } apirl_tuning;
+/* Oversampled LORs (SiddonProjector(numSamples, numAxialSamples),
+ * recon/src/SiddonProjector.cpp:28-32, 366-425, 491-548): every bin of a 3D plan is
+ * traced as ns transaxial x nas axial LORs (Sinogram2DinCylindrical3Dpet.cpp:283-299),
+ * all of whose terms go to the bin.  ns = nas = 1 restores one LOR per bin.  The
+ * projector, the scatter backprojector (plain, sensitivity, divide) and the EM driver
+ * with APIRL_BP_SCATTER follow it; the gather backprojector and batches need one LOR
+ * per bin, and oversampled projections use APIRL_PREC_REFERENCE. */
+apirl_status apirl_plan_set_oversampling(apirl_plan* plan, int ns, int nas);
 apirl_status apirl_plan_get_tuning(const apirl_plan* plan, apirl_tuning* out);
 apirl_status apirl_plan_set_tuning(apirl_plan* plan, const apirl_tuning* tuning);
 

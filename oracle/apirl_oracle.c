@@ -336,6 +336,127 @@ void orc_backproject3d(const orc_image* im, const float* sino, int nang, const f
   free(segs);
 }
 
+/* ------------------------------------------------------- oversampled bins */
+
+void orc_subsample_r(int nr, const float* r, int ns, float* r_sub) {
+  /* Sub-LOR o of bin j: r - inc/2 + inc/ns*o + inc/(2 ns)
+   * (Sinogram2DinCylindrical3Dpet.cpp:283-288), inc from the neighbouring bins
+   * (Sinogram2D.cpp:172-187). */
+  for (int j = 0; j < nr; j++) {
+    float inc;
+    if (j > 0 && j < nr - 1) inc = (r[j + 1] - r[j - 1]) / 2;
+    else if (j == 0) inc = r[1] - r[0];
+    else inc = r[nr - 1] - r[nr - 2];
+    for (int o = 0; o < ns; o++)
+      r_sub[(long)j * ns + o] = r[j] - (inc / 2) + inc / ns * o + (inc / (2 * ns));
+  }
+}
+
+int orc_ring_table_os(int nrings, float zfov, int nseg, const int* nsino, const int* minrd,
+                      const int* maxrd, int nas, float* za, float* zb) {
+  /* As orc_ring_table, with nas axial samples per sinogram: P.Z = zmean - w/2 + dz/2
+   * + p dz, dz = w / nas (SiddonProjector.cpp:335-336, 394-395). */
+  float* zring = (float*)malloc(sizeof(float) * nrings);
+  float zinc = (float)zfov / nrings;
+  for (int i = 0; i < nrings; i++) zring[i] = zinc / 2 + i * zinc;
+  float w = zfov / nrings;
+  float dzs = w / nas;
+  int total = 0;
+  for (int s = 0; s < nseg; s++) {
+    int count = 0;
+    for (int z1 = 0; z1 < 2 * nrings; z1++) {
+      int n = 0, first1 = 0, first2 = 0, last1 = 0, last2 = 0;
+      int z1a = z1;
+      for (int z2 = 0; z2 < nrings; z2++) {
+        if ((z1a - z2) <= maxrd[s] && (z1a - z2) >= minrd[s] && z1a >= 0 && z2 >= 0 &&
+            z1a < nrings && z2 < nrings) {
+          if (n == 0) { first1 = z1a; first2 = z2; }
+          last1 = z1a; last2 = z2;
+          n++;
+        }
+        z1a--;
+      }
+      if (n > 0) {
+        float z1m = (zring[first1] + zring[last1]) / 2;
+        float z2m = (zring[first2] + zring[last2]) / 2;
+        for (int q = 0; q < nas; q++) {
+          za[(long)total * nas + q] = z1m - w / 2 + dzs / 2 + q * dzs;
+          zb[(long)total * nas + q] = z2m - w / 2 + dzs / 2 + q * dzs;
+        }
+        total++;
+        count++;
+      }
+    }
+    if (count != nsino[s]) { free(zring); return -1; }
+  }
+  free(zring);
+  return total;
+}
+
+/* LOR (o, q) of an oversampled bin: xy from the sub-LOR's r, z from the axial sample. */
+static void lor3d_os(const float* ang_deg, const float* r_sub, int ns, float rscanner,
+                     const float* za, const float* zb, int nas, int s, int a, int r, int o,
+                     int q, float* p, float* v) {
+  float p1x, p1y, p2x, p2y;
+  orc_lor_points(ang_deg[a], r_sub[(long)r * ns + o], rscanner, INFINITY, &p1x, &p1y, &p2x, &p2y);
+  p[0] = p1x; p[1] = p1y; p[2] = za[(long)s * nas + q];
+  v[0] = p2x - p1x; v[1] = p2y - p1y; v[2] = zb[(long)s * nas + q] - za[(long)s * nas + q];
+}
+
+void orc_project3d_os(const orc_image* im, const float* img, int nang, const float* ang_deg,
+                      int nr, const float* r_sub, int ns, float rscanner, int nsino,
+                      const float* za, const float* zb, int nas, float* sino) {
+  /* SiddonProjector.cpp:366-425: every LOR (o, q) of the bin adds its terms to the bin
+   * (r_sub = the bins' own r when ns == 1, getPointsFromLor). */
+  orc_seg* segs = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  long nxy = (long)im->nx * im->ny, npix = nxy * im->nz;
+  for (int s = 0; s < nsino; s++)
+    for (int a = 0; a < nang; a++)
+      for (int r = 0; r < nr; r++) {
+        float acc = 0;
+        for (int o = 0; o < ns; o++)
+          for (int q = 0; q < nas; q++) {
+            float p[3], v[3];
+            lor3d_os(ang_deg, r_sub, ns, rscanner, za, zb, nas, s, a, r, o, q, p, v);
+            int n = orc_siddon3d(p[0], p[1], p[2], v[0], v[1], v[2], im, segs, SEG_CAP);
+            for (int l = 0; l < n; l++)
+              if (segs[l].i < im->nx && segs[l].j < im->ny && segs[l].k < im->nz)
+                acc += segs[l].w * 1.0f *
+                       pix_at(img, npix,
+                              (long)segs[l].k * nxy + (long)segs[l].j * im->nx + segs[l].i);
+          }
+        sino[((long)s * nang + a) * nr + r] = acc;
+      }
+  free(segs);
+}
+
+void orc_backproject3d_os(const orc_image* im, const float* sino, int nang,
+                          const float* ang_deg, int nr, const float* r_sub, int ns,
+                          float rscanner, int nsino, const float* za, const float* zb, int nas,
+                          float* img) {
+  /* SiddonProjector.cpp:491-548: zero bins skipped, every LOR (o, q) of the bin. */
+  orc_seg* segs = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  long nxy = (long)im->nx * im->ny;
+  for (int s = 0; s < nsino; s++)
+    for (int a = 0; a < nang; a++)
+      for (int r = 0; r < nr; r++) {
+        float b = sino[((long)s * nang + a) * nr + r];
+        if (b == 0) continue;
+        for (int o = 0; o < ns; o++)
+          for (int q = 0; q < nas; q++) {
+            float p[3], v[3];
+            lor3d_os(ang_deg, r_sub, ns, rscanner, za, zb, nas, s, a, r, o, q, p, v);
+            int n = orc_siddon3d(p[0], p[1], p[2], v[0], v[1], v[2], im, segs, SEG_CAP);
+            for (int l = 0; l < n; l++)
+              if (segs[l].k < im->nz && segs[l].j < im->ny && segs[l].i < im->nx &&
+                  segs[l].k >= 0 && segs[l].j >= 0 && segs[l].i >= 0)
+                img[(long)segs[l].k * nxy + (long)segs[l].j * im->nx + segs[l].i] +=
+                    segs[l].w * 1.0f * b;
+          }
+      }
+  free(segs);
+}
+
 void orc_divide_backproject3d(const orc_image* im, const float* meas, const float* est,
                               int nang, const float* ang_deg, int nr, const float* r_mm,
                               float rscanner, int nsino, const float* za, const float* zb,

@@ -69,6 +69,16 @@ void orc_project3d(const orc_image* im, const float* img, int nang, const float*
 
 /* Backprojectors accumulate into img in the reference's single-thread order
  * (SiddonProjector.cpp:34-87 for 2D, :444-568 for 3D). */
+void orc_subsample_r(int nr, const float* r, int ns, float* r_sub);
+int orc_ring_table_os(int nrings, float zfov, int nseg, const int* nsino, const int* minrd,
+                      const int* maxrd, int nas, float* za, float* zb);
+void orc_project3d_os(const orc_image* im, const float* img, int nang, const float* ang_deg,
+                      int nr, const float* r_sub, int ns, float rscanner, int nsino,
+                      const float* za, const float* zb, int nas, float* sino);
+void orc_backproject3d_os(const orc_image* im, const float* sino, int nang,
+                          const float* ang_deg, int nr, const float* r_sub, int ns,
+                          float rscanner, int nsino, const float* za, const float* zb, int nas,
+                          float* img);
 void orc_divide_backproject2d(const orc_image* im, const float* meas, const float* est,
                               int nang, const float* ang_deg, int nr, const float* r_mm,
                               float rfov, float rscanner, float* img);

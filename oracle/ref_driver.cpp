@@ -267,6 +267,9 @@ Projector* make_projector() {
     exit(7);
 #endif
   }
+  if (has_arg("ns") || has_arg("nas"))  // oversampled LORs (SiddonProjector.cpp:28-32)
+    return (Projector*)new SiddonProjector(has_arg("ns") ? iarg("ns") : 1,
+                                           has_arg("nas") ? iarg("nas") : 1);
   return (Projector*)new SiddonProjector();
 }
 

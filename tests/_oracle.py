@@ -46,6 +46,10 @@ def lib():
             "orc_project3d": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_backproject2d": (None, [P, P, _i, P, _i, P, _f, _f, P]),
             "orc_divide_backproject2d": (None, [P, P, P, _i, P, _i, P, _f, _f, P]),
+            "orc_subsample_r": (None, [_i, P, _i, P]),
+            "orc_ring_table_os": (_i, [_i, _f, _i, P, P, P, _i, P, P]),
+            "orc_project3d_os": (None, [P, P, _i, P, _i, P, _i, _f, _i, P, P, _i, P]),
+            "orc_backproject3d_os": (None, [P, P, _i, P, _i, P, _i, _f, _i, P, P, _i, P]),
             "orc_backproject3d": (None, [P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_divide_backproject3d": (None, [P, P, P, _i, P, _i, P, _f, _i, P, P, P]),
             "orc_inverse_divide": (None, [P, P, _l]),
@@ -114,6 +118,46 @@ def ring_table(rings, zfov, span, mrd):
     n = lib().orc_ring_table(rings, zfov, len(nsino), _p(nsino), _p(mn), _p(mx), _p(za), _p(zb))
     assert n == tot, (n, tot)
     return za, zb
+
+
+def subsample_r(r, ns):
+    """Radial positions of the ns sub-LORs of every bin (oversampled 3D Siddon)."""
+    r = f32(r)
+    out = np.zeros(r.size * ns, np.float32)
+    lib().orc_subsample_r(r.size, _p(r), ns, _p(out))
+    return out
+
+
+def ring_table_os(rings, zfov, span, mrd, nas):
+    nsino, mn, mx = seg_table(rings, span, mrd)
+    tot = int(nsino.sum())
+    za = np.zeros(tot * nas, np.float32)
+    zb = np.zeros(tot * nas, np.float32)
+    n = lib().orc_ring_table_os(rings, zfov, len(nsino), _p(nsino), _p(mn), _p(mx), nas, _p(za),
+                                _p(zb))
+    assert n == tot, (n, tot)
+    return za, zb
+
+
+def project3d_os(im, img, ang, r_sub, ns, rscanner, za, zb, nas):
+    img = f32(img)
+    nr = r_sub.size // ns
+    nsino = za.size // nas
+    out = np.zeros((nsino, len(ang), nr), np.float32)
+    lib().orc_project3d_os(ctypes.byref(im), _p(img), len(ang), _p(f32(ang)), nr, _p(f32(r_sub)),
+                           ns, rscanner, nsino, _p(f32(za)), _p(f32(zb)), nas, _p(out))
+    return out
+
+
+def backproject3d_os(im, sino, ang, r_sub, ns, rscanner, za, zb, nas):
+    sino = f32(sino)
+    nr = r_sub.size // ns
+    nsino = za.size // nas
+    img = np.zeros((im.nz, im.ny, im.nx), np.float32)
+    lib().orc_backproject3d_os(ctypes.byref(im), _p(sino), len(ang), _p(f32(ang)), nr,
+                               _p(f32(r_sub)), ns, rscanner, nsino, _p(f32(za)), _p(f32(zb)), nas,
+                               _p(img))
+    return img
 
 
 def siddon2d(p0, v, im, cap=4096):

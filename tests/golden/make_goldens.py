@@ -196,6 +196,21 @@ def multislice():
          volume=vol)
 
 
+def oversampled():
+    """The mid-size 3D geometry traced with 3 radial x 2 axial LORs per bin
+    (SiddonProjector(3, 2)): projection of the cylinder phantom and backprojection of
+    the measured sinogram."""
+    g = MID
+    f = np.load(os.path.join(HERE, "mid3d_project_backproject.npz"))
+    f["img"].astype(np.float32).tofile(tmpf("os_img.f32"))
+    f["meas"].astype(np.float32).tofile(tmpf("os_meas.f32"))
+    run("proj3d", **geom3d_args(g), ns=3, nas=2, img=tmpf("os_img.f32"), out=tmpf("os_proj.f32"))
+    run("back3d", **geom3d_args(g), ns=3, nas=2, sino=tmpf("os_meas.f32"),
+        out=tmpf("os_back.f32"))
+    save("oversampled.npz", geom=np.array(json.dumps(dict(g, ns=3, nas=2))),
+         proj=read("os_proj.f32"), back=read("os_back.f32").reshape(17, 64, 64))
+
+
 C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
           rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
 
@@ -217,7 +232,7 @@ def c2_rays():
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT)
-    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice"]
+    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice", "oversampled"]
     if "lists" in which:
         siddon_lists()
     if "c1" in which:
@@ -231,3 +246,5 @@ if __name__ == "__main__":
         divide_backproject()
     if "multislice" in which:
         multislice()
+    if "oversampled" in which:
+        oversampled()

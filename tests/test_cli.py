@@ -61,8 +61,8 @@ def test_duplicate_key(files):
 
 
 @pytest.mark.parametrize("extra,match", [
-    ("siddon number of samples on the detector := 2\n", "one sample per bin"),
-    ("siddon number of axial samples on the detector := 3\n", "one sample per bin"),
+    ("siddon number of samples on the detector := 0\n", "at least one sample"),
+    ("siddon number of axial samples on the detector := x\n", "at least one sample"),
     ("sensitivity filename := sens.h33\n", "not available on this path"),
     ("attenuation image filename := mu.h33\n", "not available on this path"),
 ])
