@@ -5,7 +5,8 @@
 
 The commands mirror cmd/src/OSEM.cpp and cmd/src/MLEM.cpp for the inputs this path
 covers.
-  * Input types: "Sinogram3D" (cylindrical scanner), "Sinogram2D" (2D MLEM under
+  * Input types: "Sinogram3D" (cylindrical scanner), "Sinogram3DSiemensMmr" (the mMR's
+    LOR geometry and constants, scatter backprojection), "Sinogram2D" (2D MLEM under
     either command, as in the reference) and "Sinograms2D" /
     "Sinograms2DinCylindrical3Dpet" (slice-by-slice 2D MLEM, Mlem2dMultiple, run as
     one batch; outputs <prefix>_slice_<i>_{sensitivity,final,iter_<t>}.h33 and
@@ -105,6 +106,21 @@ def run(method, par_path, log=print):
             plan.set_oversampling(*samples)
         meas = torch.from_numpy(sino["bins"]).cuda()
         is3d = True
+    elif itype == "Sinogram3DSiemensMmr":
+        # OSEM.cpp:395-403: the scanner's constants, no cylindrical keys
+        # (Sinogram3DSiemensMmr.cpp:9-27); the mMR LOR geometry (apirl_plan_set_scanner)
+        sino = read_sinogram3d(infile)
+        zfov = 260.0
+        z1, z2 = hip.ring_table_from(sino["rings"], zfov, sino["nsino"], sino["minrd"],
+                                     sino["maxrd"])
+        plan = hip.Plan(ctx, nx, ny, nz, dx, dz, sino["nproj"], sino["nr"], 297.0, 328.0,
+                        dims=3, z1=z1, z2=z2,
+                        ring_width=np.float32(zfov) / np.float32(sino["rings"]))
+        plan.set_scanner("mmr")
+        if samples != [1, 1]:
+            raise FormatError("oversampled LORs are defined for the cylindrical scanner")
+        meas = torch.from_numpy(sino["bins"]).cuda()
+        is3d = True
     elif itype == "Sinogram2D":
         # Mlem2d takes the radial FOV from the image, (dx * nx) / 2 in float
         # (Mlem2d.cpp:12, Images.cpp:378); the reference leaves the scanner radius of
@@ -141,6 +157,8 @@ def run(method, par_path, log=print):
     # backprojected by the scatter form (2D projectors have no oversampling,
     # SiddonProjector.cpp:34-193, so there the keys have no effect)
     bp_mode = hip.BP_SCATTER if (is3d and samples != [1, 1]) else hip.BP_GATHER
+    if itype == "Sinogram3DSiemensMmr":
+        bp_mode = hip.BP_SCATTER  # the gather's candidate search needs a uniform grid
     sens = torch.empty((subsets,) + plan.image_shape(), dtype=torch.float32, device="cuda")
     t0 = time.perf_counter()
     liks, times, saved = [], [], []

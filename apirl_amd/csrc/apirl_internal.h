@@ -107,6 +107,7 @@ struct apirl_plan {
   std::vector<float2> zab;         // [nsino] one axial sample per bin
   std::vector<float> z1m, z2m;     // [nsino] mean ring coordinates of each sinogram
   bool gather_tables = false;      // gather tables built (uniform grid, one LOR per bin)
+  int scanner = 0;                 // APIRL_SCANNER_*: geometry of the LOR endpoints
 };
 
 namespace apirl {
@@ -116,6 +117,12 @@ void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
                      const std::vector<float>& ang, const std::vector<float>& r,
                      std::vector<RayXY>& out);
 void build_z_table(const apirl_sino_desc& sino, std::vector<float2>& zab);
+// Ray records from explicit transaxial LOR endpoints (x1, y1, x2, y2) per bin; ok[n]
+// false marks a LOR the geometry rejects.
+void build_ray_table_pts(const apirl_image_desc& img, bool is2d, const std::vector<float4>& pts,
+                         const std::vector<char>& ok, std::vector<RayXY>& out);
+// Siemens Biograph mMR LOR endpoints of a nang x nr sinogram.
+void mmr_lor_points(int nang, int nr, std::vector<float4>& pts);
 // Oversampled bins (SiddonProjector.cpp:374-395, Sinogram2DinCylindrical3Dpet.cpp:
 // 283-299): ns radial sub-LORs per bin and nas axial ones per sinogram.
 void build_subsample_r(const std::vector<float>& r, int ns, std::vector<float>& r_sub);

@@ -399,6 +399,8 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
   if (ns < 1 || nas < 1 || ns > 16 || nas > 16 || ns * nas > 64)
     return fail(c, APIRL_ERR_INVALID, "samples per bin must be 1..16 each, 64 in all");
   if (ns == p->geo.ns && nas == p->geo.nas) return APIRL_OK;
+  if (p->scanner != APIRL_SCANNER_CYLINDRICAL)
+    return fail(c, APIRL_ERR_UNSUPPORTED, "oversampled LORs are defined for the cylindrical scanner");
   DeviceGuard g(c->device);
   // radial sub-LORs: the bin's own LOR when ns == 1 (getPointsFromLor), the
   // subsampled r otherwise (getPointsFromOverSampledLor)
@@ -436,6 +438,46 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
   p->geo.nas = nas;
   // the gather's candidate search and tables assume one LOR per bin
   p->gather_ok = p->gather_tables && ns == 1 && nas == 1;
+  return APIRL_OK;
+}
+
+apirl_status apirl_plan_set_scanner(apirl_plan* p, int scanner) {
+  if (!p) return APIRL_ERR_INVALID;
+  apirl_ctx* c = p->ctx;
+  if (scanner != APIRL_SCANNER_CYLINDRICAL && scanner != APIRL_SCANNER_MMR)
+    return fail(c, APIRL_ERR_INVALID, "unknown scanner");
+  if (scanner == p->scanner) return APIRL_OK;
+  if (p->geo.dims != 3)
+    return fail(c, APIRL_ERR_UNSUPPORTED, "scanner geometries are set on 3D plans");
+  if (p->geo.ns * p->geo.nas != 1)
+    return fail(c, APIRL_ERR_UNSUPPORTED, "set the scanner before oversampling");
+  DeviceGuard g(c->device);
+  std::vector<apirl::RayXY> rays;
+  if (scanner == APIRL_SCANNER_MMR) {
+    std::vector<float4> pts;
+    apirl::mmr_lor_points(p->geo.nang, p->geo.nr, pts);
+    std::vector<char> ok(pts.size(), 1);  // the 3D form rejects no LOR
+    apirl::build_ray_table_pts(p->img, false, pts, ok, rays);
+  } else {
+    apirl::build_ray_table(p->img, p->sino, p->ang, p->r, rays);
+  }
+  apirl::RayXY* d_rays = nullptr;
+  if (hipMalloc(&d_rays, rays.size() * sizeof(apirl::RayXY)) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(c, APIRL_ERR_NOMEM, "cannot allocate the LOR table");
+  }
+  if (hipMemcpy(d_rays, rays.data(), rays.size() * sizeof(apirl::RayXY), hipMemcpyHostToDevice) !=
+      hipSuccess) {
+    (void)hipFree(d_rays);
+    return fail(c, APIRL_ERR_HIP, "cannot upload the LOR table");
+  }
+  APIRL_HIP(c, hipStreamSynchronize(c->stream));
+  (void)hipFree(p->d_rays);
+  p->d_rays = d_rays;
+  p->scanner = scanner;
+  // the gather's candidate search assumes the plan's uniform radial grid and one
+  // angle per projection
+  p->gather_ok = p->gather_tables && scanner == APIRL_SCANNER_CYLINDRICAL;
   return APIRL_OK;
 }
 

@@ -46,17 +46,59 @@ void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
                      const std::vector<float>& ang, const std::vector<float>& r,
                      std::vector<RayXY>& out) {
   const bool is2d = sino.dims == 2;
+  std::vector<float4> pts((size_t)sino.nang * sino.nr);
+  std::vector<char> ok(pts.size());
+  for (int a = 0; a < sino.nang; a++)
+    for (int j = 0; j < sino.nr; j++) {
+      float4& q = pts[(size_t)a * sino.nr + j];
+      q = float4{0, 0, 0, 0};
+      ok[(size_t)a * sino.nr + j] = lor_points(ang[a], r[j], sino.rscanner_mm, sino.rfov_mm, is2d,
+                                               &q.x, &q.y, &q.z, &q.w);
+    }
+  build_ray_table_pts(img, is2d, pts, ok, out);
+}
+
+void mmr_lor_points(int nang, int nr, std::vector<float4>& pts) {
+  // Sinogram2DinSiemensMmr.cpp:27-50 (grids) and :175-192 (getPointsFromLor): bins of
+  // one detector element on the cylinder, r = (Rs + DOI cos(lr / Rs)) sin(lr / Rs) with
+  // lr the arc length of the bin centre; odd bins at half an angular step less; the
+  // LOR ends on the radius Rs + DOI.
+  const float Rs = 328.0f, doi = 9.6f, bin0 = 4.0891f / 2.0f;
+  const float bin = nr == 344 ? bin0 : bin0 * 344 / nr;
+  std::vector<float> r(nr), ang(nang);
+  for (int j = 0; j < nr; j++) {
+    const float lr = bin / 2 + (bin * (j - (float)(nr / 2)));
+    r[j] = (Rs + doi * cosf(lr / Rs)) * sinf(lr / Rs);
+  }
+  const float inc = (float)180.0f / nang;
+  for (int a = 0; a < nang; a++) ang[a] = a * inc;
+  const float step = nang > 1 ? ang[1] - ang[0] : 0.f;
+  const float Re = Rs + doi;
+  pts.resize((size_t)nang * nr);
+  for (int a = 0; a < nang; a++)
+    for (int j = 0; j < nr; j++) {
+      const float rad = (ang[a] - (j % 2) * step / 2) * kDegToRad;
+      const float aux = sqrtf(Re * Re - r[j] * r[j]);
+      const float c = cosf(rad), sn = sinf(rad);
+      float4& q = pts[(size_t)a * nr + j];
+      q.x = r[j] * c + sn * aux;
+      q.y = r[j] * sn - c * aux;
+      q.z = r[j] * c - sn * aux;
+      q.w = r[j] * sn + c * aux;
+    }
+}
+
+void build_ray_table_pts(const apirl_image_desc& img, bool is2d, const std::vector<float4>& pts,
+                         const std::vector<char>& ok, std::vector<RayXY>& out) {
   const float dx = img.dx, dy = img.dy;
   const float R = (dx * img.nx) / 2;
-  out.resize((size_t)sino.nang * sino.nr);
-  for (int a = 0; a < sino.nang; a++) {
-    for (int j = 0; j < sino.nr; j++) {
-      RayXY& t = out[(size_t)a * sino.nr + j];
+  out.resize(pts.size());
+  for (size_t n = 0; n < pts.size(); n++) {
+    {
+      RayXY& t = out[n];
       t = RayXY{};
-      float p1x = 0, p1y = 0, p2x = 0, p2y = 0;
-      bool valid = lor_points(ang[a], r[j], sino.rscanner_mm, sino.rfov_mm, is2d, &p1x, &p1y,
-                              &p2x, &p2y);
-      if (!valid) {
+      const float p1x = pts[n].x, p1y = pts[n].y, p2x = pts[n].z, p2y = pts[n].w;
+      if (!ok[n]) {
         t.flags = ray_flags(0, 0, 0, 0);
         continue;
       }

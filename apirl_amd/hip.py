@@ -83,6 +83,7 @@ SIGNATURES = {
     "apirl_em_reconstruct": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P, _P,
                                       _P]),
     "apirl_plan_set_oversampling": (_c_int, [_P, _c_int, _c_int]),
+    "apirl_plan_set_scanner": (_c_int, [_P, _c_int]),
     "apirl_em_reconstruct_timed": (_c_int, [_P, ctypes.POINTER(EmOptions), _P, _P, _P, _P, _P,
                                             _P, _P, _P]),
     "apirl_batch_create": (_c_int, [_P, _c_int, _c_int, ctypes.POINTER(_P)]),
@@ -279,6 +280,14 @@ class Plan:
         t = Tuning(gather_zlanes=int(n))
         self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
                        "apirl_plan_set_tuning")
+
+    SCANNERS = {"cylindrical": 0, "mmr": 1}
+
+    def set_scanner(self, name):
+        """LOR geometry of a 3D plan: "cylindrical" (the plan's grids) or "mmr"
+        (Siemens Biograph mMR: arc-corrected r, half-angle interleave, DOI radius)."""
+        self.ctx.check(self.ctx.lib.apirl_plan_set_scanner(self.h, self.SCANNERS[name]),
+                       "apirl_plan_set_scanner")
 
     def set_oversampling(self, ns, nas=1):
         """Trace every bin as ns x nas LORs (3D; the reference's SiddonProjector with

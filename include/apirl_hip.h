@@ -132,6 +132,19 @@ typedef struct apirl_tuning {
  * with APIRL_BP_SCATTER follow it; the gather backprojector and batches need one LOR
  * per bin, and oversampled projections use APIRL_PREC_REFERENCE. */
 apirl_status apirl_plan_set_oversampling(apirl_plan* plan, int ns, int nas);
+
+/* Scanner geometry of a 3D plan's LOR endpoints.  APIRL_SCANNER_CYLINDRICAL: the
+ * plan's angle and radial grids on the cylinder of rscanner_mm
+ * (Sinogram2DinCylindrical3Dpet.cpp:216-229).  APIRL_SCANNER_MMR: the Siemens
+ * Biograph mMR (Sinogram2DinSiemensMmr.cpp:27-50, 175-192; constants
+ * Sinogram3DSiemensMmr.cpp:9-27): arc-corrected radial positions of one detector
+ * element per bin, odd bins half an angular step earlier, LORs ending on the radius
+ * 328 + 9.6 mm (mean depth of interaction).  The plan's nang x nr sinogram grid
+ * keeps its shape; the ring table (z1, z2 and the ring width 260 mm / rings) stays
+ * the caller's.  The gather backprojector and batches need the cylindrical form. */
+#define APIRL_SCANNER_CYLINDRICAL 0
+#define APIRL_SCANNER_MMR 1
+apirl_status apirl_plan_set_scanner(apirl_plan* plan, int scanner);
 apirl_status apirl_plan_get_tuning(const apirl_plan* plan, apirl_tuning* out);
 apirl_status apirl_plan_set_tuning(apirl_plan* plan, const apirl_tuning* tuning);
 

@@ -457,6 +457,82 @@ void orc_backproject3d_os(const orc_image* im, const float* sino, int nang,
   free(segs);
 }
 
+/* ------------------------------------------------------------ mMR geometry */
+
+void orc_mmr_lor_table(int nang, int nr, float* pts) {
+  /* Sinogram2DinSiemensMmr.cpp:27-50: bin = 4.0891/2 mm (scaled by 344/nr),
+   * lr = bin/2 + bin (j - (float)(nr/2)), r = (Rs + DOI cos(lr/Rs)) sin(lr/Rs),
+   * phi_i = i 180/nang; :175-192: odd bins at phi - step/2, LOR ends on Rs + DOI.
+   * pts[(a nr + j) 4 ..] = x1, y1, x2, y2. */
+  const float Rs = 328.0f, doi = 9.6f, bin0 = 4.0891f / 2.0f;
+  float bin = nr == 344 ? bin0 : bin0 * 344 / nr;
+  float* r = (float*)malloc(sizeof(float) * nr);
+  float* ang = (float*)malloc(sizeof(float) * nang);
+  for (int j = 0; j < nr; j++) {
+    float lr = bin / 2 + (bin * (j - (float)(nr / 2)));
+    r[j] = (Rs + doi * cosf(lr / Rs)) * sinf(lr / Rs);
+  }
+  float inc = (float)180 / nang;
+  for (int a = 0; a < nang; a++) ang[a] = a * inc;
+  float step = nang > 1 ? ang[1] - ang[0] : 0.f;
+  float Re = Rs + doi;
+  for (int a = 0; a < nang; a++)
+    for (int j = 0; j < nr; j++) {
+      float phi = (ang[a] - (j % 2) * step / 2) * 0.01745329251994329576f;
+      float aux = sqrtf(Re * Re - r[j] * r[j]);
+      float* q = pts + ((long)a * nr + j) * 4;
+      q[0] = r[j] * cosf(phi) + sinf(phi) * aux;
+      q[1] = r[j] * sinf(phi) - cosf(phi) * aux;
+      q[2] = r[j] * cosf(phi) - sinf(phi) * aux;
+      q[3] = r[j] * sinf(phi) + cosf(phi) * aux;
+    }
+  free(r);
+  free(ang);
+}
+
+void orc_project3d_pts(const orc_image* im, const float* img, int nang, int nr, const float* pts,
+                       int nsino, const float* za, const float* zb, float* sino) {
+  /* SiddonProjector.cpp:322-441 with the LOR endpoints of each bin given. */
+  orc_seg* segs = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  long nxy = (long)im->nx * im->ny, npix = nxy * im->nz;
+  for (int s = 0; s < nsino; s++)
+    for (int a = 0; a < nang; a++)
+      for (int r = 0; r < nr; r++) {
+        const float* q = pts + ((long)a * nr + r) * 4;
+        int n = orc_siddon3d(q[0], q[1], za[s], q[2] - q[0], q[3] - q[1], zb[s] - za[s], im, segs,
+                             SEG_CAP);
+        float acc = 0;
+        for (int l = 0; l < n; l++)
+          if (segs[l].i < im->nx && segs[l].j < im->ny && segs[l].k < im->nz)
+            acc += segs[l].w * 1.0f *
+                   pix_at(img, npix, (long)segs[l].k * nxy + (long)segs[l].j * im->nx + segs[l].i);
+        sino[((long)s * nang + a) * nr + r] = acc;
+      }
+  free(segs);
+}
+
+void orc_backproject3d_pts(const orc_image* im, const float* sino, int nang, int nr,
+                           const float* pts, int nsino, const float* za, const float* zb,
+                           float* img) {
+  /* SiddonProjector.cpp:444-568 with the LOR endpoints of each bin given. */
+  orc_seg* segs = (orc_seg*)malloc(sizeof(orc_seg) * SEG_CAP);
+  long nxy = (long)im->nx * im->ny;
+  for (int s = 0; s < nsino; s++)
+    for (int a = 0; a < nang; a++)
+      for (int r = 0; r < nr; r++) {
+        float b = sino[((long)s * nang + a) * nr + r];
+        if (b == 0) continue;
+        const float* q = pts + ((long)a * nr + r) * 4;
+        int n = orc_siddon3d(q[0], q[1], za[s], q[2] - q[0], q[3] - q[1], zb[s] - za[s], im, segs,
+                             SEG_CAP);
+        for (int l = 0; l < n; l++)
+          if (segs[l].k < im->nz && segs[l].j < im->ny && segs[l].i < im->nx && segs[l].k >= 0 &&
+              segs[l].j >= 0 && segs[l].i >= 0)
+            img[(long)segs[l].k * nxy + (long)segs[l].j * im->nx + segs[l].i] += segs[l].w * 1.0f * b;
+      }
+  free(segs);
+}
+
 void orc_divide_backproject3d(const orc_image* im, const float* meas, const float* est,
                               int nang, const float* ang_deg, int nr, const float* r_mm,
                               float rscanner, int nsino, const float* za, const float* zb,

@@ -69,6 +69,12 @@ void orc_project3d(const orc_image* im, const float* img, int nang, const float*
 
 /* Backprojectors accumulate into img in the reference's single-thread order
  * (SiddonProjector.cpp:34-87 for 2D, :444-568 for 3D). */
+void orc_mmr_lor_table(int nang, int nr, float* pts);
+void orc_project3d_pts(const orc_image* im, const float* img, int nang, int nr, const float* pts,
+                       int nsino, const float* za, const float* zb, float* sino);
+void orc_backproject3d_pts(const orc_image* im, const float* sino, int nang, int nr,
+                           const float* pts, int nsino, const float* za, const float* zb,
+                           float* img);
 void orc_subsample_r(int nr, const float* r, int ns, float* r_sub);
 int orc_ring_table_os(int nrings, float zfov, int nseg, const int* nsino, const int* minrd,
                       const int* maxrd, int nas, float* za, float* zb);

@@ -24,6 +24,7 @@
 //                                                       the on-disk formats
 //   par_read file method key / itf_read header key     one key lookup: "rc<TAB>value"
 //   log_row   n in out                                 Logger::writeRowOfNumbers
+//   mmr3d     header nx ny nz dx dz img out_proj out_back   mMR projection / backprojection
 //   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
 // The reference's 2D projector checks only lower bounds and its 3D projector only
 // upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
@@ -36,6 +37,7 @@
 #undef private
 #include <Sinogram2DinCylindrical3Dpet.h>
 #include <Sinogram3DCylindricalPet.h>
+#include <Sinogram3DSiemensMmr.h>
 #include <SiddonProjector.h>
 #include <Siddon.h>
 #include <Mlem2d.h>
@@ -753,6 +755,32 @@ int cmd_sino2d_read() {
   return 0;
 }
 
+// mmr3d: a Siemens mMR sinogram read from `header` (Sinogram3DSiemensMmr(char*): mMR
+// segments and LOR geometry, its bins as read); the image nx ny nz dx dz from `img` is
+// projected into out_proj and the sinogram's bins are backprojected into out_back.
+int cmd_mmr3d() {
+  void* mem = calloc(1, sizeof(Sinogram3DSiemensMmr));
+  Sinogram3DSiemensMmr* s = new (mem) Sinogram3DSiemensMmr((char*)arg("header"));
+  Image img(size3d());
+  size_t npix = (size_t)img.getPixelCount();
+  std::vector<float> x = read_f32(arg("img"), npix);
+  memcpy(img.getPixelsPtr(), x.data(), npix * 4);
+  Image back(size3d());
+  Projector* pp = make_projector();
+  {
+    GuardedPixels guarded(&back);
+    pp->Backproject(s, &back);
+  }
+  write_f32(arg("out_back"), back.getPixelsPtr(), npix);
+  {
+    GuardedPixels guarded(&img);
+    pp->Project(&img, s);
+  }
+  std::vector<float> v = sino3d_to(s);
+  write_f32(arg("out_proj"), v.data(), v.size());
+  return 0;
+}
+
 // log_row: Logger(out).writeRowOfNumbers of the n floats in `in`.
 int cmd_log_row() {
   const int n = iarg("n");
@@ -801,6 +829,7 @@ int main(int argc, char** argv) {
   if (c == "par_read") return cmd_par_read();
   if (c == "itf_read") return cmd_itf_read();
   if (c == "log_row") return cmd_log_row();
+  if (c == "mmr3d") return cmd_mmr3d();
   if (c == "sino2d_write") return cmd_sino2d_write();
   if (c == "sino2d_read") return cmd_sino2d_read();
   fprintf(stderr, "ref_driver: unknown command %s\n", c.c_str());

@@ -47,6 +47,9 @@ def lib():
             "orc_backproject2d": (None, [P, P, _i, P, _i, P, _f, _f, P]),
             "orc_divide_backproject2d": (None, [P, P, P, _i, P, _i, P, _f, _f, P]),
             "orc_subsample_r": (None, [_i, P, _i, P]),
+            "orc_mmr_lor_table": (None, [_i, _i, P]),
+            "orc_project3d_pts": (None, [P, P, _i, _i, P, _i, P, P, P]),
+            "orc_backproject3d_pts": (None, [P, P, _i, _i, P, _i, P, P, P]),
             "orc_ring_table_os": (_i, [_i, _f, _i, P, P, P, _i, P, P]),
             "orc_project3d_os": (None, [P, P, _i, P, _i, P, _i, _f, _i, P, P, _i, P]),
             "orc_backproject3d_os": (None, [P, P, _i, P, _i, P, _i, _f, _i, P, P, _i, P]),
@@ -118,6 +121,31 @@ def ring_table(rings, zfov, span, mrd):
     n = lib().orc_ring_table(rings, zfov, len(nsino), _p(nsino), _p(mn), _p(mx), _p(za), _p(zb))
     assert n == tot, (n, tot)
     return za, zb
+
+
+def mmr_lor_table(nang, nr):
+    """Siemens mMR LOR endpoints (x1, y1, x2, y2) per bin, [nang, nr, 4]."""
+    out = np.zeros((nang, nr, 4), np.float32)
+    lib().orc_mmr_lor_table(nang, nr, _p(out))
+    return out
+
+
+def project3d_pts(im, img, pts, za, zb):
+    img = f32(img)
+    nang, nr = pts.shape[:2]
+    out = np.zeros((za.size, nang, nr), np.float32)
+    lib().orc_project3d_pts(ctypes.byref(im), _p(img), nang, nr, _p(f32(pts)), za.size,
+                            _p(f32(za)), _p(f32(zb)), _p(out))
+    return out
+
+
+def backproject3d_pts(im, sino, pts, za, zb):
+    sino = f32(sino)
+    nang, nr = pts.shape[:2]
+    img = np.zeros((im.nz, im.ny, im.nx), np.float32)
+    lib().orc_backproject3d_pts(ctypes.byref(im), _p(sino), nang, nr, _p(f32(pts)), za.size,
+                                _p(f32(za)), _p(f32(zb)), _p(img))
+    return img
 
 
 def subsample_r(r, ns):

@@ -211,6 +211,30 @@ def oversampled():
          proj=read("os_proj.f32"), back=read("os_back.f32").reshape(17, 64, 64))
 
 
+MMR = dict(nx=32, ny=32, nz=65, dx=16.0, dz=4.0, nproj=24, nr=86, rings=64, zfov=260.0, span=3,
+           mrd=2)
+
+
+def mmr():
+    """Siemens mMR geometry (Sinogram3DSiemensMmr read from an Interfile header): 24
+    angles x 86 bins (bin width scaled from 344), 64 rings, span 3, MRD 2; a cylinder
+    phantom on 32 x 32 x 65 voxels of 16 x 16 x 4 mm is projected and a seeded sinogram
+    with zero bins is backprojected by the reference's SiddonProjector."""
+    from apirl_amd import hip, interfile
+    g = MMR
+    nsino, mn, mx = hip.segment_table(g["rings"], g["span"], g["mrd"])
+    rng = np.random.default_rng(31)
+    meas = rng.poisson(2.0, (int(nsino.sum()), g["nproj"], g["nr"])).astype(np.float32)
+    hdr = interfile.write_sinogram3d(tmpf("mmr_sino"), meas, nsino, mn, mx, g["rings"], 8.18,
+                                     180.0 / g["nproj"])
+    img = phantoms.cylinder3d(g["nx"], g["nz"], g["dx"], g["dz"], radius=150.0)
+    img.tofile(tmpf("mmr_img.f32"))
+    run("mmr3d", header=hdr, nx=g["nx"], ny=g["ny"], nz=g["nz"], dx=g["dx"], dz=g["dz"],
+        img=tmpf("mmr_img.f32"), out_proj=tmpf("mmr_proj.f32"), out_back=tmpf("mmr_back.f32"))
+    save("mmr.npz", geom=np.array(json.dumps(g)), meas=meas.astype(np.uint8), img=img,
+         proj=read("mmr_proj.f32"), back=read("mmr_back.f32").reshape(g["nz"], g["ny"], g["nx"]))
+
+
 C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
           rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
 
@@ -232,7 +256,8 @@ def c2_rays():
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT)
-    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice", "oversampled"]
+    which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice", "oversampled",
+                             "mmr"]
     if "lists" in which:
         siddon_lists()
     if "c1" in which:
@@ -248,3 +273,5 @@ if __name__ == "__main__":
         multislice()
     if "oversampled" in which:
         oversampled()
+    if "mmr" in which:
+        mmr()
