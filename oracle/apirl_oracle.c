@@ -664,35 +664,76 @@ int orc_mlem2d(const orc_image* im, int nang, int nr, float rscanner, const floa
   return 0;
 }
 
-int orc_mlem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
-               const float* za, const float* zb, const float* meas, const float* init,
-               int iters, float* img, float* sens, float* lik) {
-  /* recon/src/MlemSinogram3d.cpp:89-331: update with s >= thr; one extra
-   * projection for the final likelihood (:277). */
-  long npix = (long)im->nx * im->ny * im->nz, nb = (long)nsino * nang * nr;
-  float* ang = (float*)malloc(sizeof(float) * nang);
-  float* r = (float*)malloc(sizeof(float) * nr);
-  orc_sino_grid(nang, nr, rfov, ang, r);
+/* LORs of a 3D sinogram: the cylindrical scanner's (angle, r) grid at radius
+ * rscanner, or explicit endpoints per (angle, r) bin (pts != NULL). */
+typedef struct {
+  int nang, nr;
+  const float *ang, *r, *pts;
+  float rscanner;
+} lor_source;
+
+static void project3d_src(const orc_image* im, const float* img, const lor_source* L, int nsino,
+                          const float* za, const float* zb, float* sino) {
+  if (L->pts)
+    orc_project3d_pts(im, img, L->nang, L->nr, L->pts, nsino, za, zb, sino);
+  else
+    orc_project3d(im, img, L->nang, L->ang, L->nr, L->r, L->rscanner, nsino, za, zb, sino);
+}
+
+static void backproject3d_src(const orc_image* im, const float* sino, const lor_source* L,
+                              int nsino, const float* za, const float* zb, float* img) {
+  if (L->pts)
+    orc_backproject3d_pts(im, sino, L->nang, L->nr, L->pts, nsino, za, zb, img);
+  else
+    orc_backproject3d(im, sino, L->nang, L->ang, L->nr, L->r, L->rscanner, nsino, za, zb, img);
+}
+
+static int mlem3d_core(const orc_image* im, const lor_source* L, int nsino, const float* za,
+                       const float* zb, const float* meas, const float* init, int iters,
+                       float* img, float* sens, float* lik) {
+  /* recon/src/MlemSinogram3d.cpp:89-331: sensitivity = backprojection of ones
+   * (:312-331), update with s >= thr (:209-219), one extra projection for the
+   * final likelihood (:277). */
+  long npix = (long)im->nx * im->ny * im->nz, nb = (long)nsino * L->nang * L->nr;
   float* ones = (float*)malloc(sizeof(float) * nb);
   float* est = (float*)calloc(nb, sizeof(float));
   float* bp = (float*)malloc(sizeof(float) * npix);
   for (long i = 0; i < nb; i++) ones[i] = 1;
   memset(sens, 0, sizeof(float) * npix);
-  orc_backproject3d(im, ones, nang, ang, nr, r, rscanner, nsino, za, zb, sens);
+  backproject3d_src(im, ones, L, nsino, za, zb, sens);
   float thr = orc_mlem_threshold(sens, npix);
   memcpy(img, init, sizeof(float) * npix);
   for (int t = 0; t < iters; t++) {
-    orc_project3d(im, img, nang, ang, nr, r, rscanner, nsino, za, zb, est);
+    project3d_src(im, img, L, nsino, za, zb, est);
     lik[t] = orc_likelihood(est, meas, nb, 0);
     memset(bp, 0, sizeof(float) * npix);
     orc_inverse_divide(est, meas, nb);
-    orc_backproject3d(im, est, nang, ang, nr, r, rscanner, nsino, za, zb, bp);
+    backproject3d_src(im, est, L, nsino, za, zb, bp);
     orc_em_update(img, bp, sens, npix, thr, 0);
   }
-  orc_project3d(im, img, nang, ang, nr, r, rscanner, nsino, za, zb, est);
+  project3d_src(im, img, L, nsino, za, zb, est);
   lik[iters] = orc_likelihood(est, meas, nb, 0);
-  free(ang); free(r); free(ones); free(est); free(bp);
+  free(ones); free(est); free(bp);
   return 0;
+}
+
+int orc_mlem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
+               const float* za, const float* zb, const float* meas, const float* init,
+               int iters, float* img, float* sens, float* lik) {
+  float* ang = (float*)malloc(sizeof(float) * nang);
+  float* r = (float*)malloc(sizeof(float) * nr);
+  orc_sino_grid(nang, nr, rfov, ang, r);
+  lor_source L = {nang, nr, ang, r, NULL, rscanner};
+  int rc = mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, img, sens, lik);
+  free(ang); free(r);
+  return rc;
+}
+
+int orc_mlem3d_pts(const orc_image* im, int nang, int nr, const float* pts, int nsino,
+                   const float* za, const float* zb, const float* meas, const float* init,
+                   int iters, float* img, float* sens, float* lik) {
+  lor_source L = {nang, nr, NULL, NULL, pts, 0.0f};
+  return mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, img, sens, lik);
 }
 
 int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,

@@ -126,6 +126,11 @@ int orc_mlem2d(const orc_image* im, int nang, int nr, float rscanner, const floa
 int orc_mlem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
                const float* za, const float* zb, const float* meas, const float* init,
                int iters, float* img, float* sens, float* lik);
+/* MlemSinogram3d over LORs given by their endpoints per (angle, r) bin (as
+ * orc_project3d_pts), e.g. the Siemens mMR's. */
+int orc_mlem3d_pts(const orc_image* im, int nang, int nr, const float* pts, int nsino,
+                   const float* za, const float* zb, const float* meas, const float* init,
+                   int iters, float* img, float* sens, float* lik);
 int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
                const float* za, const float* zb, const float* meas, const float* init,
                int nsubsets, int iters, float* img, float* sens_all, float* lik);

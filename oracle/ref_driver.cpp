@@ -25,6 +25,7 @@
 //   par_read file method key / itf_read header key     one key lookup: "rc<TAB>value"
 //   log_row   n in out                                 Logger::writeRowOfNumbers
 //   mmr3d     header nx ny nz dx dz img out_proj out_back   mMR projection / backprojection
+//   mmr_mlem3d header nx ny nz dx dz init iters outdir       MlemSinogram3d on an mMR sinogram
 //   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
 // The reference's 2D projector checks only lower bounds and its 3D projector only
 // upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
@@ -245,6 +246,27 @@ struct GuardedPixels {
     memcpy(orig, img->pixels, n * 4);
     img->pixels = orig;
     free(buf);
+  }
+};
+
+// The reference's SiddonProjector with every image it reads or writes held inside
+// GuardedPixels for the call, so the 3D projector's out-of-range reads before the
+// pixel array (see above) return 0 also inside the reference's EM drivers, which
+// allocate their images themselves.
+class GuardedSiddon : public SiddonProjector {
+ public:
+  using SiddonProjector::SiddonProjector;
+  bool Project(Image* image, Sinogram3D* projection) {
+    GuardedPixels g(image);
+    return SiddonProjector::Project(image, projection);
+  }
+  bool Backproject(Sinogram3D* sino, Image* image) {
+    GuardedPixels g(image);
+    return SiddonProjector::Backproject(sino, image);
+  }
+  bool DivideAndBackproject(Sinogram3D* meas, Sinogram3D* est, Image* image) {
+    GuardedPixels g(image);
+    return SiddonProjector::DivideAndBackproject(meas, est, image);
   }
 };
 
@@ -781,6 +803,30 @@ int cmd_mmr3d() {
   return 0;
 }
 
+// mmr_mlem3d: MlemSinogram3d on a Siemens mMR sinogram read from `header`.  Its
+// Copy() builds mMR segments with the scanner's constants, so the driver's copies
+// keep their geometry (the cylindrical copy constructor does not), and the
+// reconstruction is well defined.  The projector is GuardedSiddon: without the
+// guard, LORs through the row before the image read heap memory, and the
+// estimated bins of those LORs (and the likelihood) depend on what lies there.
+int cmd_mmr_mlem3d() {
+  void* mem = calloc(1, sizeof(Sinogram3DSiemensMmr));
+  Sinogram3DSiemensMmr* s = new (mem) Sinogram3DSiemensMmr((char*)arg("header"));
+  Image init(size3d());
+  size_t npix = (size_t)init.getPixelCount();
+  std::vector<float> x0 = read_f32(arg("init"), npix);
+  memcpy(init.getPixelsPtr(), x0.data(), npix * 4);
+  Projector* pp = (Projector*)new GuardedSiddon();
+  std::string dir = arg("outdir");
+  int iters = iarg("iters");
+  Mlem3dProbe mlem(s, &init, dir, dir + "/mmr_mlem3d", iters, 0, false, false, pp, pp);
+  if (!mlem.Reconstruct()) return 6;
+  write_f32((dir + "/final.f32").c_str(), mlem.getReconstructedImage()->getPixelsPtr(), npix);
+  write_f32((dir + "/sens.f32").c_str(), mlem.sens()->getPixelsPtr(), npix);
+  write_f32((dir + "/lik.f32").c_str(), mlem.lik(), iters + 1);
+  return 0;
+}
+
 // log_row: Logger(out).writeRowOfNumbers of the n floats in `in`.
 int cmd_log_row() {
   const int n = iarg("n");
@@ -820,6 +866,7 @@ int main(int argc, char** argv) {
   if (c == "back3d") return cmd_back3d();
   if (c == "osem3d") return cmd_osem3d();
   if (c == "mlem3d") return cmd_mlem3d();
+  if (c == "mmr_mlem3d") return cmd_mmr_mlem3d();
   if (c == "rays3d") return cmd_rays3d();
   if (c == "osem3d_sub") return cmd_osem3d_sub();
   if (c == "img_write") return cmd_img_write();

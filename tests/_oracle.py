@@ -63,6 +63,7 @@ def lib():
             "orc_subset_angles": (_i, [_i, _i, _i, P]),
             "orc_mlem2d": (_i, [P, _i, _i, _f, P, P, _i, P, P, P]),
             "orc_mlem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, P, P, P]),
+            "orc_mlem3d_pts": (_i, [P, _i, _i, P, _i, P, P, P, P, _i, P, P, P]),
             "orc_osem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P, P, P]),
             "orc_siddon3d_f64": (_i, [P, P, P, P, P, _i]),
             "orc_project3d_bins_f64": (None, [P, P, P, P, _f, P, P, _l, P, P, P, P]),
@@ -348,6 +349,20 @@ def mlem3d(im, nang, nr, rfov, rscanner, za, zb, meas, init, iters):
     lik = np.zeros(iters + 1, np.float32)
     lib().orc_mlem3d(ctypes.byref(im), nang, nr, rfov, rscanner, len(za), _p(f32(za)), _p(f32(zb)),
                      _p(f32(meas)), _p(f32(init)), iters, _p(img), _p(sens), _p(lik))
+    shp = (im.nz, im.ny, im.nx)
+    return img.reshape(shp), sens.reshape(shp), lik
+
+
+def mlem3d_pts(im, pts, za, zb, meas, init, iters):
+    """MlemSinogram3d over the LOR endpoints pts [nang, nr, 4] (e.g. mmr_lor_table)."""
+    nang, nr = pts.shape[:2]
+    npix = im.nx * im.ny * im.nz
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix, np.float32)
+    lik = np.zeros(iters + 1, np.float32)
+    lib().orc_mlem3d_pts(ctypes.byref(im), nang, nr, _p(f32(pts)), len(za), _p(f32(za)),
+                         _p(f32(zb)), _p(f32(meas)), _p(f32(init)), iters, _p(img), _p(sens),
+                         _p(lik))
     shp = (im.nz, im.ny, im.nx)
     return img.reshape(shp), sens.reshape(shp), lik
 

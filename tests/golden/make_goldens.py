@@ -198,8 +198,8 @@ def multislice():
 
 def oversampled():
     """The mid-size 3D geometry traced with 3 radial x 2 axial LORs per bin
-    (SiddonProjector(3, 2)): projection of the cylinder phantom and backprojection of
-    the measured sinogram."""
+    (SiddonProjector(3, 2)): projection of the cylinder phantom, backprojection of
+    the measured sinogram, and DivideAndBackproject of the divide golden's pair."""
     g = MID
     f = np.load(os.path.join(HERE, "mid3d_project_backproject.npz"))
     f["img"].astype(np.float32).tofile(tmpf("os_img.f32"))
@@ -207,8 +207,16 @@ def oversampled():
     run("proj3d", **geom3d_args(g), ns=3, nas=2, img=tmpf("os_img.f32"), out=tmpf("os_proj.f32"))
     run("back3d", **geom3d_args(g), ns=3, nas=2, sino=tmpf("os_meas.f32"),
         out=tmpf("os_back.f32"))
+    # DivideAndBackproject with the same LORs, on the measured / estimated pair of
+    # the divide golden (zero bins in every combination)
+    d = np.load(os.path.join(HERE, "divide_backproject.npz"))
+    d["meas3d"].astype(np.float32).tofile(tmpf("os_dmeas.f32"))
+    (d["est3d_x2"].astype(np.float32) * np.float32(0.5)).tofile(tmpf("os_dest.f32"))
+    run("divback3d", **geom3d_args(g), ns=3, nas=2, meas=tmpf("os_dmeas.f32"),
+        est=tmpf("os_dest.f32"), out=tmpf("os_div.f32"))
     save("oversampled.npz", geom=np.array(json.dumps(dict(g, ns=3, nas=2))),
-         proj=read("os_proj.f32"), back=read("os_back.f32").reshape(17, 64, 64))
+         proj=read("os_proj.f32"), back=read("os_back.f32").reshape(17, 64, 64),
+         divback=read("os_div.f32").reshape(17, 64, 64))
 
 
 MMR = dict(nx=32, ny=32, nz=65, dx=16.0, dz=4.0, nproj=24, nr=86, rings=64, zfov=260.0, span=3,
@@ -235,6 +243,29 @@ def mmr():
          proj=read("mmr_proj.f32"), back=read("mmr_back.f32").reshape(g["nz"], g["ny"], g["nx"]))
 
 
+def mmr_mlem3d():
+    """MlemSinogram3d of the reference on the mMR sinogram of mmr() (its Copy() keeps
+    the mMR geometry, so the reconstruction is well defined): 3 iterations from a
+    uniform image; final image, sensitivity and likelihood trace."""
+    from apirl_amd import hip, interfile
+    g = MMR
+    meas = np.load(os.path.join(HERE, "mmr.npz"))["meas"].astype(np.float32)
+    nsino, mn, mx = hip.segment_table(g["rings"], g["span"], g["mrd"])
+    hdr = interfile.write_sinogram3d(tmpf("mmr_mlem_sino"), meas, nsino, mn, mx, g["rings"], 8.18,
+                                     180.0 / g["nproj"])
+    init = np.ones((g["nz"], g["ny"], g["nx"]), np.float32)
+    init.tofile(tmpf("mmr_mlem_init.f32"))
+    out = tmpf("mmr_mlem")
+    os.makedirs(out, exist_ok=True)
+    run("mmr_mlem3d", header=hdr, nx=g["nx"], ny=g["ny"], nz=g["nz"], dx=g["dx"], dz=g["dz"],
+        init=tmpf("mmr_mlem_init.f32"), iters=3, outdir=out)
+    shp = (g["nz"], g["ny"], g["nx"])
+    save("mmr_mlem3d.npz", geom=np.array(json.dumps(dict(g, iters=3))),
+         final=np.fromfile(os.path.join(out, "final.f32"), np.float32).reshape(shp),
+         sens=np.fromfile(os.path.join(out, "sens.f32"), np.float32).reshape(shp),
+         lik=np.fromfile(os.path.join(out, "lik.f32"), np.float32))
+
+
 C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
           rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
 
@@ -257,7 +288,7 @@ if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT)
     which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice", "oversampled",
-                             "mmr"]
+                             "mmr", "mmr_mlem3d"]
     if "lists" in which:
         siddon_lists()
     if "c1" in which:
@@ -275,3 +306,5 @@ if __name__ == "__main__":
         oversampled()
     if "mmr" in which:
         mmr()
+    if "mmr_mlem3d" in which:
+        mmr_mlem3d()

@@ -354,8 +354,8 @@ def test_mid3d_osem_gather_deterministic_and_close_to_reference(torch, ctx, gold
 def test_oversampled_projector_vs_reference(torch, ctx, golden):
     """Three radial x two axial LORs per bin (apirl_plan_set_oversampling): the
     projection is bit-exact against the reference's SiddonProjector(3, 2), the scatter
-    backprojection within the per-voxel bar; the gather refuses; one LOR per bin
-    restores the plain projector bit for bit."""
+    backprojection and DivideAndBackproject within the per-voxel bar; the gather
+    refuses; one LOR per bin restores the plain projector bit for bit."""
     log("case: oversampled projector 3 x 2, mid3d")
     f = golden("oversampled.npz")
     g = json.loads(str(f["geom"]))
@@ -365,9 +365,15 @@ def test_oversampled_projector_vs_reference(torch, ctx, golden):
     proj = p.project(dev(torch, m["img"]))
     meas = dev(torch, m["meas"].astype(np.float32).reshape(p.subset_shape()))
     back = p.backproject(meas, mode=hip.BP_SCATTER)
+    d = golden("divide_backproject.npz")
+    dmeas = dev(torch, d["meas3d"].astype(np.float32).reshape(p.subset_shape()))
+    dest = dev(torch, (d["est3d_x2"].astype(np.float32) * np.float32(0.5))
+               .reshape(p.subset_shape()))
+    div = p.divide_backproject(dmeas, dest, mode=hip.BP_SCATTER)
     torch.cuda.synchronize()
     assert_bitexact(host(proj).ravel(), f["proj"])
     assert_voxel_close(host(back), f["back"])
+    assert_voxel_close(host(div), f["divback"])
     with pytest.raises(hip.ApirlError, match="one LOR per bin"):
         p.backproject(meas, mode=hip.BP_GATHER)
     with pytest.raises(hip.ApirlError, match="reference precision"):
@@ -407,6 +413,45 @@ def test_mmr_geometry_vs_reference(torch, ctx, golden):
     again = p.project(dev(torch, f["img"]))
     torch.cuda.synchronize()
     assert_bitexact(host(again), host(ref_plain))
+
+
+def test_mlem3d_on_mmr_vs_reference(torch, ctx, golden):
+    """The device-resident driver's 3D MLEM (one subset, MlemSinogram3d's rules) on
+    the mMR geometry against the reference's MlemSinogram3d run: sensitivity within
+    the per-voxel bar, the image after 3 iterations within the image bar, the
+    likelihood trace as stated below."""
+    log("case: MLEM3D on mMR, 3 iterations")
+    f = golden("mmr_mlem3d.npz")
+    g = json.loads(str(f["geom"]))
+    p = hip.Plan.for_3d(ctx, g["nx"], g["ny"], g["nz"], g["dx"], g["dz"], g["nproj"], g["nr"],
+                        297.0, 328.0, g["rings"], g["zfov"], g["span"], g["mrd"])
+    p.set_scanner("mmr")
+    meas = golden("mmr.npz")["meas"].astype(np.float32).reshape(p.subset_shape())
+    init = np.ones((g["nz"], g["ny"], g["nx"]), np.float32)
+    sens = p.sensitivity(mode=hip.BP_SCATTER)
+    img, lik = p.reconstruct(dev(torch, meas), dev(torch, init), g["iters"], subsets=1,
+                             bp_mode=hip.BP_SCATTER, osem_rules=0)
+    torch.cuda.synchronize()
+    assert_voxel_close(host(sens), f["sens"])
+    got = host(img)
+    assert np.isfinite(got).all()
+    assert rel_l2(got, f["final"]) <= IMG_RL2
+    # Likelihood: the reference adds its 518,064 float terms (of both signs) serially
+    # in float, which here rounds by up to 1.3e-4 relative; the device sums them in
+    # double.  Against the same terms summed in double (the oracle's projections of
+    # the initial and of the reference's final image) the bar is 1e-5, against the
+    # reference's float sums 5e-4.
+    lik = np.asarray(lik, np.float64)
+    im = O.image_desc(g["nx"], g["ny"], g["nz"], g["dx"], g["dz"])
+    pts = O.mmr_lor_table(g["nproj"], g["nr"])
+    za, zb = O.ring_table(g["rings"], g["zfov"], g["span"], g["mrd"])
+    for t, x in ((0, init), (g["iters"], f["final"])):
+        est = O.project3d_pts(im, x, pts, za, zb).astype(np.float64).ravel()
+        y = meas.astype(np.float64).ravel()
+        nz = est != 0
+        exact = np.sum(y[nz] * np.log(est[nz]) - est[nz])
+        assert abs(lik[t] - exact) <= 1e-5 * abs(exact), (t, lik[t], exact)
+    np.testing.assert_allclose(lik, f["lik"].astype(np.float64), rtol=5e-4)
 
 
 @pytest.mark.parametrize("which", ["mid3d", "c2"])
