@@ -157,8 +157,6 @@ def run(method, par_path, log=print):
     # backprojected by the scatter form (2D projectors have no oversampling,
     # SiddonProjector.cpp:34-193, so there the keys have no effect)
     bp_mode = hip.BP_SCATTER if (is3d and samples != [1, 1]) else hip.BP_GATHER
-    if itype == "Sinogram3DSiemensMmr":
-        bp_mode = hip.BP_SCATTER  # the gather's candidate search needs a uniform grid
     sens = torch.empty((subsets,) + plan.image_shape(), dtype=torch.float32, device="cuda")
     t0 = time.perf_counter()
     liks, times, saved = [], [], []

@@ -107,6 +107,10 @@ struct apirl_plan {
   std::vector<float2> zab;         // [nsino] one axial sample per bin
   std::vector<float> z1m, z2m;     // [nsino] mean ring coordinates of each sinogram
   bool gather_tables = false;      // gather tables built (uniform grid, one LOR per bin)
+  float2* d_angcs_odd = nullptr;   // [nang] odd bins' (cos, sin) (mMR gather search)
+  int2* d_rlut = nullptr;          // [nrl] radial position -> bin range (mMR gather search)
+  float rl0 = 0.f, rl_inv_h = 0.f;
+  int nrl = 0;
   int scanner = 0;                 // APIRL_SCANNER_*: geometry of the LOR endpoints
 };
 
@@ -163,6 +167,14 @@ size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 size_t ztable_bytes(const apirl_plan* p, int nang_sub);
 void build_xy_crossings(const apirl_image_desc& img, const std::vector<RayXY>& rays,
                         std::vector<float>& out);
+// Radial positions of the mMR's bins (ascending) and the (cos, sin) of each angle's
+// odd bins (half an angular step less), as the mMR LOR endpoints use them.
+void mmr_radial_grid(int nang, int nr, std::vector<float>& r, std::vector<float2>& angcs_odd);
+// Lookup table from a radial position to the range of bins around it, on a grid of
+// spacing h from rl0 covering the ascending positions r with two points to spare:
+// lut[k] = (first j with r[j] >= rl0 + k h, last j with r[j] <= rl0 + k h).
+void build_radial_lut(const std::vector<float>& r, std::vector<int2>& lut, float& rl0,
+                      float& inv_h);
 void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
                          std::vector<GatherSeg>& segs, float& r0, float& dr, float& rmargin,
                          bool& ok);
@@ -216,6 +228,14 @@ struct GatherArgs {
   // accumulated x crossing by which ray (a, r) enters column i, [... + nx + j] the y
   // crossing by which it enters row j
   const float* xytab;
+  // Candidate search on a non-uniform radial grid with odd bins at their own angle
+  // (mMR; null for the plan's uniform grid): angcs_odd[a] is the (cos, sin) of the
+  // odd bins of angle a; rlut[k] = (first bin with r >= rl0 + k h, last bin with
+  // r <= rl0 + k h), k < nrl, rl_inv_h = 1 / h.
+  const float2* angcs_odd;
+  const int2* rlut;
+  float rl0, rl_inv_h;
+  int nrl;
 };
 
 struct BatchHeader {

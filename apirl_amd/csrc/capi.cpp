@@ -81,7 +81,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
   if (mode != APIRL_BP_GATHER) return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
   if (!p->gather_ok)
     return fail(c, APIRL_ERR_UNSUPPORTED,
-                "gather needs a uniform radial grid and one LOR per bin (no oversampling)");
+                "gather needs its tables (a radial grid it can search) and one LOR per bin "
+                "(no oversampling)");
   if (kind == apirl::BP_DIVIDE)
     return fail(c, APIRL_ERR_UNSUPPORTED, "divide-and-backproject has the scatter form only");
   const size_t need = apirl::gather_scratch_bytes(p, subset, nsubsets);
@@ -364,6 +365,8 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
   if (p->d_xytab) (void)hipFree(p->d_xytab);
+  if (p->d_angcs_odd) (void)hipFree(p->d_angcs_odd);
+  if (p->d_rlut) (void)hipFree(p->d_rlut);
   delete p;
   return APIRL_OK;
 }
@@ -437,7 +440,7 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
   p->geo.ns = ns;
   p->geo.nas = nas;
   // the gather's candidate search and tables assume one LOR per bin
-  p->gather_ok = p->gather_tables && ns == 1 && nas == 1;
+  p->gather_ok = p->gather_tables && p->d_xytab && ns == 1 && nas == 1;
   return APIRL_OK;
 }
 
@@ -471,13 +474,59 @@ apirl_status apirl_plan_set_scanner(apirl_plan* p, int scanner) {
     (void)hipFree(d_rays);
     return fail(c, APIRL_ERR_HIP, "cannot upload the LOR table");
   }
+  // The gather's tables for the new LORs: entry crossings, and for the mMR the
+  // candidate search's odd-bin angles and radial lookup table.
+  float* d_xytab = nullptr;
+  float2* d_odd = nullptr;
+  int2* d_lut = nullptr;
+  float rl0 = 0.f, rl_inv_h = 0.f;
+  int nrl = 0;
+  bool gather = p->gather_tables;
+  if (gather) {
+    std::vector<float> xy;
+    apirl::build_xy_crossings(p->img, rays, xy);
+    gather = hipMalloc(&d_xytab, xy.size() * sizeof(float)) == hipSuccess &&
+             hipMemcpy(d_xytab, xy.data(), xy.size() * sizeof(float), hipMemcpyHostToDevice) ==
+                 hipSuccess;
+  }
+  if (gather && scanner == APIRL_SCANNER_MMR) {
+    std::vector<float> r;
+    std::vector<float2> odd;
+    std::vector<int2> lut;
+    apirl::mmr_radial_grid(p->geo.nang, p->geo.nr, r, odd);
+    apirl::build_radial_lut(r, lut, rl0, rl_inv_h);
+    nrl = (int)lut.size();
+    gather = hipMalloc(&d_odd, odd.size() * sizeof(float2)) == hipSuccess &&
+             hipMemcpy(d_odd, odd.data(), odd.size() * sizeof(float2), hipMemcpyHostToDevice) ==
+                 hipSuccess &&
+             hipMalloc(&d_lut, lut.size() * sizeof(int2)) == hipSuccess &&
+             hipMemcpy(d_lut, lut.data(), lut.size() * sizeof(int2), hipMemcpyHostToDevice) ==
+                 hipSuccess;
+  }
+  if (!gather) {
+    (void)hipGetLastError();
+    if (d_xytab) (void)hipFree(d_xytab);
+    if (d_odd) (void)hipFree(d_odd);
+    if (d_lut) (void)hipFree(d_lut);
+    d_xytab = nullptr;
+    d_odd = nullptr;
+    d_lut = nullptr;
+    nrl = 0;
+  }
   APIRL_HIP(c, hipStreamSynchronize(c->stream));
   (void)hipFree(p->d_rays);
   p->d_rays = d_rays;
+  if (p->d_xytab) (void)hipFree(p->d_xytab);
+  if (p->d_angcs_odd) (void)hipFree(p->d_angcs_odd);
+  if (p->d_rlut) (void)hipFree(p->d_rlut);
+  p->d_xytab = d_xytab;
+  p->d_angcs_odd = d_odd;
+  p->d_rlut = d_lut;
+  p->rl0 = rl0;
+  p->rl_inv_h = rl_inv_h;
+  p->nrl = nrl;
   p->scanner = scanner;
-  // the gather's candidate search assumes the plan's uniform radial grid and one
-  // angle per projection
-  p->gather_ok = p->gather_tables && scanner == APIRL_SCANNER_CYLINDRICAL;
+  p->gather_ok = gather;
   return APIRL_OK;
 }
 

@@ -243,6 +243,59 @@ __device__ __forceinline__ void z_crossings(const Geo& g, const ZRay& z, float2 
   }
 }
 
+// Range [jlo, jhi] of bins whose LORs can pass within hw of a point at signed
+// distance sc from the centre along their normal, from the radial lookup table
+// (one grid step of slack on each side for rounding); empty when jlo > jhi.
+__device__ __forceinline__ void lut_bins(const GatherArgs& A, float sc, float hw, int& jlo,
+                                         int& jhi) {
+  int kl = (int)floorf((sc - hw - A.rl0) * A.rl_inv_h) - 1;
+  int kh = (int)ceilf((sc + hw - A.rl0) * A.rl_inv_h) + 1;
+  kl = kl < 0 ? 0 : (kl > A.nrl - 1 ? A.nrl - 1 : kl);
+  kh = kh < 0 ? 0 : (kh > A.nrl - 1 ? A.nrl - 1 : kh);
+  jlo = A.rlut[kl].x;
+  jhi = A.rlut[kh].y;
+}
+
+// Radial grid of a gather launch: known at compile time for single-job launches
+// (uniform, or the lookup table), read from the arguments otherwise.
+enum GatherGrid { kGridAny = 0, kGridUniform = 1, kGridLut = 2 };
+
+template <int GRID>
+__device__ __forceinline__ bool grid_lut(const GatherArgs& A) {
+  return GRID == kGridLut || (GRID == kGridAny && A.rlut != nullptr);
+}
+
+// Candidate bins of angle a for a voxel column centred at (xc, yc): the bins whose
+// LOR passes within the column's half-width (plus the float margin) of its centre.
+// Even bins (lo0, hi0) and odd bins (lo1, hi1) get their own range; on the plan's
+// uniform grid, where every bin of an angle shares it, the two ranges are equal.
+template <int GRID>
+__device__ __forceinline__ void candidate_bins(const GatherArgs& A, int a, float xc, float yc,
+                                               float inv_dr, int& lo0, int& hi0, int& lo1,
+                                               int& hi1) {
+  const Geo& g = A.g;
+  const float2 cs = A.angcs[a];
+  const float sc = xc * cs.x + yc * cs.y;
+  const float hw = 0.5f * g.dx * (fabsf(cs.x) + fabsf(cs.y)) + g.rmargin;
+  if (!grid_lut<GRID>(A)) {
+    int jlo = (int)ceilf((sc - hw - g.r0) * inv_dr);
+    int jhi = (int)floorf((sc + hw - g.r0) * inv_dr);
+    lo0 = lo1 = jlo < 0 ? 0 : jlo;
+    hi0 = hi1 = jhi > g.nr - 1 ? g.nr - 1 : jhi;
+    return;
+  }
+  lut_bins(A, sc, hw, lo0, hi0);
+  const float2 co = A.angcs_odd[a];
+  const float so = xc * co.x + yc * co.y;
+  const float hwo = 0.5f * g.dx * (fabsf(co.x) + fabsf(co.y)) + g.rmargin;
+  lut_bins(A, so, hwo, lo1, hi1);
+}
+
+// Bin rr is a candidate of its parity's range.
+__device__ __forceinline__ bool in_parity_range(int rr, int lo0, int hi0, int lo1, int hi1) {
+  return (rr & 1) ? (rr >= lo1 && rr <= hi1) : (rr >= lo0 && rr <= hi0);
+}
+
 // Gather of one voxel (i, j, k): adds its terms to A.img[v] in the fixed order
 // (angle, radial bin, segment, sinogram).  s_zab is the axial table in LDS (3D).
 template <int DIMS, int PREC>
@@ -263,14 +316,11 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
   acc.init(A.img[v]);
   for (int ka = 0; ka < A.nang_sub; ka++) {
     const int a = A.subset + A.nsubsets * ka;
-    const float2 cs = A.angcs[a];
-    const float sc = xc * cs.x + yc * cs.y;
-    const float hw = 0.5f * g.dx * (fabsf(cs.x) + fabsf(cs.y)) + g.rmargin;
-    int jlo = (int)ceilf((sc - hw - g.r0) * inv_dr);
-    int jhi = (int)floorf((sc + hw - g.r0) * inv_dr);
-    jlo = jlo < 0 ? 0 : jlo;
-    jhi = jhi > g.nr - 1 ? g.nr - 1 : jhi;
+    int lo0, hi0, lo1, hi1;
+    candidate_bins<kGridAny>(A, a, xc, yc, inv_dr, lo0, hi0, lo1, hi1);
+    const int jlo = lo0 < lo1 ? lo0 : lo1, jhi = hi0 > hi1 ? hi0 : hi1;
     for (int rr = jlo; rr <= jhi; rr++) {
+      if (A.rlut && !in_parity_range(rr, lo0, hi0, lo1, hi1)) continue;
       if (DIMS == 3 && A.nz_pair[(long)ka * g.nr + rr] == 0) continue;
       const RayXY t = A.rays[(long)a * g.nr + rr];
       if (!(t.flags & RAY_VALID)) continue;
@@ -490,7 +540,7 @@ inline size_t column_lds_bytes(int nsino) {
 // gather_voxel tests it.  Record and table addresses are formed from a per-(angle,
 // radial bin) base and 32-bit offsets (a subset's records and table entries number
 // below 2^31).
-template <int PREC, int ZV>
+template <int PREC, int ZV, int GRID = kGridAny>
 __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2* s_zab,
                                               const int4* s_slot, int i, int j, int kb) {
   static_assert(PREC != APIRL_PREC_F64_ALPHA, "double-precision setup uses gather_voxel");
@@ -521,14 +571,11 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
   const float zmid_all = 0.5f * (zlo_all + zhi_all);
   for (int ka = 0; ka < A.nang_sub; ka++) {
     const int a = A.subset + A.nsubsets * ka;
-    const float2 cs = A.angcs[a];
-    const float sc = xc * cs.x + yc * cs.y;
-    const float hw = 0.5f * g.dx * (fabsf(cs.x) + fabsf(cs.y)) + g.rmargin;
-    int jlo = (int)ceilf((sc - hw - g.r0) * inv_dr);
-    int jhi = (int)floorf((sc + hw - g.r0) * inv_dr);
-    jlo = jlo < 0 ? 0 : jlo;
-    jhi = jhi > g.nr - 1 ? g.nr - 1 : jhi;
+    int lo0, hi0, lo1, hi1;
+    candidate_bins<GRID>(A, a, xc, yc, inv_dr, lo0, hi0, lo1, hi1);
+    const int jlo = lo0 < lo1 ? lo0 : lo1, jhi = hi0 > hi1 ? hi0 : hi1;
     for (int rr = jlo; rr <= jhi; rr++) {
+      if (grid_lut<GRID>(A) && !in_parity_range(rr, lo0, hi0, lo1, hi1)) continue;
       const unsigned pair = (unsigned)ka * (unsigned)g.nr + (unsigned)rr;
       if (A.nz_pair[pair] == 0) continue;
       const RayXY t = A.rays[(long)a * g.nr + rr];
@@ -631,7 +678,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
     if (q < nk) A.img[v0 + q * nxy] = acc[q].value();
 }
 
-template <int PREC, int ZV>
+template <int PREC, int ZV, int GRID>
 __global__ __launch_bounds__(kGBlock) void gather_column_kernel(GatherArgs A) {
   extern __shared__ float2 s_zab[];
   const Geo& g = A.g;
@@ -649,7 +696,7 @@ __global__ __launch_bounds__(kGBlock) void gather_column_kernel(GatherArgs A) {
   const int j = blockIdx.y * kTile + (threadIdx.x / kTile);
   const int kb = blockIdx.z * ZV;
   if (i >= g.nx || j >= g.ny || kb >= g.nz) return;
-  gather_column<PREC, ZV>(A, s_zab, s_slot, i, j, kb);
+  gather_column<PREC, ZV, GRID>(A, s_zab, s_slot, i, j, kb);
 }
 
 template <int DIMS, int PREC>
@@ -767,8 +814,12 @@ template <int PREC, int ZV>
 void launch_gather_column(const GatherArgs& A, hipStream_t st) {
   const Geo& g = A.g;
   const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, (g.nz + ZV - 1) / ZV);
-  hipLaunchKernelGGL((gather_column_kernel<PREC, ZV>), grid, dim3(kGBlock),
-                     column_lds_bytes(g.nsino), st, A);
+  if (A.rlut)
+    hipLaunchKernelGGL((gather_column_kernel<PREC, ZV, kGridLut>), grid, dim3(kGBlock),
+                       column_lds_bytes(g.nsino), st, A);
+  else
+    hipLaunchKernelGGL((gather_column_kernel<PREC, ZV, kGridUniform>), grid, dim3(kGBlock),
+                       column_lds_bytes(g.nsino), st, A);
 }
 
 template <int PREC>
@@ -875,6 +926,11 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   A.zsoff = p->d_zsoff;
   A.zblock = p->zblock;
   A.xytab = p->d_xytab;
+  A.angcs_odd = p->d_angcs_odd;
+  A.rlut = p->d_rlut;
+  A.rl0 = p->rl0;
+  A.rl_inv_h = p->rl_inv_h;
+  A.nrl = p->nrl;
   if (g.dims == 3 && scratch) {
     // scratch: [ZRay x nrays][non-zero flag per (angle, r)][z-crossing table]
     const size_t npair = (size_t)nang_sub * g.nr;

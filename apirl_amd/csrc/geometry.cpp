@@ -58,22 +58,67 @@ void build_ray_table(const apirl_image_desc& img, const apirl_sino_desc& sino,
   build_ray_table_pts(img, is2d, pts, ok, out);
 }
 
-void mmr_lor_points(int nang, int nr, std::vector<float4>& pts) {
-  // Sinogram2DinSiemensMmr.cpp:27-50 (grids) and :175-192 (getPointsFromLor): bins of
-  // one detector element on the cylinder, r = (Rs + DOI cos(lr / Rs)) sin(lr / Rs) with
-  // lr the arc length of the bin centre; odd bins at half an angular step less; the
-  // LOR ends on the radius Rs + DOI.
-  const float Rs = 328.0f, doi = 9.6f, bin0 = 4.0891f / 2.0f;
+namespace {
+constexpr float kMmrRs = 328.0f, kMmrDoi = 9.6f;
+
+// Sinogram2DinSiemensMmr.cpp:27-50: bins of one detector element on the cylinder,
+// r = (Rs + DOI cos(lr / Rs)) sin(lr / Rs) with lr the arc length of the bin centre;
+// angles a * 180 / nang, odd bins half an angular step less.
+void mmr_grid(int nang, int nr, std::vector<float>& r, std::vector<float>& ang, float& step) {
+  const float bin0 = 4.0891f / 2.0f;
   const float bin = nr == 344 ? bin0 : bin0 * 344 / nr;
-  std::vector<float> r(nr), ang(nang);
+  r.resize(nr);
+  ang.resize(nang);
   for (int j = 0; j < nr; j++) {
     const float lr = bin / 2 + (bin * (j - (float)(nr / 2)));
-    r[j] = (Rs + doi * cosf(lr / Rs)) * sinf(lr / Rs);
+    r[j] = (kMmrRs + kMmrDoi * cosf(lr / kMmrRs)) * sinf(lr / kMmrRs);
   }
   const float inc = (float)180.0f / nang;
   for (int a = 0; a < nang; a++) ang[a] = a * inc;
-  const float step = nang > 1 ? ang[1] - ang[0] : 0.f;
-  const float Re = Rs + doi;
+  step = nang > 1 ? ang[1] - ang[0] : 0.f;
+}
+}  // namespace
+
+void mmr_radial_grid(int nang, int nr, std::vector<float>& r, std::vector<float2>& angcs_odd) {
+  std::vector<float> ang;
+  float step;
+  mmr_grid(nang, nr, r, ang, step);
+  angcs_odd.resize(nang);
+  for (int a = 0; a < nang; a++) {
+    const double rad = ((double)ang[a] - step / 2.0) * (double)kDegToRad;
+    angcs_odd[a].x = (float)cos(rad);
+    angcs_odd[a].y = (float)sin(rad);
+  }
+}
+
+void build_radial_lut(const std::vector<float>& r, std::vector<int2>& lut, float& rl0,
+                      float& inv_h) {
+  const int nr = (int)r.size();
+  double dmin = 1e30;
+  for (int j = 1; j < nr; j++) dmin = std::min(dmin, (double)r[j] - r[j - 1]);
+  const double h = nr > 1 ? std::max(dmin / 2, 1e-3) : 1.0;
+  const double lo = (double)r[0] - 2 * h, hi = (double)r[nr - 1] + 2 * h;
+  const int n = (int)std::ceil((hi - lo) / h) + 1;
+  lut.resize(n);
+  int first = 0, last = -1;
+  for (int k = 0; k < n; k++) {
+    const double x = lo + k * h;
+    while (first < nr && r[first] < x) first++;
+    while (last + 1 < nr && r[last + 1] <= x) last++;
+    lut[k].x = first;
+    lut[k].y = last;
+  }
+  rl0 = (float)lo;
+  inv_h = (float)(1.0 / h);
+}
+
+void mmr_lor_points(int nang, int nr, std::vector<float4>& pts) {
+  // Sinogram2DinSiemensMmr.cpp:175-192 (getPointsFromLor): the LOR of bin (a, j) at
+  // distance r[j] from the centre ends on the radius Rs + DOI.
+  std::vector<float> r, ang;
+  float step;
+  mmr_grid(nang, nr, r, ang, step);
+  const float Re = kMmrRs + kMmrDoi;
   pts.resize((size_t)nang * nr);
   for (int a = 0; a < nang; a++)
     for (int j = 0; j < nr; j++) {

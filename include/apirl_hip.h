@@ -141,7 +141,8 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* plan, int ns, int nas);
  * element per bin, odd bins half an angular step earlier, LORs ending on the radius
  * 328 + 9.6 mm (mean depth of interaction).  The plan's nang x nr sinogram grid
  * keeps its shape; the ring table (z1, z2 and the ring width 260 mm / rings) stays
- * the caller's.  The gather backprojector and batches need the cylindrical form. */
+ * the caller's.  Both backprojectors and batches take either form; oversampling
+ * (apirl_plan_set_oversampling) needs the cylindrical one. */
 #define APIRL_SCANNER_CYLINDRICAL 0
 #define APIRL_SCANNER_MMR 1
 apirl_status apirl_plan_set_scanner(apirl_plan* plan, int scanner);

@@ -167,3 +167,23 @@ def test_batch_with_large_job_equals_alone(torch, ctx):
         res = run_batch(torch, ctx, plans, jobs, use_graph=use_graph)
         for k in range(len(jobs)):
             same_bits(res[k], ref[k])
+
+
+def test_batch_with_mmr_job_equals_alone(torch, ctx):
+    """A job on the mMR geometry (non-uniform radial grid, odd bins at half-step
+    angles: the grouped gather reads the job's radial lookup table) beside a
+    cylindrical 3D job: each equals, bit for bit, the job alone."""
+    log("case: batch with an mMR job beside a cylindrical 3D job")
+    g = dict(dims=3, nx=32, ny=32, nz=65, dx=16.0, dz=4.0, nproj=24, nr=86, rings=64,
+             rfov=297.0, zfov=260.0, rscanner=328.0, span=3, mrd=2)
+    mmr = make_plan(ctx, g)
+    mmr.set_scanner("mmr")
+    plans = {"mmr": mmr, "3d_tiny": make_plan(ctx, GEOMS["3d_tiny"])}
+    jobs = [("mmr", make_job(torch, mmr, g, 31, 2, 3)),
+            ("3d_tiny", make_job(torch, plans["3d_tiny"], GEOMS["3d_tiny"], 32, 2, 2))]
+    ref = [alone(plans[n], j) for n, j in jobs]
+    torch.cuda.synchronize()
+    assert np.nanmax(ref[0]) > 0
+    res = run_batch(torch, ctx, plans, jobs)
+    for k in range(len(jobs)):
+        same_bits(res[k], ref[k])

@@ -235,7 +235,7 @@ def test_cli_mmr_mlem_equals_plan(torch, ctx, golden, tmp_path):
                         297.0, 328.0, g["rings"], 260.0, g["span"], g["mrd"])
     p.set_scanner("mmr")
     x0 = torch.ones(p.image_shape(), dtype=torch.float32, device="cuda")
-    img, _ = p.reconstruct(dev(torch, meas), x0, 3, subsets=1, bp_mode=hip.BP_SCATTER,
+    img, _ = p.reconstruct(dev(torch, meas), x0, 3, subsets=1, bp_mode=hip.BP_GATHER,
                            osem_rules=0)
     got, ref = read(out + "_final"), host(img)
     assert np.isfinite(got).all() and np.isfinite(ref).all() and ref.max() > 0

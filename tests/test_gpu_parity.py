@@ -391,9 +391,9 @@ def test_oversampled_projector_vs_reference(torch, ctx, golden):
 
 
 def test_mmr_geometry_vs_reference(torch, ctx, golden):
-    """apirl_plan_set_scanner(APIRL_SCANNER_MMR): projection bit-exact and scatter
-    backprojection within the per-voxel bar against the reference's mMR sinogram; the
-    gather refuses; the cylindrical geometry comes back unchanged."""
+    """apirl_plan_set_scanner(APIRL_SCANNER_MMR): projection bit-exact, scatter and
+    gather backprojections within the per-voxel bar against the reference's mMR
+    sinogram; the cylindrical geometry (and its gather) comes back unchanged."""
     log("case: mMR geometry, 24 x 86 x 251 sinograms")
     f = golden("mmr.npz")
     g = json.loads(str(f["geom"]))
@@ -402,17 +402,26 @@ def test_mmr_geometry_vs_reference(torch, ctx, golden):
     ref_plain = p.project(dev(torch, f["img"]))
     p.set_scanner("mmr")
     proj = p.project(dev(torch, f["img"]))
-    back = p.backproject(dev(torch, f["meas"].astype(np.float32).reshape(p.subset_shape())),
-                         mode=hip.BP_SCATTER)
+    meas = dev(torch, f["meas"].astype(np.float32).reshape(p.subset_shape()))
+    back = p.backproject(meas, mode=hip.BP_SCATTER)
+    # the gather on the mMR's non-uniform grid and half-step odd angles: every voxel
+    # within the per-voxel bar, and the same bits on a second run
+    gat = p.backproject(meas, mode=hip.BP_GATHER)
+    gat2 = p.backproject(meas, mode=hip.BP_GATHER)
     torch.cuda.synchronize()
     assert_bitexact(host(proj).ravel(), f["proj"])
     assert_voxel_close(host(back), f["back"])
-    with pytest.raises(hip.ApirlError):
-        p.backproject(proj, mode=hip.BP_GATHER)
+    assert_voxel_close(host(gat), f["back"])
+    assert_bitexact(host(gat2), host(gat))
     p.set_scanner("cylindrical")
     again = p.project(dev(torch, f["img"]))
+    plain_gat = p.backproject(again, mode=hip.BP_GATHER)
+    p.set_scanner("mmr")
+    p.set_scanner("cylindrical")
+    plain_gat2 = p.backproject(again, mode=hip.BP_GATHER)
     torch.cuda.synchronize()
     assert_bitexact(host(again), host(ref_plain))
+    assert_bitexact(host(plain_gat2), host(plain_gat))
 
 
 def test_mlem3d_on_mmr_vs_reference(torch, ctx, golden):
@@ -452,6 +461,13 @@ def test_mlem3d_on_mmr_vs_reference(torch, ctx, golden):
         exact = np.sum(y[nz] * np.log(est[nz]) - est[nz])
         assert abs(lik[t] - exact) <= 1e-5 * abs(exact), (t, lik[t], exact)
     np.testing.assert_allclose(lik, f["lik"].astype(np.float64), rtol=5e-4)
+    # the same reconstruction with the gather backprojector
+    sens_g = p.sensitivity(mode=hip.BP_GATHER)
+    img_g, _ = p.reconstruct(dev(torch, meas), dev(torch, init), g["iters"], subsets=1,
+                             bp_mode=hip.BP_GATHER, osem_rules=0)
+    torch.cuda.synchronize()
+    assert_voxel_close(host(sens_g), f["sens"])
+    assert rel_l2(host(img_g), f["final"]) <= IMG_RL2
 
 
 @pytest.mark.parametrize("which", ["mid3d", "c2"])
