@@ -48,10 +48,17 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (for PMC byte calibration)")
-    ap.add_argument("--workload", choices=["osem", "batch", "serve"], default="osem",
+    ap.add_argument("--workload", choices=["osem", "batch", "serve", "shard"], default="osem",
                     help="osem: the single-job 256x256x81 OSEM line (default); batch: "
                          "64 seeded jobs of uneven size, batched vs one at a time; serve: "
-                         "open-loop arrivals over the batch, p50/p99 latency per rate")
+                         "open-loop arrivals over the batch, p50/p99 latency per rate; "
+                         "shard: one rank's share of the OSEM job sharded by sinograms")
+    ap.add_argument("--ranks", type=int, default=8,
+                    help="ranks of the sharded job (--workload shard)")
+    ap.add_argument("--shard-layout", choices=["balanced", "block", "interleaved"],
+                    default="balanced",
+                    help="sinograms of a rank: a contiguous run of about equal cost, a "
+                         "contiguous run of equal length, or every n-th one")
     ap.add_argument("--jobs", type=int, default=64)
     ap.add_argument("--rates", default="5,10,20,50,100,200")
     return ap.parse_args()
@@ -182,6 +189,88 @@ def serve_main(args):
     print(json.dumps(line), flush=True)
 
 
+def shard_main(args):
+    """One rank's share of the 256x256x81 OSEM job sharded by sinograms over
+    args.ranks GPUs (args.shard_layout): the kernels of each rank's
+    sub-iteration (fused projection + ratio, gather backprojection, update) timed on
+    this GPU at that rank's real shape, beside the whole job's."""
+    import torch
+
+    from apirl_amd import hip, phantoms, sharding
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    torch.cuda.set_device(0)
+    dev = torch.device("cuda", 0)
+    ctx = hip.Context(0)
+    g = C2
+    geo = (g["nx"], g["ny"], g["nz"], g["dx"], g["dz"], g["nproj"], g["nr"], g["rfov"],
+           g["rscanner"], g["rings"], g["zfov"], g["span"], g["mrd"])
+    full = hip.Plan.for_3d(ctx, *geo)
+    img_np, _ = phantoms.cylinder_points3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    truth = full.project(torch.from_numpy(img_np).to(dev))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(0)
+    meas = torch.poisson(truth * (COUNTS / float(truth.double().sum())), generator=gen)
+    del truth
+    x = torch.ones(full.image_shape(), dtype=torch.float32, device=dev)
+
+    def sub_iteration_ms(plan, m):
+        """Mean kernel time of one sub-iteration over the 21 subsets (HIP events)."""
+        sens = torch.empty((SUBSETS,) + plan.image_shape(), dtype=torch.float32, device=dev)
+        for s in range(SUBSETS):
+            plan.sensitivity(out=sens[s], subset=s, nsubsets=SUBSETS, mode=hip.BP_GATHER)
+        bp = torch.empty(plan.image_shape(), dtype=torch.float32, device=dev)
+        xs = x.clone()
+        times = []
+        for rep in range(2):  # the first pass warms up
+            for s in range(SUBSETS):
+                r = torch.empty(plan.subset_shape(s, SUBSETS), dtype=torch.float32, device=dev)
+                bp.zero_()
+                e0 = torch.cuda.Event(enable_timing=True)
+                e1 = torch.cuda.Event(enable_timing=True)
+                e0.record()
+                plan.project_ratio(xs, m, subset=s, nsubsets=SUBSETS, ratio=r, want_lik=False)
+                plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=hip.BP_GATHER)
+                hip.em_update(ctx, xs, bp, sens[s], 0.0, 0)
+                e1.record()
+                if rep:
+                    times.append((e0, e1))
+        torch.cuda.synchronize()
+        return float(np.mean([a.elapsed_time(b) for a, b in times]))
+
+    progress("whole job")
+    full_ms = sub_iteration_ms(full, meas)
+    per_rank = []
+    for q in range(args.ranks):
+        plan, idx = sharding.shard_plan(ctx, *geo, args.ranks, q, args.shard_layout)
+        m = meas[torch.from_numpy(idx).to(dev)].contiguous()
+        per_rank.append(sub_iteration_ms(plan, m))
+        progress(f"rank {q}: {len(idx)} sinograms, {per_rank[-1]:.3f} ms per sub-iteration")
+        del plan, m
+    worst = max(per_rank)
+    image_bytes = 4 * g["nx"] * g["ny"] * g["nz"]
+    ring_bytes = 2 * (args.ranks - 1) / args.ranks * image_bytes
+    line = {
+        "metric": f"OSEM sub-iteration kernel ms, slowest rank's share of {args.ranks} "
+                  "(256x256x81, 344x252x837, 21 subsets)",
+        "value": worst,
+        "unit": "ms",
+        "n_gpus": 1,
+        "higher_is_better": False,
+        "dtype": "f32",
+        "data": "synthetic (cylinder + 6 point sources, Poisson 3e8 counts, seed 0)",
+        "config": {"workload": "one rank's share of the 3D OSEM job sharded by sinograms",
+                   "layout": args.shard_layout, "backprojector": "gather",
+                   "ranks": args.ranks},
+        "per_rank_ms": per_rank,
+        "whole_job_ms": full_ms,
+        "share_of_whole": worst / full_ms,
+        "ideal_share": 1.0 / args.ranks,
+        "ring_allreduce_bytes_per_gpu": ring_bytes,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -192,6 +281,8 @@ def main():
         return batch_main(args)
     if args.workload == "serve":
         return serve_main(args)
+    if args.workload == "shard":
+        return shard_main(args)
     launcher = None
     if not args.no_cpu_baseline and os.path.exists(REF_DRIVER):
         from apirl_amd.spawn import Launcher
