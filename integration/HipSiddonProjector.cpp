@@ -4,6 +4,7 @@
 #include "HipSiddonProjector.h"
 
 #include <Sinogram2DinCylindrical3Dpet.h>
+#include <Sinogram2DinSiemensMmr.h>
 #include <Sinogram3DCylindricalPet.h>
 
 #include <cstdio>
@@ -128,6 +129,18 @@ apirl_plan* HipSiddonProjector::planFor(Image* image, Sinogram3D* sino) {
       z1.push_back((s2->getAxialValue1FromList(0) + s2->getAxialValue1FromList(nz - 1)) / 2);
       z2.push_back((s2->getAxialValue2FromList(0) + s2->getAxialValue2FromList(nz - 1)) / 2);
     }
+  // The LORs are those of the 2D sinograms' own class (getPointsFromLor is virtual):
+  // the Siemens mMR's (Sinogram2DinSiemensMmr.cpp:175-192), whose grids the library
+  // builds from the scanner's constants, or the cylindrical scanner's from the
+  // sinogram's angle and radial grids.  An mMR sinogram whose angles are not the full
+  // grid a * 180 / nang (a subset) is not supported.
+  const bool mmr = dynamic_cast<Sinogram2DinSiemensMmr*>(s00) != nullptr;
+  if (mmr) {
+    const float* ang = s00->getAngPtr();
+    const float inc = (float)180.0f / s00->getNumProj();
+    for (int a = 0; a < s00->getNumProj(); a++)
+      if (ang[a] != a * inc) return nullptr;
+  }
   apirl_image_desc im = image_desc(image);
   apirl_sino_desc sd;
   memset(&sd, 0, sizeof(sd));
@@ -138,22 +151,28 @@ apirl_plan* HipSiddonProjector::planFor(Image* image, Sinogram3D* sino) {
   sd.rfov_mm = s00->getRadioFov_mm();
   sd.rscanner_mm = s00->getRadioScanner_mm();
   sd.ring_width_mm = sino->getAxialFoV_mm() / sino->getNumRings();  // SiddonProjector.cpp:335
-  sd.ang_deg = s00->getAngPtr();
-  sd.r_mm = s00->getRPtr();
+  sd.ang_deg = mmr ? nullptr : s00->getAngPtr();
+  sd.r_mm = mmr ? nullptr : s00->getRPtr();
   sd.z1_mean_mm = z1.data();
   sd.z2_mean_mm = z2.data();
-  std::string key = "3";
+  std::string key = mmr ? "3m" : "3";
   append(key, &im, sizeof(im));
   append(key, &sd.nang, 3 * sizeof(int));
   append(key, &sd.rfov_mm, 3 * sizeof(float));
-  append(key, sd.ang_deg, sd.nang * sizeof(float));
-  append(key, sd.r_mm, sd.nr * sizeof(float));
+  if (!mmr) {
+    append(key, sd.ang_deg, sd.nang * sizeof(float));
+    append(key, sd.r_mm, sd.nr * sizeof(float));
+  }
   append(key, z1.data(), z1.size() * sizeof(float));
   append(key, z2.data(), z2.size() * sizeof(float));
   auto it = plans.find(key);
   if (it != plans.end()) return it->second;
   apirl_plan* p = nullptr;
   if (apirl_plan_create(ctx, &im, &sd, &p) != APIRL_OK) return nullptr;
+  if (mmr && apirl_plan_set_scanner(p, APIRL_SCANNER_MMR) != APIRL_OK) {
+    apirl_plan_destroy(p);
+    return nullptr;
+  }
   plans[key] = p;
   return p;
 }

@@ -806,9 +806,10 @@ int cmd_mmr3d() {
 // mmr_mlem3d: MlemSinogram3d on a Siemens mMR sinogram read from `header`.  Its
 // Copy() builds mMR segments with the scanner's constants, so the driver's copies
 // keep their geometry (the cylindrical copy constructor does not), and the
-// reconstruction is well defined.  The projector is GuardedSiddon: without the
-// guard, LORs through the row before the image read heap memory, and the
-// estimated bins of those LORs (and the likelihood) depend on what lies there.
+// reconstruction is well defined.  The projector is GuardedSiddon (or, with
+// projector=..., make_projector's): without the guard, LORs through the row before
+// the image read heap memory, and the estimated bins of those LORs (and the
+// likelihood) depend on what lies there.
 int cmd_mmr_mlem3d() {
   void* mem = calloc(1, sizeof(Sinogram3DSiemensMmr));
   Sinogram3DSiemensMmr* s = new (mem) Sinogram3DSiemensMmr((char*)arg("header"));
@@ -816,7 +817,7 @@ int cmd_mmr_mlem3d() {
   size_t npix = (size_t)init.getPixelCount();
   std::vector<float> x0 = read_f32(arg("init"), npix);
   memcpy(init.getPixelsPtr(), x0.data(), npix * 4);
-  Projector* pp = (Projector*)new GuardedSiddon();
+  Projector* pp = has_arg("projector") ? make_projector() : (Projector*)new GuardedSiddon();
   std::string dir = arg("outdir");
   int iters = iarg("iters");
   Mlem3dProbe mlem(s, &init, dir, dir + "/mmr_mlem3d", iters, 0, false, false, pp, pp);
