@@ -131,6 +131,49 @@ __device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, 
   return w;
 }
 
+// kUnroll steps of walk3d (below), from step m0.  TAIL: the batch may run past the
+// ray's last step, whose terms are then not added.
+template <bool TR, bool TAIL>
+__device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float* __restrict__ img,
+                                             const float* __restrict__ imgT, int nxy, int m0,
+                                             float& acc) {
+  float wv[kUnroll], pv[kUnroll];
+  bool tv[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; u++) {
+    const int ci = ray.i, cj = ray.j, ck = ray.k;
+    const bool bx = (ray.ax <= ray.ay) && (ray.ax <= ray.az);
+    const bool by = !bx && (ray.ay <= ray.az);
+    const bool bz = !bx && !by;
+    const float an = bx ? ray.ax : (by ? ray.ay : ray.az);
+    wv[u] = (an - ray.ac) * ray.L;
+    ray.ac = an;
+    const float nax = ray.ax + ray.axu, nay = ray.ay + ray.ayu, naz = ray.az + ray.azu;
+    ray.ax = bx ? nax : ray.ax;
+    ray.ay = by ? nay : ray.ay;
+    ray.az = bz ? naz : ray.az;
+    ray.i += bx ? ray.iinc : 0;
+    ray.j += by ? ray.jinc : 0;
+    ray.k += bz ? ray.kinc : 0;
+    const int li = __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
+    const bool ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz) && li >= 0 &&
+                    (!TAIL || m0 + u < ray.n);
+    int idx = li;
+    const float* base = img;
+    if (TR && ci >= 0 && cj >= 0) {
+      idx = __mul24(ck, nxy) + __mul24(ci, g.ny) + cj;
+      base = imgT;
+    }
+    tv[u] = ok;
+    pv[u] = base[ok ? idx : 0];
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; u++) {
+    const float nacc = acc + wv[u] * pv[u];
+    acc = tv[u] ? nacc : acc;
+  }
+}
+
 // 3D traversal of one ray in the reference's arithmetic, kUnroll steps per batch of
 // voxel loads (addresses never depend on loaded data), terms added in traversal
 // order.  The reference reads voxel (ci, cj, ck) when ci < nx, cj < ny, ck < nz and
@@ -140,47 +183,15 @@ __device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, 
 // offset along y, so in-image voxels are read from the transposed copy (y fastest),
 // where the wavefront's 64 parallel rays touch neighbouring addresses; out-of-image
 // reads keep the reference's index into the image.  Indices use 24-bit multiplies
-// (the plan keeps nx ny below 2^23).
+// (the plan keeps nx ny below 2^23).  Whole batches skip the step-count test; the
+// last, partial batch takes it.
 template <bool TR>
 __device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
                                         const float* __restrict__ imgT, float acc = 0.f) {
   const int nxy = g.nx * g.ny;
-  for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
-    float wv[kUnroll], pv[kUnroll];
-    bool tv[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const int ci = ray.i, cj = ray.j, ck = ray.k;
-      const bool bx = (ray.ax <= ray.ay) && (ray.ax <= ray.az);
-      const bool by = !bx && (ray.ay <= ray.az);
-      const bool bz = !bx && !by;
-      const float an = bx ? ray.ax : (by ? ray.ay : ray.az);
-      wv[u] = (an - ray.ac) * ray.L;
-      ray.ac = an;
-      const float nax = ray.ax + ray.axu, nay = ray.ay + ray.ayu, naz = ray.az + ray.azu;
-      ray.ax = bx ? nax : ray.ax;
-      ray.ay = by ? nay : ray.ay;
-      ray.az = bz ? naz : ray.az;
-      ray.i += bx ? ray.iinc : 0;
-      ray.j += by ? ray.jinc : 0;
-      ray.k += bz ? ray.kinc : 0;
-      const int li = __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
-      const bool ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz) && li >= 0 && (m0 + u < ray.n);
-      int idx = li;
-      const float* base = img;
-      if (TR && ci >= 0 && cj >= 0) {
-        idx = __mul24(ck, nxy) + __mul24(ci, g.ny) + cj;
-        base = imgT;
-      }
-      tv[u] = ok;
-      pv[u] = base[ok ? idx : 0];
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const float nacc = acc + wv[u] * pv[u];
-      acc = tv[u] ? nacc : acc;
-    }
-  }
+  int m0 = 0;
+  for (; m0 + kUnroll <= ray.n; m0 += kUnroll) walk3d_batch<TR, false>(g, ray, img, imgT, nxy, m0, acc);
+  if (m0 < ray.n) walk3d_batch<TR, true>(g, ray, img, imgT, nxy, m0, acc);
   return acc;
 }
 
