@@ -219,12 +219,18 @@ __device__ __forceinline__ void first_crossings(const Geo& g, const RayXY& t, in
 
 // n-th crossing after the first: the exact float accumulator (alpha_exact.h) or,
 // in double, a0 + n u.
+// alpha_after out of line: the column kernel needs it only for the rare crossings the
+// table does not hold, and inlined, parts of it are hoisted to every sinogram.
+__device__ __noinline__ float alpha_after_call(float a, float u, int n) {
+  return alpha_after(a, u, n);
+}
+
 template <int PREC>
 __device__ __forceinline__ typename Acc<PREC>::alpha_t crossing(typename Acc<PREC>::alpha_t a0,
                                                                typename Acc<PREC>::alpha_t u,
                                                                int n) {
   if (PREC == APIRL_PREC_F64_ALPHA) return a0 + (double)n * u;
-  else return alpha_after((float)a0, (float)u, n);
+  else return alpha_after_call((float)a0, (float)u, n);
 }
 
 template <int PREC>
