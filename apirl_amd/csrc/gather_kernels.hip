@@ -657,16 +657,17 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
             // branches); the term is added only when all hold
             const int k = kb + q;
             const int nc = (k - zk0r) * zkinc;
-            bool ok = zkinc == 0 ? k == zk0r : nc >= 0;
+            // (zkinc == 0 ? k == zk0r : nc >= 0), as lane-mask operations
+            bool ok = (k == zk0r) | ((zkinc != 0) & (nc >= 0));
             const int m = na + nb + nc;
-            ok = ok && m < tnxy + zn;
+            ok &= m < tnxy + zn;
             const bool hz = nc > 0;
             float Zt = ztv[q];
-            if (ok && hz && !(zt_pair && nc <= z.zcount)) Zt = crossing<PREC>(az0, azu, nc - 1);
+            if (ok & hz & !(zt_pair && nc <= z.zcount)) Zt = crossing<PREC>(az0, azu, nc - 1);
             const float Zp = hz ? Zt : -FLT_MAX;
             const float Zn = hz ? Zt + azu : az0;
-            ok = ok && (na <= 0 || Xp <= Zn) && (nb <= 0 || Yp <= Zn) &&
-                 (!hz || (Zp < Xn && Zp < Yn));
+            ok &= ((na <= 0) | (Xp <= Zn)) & ((nb <= 0) | (Yp <= Zn)) &
+                  (!hz | ((Zp < Xn) & (Zp < Yn)));
             float ac = Xp > Yp ? Xp : Yp;
             ac = ac > Zp ? ac : Zp;
             ac = m > 0 ? ac : tamin;
