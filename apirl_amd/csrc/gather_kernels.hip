@@ -596,12 +596,14 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
       first_crossings<PREC, 3>(g, t, iinc, jinc, ax0, axu, ay0, ayu);
       // the transaxial tests of gather_voxel as one predicate
       const int na = (i - t.i0) * iinc, nb = (j - t.j0) * jinc;
+      // (iinc == 0 ? i == i0 : na >= 0) and the same for j, as lane-mask operations
       bool okt = (t.flags & RAY_VALID) != 0;
-      okt = okt && (iinc == 0 ? i == t.i0 : na >= 0) && (jinc == 0 ? j == t.j0 : nb >= 0);
-      okt = okt && na + nb < tnxy + g.nz + 1;
+      okt &= (i == t.i0) | ((iinc != 0) & (na >= 0));
+      okt &= (j == t.j0) | ((jinc != 0) & (nb >= 0));
+      okt &= na + nb < tnxy + g.nz + 1;
       const float Xp = na > 0 ? xyi : -FLT_MAX, Xn = na > 0 ? xyi + axu : ax0;
       const float Yp = nb > 0 ? xyj : -FLT_MAX, Yn = nb > 0 ? xyj + ayu : ay0;
-      okt = okt && (na <= 0 || Xp <= Yn) && (nb <= 0 || Yp < Xn);
+      okt &= ((na <= 0) | (Xp <= Yn)) & ((nb <= 0) | (Yp < Xn));
       if (!okt) continue;
       const float lo = fmaxf(tamin, fmaxf(Xp, Yp));
       const float hi = fminf(tamax, fminf(Xn, Yn));
@@ -635,7 +637,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
           bool hit[ZV];
 #pragma unroll
           for (int q = 0; q < ZV; q++) {
-            hit[q] = q < nk && !(zmax < zlo_t[q]) && !(zmin > zhi_t[q]);
+            hit[q] = (q < nk) & !(zmax < zlo_t[q]) & !(zmin > zhi_t[q]);
             ztv[q] = 0.f;
           }
           if (zt_pair) {
