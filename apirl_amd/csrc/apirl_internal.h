@@ -164,6 +164,9 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                                      int subset, int nsubsets, int kind, int precision,
                                      void* scratch, size_t scratch_bytes, hipStream_t st);
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
+// The 3D gather's 32-bit record and table offsets hold for this subset (fewer than
+// 2^31 subset bins; angle x radial bins, sinograms and crossing-table slots below 2^24).
+bool gather_fits(const apirl_plan* p, int subset, int nsubsets);
 size_t ztable_bytes(const apirl_plan* p, int nang_sub);
 void build_xy_crossings(const apirl_image_desc& img, const std::vector<RayXY>& rays,
                         std::vector<float>& out);

@@ -258,6 +258,8 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   if (!p->gather_ok)
     return bfail(b, APIRL_ERR_UNSUPPORTED, "batched jobs use the gather backprojector");
   if (d->subsets > p->geo.nang) return bfail(b, APIRL_ERR_INVALID, "more subsets than angles");
+  if (!apirl::gather_fits(p, 0, d->subsets))  // subset 0 has the most angles
+    return bfail(b, APIRL_ERR_UNSUPPORTED, "gather: a subset has 2^31 bins or more");
   if (p->geo.dims == 3 && p->geo.nsino > b->max_nsino)
     return bfail(b, APIRL_ERR_INVALID, "sinogram count above the batch's max_sinograms");
   if ((int)b->order.size() >= b->cap) return bfail(b, APIRL_ERR_INVALID, "batch is full");

@@ -85,6 +85,9 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
                 "(no oversampling)");
   if (kind == apirl::BP_DIVIDE)
     return fail(c, APIRL_ERR_UNSUPPORTED, "divide-and-backproject has the scatter form only");
+  if (!apirl::gather_fits(p, subset, nsubsets))
+    return fail(c, APIRL_ERR_UNSUPPORTED,
+                "gather: the subset has 2^31 bins or more (use the scatter form)");
   const size_t need = apirl::gather_scratch_bytes(p, subset, nsubsets);
   if (need > p->gscratch_bytes) {
     if (p->d_gscratch) {

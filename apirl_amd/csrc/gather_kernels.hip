@@ -544,8 +544,9 @@ inline size_t column_lds_bytes(int nsino) {
 // one scan of each segment's sinograms over the union of their slabs.  Inside the
 // scan, each sinogram is tested against each voxel's own slab exactly as
 // gather_voxel tests it.  Record and table addresses are formed from a per-(angle,
-// radial bin) base and 32-bit offsets (a subset's records and table entries number
-// below 2^31).
+// radial bin) base and 32-bit offsets s npair and e npair, formed with 24-bit
+// multiplies: gather_fits admits a subset only when its records number below 2^31 and
+// npair, the sinograms and the table slots stay below 2^24, which makes both exact.
 template <int PREC, int ZV, int GRID = kGridAny>
 __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2* s_zab,
                                               const int4* s_slot, int i, int j, int kb) {
@@ -628,7 +629,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
           const float zmax = fmaxf(za_lo, za_hi), zmin = fminf(za_lo, za_hi);
           if (zmax < zlo_all) continue;
           if (zmin > zhi_all) break;
-          const ZRay z = load_zray(zr_pair + (unsigned)s * npair);
+          const ZRay z = load_zray(zr_pair + __umul24((unsigned)s, npair));
           // the table entry of each slice this sinogram can reach, loaded alongside the
           // record (its address does not depend on it) and used only when the ray
           // enters that slice by a z crossing of its own, which makes it an entry
@@ -646,7 +647,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
             for (int q = 0; q < ZV; q++) {
               int e = kb + q + sl4.x;
               e = e < sl4.y ? sl4.y : (e > sl4.z ? sl4.z : e);
-              if (hit[q]) ztv[q] = zt_pair[(unsigned)e * npair];
+              if (hit[q]) ztv[q] = zt_pair[__umul24((unsigned)e, npair)];
             }
           }
           if (z.b == 0.f) continue;  // SiddonProjector.cpp:496
@@ -963,6 +964,15 @@ hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2
   hipLaunchKernelGGL(gather_group_kernel<3>, dim3(grid), dim3(kGBlock),
                      column_lds_bytes(max_nsino), st, hdr, g3);
   return hipGetLastError();
+}
+
+bool gather_fits(const apirl_plan* p, int subset, int nsubsets) {
+  const Geo& g = p->geo;
+  if (g.dims != 3) return true;
+  const long nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long lim24 = 1L << 24;
+  return (long)g.nsino * nang_sub * g.nr < (1L << 31) && nang_sub * g.nr < lim24 &&
+         g.nsino < lim24 && p->zblock < lim24;
 }
 
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
