@@ -596,7 +596,9 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
       float ax0, axu, ay0, ayu;
       first_crossings<PREC, 3>(g, t, iinc, jinc, ax0, axu, ay0, ayu);
       // the transaxial tests of gather_voxel as one predicate
-      const int na = (i - t.i0) * iinc, nb = (j - t.j0) * jinc;
+      // crossings already taken in x and y (increments are -1, 0 or 1: a 24-bit
+      // multiply is exact and full rate)
+      const int na = __mul24(i - t.i0, iinc), nb = __mul24(j - t.j0, jinc);
       // (iinc == 0 ? i == i0 : na >= 0) and the same for j, as lane-mask operations
       bool okt = (t.flags & RAY_VALID) != 0;
       okt &= (i == t.i0) | ((iinc != 0) & (na >= 0));
@@ -659,7 +661,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
             // the tests of gather_voxel, evaluated as one predicate (selects, not
             // branches); the term is added only when all hold
             const int k = kb + q;
-            const int nc = (k - zk0r) * zkinc;
+            const int nc = __mul24(k - zk0r, zkinc);
             // (zkinc == 0 ? k == zk0r : nc >= 0), as lane-mask operations
             bool ok = (k == zk0r) | ((zkinc != 0) & (nc >= 0));
             const int m = na + nb + nc;
