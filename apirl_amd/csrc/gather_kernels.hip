@@ -666,6 +666,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
             bool ok = (k == zk0r) | ((zkinc != 0) & (nc >= 0));
             const int m = na + nb + nc;
             ok &= m < tnxy + zn;
+            if (!ok) continue;  // whole wavefronts often fail the count tests
             const bool hz = nc > 0;
             float Zt = ztv[q];
             if (ok & hz & !(zt_pair && nc <= z.zcount)) Zt = crossing<PREC>(az0, azu, nc - 1);
