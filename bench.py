@@ -275,7 +275,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(whole jobs or ray shards per GPU are described in DESIGN.md)", file=sys.stderr)
+              "(one rank's share of a sharded job: --workload shard)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batch":
         return batch_main(args)
