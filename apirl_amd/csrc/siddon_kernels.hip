@@ -156,16 +156,17 @@ __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float
     ray.j += by ? ray.jinc : 0;
     ray.k += bz ? ray.kinc : 0;
     const int li = __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
-    const bool ok = (ci < g.nx) && (cj < g.ny) && (ck < g.nz) && li >= 0 &&
-                    (!TAIL || m0 + u < ray.n);
+    const bool ok = (ci < g.nx) & (cj < g.ny) & (ck < g.nz) & (li >= 0) &
+                    (!TAIL | (m0 + u < ray.n));
     int idx = li;
     const float* base = img;
-    if (TR && ci >= 0 && cj >= 0) {
+    if (TR && ((ci >= 0) & (cj >= 0))) {
       idx = __mul24(ck, nxy) + __mul24(ci, g.ny) + cj;
       base = imgT;
     }
     tv[u] = ok;
-    pv[u] = base[ok ? idx : 0];
+    // idx >= 0 whenever ok holds, so the offset is taken unsigned (no sign extension)
+    pv[u] = base[(unsigned)(ok ? idx : 0)];
   }
 #pragma unroll
   for (int u = 0; u < kUnroll; u++) {
