@@ -519,3 +519,23 @@ def test_c2_subset_gather_vs_oracle(torch, ctx, golden):
     ref = O.backproject3d(im, host(sino), ang[idx], r, g["rscanner"], za, zb)
     assert_voxel_close(host(out), ref)
     assert_voxel_close(host(sc), ref)
+
+
+def test_gather_refuses_subsets_beyond_32bit_offsets(torch, ctx):
+    """The gather forms record offsets in 32 bits: a subset of 2^31 bins or more
+    (2048 sinograms x 1024 angles x 1024 bins, one subset) is refused with an error
+    before any scratch is allocated, by the single-job path and by batches."""
+    log("case: gather refusal above 2^31 subset bins")
+    nsino = 2048
+    z = np.full(nsino, 1.0, np.float32)
+    p = hip.Plan(ctx, 8, 8, 2, 4.0, 1.0, 1024, 1024, 16.0, 400.0, dims=3, z1=z, z2=z,
+                 ring_width=1.0)
+    img = torch.zeros(p.image_shape(), dtype=torch.float32, device="cuda")
+    sino = torch.zeros(p.subset_shape(), dtype=torch.float32, device="cuda")  # 8 GiB
+    with pytest.raises(hip.ApirlError, match="2\\^31"):
+        p.backproject(sino, out=img, mode=hip.BP_GATHER)
+    b = hip.Batch(ctx, max_jobs=2, max_sinograms=4096)
+    with pytest.raises(hip.ApirlError, match="2\\^31"):
+        b.submit(p, sino, img, img, 1, 1)
+    b.close()
+    del sino
