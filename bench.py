@@ -48,11 +48,13 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (for PMC byte calibration)")
-    ap.add_argument("--workload", choices=["osem", "batch", "serve", "shard"], default="osem",
+    ap.add_argument("--workload", choices=["osem", "batch", "serve", "shard", "c1"],
+                    default="osem",
                     help="osem: the single-job 256x256x81 OSEM line (default); batch: "
                          "64 seeded jobs of uneven size, batched vs one at a time; serve: "
                          "open-loop arrivals over the batch, p50/p99 latency per rate; "
-                         "shard: one rank's share of the OSEM job sharded by sinograms")
+                         "shard: one rank's share of the OSEM job sharded by sinograms; "
+                         "c1: 2D MLEM on 128x128 beside the reference's CPU Mlem2d")
     ap.add_argument("--ranks", type=int, default=8,
                     help="ranks of the sharded job (--workload shard)")
     ap.add_argument("--shard-layout", choices=["balanced", "block", "interleaved"],
@@ -271,6 +273,80 @@ def shard_main(args):
     print(json.dumps(line), flush=True)
 
 
+C1 = dict(nx=128, dx=2.0, nproj=192, nr=192, rfov=128.0, rscanner=400.0)
+C1_ITERS = 60
+
+
+def c1_main(args):
+    """2D MLEM, 60 iterations on 128x128 @ 2 mm from a 192 x 192 sinogram (disc and hot
+    spot, 1e6 Poisson counts, numpy seed 0), device-resident with the gather
+    backprojector, beside the reference's Mlem2d on the host's cores."""
+    launcher = None
+    if not args.no_cpu_baseline and os.path.exists(REF_DRIVER):
+        from apirl_amd.spawn import Launcher
+        launcher = Launcher().start()  # before the GPU is initialised
+    import torch
+
+    from apirl_amd import hip, phantoms
+
+    assert torch.cuda.is_available(), "bench.py needs a ROCm GPU"
+    ctx = hip.Context(0)
+    g = C1
+    plan = hip.Plan.for_2d(ctx, g["nx"], g["dx"], g["nproj"], g["nr"], g["rfov"], g["rscanner"])
+    img = torch.from_numpy(phantoms.disc2d(g["nx"], g["dx"]).reshape(plan.image_shape())).cuda()
+    truth = plan.project(img).cpu().numpy()
+    meas_np = phantoms.poisson_counts(truth, 1e6, np.random.default_rng(0)).astype(np.float32)
+    meas = torch.from_numpy(meas_np.reshape(truth.shape)).cuda()
+    init = torch.ones(plan.image_shape(), dtype=torch.float32, device="cuda")
+    for _ in range(max(args.warmup, 1)):
+        plan.reconstruct(meas, init, C1_ITERS, subsets=1, bp_mode=hip.BP_GATHER)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.reconstruct(meas, init, C1_ITERS, subsets=1, bp_mode=hip.BP_GATHER)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    it_s = args.steps * C1_ITERS / wall
+    progress(f"C1: {it_s:.1f} MLEM it/s ({wall / args.steps * 1e3:.2f} ms per 60 iterations)")
+    cpu = None
+    if launcher is not None:
+        threads = min(os.cpu_count() or 1, 16)
+        with tempfile.TemporaryDirectory() as td:
+            mp, ip = os.path.join(td, "meas.f32"), os.path.join(td, "init.f32")
+            meas_np.tofile(mp)
+            np.ones(g["nx"] * g["nx"], np.float32).tofile(ip)
+            a = [REF_DRIVER, "mlem2d", f"nx={g['nx']}", f"dx={g['dx']}", f"nproj={g['nproj']}",
+                 f"nr={g['nr']}", f"rscanner={g['rscanner']}", f"iters={C1_ITERS}",
+                 f"meas={mp}", f"init={ip}", f"outdir={td}"]
+            rc, out, err = launcher.run(a, env=dict(os.environ, OMP_NUM_THREADS=str(threads)),
+                                        timeout=600)
+        launcher.close()
+        if rc == 0:
+            rec = [ln for ln in out.splitlines() if ln.startswith('{"what"')]
+            t = json.loads(rec[-1])["seconds"]
+            cpu = {"value": C1_ITERS / t, "unit": "MLEM iters/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": f"the reference's Mlem2d, {C1_ITERS} iterations with its "
+                             f"sensitivity ({t:.2f} s), OMP_NUM_THREADS={threads}"}
+        else:
+            progress(f"cpu baseline failed (exit {rc}): {err[-500:]}")
+    line = {
+        "metric": "2D MLEM iters/s (128x128, 192x192)",
+        "value": it_s,
+        "unit": "MLEM iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "higher_is_better": True,
+        "dtype": "f32",
+        "data": "synthetic (disc + hot spot, Poisson 1e6 counts, numpy seed 0)",
+        "config": {"workload": f"2D MLEM, {C1_ITERS} iterations per step, device-resident "
+                               "(sensitivity included)", "backprojector": "gather"},
+        "ms_per_step": wall / args.steps * 1e3,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -283,6 +359,8 @@ def main():
         return serve_main(args)
     if args.workload == "shard":
         return shard_main(args)
+    if args.workload == "c1":
+        return c1_main(args)
     launcher = None
     if not args.no_cpu_baseline and os.path.exists(REF_DRIVER):
         from apirl_amd.spawn import Launcher
