@@ -68,6 +68,20 @@ struct GatherSeg {
   int pad;
 };
 
+// One geometry of a plan's LOR endpoints (APIRL_SCANNER_*) with the tables the
+// projectors and the gather read for it.  A plan keeps every form it has built and
+// points its active tables (apirl_plan::d_rays ...) at one of them.
+struct LorForm {
+  RayXY* rays = nullptr;        // [nang][nr][ns]
+  float* xytab = nullptr;       // [nang][nr][nx + ny] gather entry crossings, or null
+  float2* angcs_odd = nullptr;  // [nang] odd bins' (cos, sin) (radial lookup search)
+  int2* rlut = nullptr;         // radial position -> bin range; null on a uniform grid
+  float rl0 = 0.f, rl_inv_h = 0.f;
+  int nrl = 0;
+  bool built = false;
+  bool gather = false;          // the gather's tables exist for this form
+};
+
 }  // namespace apirl
 
 struct apirl_ctx {
@@ -99,6 +113,8 @@ struct apirl_plan {
                                    // Z slices x 64/Z columns per wavefront)
   int gather_zvox = 3;             // 3D gather voxels per lane along z (1..4, tiles only)
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
+  float* d_divbins = nullptr;      // DivideAndBackproject's bin values for the gather
+  size_t divbins_cap = 0;          //   (grown on demand)
   int* d_zsoff = nullptr;          // [2 nsino + 1] z-crossing table slots, first slices
   float* d_xytab = nullptr;        // [nang][nr][nx + ny] entry crossings (gather)
   int zblock = 0;                  // z-crossing table entries per (angle, radial bin)
@@ -106,13 +122,33 @@ struct apirl_plan {
   std::vector<float> ang, r;       // host copies of the grids
   std::vector<float2> zab;         // [nsino] one axial sample per bin
   std::vector<float> z1m, z2m;     // [nsino] mean ring coordinates of each sinogram
-  bool gather_tables = false;      // gather tables built (uniform grid, one LOR per bin)
+  bool gather_tables = false;      // the gather's candidate-search tables common to every
+                                   // LOR form (angle directions, segment runs, z slots)
+  bool uniform_r = true;           // the plan's radial grid is uniform (closed-form search)
   float2* d_angcs_odd = nullptr;   // [nang] odd bins' (cos, sin) (mMR gather search)
   int2* d_rlut = nullptr;          // [nrl] radial position -> bin range (mMR gather search)
   float rl0 = 0.f, rl_inv_h = 0.f;
   int nrl = 0;
   int scanner = 0;                 // APIRL_SCANNER_*: geometry of the LOR endpoints
+  apirl::LorForm forms[3];         // per APIRL_SCANNER_*, built on first use; the d_rays,
+                                   // d_xytab, d_angcs_odd, d_rlut above alias forms[scanner]
 };
+
+namespace apirl {
+// Points the plan's active LOR tables at form f (built); returns the previous form.
+int use_lor_form(apirl_plan* p, int f);
+// Builds form f of a 3D plan if needed (host tables, upload).  false: out of memory or
+// not defined for the plan (message in the context).
+bool build_lor_form(apirl_plan* p, int f);
+// Holds form f active for a scope (host-side pointer swap: launches already enqueued
+// keep the tables they were given).
+struct ActiveForm {
+  apirl_plan* p;
+  int prev;
+  ActiveForm(apirl_plan* plan, int f) : p(plan), prev(use_lor_form(plan, f)) {}
+  ~ActiveForm() { use_lor_form(p, prev); }
+};
+}  // namespace apirl
 
 namespace apirl {
 
@@ -127,6 +163,9 @@ void build_ray_table_pts(const apirl_image_desc& img, bool is2d, const std::vect
                          const std::vector<char>& ok, std::vector<RayXY>& out);
 // Siemens Biograph mMR LOR endpoints of a nang x nr sinogram.
 void mmr_lor_points(int nang, int nr, std::vector<float4>& pts);
+// The LORs of the reference's mMR subset sinograms (APIRL_SCANNER_MMR_SUBSET):
+// cylindrical, on the radius 328 mm, at the mMR's radial positions and angles.
+void mmr_subset_lor_points(int nang, int nr, std::vector<float4>& pts);
 // Oversampled bins (SiddonProjector.cpp:374-395, Sinogram2DinCylindrical3Dpet.cpp:
 // 283-299): ns radial sub-LORs per bin and nas axial ones per sinogram.
 void build_subsample_r(const std::vector<float>& r, int ns, std::vector<float>& r_sub);
@@ -141,6 +180,11 @@ struct FusedRatio {
   float* ratio = nullptr;           // subset sinogram
   double* lik_partials = nullptr;   // one per block, or null
   int lik_positive_only = 0;
+  // angle of the additive sinogram added to bin (s, k, r) of subset `subset`:
+  // 0 = the bin's own angle subset + nsubsets k; 1 = k, the subset-local index into
+  // the full additive sinogram, as OsemSinogram3d.cpp:172-173 with
+  // Sinogram3D::addBinToBin (Sinogram3D.cpp:857-873) reads it
+  int additive_local = 0;
 };
 hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, int subset,
                           int nsubsets, int precision, const FusedRatio* fused, int* nblocks_out,
@@ -151,6 +195,9 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
 hipError_t launch_em_update(float* x, const float* b, const float* s, size_t n, float thr,
                             int strict, hipStream_t st);
 hipError_t launch_force_positive(float* x, size_t n, hipStream_t st);
+// DivideAndBackproject's per-bin value (3D and 2D rules) into out, for its gather form.
+hipError_t launch_divide_rule(const float* meas, const float* est, float* out, size_t n,
+                              int dims, hipStream_t st);
 hipError_t launch_sum_partials(const double* partials, int n, double* out, hipStream_t st);
 // Likelihood sum y log(x) - x over x > 0 (positive_only) or x != 0, deterministic:
 // block partials into `partials`, then a fixed-order sum into *out.
@@ -167,6 +214,8 @@ size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 // The 3D gather's 32-bit record and table offsets hold for this subset (fewer than
 // 2^31 subset bins; angle x radial bins, sinograms and crossing-table slots below 2^24).
 bool gather_fits(const apirl_plan* p, int subset, int nsubsets);
+// Why a subset does not fit the gather's 32-bit offsets (nullptr when it fits).
+const char* gather_misfit(const apirl_plan* p, int subset, int nsubsets);
 size_t ztable_bytes(const apirl_plan* p, int nang_sub);
 void build_xy_crossings(const apirl_image_desc& img, const std::vector<RayXY>& rays,
                         std::vector<float>& out);

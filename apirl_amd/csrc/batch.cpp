@@ -29,6 +29,8 @@ struct BJob {
   bool is2d = false;
   const float* meas = nullptr;
   const float* add = nullptr;
+  int add_local = 1;         // additive angle per subset bin (FusedRatio::additive_local)
+  int form = 0;              // LOR form of the plan its sub-iterations use (APIRL_SCANNER_*)
   float* x = nullptr;
   size_t nvox = 0;
   int t = 0, s = 0;          // next sub-iteration to schedule
@@ -258,8 +260,8 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   if (!p->gather_ok)
     return bfail(b, APIRL_ERR_UNSUPPORTED, "batched jobs use the gather backprojector");
   if (d->subsets > p->geo.nang) return bfail(b, APIRL_ERR_INVALID, "more subsets than angles");
-  if (!apirl::gather_fits(p, 0, d->subsets))  // subset 0 has the most angles
-    return bfail(b, APIRL_ERR_UNSUPPORTED, "gather: a subset has 2^31 bins or more");
+  if (const char* why = apirl::gather_misfit(p, 0, d->subsets))  // subset 0: most angles
+    return bfail(b, APIRL_ERR_UNSUPPORTED, std::string("gather: ") + why);
   if (p->geo.dims == 3 && p->geo.nsino > b->max_nsino)
     return bfail(b, APIRL_ERR_INVALID, "sinogram count above the batch's max_sinograms");
   if ((int)b->order.size() >= b->cap) return bfail(b, APIRL_ERR_INVALID, "batch is full");
@@ -272,6 +274,17 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   j->is2d = p->geo.dims == 2;
   j->meas = d->d_meas;
   j->add = d->d_additive;
+  j->add_local = d->additive_own_angles ? 0 : 1;
+  // OSEM on the mMR traces the reference's subset LORs (apirl_em_reconstruct alike)
+  j->form = (p->scanner == APIRL_SCANNER_MMR && j->osem) ? APIRL_SCANNER_MMR_SUBSET : p->scanner;
+  if (!apirl::build_lor_form(p, j->form)) {
+    delete j;
+    return bfail(b, APIRL_ERR_NOMEM, std::string("job set-up failed: ") + p->ctx->err);
+  }
+  if (!p->forms[j->form].gather) {
+    delete j;
+    return bfail(b, APIRL_ERR_UNSUPPORTED, "batched jobs use the gather backprojector");
+  }
   j->x = d->d_out;
   j->nvox = apirl_plan_image_voxels(p);
   j->strict = (!j->osem && j->is2d) ? 1 : 0;
@@ -313,6 +326,7 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   c->stream = b->stream;
   apirl_status st = APIRL_OK;
   j->thr.assign(j->S, 0.f);
+  apirl::ActiveForm form_scope(p, j->form);
   for (int s = 0; s < j->S && st == APIRL_OK; s++) {
     st = apirl_sensitivity(p, d->d_mult, j->sens + (size_t)s * j->nvox, s, j->S, APIRL_BP_GATHER,
                            j->osem);
@@ -361,6 +375,7 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
   std::vector<std::pair<BJob*, int>> direct;  // large jobs and their subset this round
   for (int id : b->order) {
     BJob* j = b->jobs[id];
+    apirl::ActiveForm form_scope(j->plan, j->form);
     const apirl_plan* p = j->plan;
     const Geo& g = p->geo;
     const int s = j->s, S = j->S;
@@ -405,6 +420,7 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
       q.fr = FusedRatio();
       q.fr.meas = j->meas;
       q.fr.additive = j->add;
+      q.fr.additive_local = j->add_local;
       q.fr.ratio = j->ratio;
       q.fr.lik_partials = nullptr;
       q.fr.lik_positive_only = j->is2d ? 1 : 0;
@@ -485,9 +501,11 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
   for (auto& pr : direct) {
     BJob* j = pr.first;
     const int s = pr.second;
+    apirl::ActiveForm form_scope(j->plan, j->form);
     FusedRatio fr;
     fr.meas = j->meas;
     fr.additive = j->add;
+    fr.additive_local = j->add_local;
     fr.ratio = j->ratio;
     fr.lik_partials = nullptr;
     fr.lik_positive_only = 0;

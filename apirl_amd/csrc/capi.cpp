@@ -57,6 +57,59 @@ bool check_subset(const apirl_plan* p, int subset, int nsubsets) {
   return nsubsets >= 1 && subset >= 0 && subset < nsubsets && nsubsets <= p->geo.nang;
 }
 
+void free_form(apirl::LorForm& f) {
+  if (f.rays) (void)hipFree(f.rays);
+  if (f.xytab) (void)hipFree(f.xytab);
+  if (f.angcs_odd) (void)hipFree(f.angcs_odd);
+  if (f.rlut) (void)hipFree(f.rlut);
+  f = apirl::LorForm();
+}
+
+// The gather's tables of one LOR form: the entry-crossing table of its rays and,
+// when the radial grid r is not the plan's uniform one, the radial lookup table and
+// the odd bins' directions (half a step earlier when odd_half_step, the mMR's;
+// the bin's own angle otherwise).  false leaves the form without gather tables.
+bool upload_gather_form(apirl_plan* p, const std::vector<apirl::RayXY>& rays,
+                        const std::vector<float>& r, bool lut, bool odd_half_step,
+                        apirl::LorForm& f) {
+  std::vector<float> xy;
+  apirl::build_xy_crossings(p->img, rays, xy);
+  bool ok = hipMalloc(&f.xytab, xy.size() * sizeof(float)) == hipSuccess &&
+            hipMemcpy(f.xytab, xy.data(), xy.size() * sizeof(float), hipMemcpyHostToDevice) ==
+                hipSuccess;
+  if (ok && lut) {
+    std::vector<float2> odd;
+    std::vector<int2> tab;
+    if (odd_half_step) {
+      std::vector<float> rr;
+      apirl::mmr_radial_grid(p->geo.nang, p->geo.nr, rr, odd);
+    } else {
+      odd.resize(p->geo.nang);
+      for (int a = 0; a < p->geo.nang; a++) {
+        const double rad = (double)p->ang[a] * 0.01745329251994329576;
+        odd[a].x = (float)cos(rad);
+        odd[a].y = (float)sin(rad);
+      }
+    }
+    apirl::build_radial_lut(r, tab, f.rl0, f.rl_inv_h);
+    f.nrl = (int)tab.size();
+    ok = hipMalloc(&f.angcs_odd, odd.size() * sizeof(float2)) == hipSuccess &&
+         hipMemcpy(f.angcs_odd, odd.data(), odd.size() * sizeof(float2),
+                   hipMemcpyHostToDevice) == hipSuccess &&
+         hipMalloc(&f.rlut, tab.size() * sizeof(int2)) == hipSuccess &&
+         hipMemcpy(f.rlut, tab.data(), tab.size() * sizeof(int2), hipMemcpyHostToDevice) ==
+             hipSuccess;
+  }
+  if (!ok) {
+    (void)hipGetLastError();
+    apirl::RayXY* keep = f.rays;
+    f.rays = nullptr;
+    free_form(f);
+    f.rays = keep;
+  }
+  return ok;
+}
+
 // Backprojection of one subset in either form.
 // Oversampled bins are traversed in the reference's arithmetic only (and need the
 // 24-bit index walk).
@@ -68,7 +121,7 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
                              int subset, int nsubsets, int kind, int mode,
                              int precision = APIRL_PREC_REFERENCE) {
   apirl_ctx* c = p->ctx;
-  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_ALPHA)
+  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_SETUP)
     return fail(c, APIRL_ERR_INVALID, "invalid precision");
   if (mode == APIRL_BP_SCATTER) {
     if (precision != APIRL_PREC_REFERENCE)
@@ -83,11 +136,9 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     return fail(c, APIRL_ERR_UNSUPPORTED,
                 "gather needs its tables (a radial grid it can search) and one LOR per bin "
                 "(no oversampling)");
-  if (kind == apirl::BP_DIVIDE)
-    return fail(c, APIRL_ERR_UNSUPPORTED, "divide-and-backproject has the scatter form only");
-  if (!apirl::gather_fits(p, subset, nsubsets))
+  if (const char* why = apirl::gather_misfit(p, subset, nsubsets))
     return fail(c, APIRL_ERR_UNSUPPORTED,
-                "gather: the subset has 2^31 bins or more (use the scatter form)");
+                std::string("gather: ") + why + " (use the scatter form)");
   const size_t need = apirl::gather_scratch_bytes(p, subset, nsubsets);
   if (need > p->gscratch_bytes) {
     if (p->d_gscratch) {
@@ -99,6 +150,26 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     if (hipMalloc(&p->d_gscratch, need) != hipSuccess)
       return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather scratch");
     p->gscratch_bytes = need;
+  }
+  if (kind == apirl::BP_DIVIDE) {
+    // DivideAndBackproject gathered: each bin's value by the reference's rule, then a
+    // plain backprojection of it (the bins the rule zeroes add nothing, as the bins
+    // the reference skips)
+    const size_t nb = apirl_plan_subset_bins(p, subset, nsubsets);
+    if (nb > p->divbins_cap) {
+      if (p->d_divbins) {
+        APIRL_HIP(c, hipStreamSynchronize(c->stream));
+        (void)hipFree(p->d_divbins);
+      }
+      p->d_divbins = nullptr;
+      p->divbins_cap = 0;
+      if (hipMalloc(&p->d_divbins, nb * sizeof(float)) != hipSuccess)
+        return fail(c, APIRL_ERR_NOMEM, "cannot allocate the divide-and-backproject bins");
+      p->divbins_cap = nb;
+    }
+    APIRL_HIP(c, apirl::launch_divide_rule(sino, est, p->d_divbins, nb, p->geo.dims, c->stream));
+    sino = p->d_divbins;
+    kind = apirl::BP_PLAIN;
   }
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream));
@@ -287,6 +358,13 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
     geo.dr = dr;
     geo.rmargin = rm;
     p->nseg = (int)segs.size();
+    // 3D radial grids that are increasing but not uniform (explicit r_mm, e.g. the
+    // mMR's arc-corrected positions) take the radial lookup search instead of the
+    // closed form
+    bool increasing = sd.nr >= 1;
+    for (int j = 1; j < sd.nr; j++) increasing = increasing && p->r[j] > p->r[j - 1];
+    p->uniform_r = ok;
+    if (!ok && sd.dims == 3 && increasing && !segs.empty()) ok = true;
     p->gather_ok = ok;
     if (hipMalloc(&p->d_angcs, angcs.size() * sizeof(float2)) != hipSuccess ||
         hipMemcpy(p->d_angcs, angcs.data(), angcs.size() * sizeof(float2),
@@ -323,16 +401,16 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       (void)hipGetLastError();
     }
   }
-  // Entry-crossing table of the gather (independent of subset and sinogram).
-  if (p->gather_ok) {
-    std::vector<float> xy;
-    apirl::build_xy_crossings(im, rays, xy);
-    if (hipMalloc(&p->d_xytab, xy.size() * sizeof(float)) != hipSuccess ||
-        hipMemcpy(p->d_xytab, xy.data(), xy.size() * sizeof(float), hipMemcpyHostToDevice) !=
-            hipSuccess) {
-      p->gather_ok = false;
-      (void)hipGetLastError();
-    }
+  // The plan's own LORs are form APIRL_SCANNER_CYLINDRICAL: the ray table uploaded
+  // above, and for the gather its entry-crossing table (independent of subset and
+  // sinogram) and, on a radial grid that is not uniform, the radial lookup table.
+  {
+    apirl::LorForm& f = p->forms[APIRL_SCANNER_CYLINDRICAL];
+    f.rays = p->d_rays;
+    f.built = true;
+    p->gather_tables = p->gather_ok;  // candidate-search tables common to every form
+    if (p->gather_tables) f.gather = upload_gather_form(p, rays, p->r, !p->uniform_r, false, f);
+    apirl::use_lor_form(p, APIRL_SCANNER_CYLINDRICAL);
   }
   // Transposed image for the forward projector's x-major rays.
   if (hipMalloc(&p->d_imgT, (size_t)im.nx * im.ny * im.nz * sizeof(float)) != hipSuccess) {
@@ -350,7 +428,6 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
       (void)hipGetLastError();
     }
   }
-  p->gather_tables = p->gather_ok;
   *out = p;
   return APIRL_OK;
 }
@@ -358,7 +435,7 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
 apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (!p) return APIRL_OK;
   DeviceGuard g(p->ctx->device);
-  if (p->d_rays) (void)hipFree(p->d_rays);
+  for (apirl::LorForm& f : p->forms) free_form(f);  // the active tables alias one
   if (p->d_zab) (void)hipFree(p->d_zab);
   if (p->d_replicas) (void)hipFree(p->d_replicas);
   if (p->d_angcs) (void)hipFree(p->d_angcs);
@@ -366,10 +443,8 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_imgT) (void)hipFree(p->d_imgT);
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
+  if (p->d_divbins) (void)hipFree(p->d_divbins);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
-  if (p->d_xytab) (void)hipFree(p->d_xytab);
-  if (p->d_angcs_odd) (void)hipFree(p->d_angcs_odd);
-  if (p->d_rlut) (void)hipFree(p->d_rlut);
   delete p;
   return APIRL_OK;
 }
@@ -436,21 +511,22 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
     return fail(c, APIRL_ERR_HIP, "cannot upload the oversampled LOR tables");
   }
   APIRL_HIP(c, hipStreamSynchronize(c->stream));
-  (void)hipFree(p->d_rays);
+  apirl::LorForm& f = p->forms[APIRL_SCANNER_CYLINDRICAL];
+  (void)hipFree(f.rays);
   (void)hipFree(p->d_zab);
-  p->d_rays = d_rays;
+  f.rays = d_rays;
   p->d_zab = d_zab;
   p->geo.ns = ns;
   p->geo.nas = nas;
-  // the gather's candidate search and tables assume one LOR per bin
-  p->gather_ok = p->gather_tables && p->d_xytab && ns == 1 && nas == 1;
+  apirl::use_lor_form(p, APIRL_SCANNER_CYLINDRICAL);  // the gather needs one LOR per bin
   return APIRL_OK;
 }
 
 apirl_status apirl_plan_set_scanner(apirl_plan* p, int scanner) {
   if (!p) return APIRL_ERR_INVALID;
   apirl_ctx* c = p->ctx;
-  if (scanner != APIRL_SCANNER_CYLINDRICAL && scanner != APIRL_SCANNER_MMR)
+  if (scanner != APIRL_SCANNER_CYLINDRICAL && scanner != APIRL_SCANNER_MMR &&
+      scanner != APIRL_SCANNER_MMR_SUBSET)
     return fail(c, APIRL_ERR_INVALID, "unknown scanner");
   if (scanner == p->scanner) return APIRL_OK;
   if (p->geo.dims != 3)
@@ -458,78 +534,8 @@ apirl_status apirl_plan_set_scanner(apirl_plan* p, int scanner) {
   if (p->geo.ns * p->geo.nas != 1)
     return fail(c, APIRL_ERR_UNSUPPORTED, "set the scanner before oversampling");
   DeviceGuard g(c->device);
-  std::vector<apirl::RayXY> rays;
-  if (scanner == APIRL_SCANNER_MMR) {
-    std::vector<float4> pts;
-    apirl::mmr_lor_points(p->geo.nang, p->geo.nr, pts);
-    std::vector<char> ok(pts.size(), 1);  // the 3D form rejects no LOR
-    apirl::build_ray_table_pts(p->img, false, pts, ok, rays);
-  } else {
-    apirl::build_ray_table(p->img, p->sino, p->ang, p->r, rays);
-  }
-  apirl::RayXY* d_rays = nullptr;
-  if (hipMalloc(&d_rays, rays.size() * sizeof(apirl::RayXY)) != hipSuccess) {
-    (void)hipGetLastError();
-    return fail(c, APIRL_ERR_NOMEM, "cannot allocate the LOR table");
-  }
-  if (hipMemcpy(d_rays, rays.data(), rays.size() * sizeof(apirl::RayXY), hipMemcpyHostToDevice) !=
-      hipSuccess) {
-    (void)hipFree(d_rays);
-    return fail(c, APIRL_ERR_HIP, "cannot upload the LOR table");
-  }
-  // The gather's tables for the new LORs: entry crossings, and for the mMR the
-  // candidate search's odd-bin angles and radial lookup table.
-  float* d_xytab = nullptr;
-  float2* d_odd = nullptr;
-  int2* d_lut = nullptr;
-  float rl0 = 0.f, rl_inv_h = 0.f;
-  int nrl = 0;
-  bool gather = p->gather_tables;
-  if (gather) {
-    std::vector<float> xy;
-    apirl::build_xy_crossings(p->img, rays, xy);
-    gather = hipMalloc(&d_xytab, xy.size() * sizeof(float)) == hipSuccess &&
-             hipMemcpy(d_xytab, xy.data(), xy.size() * sizeof(float), hipMemcpyHostToDevice) ==
-                 hipSuccess;
-  }
-  if (gather && scanner == APIRL_SCANNER_MMR) {
-    std::vector<float> r;
-    std::vector<float2> odd;
-    std::vector<int2> lut;
-    apirl::mmr_radial_grid(p->geo.nang, p->geo.nr, r, odd);
-    apirl::build_radial_lut(r, lut, rl0, rl_inv_h);
-    nrl = (int)lut.size();
-    gather = hipMalloc(&d_odd, odd.size() * sizeof(float2)) == hipSuccess &&
-             hipMemcpy(d_odd, odd.data(), odd.size() * sizeof(float2), hipMemcpyHostToDevice) ==
-                 hipSuccess &&
-             hipMalloc(&d_lut, lut.size() * sizeof(int2)) == hipSuccess &&
-             hipMemcpy(d_lut, lut.data(), lut.size() * sizeof(int2), hipMemcpyHostToDevice) ==
-                 hipSuccess;
-  }
-  if (!gather) {
-    (void)hipGetLastError();
-    if (d_xytab) (void)hipFree(d_xytab);
-    if (d_odd) (void)hipFree(d_odd);
-    if (d_lut) (void)hipFree(d_lut);
-    d_xytab = nullptr;
-    d_odd = nullptr;
-    d_lut = nullptr;
-    nrl = 0;
-  }
-  APIRL_HIP(c, hipStreamSynchronize(c->stream));
-  (void)hipFree(p->d_rays);
-  p->d_rays = d_rays;
-  if (p->d_xytab) (void)hipFree(p->d_xytab);
-  if (p->d_angcs_odd) (void)hipFree(p->d_angcs_odd);
-  if (p->d_rlut) (void)hipFree(p->d_rlut);
-  p->d_xytab = d_xytab;
-  p->d_angcs_odd = d_odd;
-  p->d_rlut = d_lut;
-  p->rl0 = rl0;
-  p->rl_inv_h = rl_inv_h;
-  p->nrl = nrl;
-  p->scanner = scanner;
-  p->gather_ok = gather;
+  if (!apirl::build_lor_form(p, scanner)) return APIRL_ERR_NOMEM;
+  apirl::use_lor_form(p, scanner);
   return APIRL_OK;
 }
 
@@ -560,7 +566,8 @@ apirl_status apirl_project(apirl_plan* p, const float* d_img, float* d_sino, int
   if (!p || !d_img || !d_sino) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
-  if (precision < 0 || precision > 2) return fail(p->ctx, APIRL_ERR_INVALID, "invalid precision");
+  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_SETUP)
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid precision");
   if (!os_precision_ok(p, precision))
     return fail(p->ctx, APIRL_ERR_UNSUPPORTED, "oversampled LORs use the reference precision");
   DeviceGuard g(p->ctx->device);
@@ -594,12 +601,13 @@ apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const 
 
 apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float* d_meas,
                                  const float* d_additive, float* d_ratio, double* d_lik,
-                                 int subset, int nsubsets, int lik_positive_only,
-                                 int precision) {
+                                 int subset, int nsubsets, int flags, int precision) {
   if (!p || !d_img || !d_meas || !d_ratio) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
   apirl_ctx* c = p->ctx;
+  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_SETUP)
+    return fail(c, APIRL_ERR_INVALID, "invalid precision");
   if (!os_precision_ok(p, precision))
     return fail(c, APIRL_ERR_UNSUPPORTED, "oversampled LORs use the reference precision");
   DeviceGuard g(c->device);
@@ -609,7 +617,10 @@ apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float*
   fr.meas = d_meas;
   fr.additive = d_additive;
   fr.ratio = d_ratio;
-  fr.lik_positive_only = lik_positive_only;
+  if (flags & ~(APIRL_RATIO_LIK_POSITIVE | APIRL_RATIO_ADDITIVE_LOCAL))
+    return fail(c, APIRL_ERR_INVALID, "unknown project_ratio flags");
+  fr.lik_positive_only = (flags & APIRL_RATIO_LIK_POSITIVE) ? 1 : 0;
+  fr.additive_local = (flags & APIRL_RATIO_ADDITIVE_LOCAL) ? 1 : 0;
   if (d_lik) {
     apirl_status st = ensure_red(c, (size_t)nblocks);
     if (st != APIRL_OK) return st;
@@ -724,18 +735,55 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
       return st;                \
     }                           \
   } while (0)
+  // OSEM on the mMR: every subset sinogram the reference builds carries cylindrical
+  // LORs (APIRL_SCANNER_MMR_SUBSET), sensitivities included; only the final full
+  // projection keeps the mMR's (OsemSinogram3d.cpp:101-117,164-166,291-292).
+  struct FormRestore {
+    apirl_plan* p;
+    int form;
+    bool on;
+    ~FormRestore() {
+      if (on) apirl::use_lor_form(p, form);
+    }
+  } restore{p, p->scanner, false};
+  if (p->scanner == APIRL_SCANNER_MMR && opt->osem_rules) {
+    if (!apirl::build_lor_form(p, APIRL_SCANNER_MMR_SUBSET)) {
+      cleanup();
+      return APIRL_ERR_NOMEM;
+    }
+    apirl::use_lor_form(p, APIRL_SCANNER_MMR_SUBSET);
+    restore.on = true;
+  }
   // The precision mode applies to the forward projector and, for the gather form,
   // to the backprojector; the scatter form always uses the reference's arithmetic.
   const int bp_prec = opt->bp_mode == APIRL_BP_GATHER ? opt->precision : APIRL_PREC_REFERENCE;
   std::vector<float> thr(S, 0.f);
+  if (opt->sensitivity_given && !d_sens_out) {
+    cleanup();
+    return fail(c, APIRL_ERR_INVALID, "sensitivity_given needs the images in d_sens_out");
+  }
   for (int s = 0; s < S; s++) {
-    APIRL_TRY(sensitivity_impl(p, d_mult, d_sens + (size_t)s * nvox, s, S, opt->bp_mode,
-                               opt->osem_rules ? 1 : 0, bp_prec));
+    if (!opt->sensitivity_given) {
+      APIRL_TRY(sensitivity_impl(p, d_mult, d_sens + (size_t)s * nvox, s, S, opt->bp_mode,
+                                 opt->osem_rules ? 1 : 0, bp_prec));
+    } else if (opt->osem_rules) {
+      // images read from files are clamped as computed ones (OsemSinogram3d.cpp:93)
+      hipError_t e = apirl::launch_force_positive(d_sens + (size_t)s * nvox, nvox, c->stream);
+      if (e != hipSuccess) {
+        cleanup();
+        return hip_fail(c, e, "forcePositive");
+      }
+    }
     if (!opt->osem_rules) APIRL_TRY(apirl_mlem_threshold(c, d_sens + (size_t)s * nvox, nvox, &thr[s]));
   }
   // MLEM: 2D updates with s > thr (Mlem2d.cpp:175), 3D with s >= thr
   // (MlemSinogram3d.cpp:231); OSEM: threshold 0 with >= (OsemSinogram3d.cpp:28,221).
   const int strict = (!opt->osem_rules && is2d) ? 1 : 0;
+  // Likelihood rule (2D: est > 0) and the additive sinogram's angle per subset bin:
+  // the reference adds angle k of the full additive sinogram to subset bin k
+  // (OsemSinogram3d.cpp:172-173, Sinogram3D.cpp:857-873).
+  const int ratio_flags = (is2d ? APIRL_RATIO_LIK_POSITIVE : 0) |
+                          (opt->additive_own_angles ? 0 : APIRL_RATIO_ADDITIVE_LOCAL);
   if (hipMemcpyAsync(d_img, d_init, nvox * sizeof(float), hipMemcpyDeviceToDevice, c->stream) !=
       hipSuccess) {
     cleanup();
@@ -760,7 +808,7 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
       const size_t q = ((size_t)t * S + s) * 3;
       mark(q);
       APIRL_TRY(apirl_project_ratio(p, d_img, d_meas, d_additive, d_ratio,
-                                    d_lik + (size_t)t * S + s, s, S, is2d ? 1 : 0,
+                                    d_lik + (size_t)t * S + s, s, S, ratio_flags,
                                     opt->precision));
       mark(q + 1);
       if (hipMemsetAsync(d_bp, 0, nvox * sizeof(float), c->stream) != hipSuccess) {
@@ -774,10 +822,15 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
   }
   mark((size_t)T * S * 3);
   // Final likelihood.  3D drivers project the final image once more over the full
-  // sinogram (MlemSinogram3d.cpp:277, OsemSinogram3d.cpp:291-293).  Mlem2d takes it on
-  // the last estimated sinogram, which by then holds the ratio (Mlem2d.cpp:227-228).
+  // sinogram, without the additive term (MlemSinogram3d.cpp:277-278,
+  // OsemSinogram3d.cpp:291-293).  Mlem2d takes it on the last estimated sinogram,
+  // which by then holds the ratio (Mlem2d.cpp:227-228).
+  if (restore.on) {
+    apirl::use_lor_form(p, restore.form);
+    restore.on = false;
+  }
   if (!is2d) {
-    APIRL_TRY(apirl_project_ratio(p, d_img, d_meas, d_additive, d_ratio,
+    APIRL_TRY(apirl_project_ratio(p, d_img, d_meas, nullptr, d_ratio,
                                   d_lik + (size_t)T * S, 0, 1, 0, opt->precision));
   } else {
     const size_t n_last = apirl_plan_subset_bins(p, S - 1, S);
@@ -826,3 +879,62 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
 }
 
 }  // extern "C"
+
+namespace apirl {
+
+int use_lor_form(apirl_plan* p, int f) {
+  const int prev = p->scanner;
+  const LorForm& L = p->forms[f];
+  p->d_rays = L.rays;
+  p->d_xytab = L.xytab;
+  p->d_angcs_odd = L.angcs_odd;
+  p->d_rlut = L.rlut;
+  p->rl0 = L.rl0;
+  p->rl_inv_h = L.rl_inv_h;
+  p->nrl = L.nrl;
+  p->scanner = f;
+  // the gather's candidate search and tables assume one LOR per bin
+  p->gather_ok = p->gather_tables && L.gather && p->geo.ns == 1 && p->geo.nas == 1;
+  return prev;
+}
+
+bool build_lor_form(apirl_plan* p, int form) {
+  LorForm& f = p->forms[form];
+  if (f.built) return true;
+  apirl_ctx* c = p->ctx;
+  if (p->geo.dims != 3) {
+    fail(c, APIRL_ERR_UNSUPPORTED, "scanner geometries are set on 3D plans");
+    return false;
+  }
+  std::vector<RayXY> rays;
+  std::vector<float> r;
+  if (form == APIRL_SCANNER_CYLINDRICAL) {
+    build_ray_table(p->img, p->sino, p->ang, p->r, rays);
+    r = p->r;
+  } else {
+    std::vector<float4> pts;
+    if (form == APIRL_SCANNER_MMR) mmr_lor_points(p->geo.nang, p->geo.nr, pts);
+    else mmr_subset_lor_points(p->geo.nang, p->geo.nr, pts);
+    std::vector<float2> odd;
+    mmr_radial_grid(p->geo.nang, p->geo.nr, r, odd);
+    std::vector<char> ok(pts.size(), 1);  // the 3D form rejects no LOR
+    build_ray_table_pts(p->img, false, pts, ok, rays);
+  }
+  if (hipMalloc(&f.rays, rays.size() * sizeof(RayXY)) != hipSuccess ||
+      hipMemcpy(f.rays, rays.data(), rays.size() * sizeof(RayXY), hipMemcpyHostToDevice) !=
+          hipSuccess) {
+    (void)hipGetLastError();
+    free_form(f);
+    fail(c, APIRL_ERR_NOMEM, "cannot upload the LOR table");
+    return false;
+  }
+  if (p->gather_tables) {
+    if (form == APIRL_SCANNER_CYLINDRICAL)
+      f.gather = upload_gather_form(p, rays, p->r, !p->uniform_r, false, f);
+    else f.gather = upload_gather_form(p, rays, r, true, form == APIRL_SCANNER_MMR, f);
+  }
+  f.built = true;
+  return true;
+}
+
+}  // namespace apirl

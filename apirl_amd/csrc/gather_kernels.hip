@@ -171,6 +171,9 @@ struct Acc<APIRL_PREC_KAHAN> {
   __device__ void add_f64(double) {}
   __device__ float value() const { return s; }
 };
+// F64_SETUP: float crossings and weights from the double setup, summed as KAHAN.
+template <>
+struct Acc<APIRL_PREC_F64_SETUP> : Acc<APIRL_PREC_KAHAN> {};
 template <>
 struct Acc<APIRL_PREC_F64_ALPHA> {
   using alpha_t = double;
@@ -335,7 +338,11 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
       // reference's float setup from the ray table, or (F64_ALPHA, 3D) the same
       // setup carried out in double (siddon_f64.h).
       using R = typename Acc<PREC>::alpha_t;
-      constexpr bool kF64Setup = PREC == APIRL_PREC_F64_ALPHA && DIMS == 3;
+      constexpr bool kF64Setup =
+          (PREC == APIRL_PREC_F64_ALPHA || PREC == APIRL_PREC_F64_SETUP) && DIMS == 3;
+      // crossings from the (double) setup by repeated addition rather than the plan's
+      // float-setup tables
+      constexpr bool kOwnCrossings = PREC == APIRL_PREC_F64_ALPHA || kF64Setup;
       int ti0 = t.i0, tj0 = t.j0, tnxy = t.nxy;
       R tamin = t.amin, tamax = t.amax;
       R ax0, axu, ay0, ayu;
@@ -375,7 +382,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
       R Xp = -FLT_MAX, Xn, Yp = -FLT_MAX, Yn;
       Xn = ax0;
       Yn = ay0;
-      if (PREC == APIRL_PREC_F64_ALPHA) {
+      if (kOwnCrossings) {
         if (na > 0) {
           Xp = crossing<PREC>(ax0, axu, na - 1);
           Xn = Xp + axu;
@@ -447,7 +454,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           // does not depend on it); used only when the ray enters the slice by a z
           // crossing of its own, which makes it the entry zsetup wrote
           float ztv = 0.f;
-          if (PREC != APIRL_PREC_F64_ALPHA && A.ztab) {
+          if (!kOwnCrossings && A.ztab) {
             const int slot = s_soff[s + 1] - s_soff[s];
             int kk = k - s_kmin[s];
             kk = kk < 0 ? 0 : (kk >= slot ? slot - 1 : kk);
@@ -457,6 +464,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           int zk0 = z.k0, zk1 = z.k1, zkinc = z.kinc;
           R az0, azu;
           double Ld = 0.0;
+          float Lf = z.L;
           if (kF64Setup) {
             SetupZ64 zs;
             setup_z_f64(g, xy, zz.x, t.vx, t.vy, (double)(zz.y - zz.x), zs);
@@ -466,6 +474,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
             az0 = (R)zs.az;
             azu = (R)zs.azu;
             Ld = zs.L;
+            Lf = (float)zs.L;
           } else {
             z_crossings<PREC>(g, z, zz, az0, azu);
           }
@@ -481,7 +490,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           if (m >= tnxy + abs(zk1 - zk0)) continue;
           R Zp = -FLT_MAX, Zn = az0;
           if (nc > 0) {
-            if (PREC == APIRL_PREC_F64_ALPHA) {
+            if (kOwnCrossings) {
               Zp = crossing<PREC>(az0, azu, nc - 1);
             } else {
               Zp = nc <= z.zcount ? ztv : crossing<PREC>(az0, azu, nc - 1);
@@ -502,7 +511,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
           if (PREC == APIRL_PREC_F64_ALPHA) {
             acc.add_f64((an - ac) * Ld * z.b);
           } else {
-            const float w = (float)(an - ac) * z.L;
+            const float w = (float)(an - ac) * Lf;
             acc.add(w * 1.0f * z.b);  // SiddonProjector.cpp:546
           }
         }
@@ -545,12 +554,14 @@ inline size_t column_lds_bytes(int nsino) {
 // scan, each sinogram is tested against each voxel's own slab exactly as
 // gather_voxel tests it.  Record and table addresses are formed from a per-(angle,
 // radial bin) base and 32-bit offsets s npair and e npair, formed with 24-bit
-// multiplies: gather_fits admits a subset only when its records number below 2^31 and
+// multiplies: gather_fits admits a subset only when its records number below 2^31,
+// the z-table offsets stay below 2^32 (the table is held under 1 GiB) and
 // npair, the sinograms and the table slots stay below 2^24, which makes both exact.
 template <int PREC, int ZV, int GRID = kGridAny>
 __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2* s_zab,
                                               const int4* s_slot, int i, int j, int kb) {
-  static_assert(PREC != APIRL_PREC_F64_ALPHA, "double-precision setup uses gather_voxel");
+  static_assert(PREC != APIRL_PREC_F64_ALPHA && PREC != APIRL_PREC_F64_SETUP,
+                "double-precision setup uses gather_voxel");
   const Geo& g = A.g;
   const int nk = g.nz - kb < ZV ? g.nz - kb : ZV;  // voxels of this column segment (>= 1)
   const long nxy = (long)g.nx * g.ny;
@@ -842,8 +853,11 @@ template <int PREC>
 void launch_gather3(const GatherArgs& A, int zlanes, int zvox, hipStream_t st) {
   const Geo& g = A.g;
   const size_t lds = gather_lds_bytes(g.nsino);
-  if (PREC != APIRL_PREC_F64_ALPHA && zlanes <= 1 && zvox >= 1) {
-    constexpr int P = PREC == APIRL_PREC_F64_ALPHA ? APIRL_PREC_REFERENCE : PREC;
+  if (PREC != APIRL_PREC_F64_ALPHA && PREC != APIRL_PREC_F64_SETUP && zlanes <= 1 &&
+      zvox >= 1) {
+    constexpr int P =
+        (PREC == APIRL_PREC_F64_ALPHA || PREC == APIRL_PREC_F64_SETUP) ? APIRL_PREC_REFERENCE
+                                                                       : PREC;
     if (zvox == 1) launch_gather_column<P, 1>(A, st);
     else if (zvox == 2) launch_gather_column<P, 2>(A, st);
     else if (zvox == 3) launch_gather_column<P, 3>(A, st);
@@ -876,7 +890,7 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                                      int subset, int nsubsets, int kind, int precision,
                                      void* scratch, size_t scratch_bytes, hipStream_t st) {
   if (kind != BP_PLAIN && kind != BP_ONES && kind != BP_MULT) return hipErrorNotSupported;
-  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_ALPHA)
+  if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_SETUP)
     return hipErrorInvalidValue;
   const Geo& g = p->geo;
   const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
@@ -901,10 +915,13 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     if (precision == APIRL_PREC_REFERENCE)
       launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, zv, st);
     else if (precision == APIRL_PREC_KAHAN) launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, zv, st);
+    else if (precision == APIRL_PREC_F64_SETUP)
+      launch_gather3<APIRL_PREC_F64_SETUP>(A, A.zlanes, zv, st);
     else launch_gather3<APIRL_PREC_F64_ALPHA>(A, A.zlanes, zv, st);
   } else {
     if (precision == APIRL_PREC_REFERENCE) launch_gather2<APIRL_PREC_REFERENCE>(A, st);
-    else if (precision == APIRL_PREC_KAHAN) launch_gather2<APIRL_PREC_KAHAN>(A, st);
+    else if (precision == APIRL_PREC_KAHAN || precision == APIRL_PREC_F64_SETUP)
+      launch_gather2<APIRL_PREC_KAHAN>(A, st);
     else launch_gather2<APIRL_PREC_F64_ALPHA>(A, st);
   }
   return hipGetLastError();
@@ -969,13 +986,25 @@ hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2
   return hipGetLastError();
 }
 
-bool gather_fits(const apirl_plan* p, int subset, int nsubsets) {
+const char* gather_misfit(const apirl_plan* p, int subset, int nsubsets) {
   const Geo& g = p->geo;
-  if (g.dims != 3) return true;
+  if (g.dims != 3) return nullptr;
   const long nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long npair = nang_sub * g.nr;
   const long lim24 = 1L << 24;
-  return (long)g.nsino * nang_sub * g.nr < (1L << 31) && nang_sub * g.nr < lim24 &&
-         g.nsino < lim24 && p->zblock < lim24;
+  if ((long)g.nsino * npair >= (1L << 31)) return "the subset has 2^31 bins or more";
+  if (npair >= lim24) return "the subset has 2^24 (angle, radial bin) pairs or more";
+  if (g.nsino >= lim24) return "the sinogram count reaches 2^24";
+  if (p->zblock >= lim24) return "the z-crossing table slots reach 2^24 entries";
+  // the table offset e * npair (e < zblock) is a 32-bit product; the table exists only
+  // below 1 GiB, 2^28 floats, which keeps it there, and this says so explicitly
+  if (ztable_bytes(p, (int)nang_sub) > 0 && (unsigned long)p->zblock * npair >= (1UL << 32))
+    return "the z-crossing table offsets reach 2^32";
+  return nullptr;
+}
+
+bool gather_fits(const apirl_plan* p, int subset, int nsubsets) {
+  return gather_misfit(p, subset, nsubsets) == nullptr;
 }
 
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {

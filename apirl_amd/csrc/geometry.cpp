@@ -133,6 +133,23 @@ void mmr_lor_points(int nang, int nr, std::vector<float4>& pts) {
     }
 }
 
+void mmr_subset_lor_points(int nang, int nr, std::vector<float4>& pts) {
+  // Sinogram3DSiemensMmr::getSubset (Sinogram3DSiemensMmr.cpp:150-195) holds
+  // Sinogram2DinCylindrical3Dpet sinograms on the radius 328 mm (its segments are
+  // made inside the cylindrical base constructor) with the angles and the
+  // arc-corrected radial positions of the full mMR sinogram copied in; their LORs are
+  // the cylindrical 3D form (Sinogram2DinCylindrical3Dpet.cpp:216-229).
+  std::vector<float> r, ang;
+  float step;
+  mmr_grid(nang, nr, r, ang, step);
+  pts.resize((size_t)nang * nr);
+  for (int a = 0; a < nang; a++)
+    for (int j = 0; j < nr; j++) {
+      float4& q = pts[(size_t)a * nr + j];
+      lor_points(ang[a], r[j], kMmrRs, 0.f, false, &q.x, &q.y, &q.z, &q.w);
+    }
+}
+
 void build_ray_table_pts(const apirl_image_desc& img, bool is2d, const std::vector<float4>& pts,
                          const std::vector<char>& ok, std::vector<RayXY>& out) {
   const float dx = img.dx, dy = img.dy;

@@ -295,7 +295,47 @@ __device__ __forceinline__ double project_ray(
             acc = tv[u] ? nacc : acc;
           }
         }
-      } else if (PREC == APIRL_PREC_KAHAN) {
+      } else if (PREC == APIRL_PREC_F64_SETUP && DIMS == 3) {
+        // F64_SETUP: the setup in double from the float LOR (siddon_f64.h), rounded to
+        // float; then the reference's float loop (Siddon.cpp:310-340: repeated
+        // additions, the x / y / z tie order) with a Kahan-compensated float sum
+        SetupXY64 xy;
+        SetupZ64 zs;
+        setup_xy_f64(g, t.p0x, t.p0y, t.vx, t.vy, xy);
+        setup_z_f64(g, xy, za, t.vx, t.vy, (double)(zb - za), zs);
+        const int n = abs(xy.i1 - xy.i0) + abs(xy.j1 - xy.j0) + abs(zs.k1 - zs.k0);
+        float ax = (float)xy.ax, ay = (float)xy.ay, az = (float)zs.az, ac = (float)xy.amin;
+        const float axu = (float)xy.axu, ayu = (float)xy.ayu, azu = (float)zs.azu;
+        const float L = (float)zs.L;
+        const long nxy = (long)g.nx * g.ny;
+        int i = xy.i0, j = xy.j0, kk = zs.k0;
+        float c = 0.f;
+        for (int m = 0; m < n; m++) {
+          const int ci = i, cj = j, ck = kk;
+          float an;
+          if (ax <= ay && ax <= az) {
+            an = ax;
+            i += xy.iinc;
+            ax = ax + axu;
+          } else if (ay <= ax && ay <= az) {
+            an = ay;
+            j += xy.jinc;
+            ay = ay + ayu;
+          } else {
+            an = az;
+            kk += zs.kinc;
+            az = az + azu;
+          }
+          const float w = (an - ac) * L;
+          ac = an;
+          if (ci >= 0 && cj >= 0 && ck >= 0 && ci < g.nx && cj < g.ny && ck < g.nz) {
+            const float y = w * img[(long)ck * nxy + (long)cj * g.nx + ci] - c;
+            const float tt = acc + y;
+            c = (tt - acc) - y;
+            acc = tt;
+          }
+        }
+      } else if (PREC == APIRL_PREC_KAHAN || PREC == APIRL_PREC_F64_SETUP) {
         float c = 0.f;
         for (int m = 0; m < ray.n; m++) {
           int ci, cj, ck;
@@ -377,7 +417,8 @@ __device__ __forceinline__ double project_ray(
       // (Sinogram3D.cpp:584-607, 830-870; Sinogram2D.cpp:234-266, 565-581)
       const long full = ((long)s * g.nang + a) * g.nr + rr;
       float est = valid ? acc : 0.f;
-      if (fr.additive) est = est + fr.additive[full];
+      if (fr.additive)
+        est = est + fr.additive[fr.additive_local ? ((long)s * g.nang + k) * g.nr + rr : full];
       const float y = fr.meas[full];
       const bool lik_term = fr.lik_positive_only ? (est > 0.f) : (est != 0.f);
       if (lik_term) lik = (double)(y * logf(est) - est);
@@ -538,6 +579,22 @@ __global__ __launch_bounds__(kBlock) void em_update_kernel(float* __restrict__ x
   const float sv = s[i];
   const bool ok = strict ? (sv > thr) : (sv >= thr);
   x[i] = ok ? x[i] * b[i] / sv : 0.f;
+}
+
+// Per-bin value that DivideAndBackproject backprojects, for its gather form:
+// 3D (SiddonProjector.cpp:607-685): m == 0 -> 0, e != 0 -> m / e, else m;
+// 2D (:110-131): m == 0 or e == 0 -> 0, else m / e.
+__global__ __launch_bounds__(kBlock) void divide_rule_kernel(const float* __restrict__ m,
+                                                             const float* __restrict__ e,
+                                                             float* __restrict__ out, long n,
+                                                             int dims) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float mv = m[i], ev = e[i];
+  float r;
+  if (dims == 3) r = (mv == 0.f) ? 0.f : ((ev != 0.f) ? mv / ev : mv);
+  else r = (mv == 0.f || ev == 0.f) ? 0.f : mv / ev;
+  out[i] = r;
 }
 
 __global__ __launch_bounds__(kBlock) void force_positive_kernel(float* __restrict__ x, long n) {
@@ -742,6 +799,7 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
   switch (precision) {
     case APIRL_PREC_KAHAN: APIRL_LAUNCH_PROJ(APIRL_PREC_KAHAN); break;
     case APIRL_PREC_F64_ALPHA: APIRL_LAUNCH_PROJ(APIRL_PREC_F64_ALPHA); break;
+    case APIRL_PREC_F64_SETUP: APIRL_LAUNCH_PROJ(APIRL_PREC_F64_SETUP); break;
     default: APIRL_LAUNCH_PROJ(APIRL_PREC_REFERENCE); break;
   }
 #undef APIRL_LAUNCH_PROJ3
@@ -799,6 +857,14 @@ hipError_t launch_em_update(float* x, const float* b, const float* s, size_t n, 
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(em_update_kernel, dim3(grid_for((long)n)), dim3(kBlock), 0, st, x, b, s,
                      (long)n, thr, strict);
+  return hipGetLastError();
+}
+
+hipError_t launch_divide_rule(const float* meas, const float* est, float* out, size_t n,
+                              int dims, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(divide_rule_kernel, dim3(grid_for((long)n)), dim3(kBlock), 0, st, meas,
+                     est, out, (long)n, dims);
   return hipGetLastError();
 }
 

@@ -15,9 +15,13 @@ LIB_PATH = os.path.join(HERE, "libapirl_hip.so")
 APIRL_OK = 0
 BP_SCATTER = 0
 BP_GATHER = 1
+RATIO_LIK_POSITIVE = 1
+RATIO_ADDITIVE_LOCAL = 2
 PREC_REFERENCE = 0
 PREC_KAHAN = 1
-PREC_F64_ALPHA = 2
+PREC_F64 = 2          # all double
+PREC_F64_ALPHA = 2    # earlier name of PREC_F64
+PREC_F64_SETUP = 3    # double alpha setup, float stepping, Kahan float sum
 
 _c_int = ctypes.c_int
 _c_float = ctypes.c_float
@@ -38,7 +42,8 @@ class SinoDesc(ctypes.Structure):
 
 class EmOptions(ctypes.Structure):
     _fields_ = [("iterations", _c_int), ("subsets", _c_int), ("bp_mode", _c_int),
-                ("precision", _c_int), ("osem_rules", _c_int)]
+                ("precision", _c_int), ("osem_rules", _c_int), ("additive_own_angles", _c_int),
+                ("sensitivity_given", _c_int)]
 
 
 class Tuning(ctypes.Structure):
@@ -48,7 +53,7 @@ class Tuning(ctypes.Structure):
 class JobDesc(ctypes.Structure):
     _fields_ = [("plan", _P), ("iterations", _c_int), ("subsets", _c_int), ("osem_rules", _c_int),
                 ("d_meas", _P), ("d_init", _P), ("d_additive", _P), ("d_mult", _P),
-                ("d_out", _P)]
+                ("d_out", _P), ("additive_own_angles", _c_int)]
 
 
 # Every symbol the header declares, with its ctypes signature.
@@ -281,7 +286,7 @@ class Plan:
         self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
                        "apirl_plan_set_tuning")
 
-    SCANNERS = {"cylindrical": 0, "mmr": 1}
+    SCANNERS = {"cylindrical": 0, "mmr": 1, "mmr_subset": 2}
 
     def set_scanner(self, name):
         """LOR geometry of a 3D plan: "cylindrical" (the plan's grids) or "mmr"
@@ -346,12 +351,21 @@ class Plan:
         return out
 
     def project_ratio(self, img, meas, additive=None, subset=0, nsubsets=1, lik_positive_only=0,
-                      precision=PREC_REFERENCE, ratio=None, want_lik=True):
+                      precision=PREC_REFERENCE, ratio=None, want_lik=True, lik=None,
+                      additive_local=False):
+        """Fused forward step.  lik: optional float64 device tensor of one element that
+        receives the likelihood (allocated here when want_lik and lik is None).
+        additive_local: subset bin k takes the additive sinogram's angle k (the
+        reference's OSEM, OsemSinogram3d.cpp:172-173) instead of its own angle."""
         torch = _torch()
         if ratio is None:
             ratio = torch.empty(self.subset_shape(subset, nsubsets), dtype=torch.float32,
                                 device=img.device)
-        lik = torch.zeros(1, dtype=torch.float64, device=img.device) if want_lik else None
+        if lik is not None:
+            if lik.dtype != torch.float64 or lik.numel() != 1 or not lik.is_cuda:
+                raise ApirlError("lik must be a float64 GPU tensor of one element")
+        elif want_lik:
+            lik = torch.zeros(1, dtype=torch.float64, device=img.device)
         self._check_img(img)
         self._check_sino(meas, 0, 1)
         self._check_sino(ratio, subset, nsubsets)
@@ -359,7 +373,8 @@ class Plan:
             self._check_sino(additive, 0, 1)
         self.ctx.check(self.ctx.lib.apirl_project_ratio(
             self.h, _dptr(img), _dptr(meas), _dptr(additive), _dptr(ratio), _dptr(lik), subset,
-            nsubsets, lik_positive_only, precision), "apirl_project_ratio")
+            nsubsets, (RATIO_LIK_POSITIVE if lik_positive_only else 0) |
+            (RATIO_ADDITIVE_LOCAL if additive_local else 0), precision), "apirl_project_ratio")
         return ratio, lik
 
     def sensitivity(self, out=None, mult=None, subset=0, nsubsets=1, mode=BP_SCATTER,
@@ -375,16 +390,29 @@ class Plan:
 
     def reconstruct(self, meas, init, iterations, subsets=1, bp_mode=BP_SCATTER,
                     precision=PREC_REFERENCE, osem_rules=None, additive=None, mult=None,
-                    out=None, sens_out=None, times_out=None):
+                    out=None, sens_out=None, times_out=None, additive_own_angles=False,
+                    sens_in=None):
         """Device-resident MLEM (subsets=1) or OSEM; returns (image, likelihood[T*S+1]).
         times_out: optional float32 array [4, T*S] that receives each sub-iteration's
-        time in ms: whole, projection + ratio, backprojection, update."""
+        time in ms: whole, projection + ratio, backprojection, update.
+        additive_own_angles: add each subset bin's own angle of the additive sinogram
+        instead of the reference's subset-local angle (equal for one subset).
+        sens_in: sensitivity images [subsets, nz, ny, nx] to use instead of computing
+        them (the reference's "sensitivity filename"); clamped in place under OSEM."""
         torch = _torch()
         if osem_rules is None:
             osem_rules = 1 if subsets > 1 else 0
         if out is None:
             out = torch.empty(self.image_shape(), dtype=torch.float32, device=init.device)
-        opt = EmOptions(iterations, subsets, bp_mode, precision, osem_rules)
+        if sens_in is not None:
+            if sens_out is not None:
+                raise ValueError("pass sens_in or sens_out, not both")
+            if sens_in.numel() != subsets * self.nx * self.ny * self.nz or \
+                    not sens_in.is_contiguous() or sens_in.dtype != torch.float32:
+                raise ApirlError("sens_in must hold one float32 image per subset")
+            sens_out = sens_in
+        opt = EmOptions(iterations, subsets, bp_mode, precision, osem_rules,
+                        1 if additive_own_angles else 0, 1 if sens_in is not None else 0)
         lik = np.zeros(iterations * subsets + 1, np.float64)
         self._check_img(init)
         self._check_img(out)
@@ -453,14 +481,14 @@ class Batch:
                     "apirl_batch_set_launch")
 
     def submit(self, plan, meas, init, out, iterations, subsets=1, osem_rules=None,
-               additive=None, mult=None):
+               additive=None, mult=None, additive_own_angles=False):
         if osem_rules is None:
             osem_rules = 1 if subsets > 1 else 0
         plan._check_img(init)
         plan._check_img(out)
         plan._check_sino(meas, 0, 1)
         d = JobDesc(plan.h, iterations, subsets, osem_rules, _dptr(meas), _dptr(init),
-                    _dptr(additive), _dptr(mult), _dptr(out))
+                    _dptr(additive), _dptr(mult), _dptr(out), 1 if additive_own_angles else 0)
         jid = _c_int(-1)
         self._check(self.lib.apirl_batch_submit(self.h, ctypes.byref(d), ctypes.byref(jid)),
                     "apirl_batch_submit")

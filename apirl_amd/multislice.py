@@ -19,9 +19,11 @@ def fov_radius(nx, dx):
     return float(np.float32(np.float32(dx) * np.float32(nx)) / np.float32(2))
 
 
-def slice_plan(ctx, nx, dx, nproj, nr, rscanner):
-    """The 2D plan every slice uses."""
-    return hip.Plan.for_2d(ctx, nx, dx, nproj, nr, fov_radius(nx, dx), rscanner)
+def slice_plan(ctx, nx, dx, nproj, nr, rscanner, rfov=None):
+    """The 2D plan every slice uses: the radial FOV is the image's (the Mlem2d
+    override) unless given (the sinograms' own, as project / backproject read it)."""
+    return hip.Plan.for_2d(ctx, nx, dx, nproj, nr, fov_radius(nx, dx) if rfov is None else rfov,
+                           rscanner)
 
 
 def mlem2d_multiple(plan, sinos, iterations, init=None, mult=None, additive=None,

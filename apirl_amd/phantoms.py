@@ -85,3 +85,19 @@ def poisson_counts(truth, total, rng):
     s = t.sum()
     lam = t * (total / s) if s > 0 else t
     return rng.poisson(lam).astype(np.float32)
+
+
+def correction_sinograms(nsino, nang, nr, negative=False):
+    """Deterministic additive and multiplicative sinograms [nsino][nang][nr] whose
+    values vary with the angle, so a mix-up of angles between a subset and the full
+    sinogram changes the result.  additive: 0.05..0.4 counts; with negative=True about
+    one bin in 211 is -50, far below any projection there, so est + additive < 0 (the
+    reference adds it without a clamp, Sinogram3D.cpp:857-873).  multiplicative (the
+    attenuation-like factor of the sensitivity): 1..3."""
+    q, a, r = np.meshgrid(np.arange(nsino, dtype=np.int64), np.arange(nang, dtype=np.int64),
+                          np.arange(nr, dtype=np.int64), indexing="ij")
+    add = 0.05 + 0.06 * (a % 5) + 0.002 * (r % 17) + 0.001 * (q % 3)
+    if negative:
+        add = np.where((q * 131 + a * 17 + r * 7) % 211 == 0, -50.0, add)
+    mult = 1.0 + ((q * 7 + a * 3 + r) % 13) / 6.0
+    return add.astype(np.float32), mult.astype(np.float32)

@@ -400,6 +400,7 @@ def main():
     n_int = [plan.intersections(s, SUBSETS) for s in range(SUBSETS)]
     n_ray = [int(np.prod(plan.subset_shape(s, SUBSETS))) for s in range(SUBSETS)]
     ratio = torch.empty(max(n_ray), dtype=torch.float32, device=dev)
+    lik = torch.zeros(SUBSETS, dtype=torch.float64, device=dev)  # per sub-iteration
     bp = torch.empty(plan.image_shape(), dtype=torch.float32, device=dev)
     x = torch.ones(plan.image_shape(), dtype=torch.float32, device=dev)
     torch.cuda.synchronize()
@@ -417,7 +418,9 @@ def main():
                 e2 = torch.cuda.Event(enable_timing=True)
                 e3 = torch.cuda.Event(enable_timing=True)
                 e0.record()
-            plan.project_ratio(x, meas, subset=s, nsubsets=SUBSETS, ratio=r, want_lik=False)
+            # the reference evaluates the likelihood every sub-iteration
+            # (OsemSinogram3d.cpp:190): the fused epilogue reduces it with the ratio
+            plan.project_ratio(x, meas, subset=s, nsubsets=SUBSETS, ratio=r, lik=lik[s:s + 1])
             if record:
                 e1.record()
             plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=bp_mode)
@@ -449,6 +452,9 @@ def main():
         return float(np.mean([a.elapsed_time(b) for a, b, _ in ev[key]]))
 
     fwd_ms, bwd_ms, upd_ms = avg_ms("fwd"), avg_ms("bwd"), avg_ms("upd")
+    lik_last = lik.cpu().numpy()
+    if not np.all(np.isfinite(lik_last)):
+        raise RuntimeError("likelihood is not finite")
     nint_it = float(sum(n_int))
     nray_it = float(sum(n_ray))
     # algorithmic bytes per launch (averaged over the 21 subsets):
@@ -487,6 +493,7 @@ def main():
         "config": {"workload": "3D OSEM single job: 256x256x81 image, 344r x 252ang x 837 "
                                "sinograms (64 rings, span 11, MRD 60), 21 subsets",
                    "backprojector": args.bp, "parallelism": "single GPU"},
+        "likelihood_last_iteration": [float(v) for v in lik_last[[0, -1]]],
         "ray_voxel_gint_per_s": gint,
         "intersections_per_iteration": nint_it,
         "kernel_ms": {"forward_fused": fwd_ms, "backproject": bwd_ms, "update": upd_ms},

@@ -40,12 +40,19 @@ enum {
                            to the reference's single-thread result */
 };
 
-/* Precision of the forward line integral. */
+/* Arithmetic of the projector and of the gather backprojector. */
 enum {
   APIRL_PREC_REFERENCE = 0, /* fp32, the reference's operation order (bit-faithful) */
   APIRL_PREC_KAHAN = 1,     /* fp32 with Kahan-compensated accumulation along the ray */
-  APIRL_PREC_F64_ALPHA = 2  /* fp64 alpha stepping / weights, compensated fp32 sum */
+  APIRL_PREC_F64 = 2,       /* all double (3D): the Siddon setup, the alpha stepping, the
+                               weights and the sum */
+  APIRL_PREC_F64_SETUP = 3  /* the ray / plane alpha intersection setup in double (3D: alpha
+                               range, entry and exit voxels, list length, first crossings and
+                               increments, ray length), rounded to float; the stepping by
+                               repeated float additions, float weights and a Kahan-compensated
+                               float sum.  2D: as APIRL_PREC_KAHAN */
 };
+#define APIRL_PREC_F64_ALPHA APIRL_PREC_F64 /* earlier name of the all-double mode */
 
 typedef struct apirl_ctx apirl_ctx;
 typedef struct apirl_plan apirl_plan;
@@ -145,6 +152,19 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* plan, int ns, int nas);
  * (apirl_plan_set_oversampling) needs the cylindrical one. */
 #define APIRL_SCANNER_CYLINDRICAL 0
 #define APIRL_SCANNER_MMR 1
+/* APIRL_SCANNER_MMR_SUBSET: the LORs the reference's OSEM traces on mMR data.
+ * Sinogram3DSiemensMmr::getSubset (Sinogram3DSiemensMmr.cpp:150-195) constructs its
+ * subset with the parametric constructor, whose segments are built inside the
+ * cylindrical base constructor, where the virtual inicializarSegmentos is the base
+ * class's (Sinogram3DCylindricalPet): the subset's 2D sinograms are cylindrical ones
+ * on the radius 328 mm holding the mMR's arc-corrected radial positions (copied,
+ * :185-193), with no half-step for odd bins and no depth of interaction
+ * (Sinogram2DinCylindrical3Dpet.cpp:216-229).  OsemSinogram3d projects and
+ * backprojects every subset, sensitivities included, on these LORs, and only its
+ * final full projection (a Copy(), which keeps the mMR class) on the mMR's.
+ * apirl_em_reconstruct with osem_rules on an APIRL_SCANNER_MMR plan, and batch jobs
+ * likewise, switch to this form for the sub-iterations by themselves. */
+#define APIRL_SCANNER_MMR_SUBSET 2
 apirl_status apirl_plan_set_scanner(apirl_plan* plan, int scanner);
 apirl_status apirl_plan_get_tuning(const apirl_plan* plan, apirl_tuning* out);
 apirl_status apirl_plan_set_tuning(apirl_plan* plan, const apirl_tuning* tuning);
@@ -158,26 +178,36 @@ apirl_status apirl_project(apirl_plan* plan, const float* d_img, float* d_sino, 
 /* Backproject: d_img += A^T d_sino (accumulates; the caller zeroes, Mlem2d.cpp:162).
  * Replaces SiddonProjector::Backproject (2D :34-87, 3D :444-568).  precision applies
  * to the gather form: APIRL_PREC_REFERENCE sums the reference's float weights in
- * float, APIRL_PREC_KAHAN with a compensation term, APIRL_PREC_F64_ALPHA computes the
+ * float, APIRL_PREC_KAHAN with a compensation term, APIRL_PREC_F64 computes the
  * crossings and sums in double; the scatter form takes APIRL_PREC_REFERENCE only. */
 apirl_status apirl_backproject(apirl_plan* plan, const float* d_sino, float* d_img, int subset,
                                int nsubsets, int mode, int precision);
 /* DivideAndBackproject: d_img += A^T r with r the reference's ratio rule per segment
- * (3D SiddonProjector.cpp:571-704; 2D :90-144). */
+ * (3D SiddonProjector.cpp:571-704; 2D :90-144).  APIRL_BP_GATHER evaluates the rule
+ * per bin, then gathers (deterministic; w (m / e) instead of the reference's
+ * (w m) / e, within float rounding). */
 apirl_status apirl_divide_backproject(apirl_plan* plan, const float* d_meas, const float* d_est,
                                       float* d_img, int subset, int nsubsets, int mode);
 
 /* ---- fused EM steps ------------------------------------------------------------ */
-/* One forward step of an EM sub-iteration, fused: est = A x (+ additive), the
- * likelihood partial sum y log(est) - est over est != 0 (or est > 0 when
- * lik_positive_only, the 2D rule) and the ratio
+/* Flags of apirl_project_ratio. */
+#define APIRL_RATIO_LIK_POSITIVE 1  /* likelihood over est > 0 (Sinogram2D.cpp:565-581);
+                                       without it over est != 0 (Sinogram3D.cpp:584-607) */
+#define APIRL_RATIO_ADDITIVE_LOCAL 2 /* bin (s, k, r) of the subset takes the additive
+                                       sinogram's angle k (the subset-local index into
+                                       the full sinogram), as OsemSinogram3d.cpp:172-173
+                                       adds it through Sinogram3D::addBinToBin
+                                       (Sinogram3D.cpp:857-873); without it the bin's own
+                                       angle subset + nsubsets k */
+/* One forward step of an EM sub-iteration, fused: est = A x (+ additive, no clamp:
+ * Sinogram3D.cpp:857-873), the likelihood partial sum y log(est) - est and the ratio
  * ratio = (y != 0 && est != 0) ? y / est : 0 (Sinogram3D.cpp:830-856).
  * d_meas and d_additive are FULL sinograms (all angles); d_ratio is the subset
- * sinogram.  d_additive and d_lik may be NULL; *d_lik receives a double. */
+ * sinogram.  d_additive and d_lik may be NULL; *d_lik receives a double.
+ * flags: APIRL_RATIO_* (0 and 1 keep their meaning of lik_positive_only). */
 apirl_status apirl_project_ratio(apirl_plan* plan, const float* d_img, const float* d_meas,
                                  const float* d_additive, float* d_ratio, double* d_lik,
-                                 int subset, int nsubsets, int lik_positive_only,
-                                 int precision);
+                                 int subset, int nsubsets, int flags, int precision);
 /* x := cond ? x * b / s : 0, cond = s > thr (strict) or s >= thr
  * (Mlem2d.cpp:171-185, MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237). */
 apirl_status apirl_em_update(apirl_ctx* ctx, float* d_x, const float* d_b, const float* d_s,
@@ -201,6 +231,14 @@ typedef struct apirl_em_options {
   int osem_rules;      /* 1: per-subset sensitivity with forcePositive, threshold 0 with >=
                           (OsemSinogram3d); 0: MLEM rules (threshold min+1e-4(max-min),
                           strict > for 2D, >= for 3D) */
+  int additive_own_angles; /* 0: the reference's OSEM adds the additive sinogram's
+                          angle k to subset bin k (APIRL_RATIO_ADDITIVE_LOCAL); 1: each
+                          bin's own angle.  Equal for one subset. */
+  int sensitivity_given; /* 1: d_sens_out already holds the sensitivity image(s) (the
+                          reference's "sensitivity filename": OsemSinogram3d.cpp:83-95,
+                          MlemSinogram3d.cpp:132-138): they are used instead of computed,
+                          after forcePositive in place under OSEM rules, with the MLEM
+                          threshold taken from them otherwise. */
 } apirl_em_options;
 
 /* Runs the reconstruction with every buffer resident on the device.
@@ -245,6 +283,7 @@ typedef struct apirl_job_desc {
   const float* d_additive;  /* additive sinogram (full) or NULL */
   const float* d_mult;      /* multiplicative sinogram (sensitivity only) or NULL */
   float* d_out;             /* image, updated in place every sub-iteration */
+  int additive_own_angles;  /* as apirl_em_options.additive_own_angles */
 } apirl_job_desc;
 /* max_sinograms bounds the 3D jobs' sinogram count (LDS of the gather launch). */
 apirl_status apirl_batch_create(apirl_ctx* ctx, int max_jobs, int max_sinograms,

@@ -31,12 +31,27 @@ size_t bins3d(Sinogram3D* s) {
 }  // namespace
 
 HipSiddonProjector::HipSiddonProjector(int gpuId, int backprojectionMode)
-    : ctx(nullptr), mode(backprojectionMode), d_img(nullptr), d_sino(nullptr),
-      d_sino2(nullptr), cap_img(0), cap_sino(0), cap_sino2(0) {
+    : HipSiddonProjector(gpuId, backprojectionMode, 1, 1) {}
+
+HipSiddonProjector::HipSiddonProjector(int gpuId, int backprojectionMode, int numSamples,
+                                       int numAxialSamples)
+    : ctx(nullptr), mode(backprojectionMode), nsamples(numSamples), naxial(numAxialSamples),
+      d_img(nullptr), d_sino(nullptr), d_sino2(nullptr), cap_img(0), cap_sino(0),
+      cap_sino2(0) {
   if (apirl_ctx_create(gpuId, &ctx) != APIRL_OK) {
     ctx = nullptr;
     strError = "HipSiddonProjector: no usable GPU for index " + std::to_string(gpuId);
   }
+}
+
+bool HipSiddonProjector::multipleLorsRefused() {
+  // SiddonProjector.cpp:349-352: with useMultipleLorsPerBin every ring pair of a
+  // sinogram is traced; this projector traces the mean pair only, so it says so
+  // instead of returning another result
+  if (!useMultipleLorsPerBin) return false;
+  fail("HipSiddonProjector: useMultipleLorsPerBin (one LOR per ring pair) is not "
+       "supported; each bin is traced on its sinogram's mean ring pair");
+  return true;
 }
 
 HipSiddonProjector::~HipSiddonProjector() {
@@ -155,6 +170,7 @@ apirl_plan* HipSiddonProjector::planFor(Image* image, Sinogram3D* sino) {
   sd.r_mm = mmr ? nullptr : s00->getRPtr();
   sd.z1_mean_mm = z1.data();
   sd.z2_mean_mm = z2.data();
+  if (mmr && nsamples * naxial != 1) return nullptr;  // defined for the cylindrical scanner
   std::string key = mmr ? "3m" : "3";
   append(key, &im, sizeof(im));
   append(key, &sd.nang, 3 * sizeof(int));
@@ -170,6 +186,10 @@ apirl_plan* HipSiddonProjector::planFor(Image* image, Sinogram3D* sino) {
   apirl_plan* p = nullptr;
   if (apirl_plan_create(ctx, &im, &sd, &p) != APIRL_OK) return nullptr;
   if (mmr && apirl_plan_set_scanner(p, APIRL_SCANNER_MMR) != APIRL_OK) {
+    apirl_plan_destroy(p);
+    return nullptr;
+  }
+  if (nsamples * naxial != 1 && apirl_plan_set_oversampling(p, nsamples, naxial) != APIRL_OK) {
     apirl_plan_destroy(p);
     return nullptr;
   }
@@ -244,7 +264,7 @@ bool HipSiddonProjector::DivideAndBackproject(Sinogram2D* measured, Sinogram2D* 
   if (apirl_memcpy_h2d(ctx, d_img, image->getPixelsPtr(), nv * 4) != APIRL_OK ||
       apirl_memcpy_h2d(ctx, d_sino, measured->getSinogramPtr(), nb * 4) != APIRL_OK ||
       apirl_memcpy_h2d(ctx, d_sino2, estimated->getSinogramPtr(), nb * 4) != APIRL_OK ||
-      apirl_divide_backproject(p, d_sino, d_sino2, d_img, 0, 1, APIRL_BP_SCATTER) != APIRL_OK ||
+      apirl_divide_backproject(p, d_sino, d_sino2, d_img, 0, 1, mode) != APIRL_OK ||
       apirl_memcpy_d2h(ctx, image->getPixelsPtr(), d_img, nv * 4) != APIRL_OK)
     return fail("HipSiddonProjector::DivideAndBackproject");
   return true;
@@ -254,6 +274,7 @@ bool HipSiddonProjector::DivideAndBackproject(Sinogram2D* measured, Sinogram2D* 
 
 bool HipSiddonProjector::Project(Image* image, Sinogram3D* projection) {
   if (!ctx) return false;
+  if (multipleLorsRefused()) return false;
   apirl_plan* p = planFor(image, projection);
   if (!p) return fail("HipSiddonProjector::Project: unsupported 3D geometry");
   const size_t nv = apirl_plan_image_voxels(p), nb = bins3d(projection);
@@ -271,6 +292,7 @@ bool HipSiddonProjector::Project(Image* image, Sinogram3D* projection) {
 
 bool HipSiddonProjector::Backproject(Sinogram3D* projection, Image* image) {
   if (!ctx) return false;
+  if (multipleLorsRefused()) return false;
   apirl_plan* p = planFor(image, projection);
   if (!p) return fail("HipSiddonProjector::Backproject: unsupported 3D geometry");
   const size_t nv = apirl_plan_image_voxels(p), nb = bins3d(projection);
@@ -290,6 +312,7 @@ bool HipSiddonProjector::Backproject(Sinogram3D* projection, Image* image) {
 bool HipSiddonProjector::DivideAndBackproject(Sinogram3D* measured, Sinogram3D* estimated,
                                               Image* image) {
   if (!ctx) return false;
+  if (multipleLorsRefused()) return false;
   apirl_plan* p = planFor(image, measured);
   if (!p) return fail("HipSiddonProjector::DivideAndBackproject: unsupported 3D geometry");
   const size_t nv = apirl_plan_image_voxels(p), nb = bins3d(measured);
@@ -304,7 +327,7 @@ bool HipSiddonProjector::DivideAndBackproject(Sinogram3D* measured, Sinogram3D* 
   if (apirl_memcpy_h2d(ctx, d_img, image->getPixelsPtr(), nv * 4) != APIRL_OK ||
       apirl_memcpy_h2d(ctx, d_sino, hm.data(), nb * 4) != APIRL_OK ||
       apirl_memcpy_h2d(ctx, d_sino2, he.data(), nb * 4) != APIRL_OK ||
-      apirl_divide_backproject(p, d_sino, d_sino2, d_img, 0, 1, APIRL_BP_SCATTER) != APIRL_OK ||
+      apirl_divide_backproject(p, d_sino, d_sino2, d_img, 0, 1, mode) != APIRL_OK ||
       apirl_memcpy_d2h(ctx, image->getPixelsPtr(), d_img, nv * 4) != APIRL_OK)
     return fail("HipSiddonProjector::DivideAndBackproject");
   return true;

@@ -14,6 +14,8 @@
  * Semantics follow SiddonProjector (recon/src/SiddonProjector.cpp): Project
  * overwrites, Backproject and DivideAndBackproject accumulate into the caller's
  * image.  Errors return false with getLastError() filled, as the reference does.
+ * Each bin is the LOR of its sinogram's mean ring pair (SiddonProjector.cpp:349-361);
+ * with setMultipleLorsPerBin(true) (Projector.h:54-63) the 3D calls return false.
  */
 #ifndef HIP_SIDDON_PROJECTOR_H
 #define HIP_SIDDON_PROJECTOR_H
@@ -31,8 +33,15 @@
 class HipSiddonProjector : virtual Projector {
  public:
   /* gpuId: device index; backprojectionMode: APIRL_BP_GATHER (deterministic, the
-   * default) or APIRL_BP_SCATTER (fp32 atomics). */
+   * default; Backproject and DivideAndBackproject alike) or APIRL_BP_SCATTER (fp32
+   * atomics). */
   HipSiddonProjector(int gpuId = 0, int backprojectionMode = APIRL_BP_GATHER);
+  /* SiddonProjector(nSamplesOnDetector, nAxialSamplesOnDetector)
+   * (recon/src/SiddonProjector.cpp:28-32): 3D bins of a cylindrical scanner traced as
+   * numSamples x numAxialSamples LORs (apirl_plan_set_oversampling).  Oversampled
+   * bins backproject with APIRL_BP_SCATTER only (the gather takes one LOR per bin);
+   * a gather-mode projector then fails those calls with that message. */
+  HipSiddonProjector(int gpuId, int backprojectionMode, int numSamples, int numAxialSamples);
   virtual ~HipSiddonProjector();
 
   bool Project(Image* image, Sinogram2D* projection);
@@ -52,8 +61,11 @@ class HipSiddonProjector : virtual Projector {
   void copySinogramIn(Sinogram3D* s, float* host);
   void copySinogramOut(Sinogram3D* s, const float* host);
 
+  bool multipleLorsRefused();
+
   apirl_ctx* ctx;
   int mode;
+  int nsamples, naxial;  // LORs per 3D bin (transaxial, axial)
   std::string strError;
   std::map<std::string, apirl_plan*> plans;
   float *d_img, *d_sino, *d_sino2;

@@ -459,23 +459,29 @@ void orc_backproject3d_os(const orc_image* im, const float* sino, int nang,
 
 /* ------------------------------------------------------------ mMR geometry */
 
-void orc_mmr_lor_table(int nang, int nr, float* pts) {
-  /* Sinogram2DinSiemensMmr.cpp:27-50: bin = 4.0891/2 mm (scaled by 344/nr),
-   * lr = bin/2 + bin (j - (float)(nr/2)), r = (Rs + DOI cos(lr/Rs)) sin(lr/Rs),
-   * phi_i = i 180/nang; :175-192: odd bins at phi - step/2, LOR ends on Rs + DOI.
-   * pts[(a nr + j) 4 ..] = x1, y1, x2, y2. */
+/* The mMR's radial positions and angles (Sinogram2DinSiemensMmr.cpp:27-50): bin =
+ * 4.0891/2 mm (scaled by 344/nr), lr = bin/2 + bin (j - (float)(nr/2)),
+ * r = (Rs + DOI cos(lr/Rs)) sin(lr/Rs), phi_i = i 180/nang. */
+static void mmr_grids(int nang, int nr, float* r, float* ang) {
   const float Rs = 328.0f, doi = 9.6f, bin0 = 4.0891f / 2.0f;
   float bin = nr == 344 ? bin0 : bin0 * 344 / nr;
-  float* r = (float*)malloc(sizeof(float) * nr);
-  float* ang = (float*)malloc(sizeof(float) * nang);
   for (int j = 0; j < nr; j++) {
     float lr = bin / 2 + (bin * (j - (float)(nr / 2)));
     r[j] = (Rs + doi * cosf(lr / Rs)) * sinf(lr / Rs);
   }
   float inc = (float)180 / nang;
   for (int a = 0; a < nang; a++) ang[a] = a * inc;
+}
+
+void orc_mmr_lor_table(int nang, int nr, float* pts) {
+  /* Sinogram2DinSiemensMmr.cpp:175-192: odd bins at phi - step/2 (step =
+   * getAngValue(1) - getAngValue(0)), the LOR ending on Rs + DOI.
+   * pts[(a nr + j) 4 ..] = x1, y1, x2, y2. */
+  const float Re = 328.0f + 9.6f;
+  float* r = (float*)malloc(sizeof(float) * nr);
+  float* ang = (float*)malloc(sizeof(float) * nang);
+  mmr_grids(nang, nr, r, ang);
   float step = nang > 1 ? ang[1] - ang[0] : 0.f;
-  float Re = Rs + doi;
   for (int a = 0; a < nang; a++)
     for (int j = 0; j < nr; j++) {
       float phi = (ang[a] - (j % 2) * step / 2) * 0.01745329251994329576f;
@@ -485,6 +491,27 @@ void orc_mmr_lor_table(int nang, int nr, float* pts) {
       q[1] = r[j] * sinf(phi) - cosf(phi) * aux;
       q[2] = r[j] * cosf(phi) - sinf(phi) * aux;
       q[3] = r[j] * sinf(phi) + cosf(phi) * aux;
+    }
+  free(r);
+  free(ang);
+}
+
+void orc_mmr_subset_lor_table(int nang, int nr, float* pts) {
+  /* Sinogram3DSiemensMmr::getSubset (Sinogram3DSiemensMmr.cpp:150-195) constructs
+   * the subset with Sinogram3DSiemensMmr(numProj, numR, ...), whose segments are made
+   * by the Sinogram3DCylindricalPet base constructor: there the virtual
+   * inicializarSegmentos is the base class's, so the subset's 2D sinograms are
+   * Sinogram2DinCylindrical3Dpet on radioScannerMmr_mm = 328 mm.  The full sinogram's
+   * angles and arc-corrected r are copied into them (:163-193), and their LORs are
+   * the cylindrical 3D form (Sinogram2DinCylindrical3Dpet.cpp:216-229): no odd-bin
+   * half step, no depth of interaction. */
+  float* r = (float*)malloc(sizeof(float) * nr);
+  float* ang = (float*)malloc(sizeof(float) * nang);
+  mmr_grids(nang, nr, r, ang);
+  for (int a = 0; a < nang; a++)
+    for (int j = 0; j < nr; j++) {
+      float* q = pts + ((long)a * nr + j) * 4;
+      orc_lor_points(ang[a], r[j], 328.0f, INFINITY, q, q + 1, q + 2, q + 3);
     }
   free(r);
   free(ang);
@@ -690,21 +717,25 @@ static void backproject3d_src(const orc_image* im, const float* sino, const lor_
 
 static int mlem3d_core(const orc_image* im, const lor_source* L, int nsino, const float* za,
                        const float* zb, const float* meas, const float* init, int iters,
-                       float* img, float* sens, float* lik) {
-  /* recon/src/MlemSinogram3d.cpp:89-331: sensitivity = backprojection of ones
-   * (:312-331), update with s >= thr (:209-219), one extra projection for the
-   * final likelihood (:277). */
+                       const float* add, const float* mult, float* img, float* sens,
+                       float* lik) {
+  /* recon/src/MlemSinogram3d.cpp:89-331: sensitivity = backprojection of ones, or of
+   * the multiplicative sinogram when there is one (:312-331); the additive sinogram
+   * added to every projection without a clamp (:190-191, Sinogram3D.cpp:857-873);
+   * update with s >= thr (:209-219); one extra projection, without the additive
+   * term, for the final likelihood (:277-278). */
   long npix = (long)im->nx * im->ny * im->nz, nb = (long)nsino * L->nang * L->nr;
   float* ones = (float*)malloc(sizeof(float) * nb);
   float* est = (float*)calloc(nb, sizeof(float));
   float* bp = (float*)malloc(sizeof(float) * npix);
-  for (long i = 0; i < nb; i++) ones[i] = 1;
+  for (long i = 0; i < nb; i++) ones[i] = mult ? mult[i] : 1;
   memset(sens, 0, sizeof(float) * npix);
   backproject3d_src(im, ones, L, nsino, za, zb, sens);
   float thr = orc_mlem_threshold(sens, npix);
   memcpy(img, init, sizeof(float) * npix);
   for (int t = 0; t < iters; t++) {
     project3d_src(im, img, L, nsino, za, zb, est);
+    if (add) orc_add(est, add, nb);
     lik[t] = orc_likelihood(est, meas, nb, 0);
     memset(bp, 0, sizeof(float) * npix);
     orc_inverse_divide(est, meas, nb);
@@ -724,7 +755,7 @@ int orc_mlem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner
   float* r = (float*)malloc(sizeof(float) * nr);
   orc_sino_grid(nang, nr, rfov, ang, r);
   lor_source L = {nang, nr, ang, r, NULL, rscanner};
-  int rc = mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, img, sens, lik);
+  int rc = mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, NULL, NULL, img, sens, lik);
   free(ang); free(r);
   return rc;
 }
@@ -733,54 +764,140 @@ int orc_mlem3d_pts(const orc_image* im, int nang, int nr, const float* pts, int 
                    const float* za, const float* zb, const float* meas, const float* init,
                    int iters, float* img, float* sens, float* lik) {
   lor_source L = {nang, nr, NULL, NULL, pts, 0.0f};
-  return mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, img, sens, lik);
+  return mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, NULL, NULL, img, sens, lik);
+}
+
+int orc_mlem3d_pts_corr(const orc_image* im, int nang, int nr, const float* pts, int nsino,
+                        const float* za, const float* zb, const float* meas, const float* init,
+                        int iters, const float* add, const float* mult, float* img, float* sens,
+                        float* lik) {
+  lor_source L = {nang, nr, NULL, NULL, pts, 0.0f};
+  return mlem3d_core(im, &L, nsino, za, zb, meas, init, iters, add, mult, img, sens, lik);
 }
 
 int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
                const float* za, const float* zb, const float* meas, const float* init,
                int nsubsets, int iters, float* img, float* sens_all, float* lik) {
-  /* recon/src/OsemSinogram3d.cpp:24-346: per-subset sensitivity (backprojection of
-   * ones over the subset, then forcePositive), threshold 0 with >=, image updated
-   * after every subset; final likelihood from a full projection. */
-  long npix = (long)im->nx * im->ny * im->nz, nb = (long)nsino * nang * nr;
+  return orc_osem3d_corr(im, nang, nr, rfov, rscanner, nsino, za, zb, meas, init, nsubsets,
+                         iters, NULL, NULL, 1, img, sens_all, lik);
+}
+
+static int osem3d_core(const orc_image* im, const lor_source* Lsub, const lor_source* Lfull,
+                       int nsino, const float* za, const float* zb, const float* meas,
+                       const float* init, int nsubsets, int iters, const float* add,
+                       const float* mult, int add_local, float* img, float* sens_all,
+                       float* lik);
+
+int orc_osem3d_corr(const orc_image* im, int nang, int nr, float rfov, float rscanner,
+                    int nsino, const float* za, const float* zb, const float* meas,
+                    const float* init, int nsubsets, int iters, const float* add,
+                    const float* mult, int add_local, float* img, float* sens_all,
+                    float* lik) {
   float* ang = (float*)malloc(sizeof(float) * nang);
   float* r = (float*)malloc(sizeof(float) * nr);
   orc_sino_grid(nang, nr, rfov, ang, r);
+  lor_source L = {nang, nr, ang, r, NULL, rscanner};
+  int rc = osem3d_core(im, &L, &L, nsino, za, zb, meas, init, nsubsets, iters, add, mult,
+                       add_local, img, sens_all, lik);
+  free(ang); free(r);
+  return rc;
+}
+
+int orc_osem3d_pts_corr(const orc_image* im, int nang, int nr, const float* pts_sub,
+                        const float* pts_full, int nsino, const float* za, const float* zb,
+                        const float* meas, const float* init, int nsubsets, int iters,
+                        const float* add, const float* mult, int add_local, float* img,
+                        float* sens_all, float* lik) {
+  lor_source Ls = {nang, nr, NULL, NULL, pts_sub, 0.0f};
+  lor_source Lf = {nang, nr, NULL, NULL, pts_full, 0.0f};
+  return osem3d_core(im, &Ls, &Lf, nsino, za, zb, meas, init, nsubsets, iters, add, mult,
+                     add_local, img, sens_all, lik);
+}
+
+/* The LORs of subset s's angles idx[0..na-1] taken from L (grid or endpoint form). */
+static void subset_source(const lor_source* L, const int* idx, int na, float* sang,
+                          float* spts, lor_source* out) {
+  *out = *L;
+  out->nang = na;
+  if (L->pts) {
+    for (int k = 0; k < na; k++)
+      memcpy(spts + (long)k * L->nr * 4, L->pts + (long)idx[k] * L->nr * 4,
+             sizeof(float) * 4 * L->nr);
+    out->pts = spts;
+  } else {
+    for (int k = 0; k < na; k++) sang[k] = L->ang[idx[k]];
+    out->ang = sang;
+  }
+}
+
+static int osem3d_core(const orc_image* im, const lor_source* Lsub, const lor_source* Lfull,
+                       int nsino, const float* za, const float* zb, const float* meas,
+                       const float* init, int nsubsets, int iters, const float* add,
+                       const float* mult, int add_local, float* img, float* sens_all,
+                       float* lik) {
+  const int nang = Lsub->nang, nr = Lsub->nr;
+  /* recon/src/OsemSinogram3d.cpp:24-346: per-subset sensitivity (backprojection of
+   * ones, or of the multiplicative sinogram's subset, over the subset, then
+   * forcePositive; :327-346, :101-117), threshold 0 with >=, image updated after
+   * every subset; the additive sinogram added to the subset's projection without a
+   * clamp (:172-173) -- Sinogram3D::addBinToBin (Sinogram3D.cpp:857-873) indexes
+   * the FULL additive sinogram with the subset's own angle index k, so subset bin
+   * (q, k, r) gets additive bin (q, k, r) (add_local = 1; add_local = 0 takes the
+   * bin's own full angle instead); final likelihood from a full projection (over
+   * Lfull's LORs) without the additive term (:291-293).  Subset bins take Lsub's
+   * LORs: the grid's, or for the mMR the subset's own (orc_mmr_lor_table_subsets). */
+  long npix = (long)im->nx * im->ny * im->nz, nb = (long)nsino * nang * nr;
   int* idx = (int*)malloc(sizeof(int) * nang);
   float* sang = (float*)malloc(sizeof(float) * nang);
+  float* spts = (float*)malloc(sizeof(float) * 4 * (size_t)nang * nr);
   float* ysub = (float*)malloc(sizeof(float) * nb);
   float* est = (float*)malloc(sizeof(float) * nb);
   float* bp = (float*)malloc(sizeof(float) * npix);
+  lor_source Ls;
   for (int s = 0; s < nsubsets; s++) {
     int na = orc_subset_angles(nang, nsubsets, s, idx);
-    for (int k = 0; k < na; k++) sang[k] = ang[idx[k]];
+    subset_source(Lsub, idx, na, sang, spts, &Ls);
     long nbs = (long)nsino * na * nr;
-    for (long i = 0; i < nbs; i++) ysub[i] = 1;
+    if (mult) {
+      for (int q = 0; q < nsino; q++)
+        for (int k = 0; k < na; k++)
+          memcpy(ysub + ((long)q * na + k) * nr, mult + ((long)q * nang + idx[k]) * nr,
+                 sizeof(float) * nr);
+    } else {
+      for (long i = 0; i < nbs; i++) ysub[i] = 1;
+    }
     float* sens = sens_all + (long)s * npix;
     memset(sens, 0, sizeof(float) * npix);
-    orc_backproject3d(im, ysub, na, sang, nr, r, rscanner, nsino, za, zb, sens);
+    backproject3d_src(im, ysub, &Ls, nsino, za, zb, sens);
     orc_force_positive(sens, npix);
   }
   memcpy(img, init, sizeof(float) * npix);
   for (int t = 0; t < iters; t++)
     for (int s = 0; s < nsubsets; s++) {
       int na = orc_subset_angles(nang, nsubsets, s, idx);
-      for (int k = 0; k < na; k++) sang[k] = ang[idx[k]];
+      subset_source(Lsub, idx, na, sang, spts, &Ls);
       long nbs = (long)nsino * na * nr;
       for (int q = 0; q < nsino; q++)
         for (int k = 0; k < na; k++)
           memcpy(ysub + ((long)q * na + k) * nr, meas + ((long)q * nang + idx[k]) * nr,
                  sizeof(float) * nr);
-      orc_project3d(im, img, na, sang, nr, r, rscanner, nsino, za, zb, est);
+      project3d_src(im, img, &Ls, nsino, za, zb, est);
+      if (add)
+        for (int q = 0; q < nsino; q++)
+          for (int k = 0; k < na; k++) {
+            float* e = est + ((long)q * na + k) * nr;
+            const float* ad = add + ((long)q * nang + (add_local ? k : idx[k])) * nr;
+            for (int j = 0; j < nr; j++) e[j] = e[j] + ad[j];
+          }
       lik[t * nsubsets + s] = orc_likelihood(est, ysub, nbs, 0);
       memset(bp, 0, sizeof(float) * npix);
       orc_inverse_divide(est, ysub, nbs);
-      orc_backproject3d(im, est, na, sang, nr, r, rscanner, nsino, za, zb, bp);
+      backproject3d_src(im, est, &Ls, nsino, za, zb, bp);
       orc_em_update(img, bp, sens_all + (long)s * npix, npix, 0.0f, 0);
     }
-  orc_project3d(im, img, nang, ang, nr, r, rscanner, nsino, za, zb, est);
+  project3d_src(im, img, Lfull, nsino, za, zb, est);
   lik[iters * nsubsets] = orc_likelihood(est, meas, nb, 0);
-  free(ang); free(r); free(idx); free(sang); free(ysub); free(est); free(bp);
+  free(idx); free(sang); free(spts); free(ysub); free(est); free(bp);
   return 0;
 }
 

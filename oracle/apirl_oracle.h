@@ -70,6 +70,11 @@ void orc_project3d(const orc_image* im, const float* img, int nang, const float*
 /* Backprojectors accumulate into img in the reference's single-thread order
  * (SiddonProjector.cpp:34-87 for 2D, :444-568 for 3D). */
 void orc_mmr_lor_table(int nang, int nr, float* pts);
+/* The LORs of the reference's mMR SUBSET sinograms (Sinogram3DSiemensMmr::getSubset,
+ * Sinogram3DSiemensMmr.cpp:150-195): cylindrical (Sinogram2DinCylindrical3Dpet.cpp:
+ * 216-229) on the radius 328 mm at the mMR's angles and arc-corrected r -- the subset
+ * is built with cylindrical 2D sinograms; [nang][nr][4] for every full-grid angle. */
+void orc_mmr_subset_lor_table(int nang, int nr, float* pts);
 void orc_project3d_pts(const orc_image* im, const float* img, int nang, int nr, const float* pts,
                        int nsino, const float* za, const float* zb, float* sino);
 void orc_backproject3d_pts(const orc_image* im, const float* sino, int nang, int nr,
@@ -134,6 +139,30 @@ int orc_mlem3d_pts(const orc_image* im, int nang, int nr, const float* pts, int 
 int orc_osem3d(const orc_image* im, int nang, int nr, float rfov, float rscanner, int nsino,
                const float* za, const float* zb, const float* meas, const float* init,
                int nsubsets, int iters, float* img, float* sens_all, float* lik);
+/* orc_osem3d with the corrections of OsemSinogram3d.cpp:101-117, 172-173, 327-346:
+ * add (full additive sinogram, may be NULL) added to each subset projection without a
+ * clamp, subset bin (q, k, r) taking additive bin (q, k, r) when add_local (the
+ * reference's Sinogram3D::addBinToBin on a subset, Sinogram3D.cpp:857-873) or its own
+ * full angle otherwise; mult (full multiplicative sinogram, may be NULL) backprojected
+ * subset by subset as the sensitivity; the final likelihood without the additive term. */
+int orc_osem3d_corr(const orc_image* im, int nang, int nr, float rfov, float rscanner,
+                    int nsino, const float* za, const float* zb, const float* meas,
+                    const float* init, int nsubsets, int iters, const float* add,
+                    const float* mult, int add_local, float* img, float* sens_all,
+                    float* lik);
+/* orc_osem3d_corr over LORs given by endpoints: subset bins take pts_sub (e.g.
+ * orc_mmr_lor_table_subsets), the final full projection pts_full. */
+int orc_osem3d_pts_corr(const orc_image* im, int nang, int nr, const float* pts_sub,
+                        const float* pts_full, int nsino, const float* za, const float* zb,
+                        const float* meas, const float* init, int nsubsets, int iters,
+                        const float* add, const float* mult, int add_local, float* img,
+                        float* sens_all, float* lik);
+/* orc_mlem3d_pts with MlemSinogram3d's corrections (additive sinogram in every
+ * projection but the final one, multiplicative sinogram as the sensitivity). */
+int orc_mlem3d_pts_corr(const orc_image* im, int nang, int nr, const float* pts, int nsino,
+                        const float* za, const float* zb, const float* meas, const float* init,
+                        int iters, const float* add, const float* mult, float* img, float* sens,
+                        float* lik);
 
 /* FP64 oracle: the reference's 3D Siddon (recon/src/Siddon.cpp:13-344) carried out
  * in double from the same float LORs, with double accumulation (used to report the

@@ -51,8 +51,10 @@ $(OUT)/medcon_%.o: $(REF)/data/medcon/%.c | $(OUT)
 $(OUT)/libapirl_ref.so: $(OBJS)
 	$(CXX) -shared -fopenmp -o $@ $^ -lm
 
+# -fno-access-control: the driver swaps Image's private pixel pointer for a guarded
+# copy (see ref_driver.cpp); the reference's headers stay untouched.
 $(OUT)/ref_driver: oracle/ref_driver.cpp $(OUT)/libapirl_ref.so
-	$(CXX) $(OPT) -fopenmp $(INC) -w -o $@ $< -L$(OUT) -lapirl_ref -Wl,-rpath,'$$ORIGIN' -lm
+	$(CXX) $(OPT) -fopenmp -fno-access-control $(INC) -w -o $@ $< -L$(OUT) -lapirl_ref -Wl,-rpath,'$$ORIGIN' -lm
 
 clean:
 	rm -rf $(OUT)
@@ -65,6 +67,6 @@ HIPLIB := apirl_amd/libapirl_hip.so
 hip: $(OUT)/ref_hip_driver
 
 $(OUT)/ref_hip_driver: oracle/ref_driver.cpp integration/HipSiddonProjector.cpp integration/HipSiddonProjector.h $(OUT)/libapirl_ref.so $(HIPLIB)
-	$(CXX) $(OPT) -fopenmp -DWITH_HIP $(INC) -Iinclude -Iintegration -w -o $@ oracle/ref_driver.cpp integration/HipSiddonProjector.cpp -L$(OUT) -lapirl_ref -Lapirl_amd -lapirl_hip -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../apirl_amd'
+	$(CXX) $(OPT) -fopenmp -fno-access-control -DWITH_HIP $(INC) -Iinclude -Iintegration -w -o $@ oracle/ref_driver.cpp integration/HipSiddonProjector.cpp -L$(OUT) -lapirl_ref -Lapirl_amd -lapirl_hip -Wl,-rpath,'$$ORIGIN' -Wl,-rpath,'$$ORIGIN/../../apirl_amd'
 
 .PHONY: hip

@@ -16,7 +16,7 @@
 //   divback3d <geom3d> meas est out
 //   proj3d    <geom3d> img out
 //   back3d    <geom3d> sino out
-//   osem3d    <geom3d> subsets iters meas init outdir
+//   osem3d    <geom3d> subsets iters meas init outdir [add mult guard]
 //   mlem3d    <geom3d> iters meas init outdir
 //   rays3d    <geom3d> img start stride out             sampled reference rays
 //   osem3d_sub <geom3d> subsets subset meas             one timed OSEM sub-iteration
@@ -25,17 +25,19 @@
 //   par_read file method key / itf_read header key     one key lookup: "rc<TAB>value"
 //   log_row   n in out                                 Logger::writeRowOfNumbers
 //   mmr3d     header nx ny nz dx dz img out_proj out_back   mMR projection / backprojection
-//   mmr_mlem3d header nx ny nz dx dz init iters outdir       MlemSinogram3d on an mMR sinogram
+//   mmr_mlem3d header nx ny nz dx dz init iters outdir [add mult]  MlemSinogram3d, mMR
+//   mmr_osem3d header nx ny nz dx dz init iters subsets outdir [add mult]
+//                                                       OsemSinogram3d on an mMR sinogram
+//   mmr_subset_proj header nx ny nz dx dz img subset subsets out   projection of a subset
 //   <geom3d> = nx ny nz dx dz nproj nr rings rfov zfov rscanner span mrd
 // The reference's 2D projector checks only lower bounds and its 3D projector only
 // upper bounds (SiddonProjector.cpp:178,417), so an entry voxel index of -1 or n can
 // read outside the pixel array (undefined behaviour; it segfaults when the array is
 // page-aligned).  For the calls this driver makes directly, the pixel array is moved
-// into a buffer with zeroed guard zones, so such a read returns 0.  Access to the
-// private pointer is the only reason for the macro below.
-#define private public
+// into a buffer with zeroed guard zones, so such a read returns 0.  That swap needs
+// Image's private pixel pointer: oracle/ref.mk compiles this driver with
+// -fno-access-control, leaving the reference's headers as they are.
 #include <Images.h>
-#undef private
 #include <Sinogram2DinCylindrical3Dpet.h>
 #include <Sinogram3DCylindricalPet.h>
 #include <Sinogram3DSiemensMmr.h>
@@ -214,15 +216,28 @@ struct Mlem2dProbe : Mlem2d {
   float* lik() { return likelihoodValues; }
   Image* sens() { return sensitivityImage; }
 };
+// corrections(): the additive / multiplicative sinograms set as the reference's
+// setAdditiveProjection / setMultiplicativeProjection leave them
+// (MlemSinogram3d.cpp:17-42), but from sinograms built here: those loaders copy the
+// input sinogram, and the cylindrical copy constructor leaves the scanner radius
+// unset, which getSubset then passes on to the multiplicative subsets.
 struct Mlem3dProbe : MlemSinogram3d {
   using MlemSinogram3d::MlemSinogram3d;
   float* lik() { return likelihoodValues; }
   Image* sens() { return sensitivityImage; }
+  void corrections(Sinogram3D* add, Sinogram3D* mult) {
+    if (add) { additiveProjection = add; enableAdditiveTerm = true; }
+    if (mult) { multiplicativeProjection = mult; enableMultiplicativeTerm = true; }
+  }
 };
 struct OsemProbe : OsemSinogram3d {
   using OsemSinogram3d::OsemSinogram3d;
   float* lik() { return likelihoodValues; }
   Image* sens(int s) { return sensitivityImages[s]; }
+  void corrections(Sinogram3D* add, Sinogram3D* mult) {
+    if (add) { additiveProjection = add; enableAdditiveTerm = true; }
+    if (mult) { multiplicativeProjection = mult; enableMultiplicativeTerm = true; }
+  }
 };
 
 
@@ -285,7 +300,14 @@ Projector* make_projector() {
 #ifdef WITH_HIP
     int mode = (has_arg("bp") && strcmp(arg("bp"), "scatter") == 0) ? APIRL_BP_SCATTER
                                                                     : APIRL_BP_GATHER;
-    return (Projector*)new HipSiddonProjector(0, mode);
+    Projector* hp = (has_arg("ns") || has_arg("nas"))
+                        ? (Projector*)new HipSiddonProjector(0, mode,
+                                                             has_arg("ns") ? iarg("ns") : 1,
+                                                             has_arg("nas") ? iarg("nas") : 1)
+                        : (Projector*)new HipSiddonProjector(0, mode);
+    // multilor=1: Projector::setMultipleLorsPerBin(true) (Projector.h:60)
+    if (has_arg("multilor")) hp->setMultipleLorsPerBin(iarg("multilor") != 0);
+    return hp;
 #else
     fprintf(stderr, "ref_driver: built without the HIP projector\n");
     exit(7);
@@ -524,6 +546,16 @@ int cmd_divback3d() {
   return 0;
 }
 
+// Optional sinogram argument `key` (raw f32 in the layout of `like`): a sinogram
+// of the same class and geometry holding it, or NULL.
+Sinogram3D* optional_sino(const char* key, Sinogram3D* like) {
+  if (!has_arg(key)) return nullptr;
+  Sinogram3D* c = like->Copy();
+  std::vector<float> v = read_f32(arg(key), bins3d(like));
+  sino3d_from(c, v.data());
+  return c;
+}
+
 int cmd_osem3d() {
   Image init(size3d());
   size_t npix = (size_t)init.getPixelCount();
@@ -532,10 +564,27 @@ int cmd_osem3d() {
   Sinogram3DCylindricalPet* s = make_sino3d();
   std::vector<float> y = read_f32(arg("meas"), bins3d(s));
   sino3d_from(s, y.data());
-  Projector* pp = make_projector();
+  // guard=1: the reference's projector with its images in zeroed guard zones
+  Projector* pp = (has_arg("guard") && iarg("guard")) ? (Projector*)new GuardedSiddon()
+                                                      : make_projector();
   std::string dir = arg("outdir");
   int iters = iarg("iters"), subsets = iarg("subsets");
   OsemProbe osem(s, &init, dir, dir + "/osem3d", iters, 0, false, false, pp, pp, subsets);
+  // add= / mult=: additive and multiplicative sinograms (full layout), each held in a
+  // cylindrical sinogram built like the input (make_sino3d keeps the scanner radius)
+  Sinogram3D* add = nullptr;
+  Sinogram3D* mult = nullptr;
+  if (has_arg("add")) {
+    add = make_sino3d();
+    std::vector<float> v = read_f32(arg("add"), bins3d(add));
+    sino3d_from(add, v.data());
+  }
+  if (has_arg("mult")) {
+    mult = make_sino3d();
+    std::vector<float> v = read_f32(arg("mult"), bins3d(mult));
+    sino3d_from(mult, v.data());
+  }
+  osem.corrections(add, mult);
   double t0 = now_s();
   if (!osem.Reconstruct()) return 6;
   report_time("osem3d", t0);
@@ -821,10 +870,59 @@ int cmd_mmr_mlem3d() {
   std::string dir = arg("outdir");
   int iters = iarg("iters");
   Mlem3dProbe mlem(s, &init, dir, dir + "/mmr_mlem3d", iters, 0, false, false, pp, pp);
+  // add= / mult=: additive and multiplicative sinograms; the mMR Copy() keeps the
+  // scanner geometry (Sinogram3DSiemensMmr.cpp:74-87)
+  mlem.corrections(optional_sino("add", s), optional_sino("mult", s));
   if (!mlem.Reconstruct()) return 6;
   write_f32((dir + "/final.f32").c_str(), mlem.getReconstructedImage()->getPixelsPtr(), npix);
   write_f32((dir + "/sens.f32").c_str(), mlem.sens()->getPixelsPtr(), npix);
   write_f32((dir + "/lik.f32").c_str(), mlem.lik(), iters + 1);
+  return 0;
+}
+
+// mmr_osem3d: OsemSinogram3d on a Siemens mMR sinogram read from `header`, with
+// `subsets` subsets.  Its subsets are Sinogram3DSiemensMmr::getSubset copies
+// (Sinogram3DSiemensMmr.cpp:150-195), whose odd bins take half of the subset's own
+// angular step (Sinogram2DinSiemensMmr.cpp:180-181).  add= / mult= as mmr_mlem3d.
+int cmd_mmr_osem3d() {
+  void* mem = calloc(1, sizeof(Sinogram3DSiemensMmr));
+  Sinogram3DSiemensMmr* s = new (mem) Sinogram3DSiemensMmr((char*)arg("header"));
+  Image init(size3d());
+  size_t npix = (size_t)init.getPixelCount();
+  std::vector<float> x0 = read_f32(arg("init"), npix);
+  memcpy(init.getPixelsPtr(), x0.data(), npix * 4);
+  Projector* pp = has_arg("projector") ? make_projector() : (Projector*)new GuardedSiddon();
+  std::string dir = arg("outdir");
+  int iters = iarg("iters"), subsets = iarg("subsets");
+  OsemProbe osem(s, &init, dir, dir + "/mmr_osem3d", iters, 0, false, false, pp, pp, subsets);
+  osem.corrections(optional_sino("add", s), optional_sino("mult", s));
+  if (!osem.Reconstruct()) return 6;
+  write_f32((dir + "/final.f32").c_str(), osem.getReconstructedImage()->getPixelsPtr(), npix);
+  std::vector<float> sens(npix * subsets);
+  for (int k = 0; k < subsets; k++) memcpy(sens.data() + k * npix, osem.sens(k)->getPixelsPtr(), npix * 4);
+  write_f32((dir + "/sens.f32").c_str(), sens.data(), sens.size());
+  write_f32((dir + "/lik.f32").c_str(), osem.lik(), iters * subsets + 1);
+  return 0;
+}
+
+// mmr_subset_proj: Project of `img` onto subset `subset` of `subsets` of the mMR
+// sinogram read from `header` (Sinogram3DSiemensMmr::getSubset), as the reference's
+// OSEM projects it.
+int cmd_mmr_subset_proj() {
+  void* mem = calloc(1, sizeof(Sinogram3DSiemensMmr));
+  Sinogram3DSiemensMmr* s = new (mem) Sinogram3DSiemensMmr((char*)arg("header"));
+  Image img(size3d());
+  size_t npix = (size_t)img.getPixelCount();
+  std::vector<float> x = read_f32(arg("img"), npix);
+  memcpy(img.getPixelsPtr(), x.data(), npix * 4);
+  Sinogram3D* sub = s->getSubset(iarg("subset"), iarg("subsets"));
+  Projector* pp = make_projector();
+  {
+    GuardedPixels guarded(&img);
+    pp->Project(&img, sub);
+  }
+  std::vector<float> v = sino3d_to(sub);
+  write_f32(arg("out"), v.data(), v.size());
   return 0;
 }
 
@@ -855,6 +953,8 @@ int main(int argc, char** argv) {
     g_args[std::string(argv[i], eq - argv[i])] = eq + 1;
   }
   std::string c = argv[1];
+  if (c == "mmr_osem3d") return cmd_mmr_osem3d();
+  if (c == "mmr_subset_proj") return cmd_mmr_subset_proj();
   if (c == "siddon2d") return cmd_siddon2d();
   if (c == "siddon3d") return cmd_siddon3d();
   if (c == "proj2d") return cmd_proj2d();

@@ -4,7 +4,6 @@ reference's Siddon path.  Test infrastructure only -- imported by tests/, by
 the checker, never as the thing measured or shipped."""
 import ctypes
 import os
-import subprocess
 
 import numpy as np
 
@@ -33,7 +32,10 @@ def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB_PATH):
-            subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+            # built by __graft_entry__.build() (make -C oracle); never built here, where
+            # the process may already have initialised the GPU and must not fork a build
+            raise RuntimeError(f"{LIB_PATH} is missing: run `make -C oracle` (or "
+                               "__graft_entry__.build()) first")
         L = ctypes.CDLL(LIB_PATH)
         P = _P
         sig = {
@@ -65,6 +67,12 @@ def lib():
             "orc_mlem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, P, P, P]),
             "orc_mlem3d_pts": (_i, [P, _i, _i, P, _i, P, P, P, P, _i, P, P, P]),
             "orc_osem3d": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P, P, P]),
+            "orc_osem3d_corr": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P, P, _i, P,
+                                     P, P]),
+            "orc_osem3d_pts_corr": (_i, [P, _i, _i, P, P, _i, P, P, P, P, _i, _i, P, P, _i,
+                                         P, P, P]),
+            "orc_mlem3d_pts_corr": (_i, [P, _i, _i, P, _i, P, P, P, P, _i, P, P, P, P, P]),
+            "orc_mmr_subset_lor_table": (None, [_i, _i, P]),
             "orc_siddon3d_f64": (_i, [P, P, P, P, P, _i]),
             "orc_project3d_bins_f64": (None, [P, P, P, P, _f, P, P, _l, P, P, P, P]),
             "orc_osem3d_f64": (_i, [P, _i, _i, _f, _f, _i, P, P, P, P, _i, _i, P]),
@@ -128,6 +136,14 @@ def mmr_lor_table(nang, nr):
     """Siemens mMR LOR endpoints (x1, y1, x2, y2) per bin, [nang, nr, 4]."""
     out = np.zeros((nang, nr, 4), np.float32)
     lib().orc_mmr_lor_table(nang, nr, _p(out))
+    return out
+
+
+def mmr_subset_lor_table(nang, nr):
+    """LOR endpoints of the reference's mMR subset sinograms (cylindrical on 328 mm at
+    the mMR's grids), [nang, nr, 4] over the full angle grid."""
+    out = np.zeros((nang, nr, 4), np.float32)
+    lib().orc_mmr_subset_lor_table(nang, nr, _p(out))
     return out
 
 
@@ -377,3 +393,54 @@ def osem3d(im, nang, nr, rfov, rscanner, za, zb, meas, init, nsub, iters):
                      _p(lik))
     shp = (im.nz, im.ny, im.nx)
     return img.reshape(shp), sens.reshape((nsub,) + shp), lik
+
+
+def osem3d_corr(im, nang, nr, rfov, rscanner, za, zb, meas, init, nsub, iters, add=None,
+                mult=None, add_local=True):
+    """OsemSinogram3d with an additive (no clamp; subset bin k takes the additive's angle
+    k when add_local, as the reference does) and a multiplicative (sensitivity) term."""
+    npix = im.nx * im.ny * im.nz
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix * nsub, np.float32)
+    lik = np.zeros(iters * nsub + 1, np.float32)
+    lib().orc_osem3d_corr(ctypes.byref(im), nang, nr, rfov, rscanner, len(za), _p(f32(za)),
+                          _p(f32(zb)), _p(f32(meas)), _p(f32(init)), nsub, iters,
+                          _p(f32(add)) if add is not None else None,
+                          _p(f32(mult)) if mult is not None else None, 1 if add_local else 0,
+                          _p(img), _p(sens), _p(lik))
+    shp = (im.nz, im.ny, im.nx)
+    return img.reshape(shp), sens.reshape((nsub,) + shp), lik
+
+
+def osem3d_pts_corr(im, pts_sub, pts_full, za, zb, meas, init, nsub, iters, add=None,
+                    mult=None, add_local=True):
+    """osem3d_corr over LOR endpoints: subset bins on pts_sub, the final projection on
+    pts_full (both [nang, nr, 4])."""
+    nang, nr = pts_full.shape[:2]
+    npix = im.nx * im.ny * im.nz
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix * nsub, np.float32)
+    lik = np.zeros(iters * nsub + 1, np.float32)
+    lib().orc_osem3d_pts_corr(ctypes.byref(im), nang, nr, _p(f32(pts_sub)), _p(f32(pts_full)),
+                              len(za), _p(f32(za)), _p(f32(zb)), _p(f32(meas)), _p(f32(init)),
+                              nsub, iters, _p(f32(add)) if add is not None else None,
+                              _p(f32(mult)) if mult is not None else None,
+                              1 if add_local else 0, _p(img), _p(sens), _p(lik))
+    shp = (im.nz, im.ny, im.nx)
+    return img.reshape(shp), sens.reshape((nsub,) + shp), lik
+
+
+def mlem3d_pts_corr(im, pts, za, zb, meas, init, iters, add=None, mult=None):
+    """MlemSinogram3d over LOR endpoints with the additive and multiplicative terms."""
+    nang, nr = pts.shape[:2]
+    npix = im.nx * im.ny * im.nz
+    img = np.zeros(npix, np.float32)
+    sens = np.zeros(npix, np.float32)
+    lik = np.zeros(iters + 1, np.float32)
+    lib().orc_mlem3d_pts_corr(ctypes.byref(im), nang, nr, _p(f32(pts)), len(za), _p(f32(za)),
+                              _p(f32(zb)), _p(f32(meas)), _p(f32(init)), iters,
+                              _p(f32(add)) if add is not None else None,
+                              _p(f32(mult)) if mult is not None else None,
+                              _p(img), _p(sens), _p(lik))
+    shp = (im.nz, im.ny, im.nx)
+    return img.reshape(shp), sens.reshape(shp), lik

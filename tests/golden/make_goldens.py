@@ -266,6 +266,103 @@ def mmr_mlem3d():
          lik=np.fromfile(os.path.join(out, "lik.f32"), np.float32))
 
 
+def corrections():
+    """The reference's OsemSinogram3d with an additive and a multiplicative sinogram
+    (phantoms.correction_sinograms, angle-varying) on the mid-size geometry: run
+    "pos" with both terms, run "neg" with an additive term below minus the projection
+    on some bins (no clamp).  Then the mMR geometry: the reference's OsemSinogram3d
+    with 3 subsets (subset LORs of Sinogram3DSiemensMmr::getSubset), its
+    MlemSinogram3d with both terms, and the projection of subset 1 of 3."""
+    from apirl_amd import hip, interfile
+    g = MID_OSEM
+    f = np.load(os.path.join(HERE, "mid3d_project_backproject.npz"))
+    meas = f["meas"].astype(np.float32)
+    meas.tofile(tmpf("c_meas.f32"))
+    nsino, nang, nr = meas.shape if meas.ndim == 3 else (meas.size // (g["nproj"] * g["nr"]),
+                                                          g["nproj"], g["nr"])
+    init = np.ones((g["nz"], g["ny"], g["nx"]), np.float32)
+    init.tofile(tmpf("c_init.f32"))
+    out = {}
+    for label, neg, use_mult in (("pos", False, True), ("neg", True, False)):
+        add, mult = phantoms.correction_sinograms(nsino, nang, nr, negative=neg)
+        add.tofile(tmpf(f"c_add_{label}.f32"))
+        mult.tofile(tmpf(f"c_mult_{label}.f32"))
+        d = tmpf(f"c_out_{label}")
+        os.makedirs(d, exist_ok=True)
+        kw = dict(add=tmpf(f"c_add_{label}.f32"))
+        if use_mult:
+            kw["mult"] = tmpf(f"c_mult_{label}.f32")
+        run("osem3d", **geom3d_args(g), subsets=7, iters=2, meas=tmpf("c_meas.f32"),
+            init=tmpf("c_init.f32"), outdir=d, guard=1, **kw)
+        shp = (g["nz"], g["ny"], g["nx"])
+        sens = np.fromfile(os.path.join(d, "sens.f32"), np.float32).reshape((7,) + shp)
+        out[f"{label}_final"] = np.fromfile(os.path.join(d, "final.f32"), np.float32).reshape(shp)
+        if use_mult:  # without it the sensitivities are mid3d_osem.npz's
+            out[f"{label}_sens0"] = sens[0]
+            out[f"{label}_sens3"] = sens[3]
+        out[f"{label}_lik"] = np.fromfile(os.path.join(d, "lik.f32"), np.float32)
+    # mMR OSEM with 3 subsets, on an image inside the radial FOV (voxels every LOR
+    # family of a subset covers, so the reference's 0/0 update stays in the top
+    # slice, which no LOR visits); counts from mmr_osem_counts()
+    import hashlib
+    go = MMR_OSEM
+    ns_, mn, mx = hip.segment_table(go["rings"], go["span"], go["mrd"])
+    ometa = mmr_osem_counts(int(ns_.sum()))
+    ohdr = interfile.write_sinogram3d(tmpf("c_mmro_sino"), ometa, ns_, mn, mx, go["rings"],
+                                      4.09, 180.0 / go["nproj"])
+    oshp = (go["nz"], go["ny"], go["nx"])
+    np.ones(oshp, np.float32).tofile(tmpf("c_mmro_init.f32"))
+    d = tmpf("c_mmr_osem")
+    os.makedirs(d, exist_ok=True)
+    run("mmr_osem3d", header=ohdr, nx=go["nx"], ny=go["ny"], nz=go["nz"], dx=go["dx"],
+        dz=go["dz"], init=tmpf("c_mmro_init.f32"), iters=go["iters"], subsets=go["subsets"],
+        outdir=d)
+    out["mmr_osem_final"] = np.fromfile(os.path.join(d, "final.f32"), np.float32).reshape(oshp)
+    out["mmr_osem_sens0"] = np.fromfile(os.path.join(d, "sens.f32"),
+                                        np.float32).reshape((go["subsets"],) + oshp)[0]
+    out["mmr_osem_lik"] = np.fromfile(os.path.join(d, "lik.f32"), np.float32)
+    out["mmr_osem_meas_sha256"] = np.array(hashlib.sha256(ometa.tobytes()).hexdigest())
+    # mMR: MLEM3D with corrections, one subset's projection
+    gm = MMR
+    mm = np.load(os.path.join(HERE, "mmr.npz"))
+    mmeas = mm["meas"].astype(np.float32)
+    ns_, mn, mx = hip.segment_table(gm["rings"], gm["span"], gm["mrd"])
+    hdr = interfile.write_sinogram3d(tmpf("c_mmr_sino"), mmeas, ns_, mn, mx, gm["rings"], 8.18,
+                                     180.0 / gm["nproj"])
+    shp = (gm["nz"], gm["ny"], gm["nx"])
+    np.ones(shp, np.float32).tofile(tmpf("c_mmr_init.f32"))
+    madd, mmult = phantoms.correction_sinograms(int(ns_.sum()), gm["nproj"], gm["nr"])
+    madd.tofile(tmpf("c_mmr_add.f32"))
+    mmult.tofile(tmpf("c_mmr_mult.f32"))
+    d = tmpf("c_mmr_mlem")
+    os.makedirs(d, exist_ok=True)
+    run("mmr_mlem3d", header=hdr, nx=gm["nx"], ny=gm["ny"], nz=gm["nz"], dx=gm["dx"],
+        dz=gm["dz"], init=tmpf("c_mmr_init.f32"), iters=2, outdir=d,
+        add=tmpf("c_mmr_add.f32"), mult=tmpf("c_mmr_mult.f32"))
+    out["mmr_mlem_final"] = np.fromfile(os.path.join(d, "final.f32"), np.float32).reshape(shp)
+    out["mmr_mlem_sens"] = np.fromfile(os.path.join(d, "sens.f32"), np.float32).reshape(shp)
+    out["mmr_mlem_lik"] = np.fromfile(os.path.join(d, "lik.f32"), np.float32)
+    mm["img"].tofile(tmpf("c_mmr_img.f32"))
+    run("mmr_subset_proj", header=hdr, nx=gm["nx"], ny=gm["ny"], nz=gm["nz"], dx=gm["dx"],
+        dz=gm["dz"], img=tmpf("c_mmr_img.f32"), subset=1, subsets=3, out=tmpf("c_mmr_sp.f32"))
+    out["mmr_subset1of3_proj"] = read("c_mmr_sp.f32")
+    save("corrections.npz", geom=np.array(json.dumps(dict(g, subsets=7, iters=2))),
+         geom_mmr=np.array(json.dumps(dict(gm, mlem_iters=2))),
+         geom_mmr_osem=np.array(json.dumps(go)), **out)
+
+
+# mMR OSEM case: a 192 mm image inside the 297 mm radial FOV, 172 radial bins
+MMR_OSEM = dict(nx=32, ny=32, nz=65, dx=6.0, dz=4.0, nproj=24, nr=172, rings=64, zfov=260.0,
+                span=3, mrd=2, subsets=3, iters=2, seed=7)
+
+
+def mmr_osem_counts(nsino):
+    """Seeded Poisson(2) counts of the mMR OSEM case (numpy default_rng(seed))."""
+    g = MMR_OSEM
+    rng = np.random.default_rng(g["seed"])
+    return rng.poisson(2.0, (nsino, g["nproj"], g["nr"])).astype(np.float32)
+
+
 C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=64,
           rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
 
@@ -284,11 +381,52 @@ def c2_rays():
          proj=rec[:, 3])
 
 
+def c2_osem():
+    """The reference's OsemSinogram3d on the full 256x256x81 / 344x252x837 workload:
+    one iteration of 21 subsets from ones.  The measured sinogram is
+    phantoms.poisson_counts(reference projection of the cylinder-and-points phantom,
+    3e8, default_rng(0)); the fixture keeps its sha256 and the projection's, so the GPU
+    test can rebuild the same input from its own (bit-exact) projection.  The image
+    is kept as per-slice sums and a seeded sample of 131072 voxels; run with one
+    thread, so the reference's float atomics add in a fixed order."""
+    import hashlib
+    g = C2
+    img, _ = phantoms.cylinder_points3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    img.tofile(tmpf("c2o_img.f32"))
+    # every bin is written by one thread: the projection does not depend on threads
+    run("proj3d", threads=8, **geom3d_args(g), img=tmpf("c2o_img.f32"), out=tmpf("c2o_proj.f32"))
+    proj = read("c2o_proj.f32")
+    meas = phantoms.poisson_counts(proj, 3e8, np.random.default_rng(0))
+    meas.tofile(tmpf("c2o_meas.f32"))
+    init = np.ones((g["nz"], g["ny"], g["nx"]), np.float32)
+    init.tofile(tmpf("c2o_init.f32"))
+    d = tmpf("c2o_out")
+    os.makedirs(d, exist_ok=True)
+    run("osem3d", threads=1, **geom3d_args(g), subsets=21, iters=1, meas=tmpf("c2o_meas.f32"),
+        init=tmpf("c2o_init.f32"), outdir=d, guard=1)
+    shp = (g["nz"], g["ny"], g["nx"])
+    final = np.fromfile(os.path.join(d, "final.f32"), np.float32).reshape(shp)
+    sens = np.fromfile(os.path.join(d, "sens.f32"), np.float32).reshape((21,) + shp)
+    rng = np.random.default_rng(123)
+    idx = np.sort(rng.choice(final.size, 131072, replace=False)).astype(np.int64)
+    save("c2_osem.npz", geom=np.array(json.dumps(dict(g, subsets=21, iters=1, counts=3e8,
+                                                      seed=0))),
+         proj_sha256=np.array(hashlib.sha256(proj.tobytes()).hexdigest()),
+         meas_sha256=np.array(hashlib.sha256(meas.tobytes()).hexdigest()),
+         meas_total=np.array(float(meas.sum(dtype=np.float64))),
+         sample_idx=idx, final_sample=final.ravel()[idx],
+         final_slice_sums=np.nansum(final, axis=(1, 2), dtype=np.float64),
+         final_nan=np.array(int(np.isnan(final).sum())),
+         sens_sums=sens.sum(axis=(1, 2, 3), dtype=np.float64),
+         sens0_sample=sens[0].ravel()[idx],
+         lik=np.fromfile(os.path.join(d, "lik.f32"), np.float32))
+
+
 if __name__ == "__main__":
     if not os.path.exists(DRIVER):
         subprocess.check_call(["make", "-f", "oracle/ref.mk", "-j8"], cwd=ROOT)
     which = sys.argv[1:] or ["lists", "c1", "mid3d", "c2", "divide", "multislice", "oversampled",
-                             "mmr", "mmr_mlem3d"]
+                             "mmr", "mmr_mlem3d", "corrections"]
     if "lists" in which:
         siddon_lists()
     if "c1" in which:
@@ -308,3 +446,7 @@ if __name__ == "__main__":
         mmr()
     if "mmr_mlem3d" in which:
         mmr_mlem3d()
+    if "corrections" in which:
+        corrections()
+    if "c2_osem" in which:  # minutes of reference CPU time: only when named
+        c2_osem()

@@ -63,7 +63,6 @@ def test_duplicate_key(files):
 @pytest.mark.parametrize("extra,match", [
     ("siddon number of samples on the detector := 0\n", "at least one sample"),
     ("siddon number of axial samples on the detector := x\n", "at least one sample"),
-    ("sensitivity filename := sens.h33\n", "not available on this path"),
     ("attenuation image filename := mu.h33\n", "not available on this path"),
 ])
 def test_unsupported_options_refused(files, extra, match):
@@ -110,3 +109,50 @@ def test_log_layout(tmp_path):
     assert lines[16:18] == ["Tiempos de UpdatePixel por Iteración [mseg]", "6, 7"]
     assert lines[18:20] == ["Likelihood por Iteración:", "1.5, 2.5, 3.5"]
     assert text.endswith("\r\n") and "\n" not in text.replace("\r\n", "")
+
+
+def test_gpu_block_size_keys(tmp_path):
+    """ProcessBlockSizeString (readCudaParameters.cpp:23-49): the text between the
+    last '{' and '}' split at blanks and commas; missing braces or elements refused."""
+    path = tmp_path / "m.par"
+    path.write_text("MLEM Parameters :=\nprojector block size := {128, 2,1}\n"
+                    "backprojector block size := 64,1,1\npixel update block size := {8,1}\n")
+    par = interfile.ParFile(str(path), "MLEM")
+    assert cli._block_size(par, "projector block size") == (128, 2, 1)
+    with pytest.raises(interfile.FormatError, match="no {}"):
+        cli._block_size(par, "backprojector block size")
+    with pytest.raises(interfile.FormatError, match="x, y and z"):
+        cli._block_size(par, "pixel update block size")
+    with pytest.raises(interfile.FormatError, match="not found"):
+        cli._block_size(par, "gpu id")
+
+
+def test_subset_keys_read_together(tmp_path):
+    """"number of subsets" and "subset index" count only together
+    (project.cpp:226-237); an index outside the subsets is refused."""
+    def par(text):
+        path = tmp_path / "p.par"
+        path.write_text("Projection Parameters :=\n" + text)
+        return interfile.ParFile(str(path), "Projection")
+    assert cli._subset_keys(par("number of subsets := 7\nsubset index := 2\n")) == (7, 2)
+    assert cli._subset_keys(par("number of subsets := 7\n")) == (0, 0)
+    with pytest.raises(interfile.FormatError, match="subset index"):
+        cli._subset_keys(par("number of subsets := 7\nsubset index := 7\n"))
+
+
+@pytest.mark.parametrize("cmd,method,keys,match", [
+    ("project", "Projection", {"input file": "i.h33", "output type": "Sinogram3D",
+                               "projector": "ConeOfResponse", "output filename": "o",
+                               "output projection": "s.h33"}, "ConeOfResponse"),
+    ("backproject", "Backproject", {"input type": "Sinogram3D", "input file": "s.h33",
+                                    "backprojector": "Siddon", "output image": "i.h33"},
+     "output filename"),
+    ("cuMLEM", "MLEM", {"input type": "Sinogram3D", "projector block size": "{1,1,1}",
+                        "backprojector block size": "{1,1,1}"}, "pixel update block size"),
+])
+def test_projection_commands_refuse_before_the_device(tmp_path, capsys, cmd, method, keys, match):
+    path = tmp_path / "c.par"
+    path.write_text(f"{method} Parameters :=\n" +
+                    "".join(f"{k} := {v}\n" for k, v in keys.items()))
+    assert cli.main([cmd, str(path)]) == 1
+    assert match in capsys.readouterr().err

@@ -187,3 +187,58 @@ def test_batch_with_mmr_job_equals_alone(torch, ctx):
     res = run_batch(torch, ctx, plans, jobs)
     for k in range(len(jobs)):
         same_bits(res[k], ref[k])
+
+
+def c3_specs():
+    """One job of every geometry of the batched-jobs workload (2D 128x128 / 192x192,
+    2D 256x256 / 344x252, 3D 128x128x47 / 192x192x233, 3D 256x256x81 /
+    344x252x837), seeded, with uneven subsets and an attenuation sinogram on half."""
+    from apirl_amd import workloads
+    kinds = list(workloads.GEOMETRIES)
+    subsets = {"2d_128": 5, "2d_256": 21, "3d_small": 7, "3d_c2": 21}
+    iters = {"2d_128": 3, "2d_256": 2, "3d_small": 2, "3d_c2": 1}
+    return [dict(seed=100 + k, kind=kind, subsets=subsets[kind], iterations=iters[kind],
+                 counts=float(10 ** (5 + k)), attenuation=bool(k % 2))
+            for k, kind in enumerate(kinds)]
+
+
+def alone_spec(plans, spec, d):
+    img, _ = plans[spec["kind"]].reconstruct(d["meas"], d["init"], spec["iterations"],
+                                             subsets=spec["subsets"], bp_mode=hip.BP_GATHER,
+                                             mult=d["mult"])
+    return host(img)
+
+
+def test_c3_every_job_type_batched_equals_alone(torch, ctx):
+    """The batched-jobs workload's four job types in one batch (the 256x256x81 job on
+    the single-job kernels after each grouped round, the others in the grouped
+    launches): each image equals, bit for bit, the job run alone."""
+    from apirl_amd import serving
+    log("case: one job of each C3 geometry in one batch")
+    specs = c3_specs()
+    plans, jobs = serving.materialise_all(ctx, specs, torch)
+    ref = [alone_spec(plans, s, d) for s, d in jobs]
+    torch.cuda.synchronize()
+    dt, choice = serving.run_batched(ctx, plans, jobs, torch, max_jobs=8, tune=True)
+    log(f"  batched in {dt:.3f} s, autotuned {choice}")
+    for k, (spec, d) in enumerate(jobs):
+        log(f"  {spec['kind']}: S={spec['subsets']} T={spec['iterations']}")
+        same_bits(host(d["out"]), ref[k])
+
+
+@pytest.mark.parametrize("rate", [5.0, 200.0])
+def test_c4_serving_job_types_at_two_rates(torch, ctx, rate):
+    """Serving (open-loop arrivals, admission by HBM footprint) over the four job
+    types at a low and a high arrival rate: every image equals the job alone."""
+    from apirl_amd import serving
+    log(f"case: serving the C3 job types at {rate} jobs/s")
+    specs = c3_specs()
+    plans, jobs = serving.materialise_all(ctx, specs, torch)
+    ref = [alone_spec(plans, s, d) for s, d in jobs]
+    torch.cuda.synchronize()
+    lat, wall = serving.serve(ctx, plans, jobs, rate=rate, torch=torch, max_jobs=8)
+    log(f"  p50 {np.percentile(lat, 50):.3f} s, p99 {np.percentile(lat, 99):.3f} s, "
+        f"{len(jobs) / wall:.2f} jobs/s")
+    assert np.all(np.isfinite(lat)) and np.all(lat > 0)
+    for k, (_, d) in enumerate(jobs):
+        same_bits(host(d["out"]), ref[k])

@@ -240,3 +240,142 @@ def test_cli_mmr_mlem_equals_plan(torch, ctx, golden, tmp_path):
     got, ref = read(out + "_final"), host(img)
     assert np.isfinite(got).all() and np.isfinite(ref).all() and ref.max() > 0
     assert rel_l2(got, ref) <= IMG_RL2
+
+
+def test_cli_osem_sensitivity_filename_reload(torch, ctx, golden, tmp_path):
+    """"sensitivity filename" (OsemSinogram3d.cpp:83-95): a second OSEM run that reads
+    the first run's <prefix>_sensitivity_subset_<s>.h33 gives the same images bit for
+    bit and writes no sensitivity of its own."""
+    log("case: CLI OSEM, sensitivity images reloaded from the first run")
+    f = golden("mid3d_osem.npz")
+    g = json.loads(str(f["geom"]))
+    meas, keys = _sino3d_inputs(tmp_path, g, f["meas"])
+    common = dict(keys, **{"number of iterations": g["iters"], "number of subsets": g["subsets"]})
+    a, b = str(tmp_path / "first"), str(tmp_path / "second")
+    assert cli.main(["OSEM", write_par(tmp_path / "a.par", "OSEM", **common,
+                                       **{"output filename prefix": a})]) == 0
+    assert cli.main(["OSEM", write_par(tmp_path / "b.par", "OSEM", **common,
+                                       **{"output filename prefix": b,
+                                          "sensitivity filename": a + "_sensitivity"})]) == 0
+    assert_bitexact(read(b + "_final"), read(a + "_final"), nan_ok=True)
+    assert not os.path.exists(b + "_sensitivity_subset_0.h33")
+
+
+def test_cli_project_backproject_vs_reference(torch, ctx, golden, tmp_path):
+    """project / backproject (cmd/src/project.cpp, backproject.cpp) on a Sinogram3D:
+    the whole projection bit-exact and the backprojection within 1e-5 per voxel of
+    the reference's SiddonProjector (mid3d goldens); subset 2 of 7 bit-exact against
+    the oracle's projection of the subset's angles."""
+    import _oracle as O
+    log("case: CLI project / backproject, Sinogram3D, mid3d geometry")
+    f = golden("mid3d_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    meas, keys = _sino3d_inputs(tmp_path, g, f["meas"])
+    img = interfile.write_image(str(tmp_path / "img"), f["img"].reshape(g["nz"], g["ny"], g["nx"]),
+                                g["dx"], g["dx"], g["dz"])
+    cyl = {k: keys[k] for k in ("cylindrical pet radius (in mm)", "radius fov (in mm)",
+                                "axial fov (in mm)")}
+    out = str(tmp_path / "proj")
+    assert cli.main(["project", write_par(tmp_path / "p.par", "Projection", **cyl, **{
+        "input file": img, "output type": "Sinogram3D", "projector": "Siddon",
+        "output filename": out, "output projection": keys["input file"]})]) == 0
+    got = interfile.read_sinogram3d(out + ".h33")["bins"]
+    assert_bitexact(got.ravel(), f["proj"])
+    out2 = str(tmp_path / "proj_s2")
+    assert cli.main(["project", write_par(tmp_path / "p2.par", "Projection", **cyl, **{
+        "input file": img, "output type": "Sinogram3D", "projector": "CuSiddonProjector",
+        "output filename": out2, "output projection": keys["input file"],
+        "number of subsets": 7, "subset index": 2})]) == 0
+    sub = interfile.read_sinogram3d(out2 + ".h33")["bins"]
+    im = O.image_desc(g["nx"], g["ny"], g["nz"], g["dx"], g["dz"])
+    ang, r = O.sino_grid(g["nproj"], g["nr"], g["rfov"])
+    za, zb = O.ring_table(g["rings"], g["zfov"], g["span"], g["mrd"])
+    idx = O.subset_angles(g["nproj"], 7, 2)
+    ref = O.project3d(im, f["img"], np.ascontiguousarray(ang[idx]), r, g["rscanner"], za, zb)
+    assert_bitexact(sub.ravel(), ref.ravel())
+    bimg = interfile.write_image(str(tmp_path / "zero"), np.zeros((g["nz"], g["ny"], g["nx"]),
+                                 np.float32), g["dx"], g["dx"], g["dz"])
+    bout = str(tmp_path / "back")
+    assert cli.main(["backproject", write_par(tmp_path / "b.par", "Backproject", **cyl, **{
+        "input type": "Sinogram3D", "input file": keys["input file"],
+        "backprojector": "Siddon", "output image": bimg, "output filename": bout})]) == 0
+    assert_voxel_close(read(bout), f["back"])
+
+
+def test_cli_mmr_subset_projection_and_cumlem(torch, ctx, golden, tmp_path):
+    """project onto subset 1 of 3 of an mMR sinogram: bit-exact against the
+    reference's projection of that getSubset copy.  cuMLEM (cmdGPU/src/cuMLEM.cpp) with
+    its GPU keys on Sinogram3DSiemensMmr input: MLEM without "number of subsets"
+    (the reference's MlemSinogram3d golden), OSEM with it (its OsemSinogram3d golden),
+    both within the image bar."""
+    log("case: CLI mMR subset projection, cuMLEM MLEM and OSEM")
+    c = golden("corrections.npz")
+    gm = json.loads(str(c["geom_mmr"]))
+    mm = golden("mmr.npz")
+    nsino, mn, mx = hip.segment_table(gm["rings"], gm["span"], gm["mrd"])
+    meas = mm["meas"].astype(np.float32).reshape(int(nsino.sum()), gm["nproj"], gm["nr"])
+    sino = interfile.write_sinogram3d(str(tmp_path / "mmr"), meas, nsino, mn, mx, gm["rings"],
+                                      8.18, 180.0 / gm["nproj"])
+    shp = (gm["nz"], gm["ny"], gm["nx"])
+    img = interfile.write_image(str(tmp_path / "img"), mm["img"].reshape(shp), gm["dx"], gm["dx"],
+                                gm["dz"])
+    out = str(tmp_path / "sub")
+    assert cli.main(["project", write_par(tmp_path / "p.par", "Projection", **{
+        "input file": img, "output type": "Sinogram3DSiemensMmr", "projector": "Siddon",
+        "output filename": out, "output projection": sino, "number of subsets": 3,
+        "subset index": 1})]) == 0
+    got = interfile.read_sinogram3d(out + ".h33")["bins"]
+    assert_bitexact(got.ravel(), c["mmr_subset1of3_proj"])
+    init = interfile.write_image(str(tmp_path / "init"), np.ones(shp, np.float32), gm["dx"],
+                                 gm["dx"], gm["dz"])
+    gpu = {"projector block size": "{128,1,1}", "backprojector block size": "{128, 1, 1}",
+           "pixel update block size": "{512,1,1}", "gpu id": 0,
+           "forwardprojector": "CuSiddonProjector", "backprojector": "CuSiddonProjector",
+           "input type": "Sinogram3DSiemensMmr", "input file": sino, "initial estimate": init}
+    m = golden("mmr_mlem3d.npz")
+    out = str(tmp_path / "cumlem")
+    assert cli.main(["cuMLEM", write_par(tmp_path / "m.par", "MLEM", **gpu, **{
+        "output filename prefix": out, "number of iterations": 3})]) == 0
+    assert rel_l2(read(out + "_final"), m["final"]) <= IMG_RL2
+    # OSEM: the mMR OSEM case (192 mm image inside the radial FOV, seeded counts)
+    go = json.loads(str(c["geom_mmr_osem"]))
+    ns2, mn2, mx2 = hip.segment_table(go["rings"], go["span"], go["mrd"])
+    counts = np.random.default_rng(go["seed"]).poisson(
+        2.0, (int(ns2.sum()), go["nproj"], go["nr"])).astype(np.float32)
+    osino = interfile.write_sinogram3d(str(tmp_path / "mmro"), counts, ns2, mn2, mx2, go["rings"],
+                                       4.09, 180.0 / go["nproj"])
+    oshp = (go["nz"], go["ny"], go["nx"])
+    oinit = interfile.write_image(str(tmp_path / "oinit"), np.ones(oshp, np.float32), go["dx"],
+                                  go["dx"], go["dz"])
+    out = str(tmp_path / "cuosem")
+    assert cli.main(["cuMLEM", write_par(tmp_path / "o.par", "MLEM", **dict(
+        gpu, **{"input file": osino, "initial estimate": oinit}), **{
+        "output filename prefix": out, "number of iterations": go["iters"],
+        "number of subsets": go["subsets"]})]) == 0
+    final, ref = read(out + "_final"), c["mmr_osem_final"]
+    assert np.array_equal(np.isnan(final), np.isnan(ref))
+    assert rel_l2(final, ref) <= IMG_RL2
+    # a block size without its braces is refused, as the reference refuses it
+    bad = dict(gpu, **{"projector block size": "128,1,1"})
+    assert cli.main(["cuMLEM", write_par(tmp_path / "bad.par", "MLEM", **bad, **{
+        "output filename prefix": out, "number of iterations": 1})]) == 1
+
+
+def test_cli_project_2d_subset_vs_reference(torch, ctx, golden, tmp_path):
+    """project / backproject on Sinogram2D (the 2D LOR form with the r <= rfov
+    rule): the whole projection bit-exact against the reference's (c1 golden) and
+    the gather backprojection bit-exact as well."""
+    log("case: CLI project / backproject, Sinogram2D, c1 geometry")
+    f = golden("c1_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    img = interfile.write_image(str(tmp_path / "img"),
+                                f["img"].reshape(1, g["nx"], g["nx"]), g["dx"], g["dx"], 1.0)
+    nproj, nr = g["nproj"], g["nr"]
+    sample = interfile.write_sinogram2d(str(tmp_path / "s"), np.zeros((nproj, nr), np.float32))
+    cyl = {"radius fov (in mm)": g["rfov"], "cylindrical pet radius (in mm)": g["rscanner"],
+           "axial fov (in mm)": 0}
+    out = str(tmp_path / "p")
+    assert cli.main(["project", write_par(tmp_path / "p.par", "Projection", **cyl, **{
+        "input file": img, "output type": "Sinogram2D", "projector": "Siddon",
+        "output filename": out, "output projection": sample})]) == 0
+    assert_bitexact(interfile.read_sinogram2d(out + ".h33").ravel(), f["proj"])

@@ -152,3 +152,101 @@ def test_reference_mlem3d_on_mmr_with_hip_gather(golden):
     e = np.linalg.norm(final - ref) / np.linalg.norm(ref)
     print(f"mMR MLEM3D via reference driver + HIP projector: rel L2 {e:.2e}", flush=True)
     assert e <= 1e-4
+
+
+def run_refused(cmd, **kw):
+    """Runs the driver expecting HipSiddonProjector to refuse the call: the reference's
+    drivers ignore the projector's return value, so the refusal shows as its message."""
+    from conftest import LAUNCHER
+    args = [DRV, cmd] + [f"{k}={v}" for k, v in kw.items()]
+    print("case:", " ".join(args[1:3]), flush=True)
+    rc, out, err = LAUNCHER.run(args, env=dict(os.environ, OMP_NUM_THREADS="1"), timeout=300)
+    return rc, err
+
+
+@needs_driver
+def test_reference_mmr_osem_with_hip_gather(golden):
+    """OsemSinogram3d of the reference on the mMR sinogram with 3 subsets, with
+    HipSiddonProjector: the subset sinograms the reference builds are cylindrical ones
+    at the mMR's grids (Sinogram3DSiemensMmr::getSubset), which the binding takes
+    from their class, so the image meets the bar against the reference's own run."""
+    sys.path.insert(0, ROOT)
+    from apirl_amd import hip, interfile
+    c = golden("corrections.npz")
+    g = json.loads(str(c["geom_mmr_osem"]))
+    nsino, mn, mx = hip.segment_table(g["rings"], g["span"], g["mrd"])
+    counts = np.random.default_rng(g["seed"]).poisson(
+        2.0, (int(nsino.sum()), g["nproj"], g["nr"])).astype(np.float32)
+    with tempfile.TemporaryDirectory() as td:
+        hdr = interfile.write_sinogram3d(os.path.join(td, "mmro"), counts, nsino, mn, mx,
+                                         g["rings"], 4.09, 180.0 / g["nproj"])
+        np.ones((g["nz"], g["ny"], g["nx"]), np.float32).tofile(os.path.join(td, "init.f32"))
+        run("mmr_osem3d", projector="hip", bp="gather", header=hdr, nx=g["nx"], ny=g["ny"],
+            nz=g["nz"], dx=g["dx"], dz=g["dz"], init=os.path.join(td, "init.f32"),
+            iters=g["iters"], subsets=g["subsets"], outdir=td)
+        final = np.fromfile(os.path.join(td, "final.f32"), np.float32)
+    ref = c["mmr_osem_final"].ravel()
+    assert np.array_equal(np.isnan(final), np.isnan(ref))
+    m = np.isfinite(ref)
+    e = np.linalg.norm(final[m].astype(np.float64) - ref[m]) / np.linalg.norm(ref[m].astype(np.float64))
+    print(f"mMR OSEM via reference driver + HIP projector: rel L2 {e:.2e}", flush=True)
+    assert e <= 1e-4
+
+
+@needs_driver
+def test_reference_divide_backproject_with_hip_gather(golden):
+    """Projector::DivideAndBackproject through the binding in gather mode: within
+    1e-5 per voxel of the reference's, bit-identical from run to run."""
+    f = golden("divide_backproject.npz")
+    g = json.loads(str(f["geom3d"]))
+    outs = []
+    with tempfile.TemporaryDirectory() as td:
+        f["meas3d"].astype(np.float32).tofile(os.path.join(td, "m.f32"))
+        (f["est3d_x2"].astype(np.float32) * np.float32(0.5)).tofile(os.path.join(td, "e.f32"))
+        for rep in range(2):
+            out = os.path.join(td, f"d{rep}.f32")
+            run("divback3d", projector="hip", bp="gather", **{k: g[k] for k in GEOM3},
+                meas=os.path.join(td, "m.f32"), est=os.path.join(td, "e.f32"), out=out)
+            outs.append(np.fromfile(out, np.float32))
+    assert voxel_rel(outs[0], f["img3d"]) <= 1e-5
+    assert np.array_equal(bits(outs[0]), bits(outs[1]))
+
+
+@needs_driver
+def test_reference_oversampled_with_hip_projector(golden):
+    """HipSiddonProjector(gpu, scatter, 3, 2), the counterpart of
+    SiddonProjector(3, 2): projection bit-exact, backprojection within 1e-5 per voxel
+    of the reference's; a gather-mode projector refuses oversampled backprojection."""
+    f = golden("oversampled.npz")
+    g = json.loads(str(f["geom"]))
+    m = golden("mid3d_project_backproject.npz")
+    geo = {k: g[k] for k in GEOM3}
+    with tempfile.TemporaryDirectory() as td:
+        m["img"].astype(np.float32).tofile(os.path.join(td, "img.f32"))
+        m["meas"].astype(np.float32).tofile(os.path.join(td, "meas.f32"))
+        run("proj3d", projector="hip", bp="scatter", ns=g["ns"], nas=g["nas"], **geo,
+            img=os.path.join(td, "img.f32"), out=os.path.join(td, "p.f32"))
+        run("back3d", projector="hip", bp="scatter", ns=g["ns"], nas=g["nas"], **geo,
+            sino=os.path.join(td, "meas.f32"), out=os.path.join(td, "b.f32"))
+        proj = np.fromfile(os.path.join(td, "p.f32"), np.float32)
+        back = np.fromfile(os.path.join(td, "b.f32"), np.float32)
+        rc, err = run_refused("back3d", projector="hip", bp="gather", ns=g["ns"], nas=g["nas"],
+                              **geo, sino=os.path.join(td, "meas.f32"),
+                              out=os.path.join(td, "b2.f32"))
+    assert np.array_equal(bits(proj), bits(f["proj"]))
+    assert voxel_rel(back, f["back"]) <= 1e-5
+    assert "one LOR per bin" in err
+
+
+@needs_driver
+def test_reference_multiple_lors_per_bin_refused(golden):
+    """setMultipleLorsPerBin(true) (Projector.h:60): the binding traces each bin on its
+    sinogram's mean ring pair only, so it returns false and says why instead of
+    returning mean-pair results."""
+    f = golden("mid3d_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    with tempfile.TemporaryDirectory() as td:
+        f["img"].astype(np.float32).tofile(os.path.join(td, "img.f32"))
+        rc, err = run_refused("proj3d", projector="hip", multilor=1, **{k: g[k] for k in GEOM3},
+                              img=os.path.join(td, "img.f32"), out=os.path.join(td, "p.f32"))
+    assert "useMultipleLorsPerBin" in err

@@ -186,7 +186,8 @@ def test_mid3d_divide_backproject_vs_oracle(torch, ctx, golden):
 @pytest.mark.parametrize("dims", [2, 3])
 def test_divide_backproject_vs_reference(torch, ctx, golden, dims):
     """DivideAndBackproject against the reference's own output, with measured and
-    estimated bins zero in every combination (2D: C1 geometry, 3D: mid-size)."""
+    estimated bins zero in every combination (2D: C1 geometry, 3D: mid-size), in
+    both forms; the gather form is bit-identical from run to run."""
     log(f"case: {dims}D divide-and-backproject vs reference")
     f = golden("divide_backproject.npz")
     g = json.loads(str(f[f"geom{dims}d"]))
@@ -194,9 +195,14 @@ def test_divide_backproject_vs_reference(torch, ctx, golden, dims):
     meas = f[f"meas{dims}d"].astype(np.float32)
     est = f[f"est{dims}d_x2"].astype(np.float32) * np.float32(0.5)
     shape = (1,) + meas.shape if dims == 2 else meas.shape
-    out = p.divide_backproject(dev(torch, meas.reshape(shape)), dev(torch, est.reshape(shape)))
+    m, e = dev(torch, meas.reshape(shape)), dev(torch, est.reshape(shape))
+    out = p.divide_backproject(m, e)
+    gat = p.divide_backproject(m, e, mode=hip.BP_GATHER)
+    gat2 = p.divide_backproject(m, e, mode=hip.BP_GATHER)
     torch.cuda.synchronize()
     assert_voxel_close(host(out).ravel(), f[f"img{dims}d"])
+    assert_voxel_close(host(gat).ravel(), f[f"img{dims}d"])
+    assert_bitexact(host(gat2), host(gat))
 
 
 def test_mid3d_fused_ratio_and_likelihood(torch, ctx, golden):

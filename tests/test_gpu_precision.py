@@ -4,13 +4,14 @@ exit voxels by floor(), list length |di|+|dj|+|dk|), summed in double; pinned in
 test_oracle_f64.py.
 
 Modes: APIRL_PREC_REFERENCE (the reference's float arithmetic), APIRL_PREC_KAHAN
-(same weights, compensated sums), APIRL_PREC_F64_ALPHA (the whole Siddon setup,
-crossings, weights and sums in double).  The float modes differ from the FP64 oracle
+(same weights, compensated sums), APIRL_PREC_F64 (the whole Siddon setup, crossings,
+weights and sums in double) and APIRL_PREC_F64_SETUP (only the ray / plane alpha
+intersection setup in double; float stepping and weights, compensated float sums).  The float modes differ from the FP64 oracle
 mostly through discrete decisions, not summation: LORs leave the square FOV exactly
 on its faces and some lie on voxel faces (at 0 and 90 degrees, and the central
 direct plane at z = H/2 with the reference's dz/2 axial offset), so float and double
-floor() put the exit voxel or a whole row on different sides.  F64_ALPHA makes the
-double decisions.  Workload: the hot-spheres phantom (cylinder with 6 spheres of
+floor() put the exit voxel or a whole row on different sides.  F64 and F64_SETUP
+make the double decisions.  Workload: the hot-spheres phantom (cylinder with 6 spheres of
 diameter 10-37 mm at 4:1 contrast) at 1e9 counts.  Per projection on the
 256x256x81 / 344x252x837 geometry (sampled bins of one 12-angle subset), per
 backprojection and per final image (10 OSEM iterations x 21 subsets) on a 64x64x17
@@ -18,10 +19,12 @@ image with a 122 x 84 x 73 sinogram, whose FP64 OSEM the oracle finishes in
 seconds.  Its radial FOV covers the square image at every angle and each subset has
 4 angles: with fewer, some voxels get a zero sensitivity in one subset while other
 subsets still reach them, and the reference's 0/0 update (OsemSinogram3d.cpp:
-217-237) spreads NaN over the whole image.  Bars: F64_ALPHA within 1e-7 relative L2
-of the oracle for line integrals and backprojections (its double sums are rounded
-once to float) and 1e-4 for the image (stored in float between sub-iterations); the
-float modes' errors are printed."""
+217-237) spreads NaN over the whole image.  Bars: F64 within 1e-7 relative L2 of
+the oracle for line integrals and backprojections (its double sums are rounded once
+to float), F64_SETUP within 1e-5 (its crossings are float sums); both within 1e-4
+for the image (stored in float between sub-iterations); the float modes' errors are
+printed.  The gather backprojection of one 12-angle subset of the full 256x256x81 /
+344x252x837 geometry is compared with the FP64 oracle voxel by voxel as well."""
 import numpy as np
 import pytest
 
@@ -35,7 +38,8 @@ C2 = dict(nx=256, ny=256, nz=81, dx=2.0445, dz=2.027, nproj=252, nr=344, rings=6
           rfov=261.696, zfov=164.187, rscanner=400.0, span=11, mrd=60)
 MID = dict(nx=64, ny=64, nz=17, dx=2.0445, dz=2.027, nproj=84, nr=122, rings=13, rfov=93.0,
            zfov=34.459, rscanner=400.0, span=5, mrd=8)
-MODES = {"reference": hip.PREC_REFERENCE, "kahan": hip.PREC_KAHAN, "f64_alpha": hip.PREC_F64_ALPHA}
+MODES = {"reference": hip.PREC_REFERENCE, "kahan": hip.PREC_KAHAN, "f64": hip.PREC_F64,
+         "f64_setup": hip.PREC_F64_SETUP}
 
 
 def errors(a, ref):
@@ -77,7 +81,8 @@ def test_projection_modes_vs_f64_oracle_c2(torch, ctx):
     errs = {k: errors(v[s, ka, q], ref) for k, v in out.items()}
     for k, (rl2, mx) in errs.items():
         log(f"  projection {k:10s}: rel L2 {rl2:.3e}, max abs {mx:.3e}")
-    assert errs["f64_alpha"][0] <= 1e-7  # the double sums rounded once to float
+    assert errs["f64"][0] <= 1e-7  # the double sums rounded once to float
+    assert errs["f64_setup"][0] <= 1e-5
     assert errs["reference"][0] <= 5e-2 and errs["kahan"][0] <= 5e-2
 
 
@@ -96,7 +101,8 @@ def test_backprojection_modes_vs_f64_oracle(torch, ctx):
         assert np.array_equal(a.view(np.int32), b.view(np.int32))  # deterministic
         errs[k] = errors(a, ref)
         log(f"  backprojection {k:10s}: rel L2 {errs[k][0]:.3e}, max abs {errs[k][1]:.3e}")
-    assert errs["f64_alpha"][0] <= 1e-7  # the double sums rounded once to float
+    assert errs["f64"][0] <= 1e-7  # the double sums rounded once to float
+    assert errs["f64_setup"][0] <= 1e-5
     assert errs["reference"][0] <= 5e-2 and errs["kahan"][0] <= 5e-2
 
 
@@ -119,5 +125,27 @@ def test_osem_modes_vs_f64_oracle(torch, ctx):
         assert np.array_equal(np.isnan(a), np.isnan(ref))
         rl2, mx = errors(a, ref)
         log(f"  OSEM image {k:10s}: rel L2 {rl2:.3e}, max abs {mx:.3e}")
-        if k == "f64_alpha":
+        if k in ("f64", "f64_setup"):
             assert rl2 <= 1e-4
+
+
+def test_backprojection_modes_vs_f64_oracle_c2(torch, ctx):
+    """The gather backprojection of subset 0 of 21 on the full 256x256x81 /
+    344x252x837 geometry, every voxel against the FP64 oracle (about ten seconds of
+    one CPU core), per mode."""
+    log("case: precision of the 256x256x81 gather backprojection, subset 0 of 21")
+    g = C2
+    p = plan3d(ctx, g)
+    img = phantoms.cylinder_spheres3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    sino = host(p.project(dev(torch, img), subset=0, nsubsets=21))
+    im, ang, r, za, zb = geometry(g)
+    idx = O.subset_angles(g["nproj"], 21, 0)
+    ref = O.backproject3d_f64(im, sino, np.ascontiguousarray(ang[idx]), r, g["rscanner"], za, zb)
+    s = dev(torch, sino)
+    errs = {}
+    for k, m in MODES.items():
+        a = host(p.backproject(s, subset=0, nsubsets=21, mode=hip.BP_GATHER, precision=m))
+        errs[k] = errors(a, ref)
+        log(f"  backprojection {k:10s}: rel L2 {errs[k][0]:.3e}, max abs {errs[k][1]:.3e}")
+    assert errs["f64"][0] <= 1e-7
+    assert errs["f64_setup"][0] <= 1e-5
