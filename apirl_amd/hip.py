@@ -47,7 +47,11 @@ class EmOptions(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("gather_zlanes", _c_int), ("gather_zvox", _c_int)]
+    _fields_ = [("gather_zlanes", _c_int), ("gather_zvox", _c_int), ("gather_xcd", _c_int)]
+
+
+GATHER_XCD_ROUND_ROBIN = 1
+GATHER_XCD_WEDGES = 2
 
 
 class JobDesc(ctypes.Structure):
@@ -299,6 +303,19 @@ class Plan:
         numSamples / numAxialSamples).  Gather and batches need ns = nas = 1."""
         self.ctx.check(self.ctx.lib.apirl_plan_set_oversampling(self.h, int(ns), int(nas)),
                        "apirl_plan_set_oversampling")
+
+    def set_tuning(self, **fields):
+        """Sets apirl_tuning fields (gather_zlanes, gather_zvox, gather_xcd); the
+        results do not depend on them."""
+        t = Tuning(**{k: int(v) for k, v in fields.items()})
+        self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
+                       "apirl_plan_set_tuning")
+
+    def get_tuning(self):
+        t = Tuning()
+        self.ctx.check(self.ctx.lib.apirl_plan_get_tuning(self.h, ctypes.byref(t)),
+                       "apirl_plan_get_tuning")
+        return {k: getattr(t, k) for k, _ in Tuning._fields_}
 
     @property
     def gather_zvox(self):

@@ -480,8 +480,8 @@ def test_mlem3d_on_mmr_vs_reference(torch, ctx, golden):
 def test_gather_voxel_mappings_bit_identical(torch, ctx, golden, which):
     """The 3D gather's launch shapes give every voxel the same terms in the same
     order: the per-voxel kernel (gather_zlanes = 2) and the column kernel with 1-4
-    voxels per lane (gather_zvox) agree bit for bit, in both float precision modes,
-    on a sinogram with zero bins."""
+    voxels per lane (gather_zvox), in grid order or in XCD sectors (gather_xcd), agree
+    bit for bit, in both float precision modes, on a sinogram with zero bins."""
     log(f"case: gather voxel mappings, {which}")
     if which == "mid3d":
         g = json.loads(str(golden("mid3d_project_backproject.npz")["geom"]))
@@ -498,12 +498,15 @@ def test_gather_voxel_mappings_bit_identical(torch, ctx, golden, which):
         p.gather_zlanes = 2
         ref = host(p.backproject(sd, subset=1, nsubsets=nsub, mode=hip.BP_GATHER, precision=prec))
         p.gather_zlanes = 1
-        for zv in (1, 2, 3, 4):
-            p.gather_zvox = zv
-            out = p.backproject(sd, subset=1, nsubsets=nsub, mode=hip.BP_GATHER, precision=prec)
-            torch.cuda.synchronize()
-            assert_bitexact(host(out), ref)
-    p.gather_zvox = 3
+        for xcd in (hip.GATHER_XCD_ROUND_ROBIN, hip.GATHER_XCD_WEDGES):
+            p.set_tuning(gather_xcd=xcd)
+            for zv in (1, 2, 3, 4):
+                p.gather_zvox = zv
+                out = p.backproject(sd, subset=1, nsubsets=nsub, mode=hip.BP_GATHER,
+                                    precision=prec)
+                torch.cuda.synchronize()
+                assert_bitexact(host(out), ref)
+    p.set_tuning(gather_zvox=3, gather_xcd=hip.GATHER_XCD_ROUND_ROBIN)
 
 
 def test_c2_subset_gather_vs_oracle(torch, ctx, golden):
