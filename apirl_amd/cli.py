@@ -69,7 +69,7 @@ def _sens_files(par, is3d, osem, subsets):
     return np.stack([read_image(q)[0] for q in paths]).astype(np.float32)
 
 
-def run(method, par_path, log=print, gpu=0, mmr_osem_if_subsets=False):
+def run(method, par_path, log=print, gpu=0, mmr_osem_if_subsets=False, projectors=None):
     import torch
 
     if not par_path.endswith(".par"):
@@ -90,11 +90,12 @@ def run(method, par_path, log=print, gpu=0, mmr_osem_if_subsets=False):
     if iters < 0 or subsets < 1 or interval < 0:
         raise FormatError(f"{par_path}: negative iterations or interval, or no subsets")
     # both projector names are required (readParameters.cpp:117-158)
+    projectors = PROJECTORS if projectors is None else projectors
     for key in ("forwardprojector", "backprojector"):
         name = par.get(key, required=True).strip()
-        if name not in PROJECTORS:
+        if name not in projectors:
             raise FormatError(f"{key} '{name}' is not available on this path "
-                              f"(use one of {', '.join(PROJECTORS)})")
+                              f"(use one of {', '.join(projectors)})")
     # LORs per bin (readParameters.cpp:162-203; absent keys mean one)
     samples = []
     for key in ("siddon number of samples on the detector",
@@ -547,7 +548,9 @@ def run_cumlem(par_path, log=print):
     itype = par.get("input type", required=True)
     if itype not in ("Sinogram3D", "Sinogram3DSiemensMmr"):
         raise FormatError(f"input type '{itype}' is not available to cuMLEM on this path")
-    return run("MLEM", par_path, log=log, gpu=_gpu_id(par), mmr_osem_if_subsets=True)
+    # cuMLEM.cpp:279-316: "CuSiddonProjector" names the GPU Siddon projector
+    return run("MLEM", par_path, log=log, gpu=_gpu_id(par), mmr_osem_if_subsets=True,
+               projectors=GPU_PROJECTORS)
 
 
 COMMANDS = {"OSEM": lambda p: run("OSEM", p), "MLEM": lambda p: run("MLEM", p),
@@ -562,7 +565,7 @@ def main(argv=None):
         return 1
     try:
         COMMANDS[argv[0]](os.path.abspath(argv[1]))
-    except (FormatError, hip.ApirlError, KeyError, ValueError) as e:
+    except (FormatError, hip.ApirlError, KeyError, ValueError, OSError) as e:
         print(f"{argv[0]}: {e}", file=sys.stderr)
         return 1
     return 0

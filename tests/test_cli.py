@@ -140,6 +140,11 @@ def test_subset_keys_read_together(tmp_path):
         cli._subset_keys(par("number of subsets := 7\nsubset index := 7\n"))
 
 
+RUN_KEYS = {"input type": "Sinogram3D", "input file": "s.h33", "initial estimate": "no_init.h33",
+            "output filename prefix": "o", "number of iterations": "1",
+            "backprojector": "Siddon"}
+
+
 @pytest.mark.parametrize("cmd,method,keys,match", [
     ("project", "Projection", {"input file": "i.h33", "output type": "Sinogram3D",
                                "projector": "ConeOfResponse", "output filename": "o",
@@ -149,6 +154,14 @@ def test_subset_keys_read_together(tmp_path):
      "output filename"),
     ("cuMLEM", "MLEM", {"input type": "Sinogram3D", "projector block size": "{1,1,1}",
                         "backprojector block size": "{1,1,1}"}, "pixel update block size"),
+    # the CPU command knows no GPU projector name (readParameters.cpp:117-158) ...
+    ("MLEM", "MLEM", dict(RUN_KEYS, forwardprojector="CuSiddonProjector"), "CuSiddonProjector"),
+    # ... cuMLEM does (cuMLEM.cpp:279-316): it goes on to read the initial estimate
+    ("cuMLEM", "MLEM", dict(RUN_KEYS, forwardprojector="CuSiddonProjector",
+                            backprojector="CuSiddonProjector",
+                            **{"projector block size": "{1,1,1}",
+                               "backprojector block size": "{1,1,1}",
+                               "pixel update block size": "{1,1,1}"}), "no_init.h33"),
 ])
 def test_projection_commands_refuse_before_the_device(tmp_path, capsys, cmd, method, keys, match):
     path = tmp_path / "c.par"
