@@ -112,9 +112,11 @@ struct apirl_plan {
   int gather_zlanes = 1;           // 3D gather voxel mapping (1: 16x16 slice tiles; Z>1:
                                    // Z slices x 64/Z columns per wavefront)
   int gather_zvox = 3;             // 3D gather voxels per lane along z (1..4, tiles only)
-  int gather_xcd = APIRL_GATHER_XCD_ROUND_ROBIN;  // column kernel's tile order
-  int* d_tilemap = nullptr;        // its tile map (APIRL_GATHER_XCD_WEDGES), for zv below
-  int tilemap_zv = 0, tilemap_slots = 0;
+  int gather_xcd = APIRL_GATHER_XCD_WEDGES;  // column kernel's tile order
+  // its tile maps (APIRL_GATHER_XCD_WEDGES), one per voxels-per-lane depth 1..4, built
+  // on first use and kept until the plan ends (captured launches may hold them)
+  int* d_tilemap[5] = {};
+  int tilemap_slots[5] = {};
   void* d_gscratch = nullptr;      // per-ray axial setup of the gather (grown on demand)
   float* d_divbins = nullptr;      // DivideAndBackproject's bin values for the gather
   size_t divbins_cap = 0;          //   (grown on demand)

@@ -110,6 +110,33 @@ bool upload_gather_form(apirl_plan* p, const std::vector<apirl::RayXY>& rays,
   return ok;
 }
 
+// The sector order of the gather column kernel's tiles (APIRL_GATHER_XCD_WEDGES) for
+// the plan's voxels-per-lane depth: built once per depth and kept until the plan
+// ends, so a launch captured with one map never sees it freed.  False when it cannot
+// be allocated; 2D plans and the other orders need none.
+bool ensure_tile_map(apirl_plan* p) {
+  const int zv = p->gather_zvox;
+  if (p->gather_xcd != APIRL_GATHER_XCD_WEDGES || p->geo.dims != 3 || zv < 1 || zv > 4 ||
+      p->d_tilemap[zv])
+    return true;
+  std::vector<int> map;
+  apirl::build_wedge_tile_map((p->geo.nx + 15) / 16, (p->geo.ny + 15) / 16,
+                              (p->geo.nz + zv - 1) / zv, map);
+  int* d = nullptr;
+  if (hipMalloc(&d, map.size() * sizeof(int)) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  if (hipMemcpy(d, map.data(), map.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(d);
+    return false;
+  }
+  p->d_tilemap[zv] = d;
+  p->tilemap_slots[zv] = (int)map.size();
+  return true;
+}
+
 // Backprojection of one subset in either form.
 // Oversampled bins are traversed in the reference's arithmetic only (and need the
 // 24-bit index walk).
@@ -171,26 +198,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     sino = p->d_divbins;
     kind = apirl::BP_PLAIN;
   }
-  if (p->gather_xcd == APIRL_GATHER_XCD_WEDGES && p->geo.dims == 3 &&
-      p->tilemap_zv != p->gather_zvox) {
-    // the sector order of the column kernel's tiles for this voxels-per-lane depth
-    const int zv = p->gather_zvox;
-    std::vector<int> map;
-    apirl::build_wedge_tile_map((p->geo.nx + 15) / 16, (p->geo.ny + 15) / 16,
-                                (p->geo.nz + zv - 1) / zv, map);
-    if (p->d_tilemap) {
-      APIRL_HIP(c, hipStreamSynchronize(c->stream));
-      (void)hipFree(p->d_tilemap);
-    }
-    p->d_tilemap = nullptr;
-    p->tilemap_zv = 0;
-    if (hipMalloc(&p->d_tilemap, map.size() * sizeof(int)) != hipSuccess)
-      return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
-    APIRL_HIP(c, hipMemcpy(p->d_tilemap, map.data(), map.size() * sizeof(int),
-                           hipMemcpyHostToDevice));
-    p->tilemap_zv = zv;
-    p->tilemap_slots = (int)map.size();
-  }
+  if (!ensure_tile_map(p))
+    return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream));
   return APIRL_OK;
@@ -432,6 +441,9 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
     if (p->gather_tables) f.gather = upload_gather_form(p, rays, p->r, !p->uniform_r, false, f);
     apirl::use_lor_form(p, APIRL_SCANNER_CYLINDRICAL);
   }
+  // The gather's tile order for the default depth (without it the column kernel
+  // keeps the grid's order).
+  (void)ensure_tile_map(p);
   // Transposed image for the forward projector's x-major rays.
   if (hipMalloc(&p->d_imgT, (size_t)im.nx * im.ny * im.nz * sizeof(float)) != hipSuccess) {
     p->d_imgT = nullptr;
@@ -464,7 +476,8 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_segs) (void)hipFree(p->d_segs);
   if (p->d_gscratch) (void)hipFree(p->d_gscratch);
   if (p->d_divbins) (void)hipFree(p->d_divbins);
-  if (p->d_tilemap) (void)hipFree(p->d_tilemap);
+  for (int* m : p->d_tilemap)
+    if (m) (void)hipFree(m);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
   delete p;
   return APIRL_OK;
@@ -585,6 +598,9 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
       return fail(p->ctx, APIRL_ERR_INVALID, "unknown gather_xcd");
     p->gather_xcd = t->gather_xcd;
   }
+  DeviceGuard g(p->ctx->device);
+  if (!ensure_tile_map(p))
+    return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   return APIRL_OK;
 }
 

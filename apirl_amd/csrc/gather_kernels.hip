@@ -978,10 +978,11 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   A.rl_inv_h = p->rl_inv_h;
   A.nrl = p->nrl;
   // the column kernel's sector order when selected and built for the current depth
-  const bool wedge = p->gather_xcd == APIRL_GATHER_XCD_WEDGES && p->d_tilemap &&
-                     p->tilemap_zv == p->gather_zvox;
-  A.tile_map = wedge ? p->d_tilemap : nullptr;
-  A.tile_slots = wedge ? p->tilemap_slots : 0;
+  const int zv = p->gather_zvox;
+  const bool wedge = p->gather_xcd == APIRL_GATHER_XCD_WEDGES && zv >= 1 && zv <= 4 &&
+                     p->d_tilemap[zv];
+  A.tile_map = wedge ? p->d_tilemap[zv] : nullptr;
+  A.tile_slots = wedge ? p->tilemap_slots[zv] : 0;
   if (g.dims == 3 && scratch) {
     // scratch: [ZRay x nrays][non-zero flag per (angle, r)][z-crossing table]
     const size_t npair = (size_t)nang_sub * g.nr;

@@ -135,7 +135,9 @@ typedef struct apirl_tuning {
 /* gather_xcd: APIRL_GATHER_XCD_ROUND_ROBIN takes the 16 x 16 x zvox tiles in grid
  * order, which the hardware deals to the 8 XCDs in turn; APIRL_GATHER_XCD_WEDGES gives
  * each XCD the tiles of one 45-degree sector of the image (every z block), so the
- * candidate rays a tile shares with its neighbours are fetched into one XCD's L2. */
+ * candidate rays a tile shares with its neighbours are fetched into one XCD's L2
+ * (the default: 2.5 ms against 2.9 ms for one 12-angle subset of the 256x256x81 /
+ * 344x252x837 workload). */
 #define APIRL_GATHER_XCD_ROUND_ROBIN 1
 #define APIRL_GATHER_XCD_WEDGES 2
 /* Oversampled LORs (SiddonProjector(numSamples, numAxialSamples),
