@@ -88,6 +88,9 @@ struct apirl_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
+  // this device's copy of the file-scope partials array of apirl_project_ratio_noalloc
+  // (hipGetSymbolAddress, taken once per device); null if that failed
+  double* d_noalloc = nullptr;
   // scratch owned by the context (likelihood partials, reductions)
   double* d_red = nullptr;
   size_t red_cap = 0;
@@ -204,6 +207,11 @@ hipError_t launch_force_positive(float* x, size_t n, hipStream_t st);
 hipError_t launch_divide_rule(const float* meas, const float* est, float* out, size_t n,
                               int dims, hipStream_t st);
 hipError_t launch_sum_partials(const double* partials, int n, double* out, hipStream_t st);
+// Partials of apirl_project_ratio_noalloc: a file-scope device array of
+// siddon_kernels.hip of kNoallocPartials doubles (one per workgroup of 256 bins).
+constexpr int kNoallocPartials = 1 << 19;
+// The current device's address of that array (hipGetSymbolAddress).
+hipError_t noalloc_scratch_address(double** out);
 // Likelihood sum y log(x) - x over x > 0 (positive_only) or x != 0, deterministic:
 // block partials into `partials`, then a fixed-order sum into *out.
 hipError_t launch_likelihood(const float* x, const float* y, long n, int positive_only,

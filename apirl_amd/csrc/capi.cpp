@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <vector>
 
 #include "apirl_internal.h"
@@ -209,6 +211,24 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
 
 extern "C" {
 
+// Per-device address of the allocation-free entry point's partials array, taken once
+// per device with that device current.
+double* noalloc_scratch_for(int device) {
+  static std::mutex mu;
+  static std::map<int, double*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(device);
+  if (it != addr.end()) return it->second;
+  double* p = nullptr;
+  if (apirl::noalloc_scratch_address(&p) != hipSuccess) {
+    (void)hipGetLastError();
+    p = nullptr;  // the entry point then reports the failure; not cached, so retried
+    return p;
+  }
+  addr[device] = p;
+  return p;
+}
+
 apirl_status apirl_ctx_create(int device, apirl_ctx** out) {
   if (!out) return APIRL_ERR_INVALID;
   *out = nullptr;
@@ -223,6 +243,7 @@ apirl_status apirl_ctx_create(int device, apirl_ctx** out) {
     delete c;
     return APIRL_ERR_NOMEM;
   }
+  c->d_noalloc = noalloc_scratch_for(device);
   *out = c;
   return APIRL_OK;
 }
@@ -642,9 +663,14 @@ apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const 
   return backproject_any(p, d_meas, d_est, d_img, subset, nsubsets, apirl::BP_DIVIDE, mode);
 }
 
-apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float* d_meas,
-                                 const float* d_additive, float* d_ratio, double* d_lik,
-                                 int subset, int nsubsets, int flags, int precision) {
+namespace {
+// The fused forward step.  noalloc: the likelihood partials go to the file-scope
+// array of this GPU (no allocation; up to kNoallocPartials blocks), otherwise to the
+// context's buffer, grown on demand.
+apirl_status project_ratio_impl(apirl_plan* p, const float* d_img, const float* d_meas,
+                                const float* d_additive, float* d_ratio, double* d_lik,
+                                int subset, int nsubsets, int flags, int precision,
+                                bool noalloc) {
   if (!p || !d_img || !d_meas || !d_ratio) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
@@ -665,14 +691,41 @@ apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float*
   fr.lik_positive_only = (flags & APIRL_RATIO_LIK_POSITIVE) ? 1 : 0;
   fr.additive_local = (flags & APIRL_RATIO_ADDITIVE_LOCAL) ? 1 : 0;
   if (d_lik) {
-    apirl_status st = ensure_red(c, (size_t)nblocks);
-    if (st != APIRL_OK) return st;
-    fr.lik_partials = c->d_red;
+    if (noalloc) {
+      if (nblocks > apirl::kNoallocPartials)
+        return fail(c, APIRL_ERR_INVALID,
+                    "apirl_project_ratio_noalloc: the likelihood of more than "
+                    "APIRL_NOALLOC_MAX_BINS bins per call needs apirl_project_ratio");
+      if (!c->d_noalloc)
+        return fail(c, APIRL_ERR_HIP, "the partials array's device address is unavailable");
+      fr.lik_partials = c->d_noalloc;
+    } else {
+      apirl_status st = ensure_red(c, (size_t)nblocks);
+      if (st != APIRL_OK) return st;
+      fr.lik_partials = c->d_red;
+    }
   }
   APIRL_HIP(c, apirl::launch_project(p, d_img, nullptr, subset, nsubsets, precision, &fr,
                                      nullptr, c->stream));
-  if (d_lik) APIRL_HIP(c, apirl::launch_sum_partials(c->d_red, nblocks, d_lik, c->stream));
+  if (d_lik)
+    APIRL_HIP(c, apirl::launch_sum_partials(fr.lik_partials, nblocks, d_lik, c->stream));
   return APIRL_OK;
+}
+}  // namespace
+
+apirl_status apirl_project_ratio(apirl_plan* p, const float* d_img, const float* d_meas,
+                                 const float* d_additive, float* d_ratio, double* d_lik,
+                                 int subset, int nsubsets, int flags, int precision) {
+  return project_ratio_impl(p, d_img, d_meas, d_additive, d_ratio, d_lik, subset, nsubsets,
+                            flags, precision, false);
+}
+
+apirl_status apirl_project_ratio_noalloc(apirl_plan* p, const float* d_img,
+                                         const float* d_meas, const float* d_additive,
+                                         float* d_ratio, double* d_lik, int subset,
+                                         int nsubsets, int flags, int precision) {
+  return project_ratio_impl(p, d_img, d_meas, d_additive, d_ratio, d_lik, subset, nsubsets,
+                            flags, precision, true);
 }
 
 apirl_status apirl_em_update(apirl_ctx* c, float* d_x, const float* d_b, const float* d_s,

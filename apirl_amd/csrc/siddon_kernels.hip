@@ -875,6 +875,19 @@ hipError_t launch_force_positive(float* x, size_t n, hipStream_t st) {
   return hipGetLastError();
 }
 
+// The likelihood's block partials of apirl_project_ratio_noalloc: 2^19 doubles
+// (4 MiB) per GPU, resident until the process ends.  Every block writes its own entry
+// before sum_partials_kernel reads them, so nothing needs clearing; calls that overlap
+// on one GPU share the array and race.
+static __device__ __align__(256) double scratch[kNoallocPartials];
+
+hipError_t noalloc_scratch_address(double** out) {
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  *out = e == hipSuccess ? static_cast<double*>(ptr) : nullptr;
+  return e;
+}
+
 hipError_t launch_sum_partials(const double* partials, int n, double* out, hipStream_t st) {
   hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, st, partials, n, out);
   return hipGetLastError();

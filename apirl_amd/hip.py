@@ -86,6 +86,8 @@ SIGNATURES = {
     "apirl_backproject": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_divide_backproject": (_c_int, [_P, _P, _P, _P, _c_int, _c_int, _c_int]),
     "apirl_project_ratio": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
+    "apirl_project_ratio_noalloc": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int,
+                                             _c_int]),
     "apirl_em_update": (_c_int, [_P, _P, _P, _P, _c_size, _c_float, _c_int]),
     "apirl_sensitivity": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_mlem_threshold": (_c_int, [_P, _P, _c_size, ctypes.POINTER(_c_float)]),

@@ -217,6 +217,21 @@ apirl_status apirl_divide_backproject(apirl_plan* plan, const float* d_meas, con
 apirl_status apirl_project_ratio(apirl_plan* plan, const float* d_img, const float* d_meas,
                                  const float* d_additive, float* d_ratio, double* d_lik,
                                  int subset, int nsubsets, int flags, int precision);
+/* apirl_project_ratio that allocates nothing.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The likelihood's block partials go
+ * to a file-scope device array of 2^19 doubles (4 MiB per GPU, held until the process
+ * ends) instead of the context's buffer, so a call with d_lik != NULL takes at most
+ * APIRL_NOALLOC_MAX_BINS subset bins (larger ones fail with APIRL_ERR_INVALID); with
+ * d_lik == NULL any size works.  The plan's own buffers (allocated by
+ * apirl_plan_create) are used as by apirl_project_ratio; nothing is allocated or
+ * freed.  Calls that overlap on one GPU share the array and race: serialise them
+ * (one stream, or events). */
+#define APIRL_NOALLOC_MAX_BINS ((size_t)1 << 27)
+apirl_status apirl_project_ratio_noalloc(apirl_plan* plan, const float* d_img,
+                                         const float* d_meas, const float* d_additive,
+                                         float* d_ratio, double* d_lik, int subset,
+                                         int nsubsets, int flags, int precision);
 /* x := cond ? x * b / s : 0, cond = s > thr (strict) or s >= thr
  * (Mlem2d.cpp:171-185, MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237). */
 apirl_status apirl_em_update(apirl_ctx* ctx, float* d_x, const float* d_b, const float* d_s,
