@@ -105,7 +105,10 @@ struct apirl_plan {
   apirl::Geo geo{};
   apirl::RayXY* d_rays = nullptr;  // [nang][nr][ns]
   float2* d_zab = nullptr;         // [nsino][nas] (P1.Z, P2.Z), 3D only
-  float* d_replicas = nullptr;     // [8][nvox] per-XCD scatter targets (zero between calls)
+  // [16][nvox] per-XCD scatter targets, 8 in the image layout and 8 transposed
+  // ([z][x][y]), allocated on the first scatter backprojection and zero between calls
+  float* d_replicas = nullptr;
+  bool replicas_failed = false;    // their allocation failed: direct atomics on the image
   float* d_imgT = nullptr;         // [nz][nx][ny] transposed copy of the projected image
   int* d_zorder = nullptr;         // [nsino] sinograms sorted by mean axial coordinate
   float2* d_angcs = nullptr;       // [nang] (cos, sin) for the gather candidate search

@@ -156,6 +156,20 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     if (precision != APIRL_PREC_REFERENCE)
       return fail(c, APIRL_ERR_UNSUPPORTED,
                   "the scatter backprojector has the reference precision only");
+    if (!p->d_replicas && !p->replicas_failed) {
+      // the per-XCD scatter targets (16 image copies), zeroed here and re-zeroed by
+      // the reduction after every use; without them the atomics go to the image
+      const size_t bytes = (size_t)16 * p->geo.nx * p->geo.ny * p->geo.nz * sizeof(float);
+      if (hipMalloc(&p->d_replicas, bytes) != hipSuccess) {
+        p->d_replicas = nullptr;
+        p->replicas_failed = true;
+        (void)hipGetLastError();
+      } else if (hipMemsetAsync(p->d_replicas, 0, bytes, c->stream) != hipSuccess) {
+        (void)hipFree(p->d_replicas);
+        p->d_replicas = nullptr;
+        return fail(c, APIRL_ERR_HIP, "cannot clear the scatter backprojector's copies");
+      }
+    }
     APIRL_HIP(c, apirl::launch_backproject(p, sino, est, img, subset, nsubsets, kind, mode,
                                            c->stream));
     return APIRL_OK;
@@ -469,17 +483,6 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
   if (hipMalloc(&p->d_imgT, (size_t)im.nx * im.ny * im.nz * sizeof(float)) != hipSuccess) {
     p->d_imgT = nullptr;
     (void)hipGetLastError();
-  }
-  // Per-XCD scatter targets for the backprojector: 8 image copies, zeroed here and
-  // re-zeroed by the reduction after every use.
-  {
-    const size_t bytes = (size_t)8 * im.nx * im.ny * im.nz * sizeof(float);
-    if (hipMalloc(&p->d_replicas, bytes) != hipSuccess ||
-        hipMemset(p->d_replicas, 0, bytes) != hipSuccess) {
-      if (p->d_replicas) (void)hipFree(p->d_replicas);
-      p->d_replicas = nullptr;  // fall back to direct atomics on the image
-      (void)hipGetLastError();
-    }
   }
   *out = p;
   return APIRL_OK;

@@ -175,6 +175,62 @@ __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float
   }
 }
 
+// kUnroll steps of walk3d from step m0 for a batch that stays inside the image.  A
+// ray's voxel index moves monotonically along each axis, so when the first and the
+// last voxel of a batch lie inside the image every voxel between them does: the
+// reference's bounds tests all pass and each term reads the in-image voxel, so they
+// and the out-of-image index are skipped.  A lane whose batch reaches outside the
+// image, or past the ray's last step, takes walk3d_batch from the same state.
+template <bool TR>
+__device__ __forceinline__ void walk3d_batch_inside(const Geo& g, Ray& ray,
+                                                    const float* __restrict__ img,
+                                                    const float* __restrict__ imgT, int nxy,
+                                                    int m0, float& acc) {
+  Ray r = ray;
+  float wv[kUnroll];
+  int iv[kUnroll];
+  int ei = 0, ej = 0, ek = 0;  // the batch's last voxel
+#pragma unroll
+  for (int u = 0; u < kUnroll; u++) {
+    const int ci = r.i, cj = r.j, ck = r.k;
+    const bool bx = (r.ax <= r.ay) && (r.ax <= r.az);
+    const bool by = !bx && (r.ay <= r.az);
+    const bool bz = !bx && !by;
+    const float an = bx ? r.ax : (by ? r.ay : r.az);
+    wv[u] = (an - r.ac) * r.L;
+    r.ac = an;
+    const float nax = r.ax + r.axu, nay = r.ay + r.ayu, naz = r.az + r.azu;
+    r.ax = bx ? nax : r.ax;
+    r.ay = by ? nay : r.ay;
+    r.az = bz ? naz : r.az;
+    r.i += bx ? r.iinc : 0;
+    r.j += by ? r.jinc : 0;
+    r.k += bz ? r.kinc : 0;
+    iv[u] = TR ? __mul24(ck, nxy) + __mul24(ci, g.ny) + cj
+               : __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
+    if (u == kUnroll - 1) {
+      ei = ci;
+      ej = cj;
+      ek = ck;
+    }
+  }
+  const bool inside = ((unsigned)ray.i < (unsigned)g.nx) & ((unsigned)ray.j < (unsigned)g.ny) &
+                      ((unsigned)ray.k < (unsigned)g.nz) & ((unsigned)ei < (unsigned)g.nx) &
+                      ((unsigned)ej < (unsigned)g.ny) & ((unsigned)ek < (unsigned)g.nz) &
+                      (m0 + kUnroll <= ray.n);
+  if (inside) {
+    const float* __restrict__ base = TR ? imgT : img;
+    float pv[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) pv[u] = base[(unsigned)iv[u]];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) acc = acc + wv[u] * pv[u];
+    ray = r;
+  } else {
+    walk3d_batch<TR, true>(g, ray, img, imgT, nxy, m0, acc);
+  }
+}
+
 // 3D traversal of one ray in the reference's arithmetic, kUnroll steps per batch of
 // voxel loads (addresses never depend on loaded data), terms added in traversal
 // order.  The reference reads voxel (ci, cj, ck) when ci < nx, cj < ny, ck < nz and
@@ -184,15 +240,14 @@ __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float
 // offset along y, so in-image voxels are read from the transposed copy (y fastest),
 // where the wavefront's 64 parallel rays touch neighbouring addresses; out-of-image
 // reads keep the reference's index into the image.  Indices use 24-bit multiplies
-// (the plan keeps nx ny below 2^23).  Whole batches skip the step-count test; the
-// last, partial batch takes it.
+// (the plan keeps nx ny below 2^23).  Batches inside the image skip the bounds and
+// step-count tests; the others, and the last, partial batch, take them.
 template <bool TR>
 __device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
                                         const float* __restrict__ imgT, float acc = 0.f) {
   const int nxy = g.nx * g.ny;
-  int m0 = 0;
-  for (; m0 + kUnroll <= ray.n; m0 += kUnroll) walk3d_batch<TR, false>(g, ray, img, imgT, nxy, m0, acc);
-  if (m0 < ray.n) walk3d_batch<TR, true>(g, ray, img, imgT, nxy, m0, acc);
+  for (int m0 = 0; m0 < ray.n; m0 += kUnroll)
+    walk3d_batch_inside<TR>(g, ray, img, imgT, nxy, m0, acc);
   return acc;
 }
 
@@ -483,10 +538,18 @@ __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
   // lanes over sinograms, which puts every lane on its own line).
   const long idx = (long)blockIdx.x * kBlock + threadIdx.x;
   if (idx >= nrays) return;
-  // With REPLICA, `img` holds one image per XCD and each wave adds into its own die's
-  // copy, so the atomics of a cache line stay in one L2 instead of moving between
-  // dies; reduce_replicas_kernel folds the copies afterwards.
-  if (REPLICA) img += (long)xcc_id() * ((long)g.nx * g.ny * g.nz);
+  // With REPLICA, `img` holds 2 x 8 image copies: each wave adds into its own die's
+  // copies, so the atomics of a cache line stay in one L2 instead of moving between
+  // dies.  Rays running mostly along x (whose 64 lanes are offset along y) add their
+  // in-image voxels into the die's transposed copy (y fastest), where one atomic
+  // instruction of the wavefront touches a few cache lines instead of one line per
+  // lane.  reduce_replicas_kernel folds the copies afterwards.
+  float* imgT = img;
+  if (REPLICA) {
+    const long nv = (long)g.nx * g.ny * g.nz;
+    imgT = img + (long)(kNumXcd + xcc_id()) * nv;
+    img += (long)xcc_id() * nv;
+  }
   int s, a, rr, k;
   ray_coords(g, idx, nang_sub, subset, nsubsets, s, a, rr, k);
   float b = 1.f, e = 0.f;
@@ -514,6 +577,7 @@ __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
   Ray ray;
   const bool valid = g.dims == 3 ? setup_ray<3>(g, t, za, zb, ray) : setup_ray<2>(g, t, za, zb, ray);
   if (!valid) continue;
+  const bool tr = REPLICA && (t.flags & RAY_XMAJOR);
   for (int m = 0; m < ray.n; m++) {
     int ci, cj, ck;
     const float w = step(ray, ci, cj, ck);
@@ -530,7 +594,13 @@ __global__ __launch_bounds__(kBlock) void backproject_scatter_kernel(
     }
     if (ok) {
       const long li = (long)ck * nxy + (long)cj * g.nx + ci;
-      if (li >= 0 && li < npix) unsafeAtomicAdd(&img[li], term);
+      // Only a voxel inside the image goes to the transposed copy.  DivideAndBackproject
+      // tests the upper bounds only (SiddonProjector.cpp:666-667): there a negative ci
+      // or cj names the voxel li of the image layout (as the reference does), which has
+      // no transposed counterpart, and a negative ck gives li < 0, which is skipped.
+      if (tr && ci >= 0 && cj >= 0 && ck >= 0)
+        unsafeAtomicAdd(&imgT[(long)ck * nxy + (long)ci * g.ny + cj], term);
+      else if (li >= 0 && li < npix) unsafeAtomicAdd(&img[li], term);
     }
   }
   }
@@ -667,18 +737,46 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(const float* __restr
 
 // img[v] += rep[0][v] + ... + rep[7][v] (fixed order), and clears the replicas for
 // the next backprojection.
+// img += the 8 image-layout copies rep[0..7] and the 8 transposed copies rep[8..15]
+// (layout [z][x][y]), which are cleared for the next use.  One 32 x 32 tile of one
+// slice per workgroup: the transposed sums pass through LDS, so every copy is read
+// and cleared with coalesced accesses.  Summation order per voxel: the image-layout
+// copies by die, then the transposed ones by die.
 __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restrict__ img,
-                                                                 float* __restrict__ rep,
-                                                                 long n) {
-  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  float acc = 0.f;
+                                                                 float* __restrict__ rep, int nx,
+                                                                 int ny) {
+  __shared__ float tile[32][33];
+  const long n = (long)nx * ny * gridDim.z;
+  const long base = (long)blockIdx.z * nx * ny;
+  const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  for (int r = ty; r < 32; r += 8) {
+    const int x = x0 + r, y = y0 + tx;
+    float acc = 0.f;
+    if (x < nx && y < ny) {
+      const long i = base + (long)x * ny + y;
 #pragma unroll
-  for (int x = 0; x < kNumXcd; x++) {
-    acc += rep[(long)x * n + i];
-    rep[(long)x * n + i] = 0.f;
+      for (int c = kNumXcd; c < 2 * kNumXcd; c++) {
+        acc += rep[(long)c * n + i];
+        rep[(long)c * n + i] = 0.f;
+      }
+    }
+    tile[r][tx] = acc;
   }
-  img[i] += acc;
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int y = y0 + r, x = x0 + tx;
+    if (x < nx && y < ny) {
+      const long i = base + (long)y * nx + x;
+      float acc = 0.f;
+#pragma unroll
+      for (int c = 0; c < kNumXcd; c++) {
+        acc += rep[(long)c * n + i];
+        rep[(long)c * n + i] = 0.f;
+      }
+      img[i] += acc + tile[tx][r];
+    }
+  }
 }
 
 // out[k][i][j] = in[k][j][i] through a 32 x 32 LDS tile (coalesced both ways).
@@ -816,7 +914,6 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
   const long nrays = (long)g.nsino * nang_sub * g.nr;
   if (nrays == 0) return hipSuccess;
   const unsigned grid = grid_for(nrays);
-  const long nvox = (long)g.nx * g.ny * g.nz;
   float* rep = p->d_replicas;
   float* target = rep ? rep : img;
 #define APIRL_LAUNCH_BP(K)                                                                   \
@@ -836,8 +933,8 @@ hipError_t launch_backproject(const apirl_plan* p, const float* sino, const floa
   }
 #undef APIRL_LAUNCH_BP
   if (rep)
-    hipLaunchKernelGGL(reduce_replicas_kernel, dim3(grid_for(nvox)), dim3(kBlock), 0, st, img,
-                       rep, nvox);
+    hipLaunchKernelGGL(reduce_replicas_kernel, dim3((g.nx + 31) / 32, (g.ny + 31) / 32, g.nz),
+                       dim3(kBlock), 0, st, img, rep, g.nx, g.ny);
   return hipGetLastError();
 }
 
