@@ -100,18 +100,38 @@ def cpu_baseline(meas_host, threads, launcher):
                       f"OMP_NUM_THREADS={threads}; iterations/s = 1/(21 t)"}
 
 
-def measured_traffic(entry):
-    """HBM bytes per launch of one entry point (all of its kernels), as measured by the
-    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/pmc_traffic_c2.md
-    (lines "traffic <entry> <bytes>"); None when no measurement is committed."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_c2.md")
+def profile_figure(name, key, entry):
+    """A per-launch figure of one entry point committed under profiles/ by its rocprofv3
+    passes: lines "<key> <entry> <value>"; None when no measurement is committed."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
         f = line.split()
-        if len(f) == 3 and f[0] == "traffic" and f[1] == entry:
+        if len(f) == 3 and f[0] == key and f[1] == entry:
             return float(f[2])
     return None
+
+
+def measured_traffic(entry):
+    """HBM bytes per launch of one entry point (all of its kernels), as measured by the
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes summarised in profiles/pmc_traffic_c2.md."""
+    return profile_figure("pmc_traffic_c2.md", "traffic", entry)
+
+
+# MI355X: 256 CUs; one VALU wave-instruction issued per cycle per CU at 2.4 GHz
+VALU_ISSUE_PER_S = 256 * 2.4e9
+
+
+def valu_issue(entry, ms):
+    """The VALU issue bound of the entry point's main kernel: its VALU wave-instructions
+    per launch (SQ_INSTS_VALU, profiles/sq_counters_c2.md) at one per cycle per CU,
+    against the measured launch time."""
+    v = profile_figure("sq_counters_c2.md", "valu", entry)
+    if v is None:
+        return None
+    t = v / VALU_ISSUE_PER_S * 1e3
+    return {"wave_instructions": v, "issue_ms": t, "frac": t / ms}
 
 
 def batch_main(args):
@@ -499,7 +519,8 @@ def main():
         "kernel_ms": {"forward_fused": fwd_ms, "backproject": bwd_ms, "update": upd_ms},
         "roofline": {"bound": "hbm", "kernel": dom_entry,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_entry)},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_entry),
+                     "valu_issue": valu_issue(dom_entry, dom_ms)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
