@@ -182,10 +182,10 @@ def run(method, par_path, log=print, gpu=0, mmr_osem_if_subsets=False, projector
         # reproduced, and said so
         print("note: as the reference's OSEM, subset angle k takes the additive "
               "sinogram's angle k (not the subset's own angle s + S k)", file=sys.stderr)
-    # the gather backprojector needs one LOR per bin; oversampled 3D bins are
-    # backprojected by the scatter form (2D projectors have no oversampling,
-    # SiddonProjector.cpp:34-193, so there the keys have no effect)
-    bp_mode = hip.BP_SCATTER if (is3d and samples != [1, 1]) else hip.BP_GATHER
+    # the deterministic gather backprojector, oversampled 3D bins included (their LORs
+    # gathered in turn); 2D projectors have no oversampling (SiddonProjector.cpp:
+    # 34-193), so there the keys have no effect
+    bp_mode = hip.BP_GATHER
     sens = torch.empty((subsets,) + plan.image_shape(), dtype=torch.float32, device="cuda")
     given = _sens_files(par, is3d, osem_rules, subsets)
     if given is not None:

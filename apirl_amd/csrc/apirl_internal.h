@@ -109,6 +109,11 @@ struct apirl_plan {
   // ([z][x][y]), allocated on the first scatter backprojection and zero between calls
   float* d_replicas = nullptr;
   bool replicas_failed = false;    // their allocation failed: direct atomics on the image
+  // Oversampled plans (ns x nas LORs per bin): one plan of one LOR per bin for each
+  // radial sub-LOR o and axial sample q (index o * nas + q), whose gathers add up to
+  // the oversampled backprojection; built on the first gather, freed with the plan or
+  // when the oversampling changes
+  std::vector<apirl_plan*> os_parts;
   float* d_imgT = nullptr;         // [nz][nx][ny] transposed copy of the projected image
   int* d_zorder = nullptr;         // [nsino] sinograms sorted by mean axial coordinate
   float2* d_angcs = nullptr;       // [nang] (cos, sin) for the gather candidate search
@@ -215,6 +220,10 @@ hipError_t launch_sum_partials(const double* partials, int n, double* out, hipSt
 constexpr int kNoallocPartials = 1 << 19;
 // The current device's address of that array (hipGetSymbolAddress).
 hipError_t noalloc_scratch_address(double** out);
+// The one-LOR-per-bin plans of an oversampled plan (apirl_plan::os_parts); false with
+// the context's error set when they cannot be built.
+bool ensure_os_parts(apirl_plan* p);
+void free_os_parts(apirl_plan* p);
 // Likelihood sum y log(x) - x over x > 0 (positive_only) or x != 0, deterministic:
 // block partials into `partials`, then a fixed-order sum into *out.
 hipError_t launch_likelihood(const float* x, const float* y, long n, int positive_only,

@@ -121,6 +121,22 @@ size_t max_gather_scratch(const apirl_plan* p, int S) {
   return m;
 }
 
+// The direct path's gather backprojection of a job's ratio into its bp: the plan's
+// own, or for oversampled bins those of its plans of one LOR per bin, in turn.
+hipError_t backproject_job(const BJob* j, int s, hipStream_t st) {
+  const apirl_plan* p = j->plan;
+  if (p->geo.ns * p->geo.nas == 1)
+    return launch_backproject_gather(p, j->ratio, j->bp, s, j->S, BP_PLAIN, APIRL_PREC_REFERENCE,
+                                     j->zscr, j->zscr_bytes, st);
+  for (const apirl_plan* h : p->os_parts) {
+    const hipError_t e = launch_backproject_gather(h, j->ratio, j->bp, s, j->S, BP_PLAIN,
+                                                   APIRL_PREC_REFERENCE, j->zscr, j->zscr_bytes,
+                                                   st);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 hipError_t enqueue_round(apirl_batch* b) {
   BatchHeader* hdr = (BatchHeader*)b->d_tab;
   hipError_t e;
@@ -231,7 +247,13 @@ size_t apirl_batch_job_bytes(const apirl_plan* p, int subsets) {
   const size_t nvox = apirl_plan_image_voxels(p);
   size_t bytes = (size_t)subsets * nvox * sizeof(float) + nvox * sizeof(float) +
                  max_subset_bins(p, subsets) * sizeof(float);
-  if (p->geo.dims == 3) bytes += max_gather_scratch(p, subsets);
+  if (p->geo.dims == 3) {
+    // oversampled bins: the largest gather scratch of the plans of one LOR per bin
+    // once they are built (the job's submission builds them), the plan's own before
+    size_t z = max_gather_scratch(p, subsets);
+    for (const apirl_plan* h : p->os_parts) z = std::max(z, max_gather_scratch(h, subsets));
+    bytes += z;
+  }
   return bytes;
 }
 
@@ -257,11 +279,19 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
     return APIRL_ERR_INVALID;
   apirl_plan* p = d->plan;
   if (p->ctx != b->ctx) return bfail(b, APIRL_ERR_INVALID, "plan belongs to another context");
-  if (!p->gather_ok)
-    return bfail(b, APIRL_ERR_UNSUPPORTED, "batched jobs use the gather backprojector");
   if (d->subsets > p->geo.nang) return bfail(b, APIRL_ERR_INVALID, "more subsets than angles");
-  if (const char* why = apirl::gather_misfit(p, 0, d->subsets))  // subset 0: most angles
-    return bfail(b, APIRL_ERR_UNSUPPORTED, std::string("gather: ") + why);
+  // Oversampled bins (ns x nas LORs): the job runs with the single-job kernels, its
+  // backprojection the gathers of the plans of one LOR per bin (apirl_plan::os_parts).
+  const bool os = p->geo.ns * p->geo.nas > 1;
+  if (os && !apirl::ensure_os_parts(p))
+    return bfail(b, APIRL_ERR_UNSUPPORTED, std::string("oversampled job: ") + p->ctx->err);
+  std::vector<apirl_plan*> gplans = os ? p->os_parts : std::vector<apirl_plan*>{p};
+  for (apirl_plan* q : gplans) {
+    if (!q->gather_ok)
+      return bfail(b, APIRL_ERR_UNSUPPORTED, "batched jobs use the gather backprojector");
+    if (const char* why = apirl::gather_misfit(q, 0, d->subsets))  // subset 0: most angles
+      return bfail(b, APIRL_ERR_UNSUPPORTED, std::string("gather: ") + why);
+  }
   if (p->geo.dims == 3 && p->geo.nsino > b->max_nsino)
     return bfail(b, APIRL_ERR_INVALID, "sinogram count above the batch's max_sinograms");
   if ((int)b->order.size() >= b->cap) return bfail(b, APIRL_ERR_INVALID, "batch is full");
@@ -281,7 +311,7 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
     delete j;
     return bfail(b, APIRL_ERR_NOMEM, std::string("job set-up failed: ") + p->ctx->err);
   }
-  if (!p->forms[j->form].gather) {
+  if (!os && !p->forms[j->form].gather) {
     delete j;
     return bfail(b, APIRL_ERR_UNSUPPORTED, "batched jobs use the gather backprojector");
   }
@@ -291,7 +321,9 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   const size_t sens_b = (size_t)j->S * j->nvox * sizeof(float);
   const size_t ratio_b = max_subset_bins(p, j->S) * sizeof(float);
   const size_t bp_b = j->nvox * sizeof(float);
-  const size_t z_b = j->is2d ? 0 : max_gather_scratch(p, j->S);
+  size_t z_b = 0;
+  if (!j->is2d)
+    for (apirl_plan* q : gplans) z_b = std::max(z_b, max_gather_scratch(q, j->S));
   auto get = [&](size_t n) -> void* {
     void* q = balloc(b, n);
     if (q) j->bufs.emplace_back(q, n);
@@ -302,7 +334,7 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   j->bp = (float*)get(bp_b);
   if (z_b) j->zscr = get(z_b);
   j->zscr_bytes = z_b;
-  j->large = !j->is2d && j->nvox >= kLargeVoxels;
+  j->large = (!j->is2d && j->nvox >= kLargeVoxels) || os;
   if (!j->sens || !j->ratio || !j->bp || (z_b && !j->zscr) ||
       hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming) != hipSuccess) {
     release_bufs(b, j);
@@ -512,9 +544,7 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
     if ((e = launch_project(j->plan, j->x, nullptr, s, j->S, APIRL_PREC_REFERENCE, &fr, nullptr,
                             b->stream)) != hipSuccess ||
         (e = hipMemsetAsync(j->bp, 0, j->nvox * sizeof(float), b->stream)) != hipSuccess ||
-        (e = launch_backproject_gather(j->plan, j->ratio, j->bp, s, j->S, BP_PLAIN,
-                                       APIRL_PREC_REFERENCE, j->zscr, j->zscr_bytes,
-                                       b->stream)) != hipSuccess ||
+        (e = backproject_job(j, s, b->stream)) != hipSuccess ||
         (e = launch_em_update(j->x, j->bp, j->sens + (size_t)s * j->nvox, j->nvox, j->thr[s],
                               j->strict, b->stream)) != hipSuccess)
       return bfail(b, APIRL_ERR_HIP, std::string("large-job launch failed: ") +

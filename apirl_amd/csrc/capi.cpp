@@ -175,6 +175,19 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     return APIRL_OK;
   }
   if (mode != APIRL_BP_GATHER) return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
+  if (p->geo.ns * p->geo.nas > 1) {
+    // every LOR of the oversampled bins (SiddonProjector.cpp:491-548): the gathers of
+    // the plans of one LOR per bin, added in turn into the image
+    if (precision != APIRL_PREC_REFERENCE)
+      return fail(c, APIRL_ERR_UNSUPPORTED, "oversampled LORs use the reference precision");
+    if (!apirl::ensure_os_parts(p)) return APIRL_ERR_UNSUPPORTED;
+    for (apirl_plan* h : p->os_parts) {
+      const apirl_status st = backproject_any(h, sino, est, img, subset, nsubsets, kind, mode,
+                                              precision);
+      if (st != APIRL_OK) return st;
+    }
+    return APIRL_OK;
+  }
   if (!p->gather_ok)
     return fail(c, APIRL_ERR_UNSUPPORTED,
                 "gather needs its tables (a radial grid it can search) and one LOR per bin "
@@ -325,8 +338,12 @@ apirl_status apirl_memset(apirl_ctx* ctx, void* dptr, int value, size_t bytes) {
   return APIRL_OK;
 }
 
-apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
-                               const apirl_sino_desc* sino, apirl_plan** out) {
+namespace {
+// apirl_plan_create; zab: the axial table to use instead of the one built from the
+// mean ring coordinates (the parts of an oversampled plan)
+apirl_status plan_create_impl(apirl_ctx* ctx, const apirl_image_desc* image,
+                              const apirl_sino_desc* sino, const std::vector<float2>* zab,
+                              apirl_plan** out) {
   if (!ctx || !image || !sino || !out) return APIRL_ERR_INVALID;
   *out = nullptr;
   const apirl_image_desc& im = *image;
@@ -385,7 +402,8 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
   if (sd.dims == 3) {
     p->z1m.assign(sd.z1_mean_mm, sd.z1_mean_mm + sd.nsino);
     p->z2m.assign(sd.z2_mean_mm, sd.z2_mean_mm + sd.nsino);
-    apirl::build_z_table(sd, p->zab);
+    if (zab) p->zab = *zab;
+    else apirl::build_z_table(sd, p->zab);
     if (hipMalloc(&p->d_zab, p->zab.size() * sizeof(float2)) != hipSuccess ||
         hipMemcpy(p->d_zab, p->zab.data(), p->zab.size() * sizeof(float2),
                   hipMemcpyHostToDevice) != hipSuccess) {
@@ -487,6 +505,66 @@ apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
   *out = p;
   return APIRL_OK;
 }
+}  // namespace
+
+apirl_status apirl_plan_create(apirl_ctx* ctx, const apirl_image_desc* image,
+                               const apirl_sino_desc* sino, apirl_plan** out) {
+  return plan_create_impl(ctx, image, sino, nullptr, out);
+}
+
+extern "C++" {
+namespace apirl {
+
+void free_os_parts(apirl_plan* p) {
+  for (apirl_plan* h : p->os_parts) apirl_plan_destroy(h);
+  p->os_parts.clear();
+}
+
+bool ensure_os_parts(apirl_plan* p) {
+  const int ns = p->geo.ns, nas = p->geo.nas;
+  if (ns * nas == 1 || !p->os_parts.empty()) return true;
+  apirl_ctx* c = p->ctx;
+  // the LORs of the oversampled tables (apirl_plan_set_oversampling): radial sub-LOR o
+  // of bin j at r_sub[j ns + o], axial sample q of sinogram s at zab[s nas + q]
+  std::vector<float> r_sub;
+  apirl::build_subsample_r(p->r, ns, r_sub);
+  std::vector<float2> zab_os;
+  apirl::build_z_table_os(p->z1m, p->z2m, p->sino.ring_width_mm, nas, zab_os);
+  const int nr = p->sino.nr, nsino = p->sino.nsino;
+  std::vector<float> r_o(nr), z1(nsino), z2(nsino);
+  std::vector<float2> zab_q(nsino);
+  for (int o = 0; o < ns; o++)
+    for (int q = 0; q < nas; q++) {
+      for (int j = 0; j < nr; j++) r_o[j] = r_sub[(size_t)j * ns + o];
+      for (int s = 0; s < nsino; s++) {
+        zab_q[s] = zab_os[(size_t)s * nas + q];
+        z1[s] = zab_q[s].x;
+        z2[s] = zab_q[s].y;
+      }
+      apirl_sino_desc sd = p->sino;
+      sd.ang_deg = p->ang.data();
+      sd.r_mm = r_o.data();
+      sd.z1_mean_mm = z1.data();
+      sd.z2_mean_mm = z2.data();
+      apirl_plan* h = nullptr;
+      if (plan_create_impl(c, &p->img, &sd, &zab_q, &h) != APIRL_OK || !h->gather_ok) {
+        if (h) {
+          apirl_plan_destroy(h);
+          fail(c, APIRL_ERR_UNSUPPORTED, "the gather cannot search the oversampled LORs' grid");
+        }
+        free_os_parts(p);
+        return false;
+      }
+      h->gather_zvox = p->gather_zvox;
+      h->gather_zlanes = p->gather_zlanes;
+      h->gather_xcd = p->gather_xcd;
+      p->os_parts.push_back(h);
+    }
+  return true;
+}
+
+}  // namespace apirl
+}  // extern "C++"
 
 apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (!p) return APIRL_OK;
@@ -494,6 +572,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   for (apirl::LorForm& f : p->forms) free_form(f);  // the active tables alias one
   if (p->d_zab) (void)hipFree(p->d_zab);
   if (p->d_replicas) (void)hipFree(p->d_replicas);
+  apirl::free_os_parts(p);
   if (p->d_angcs) (void)hipFree(p->d_angcs);
   if (p->d_zorder) (void)hipFree(p->d_zorder);
   if (p->d_imgT) (void)hipFree(p->d_imgT);
@@ -569,6 +648,7 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
     return fail(c, APIRL_ERR_HIP, "cannot upload the oversampled LOR tables");
   }
   APIRL_HIP(c, hipStreamSynchronize(c->stream));
+  apirl::free_os_parts(p);
   apirl::LorForm& f = p->forms[APIRL_SCANNER_CYLINDRICAL];
   (void)hipFree(f.rays);
   (void)hipFree(p->d_zab);

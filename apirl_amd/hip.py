@@ -302,7 +302,8 @@ class Plan:
 
     def set_oversampling(self, ns, nas=1):
         """Trace every bin as ns x nas LORs (3D; the reference's SiddonProjector with
-        numSamples / numAxialSamples).  Gather and batches need ns = nas = 1."""
+        numSamples / numAxialSamples), in the reference precision; the gather takes a
+        bin's ns x nas LORs in turn, batches run such jobs with the single-job kernels."""
         self.ctx.check(self.ctx.lib.apirl_plan_set_oversampling(self.h, int(ns), int(nas)),
                        "apirl_plan_set_oversampling")
 

@@ -144,9 +144,11 @@ typedef struct apirl_tuning {
  * recon/src/SiddonProjector.cpp:28-32, 366-425, 491-548): every bin of a 3D plan is
  * traced as ns transaxial x nas axial LORs (Sinogram2DinCylindrical3Dpet.cpp:283-299),
  * all of whose terms go to the bin.  ns = nas = 1 restores one LOR per bin.  The
- * projector, the scatter backprojector (plain, sensitivity, divide) and the EM driver
- * with APIRL_BP_SCATTER follow it; the gather backprojector and batches need one LOR
- * per bin, and oversampled projections use APIRL_PREC_REFERENCE. */
+ * projector, both backprojectors (plain, sensitivity, divide), the EM driver and
+ * batches follow it, in APIRL_PREC_REFERENCE.  The gather takes the ns x nas LORs of
+ * a bin as ns x nas plans of one LOR per bin (built on its first call, about the
+ * plan's own size each), gathered in turn into the image; batches run such jobs with
+ * the single-job kernels after each grouped round. */
 apirl_status apirl_plan_set_oversampling(apirl_plan* plan, int ns, int nas);
 
 /* Scanner geometry of a 3D plan's LOR endpoints.  APIRL_SCANNER_CYLINDRICAL: the

@@ -38,9 +38,8 @@ class HipSiddonProjector : virtual Projector {
   HipSiddonProjector(int gpuId = 0, int backprojectionMode = APIRL_BP_GATHER);
   /* SiddonProjector(nSamplesOnDetector, nAxialSamplesOnDetector)
    * (recon/src/SiddonProjector.cpp:28-32): 3D bins of a cylindrical scanner traced as
-   * numSamples x numAxialSamples LORs (apirl_plan_set_oversampling).  Oversampled
-   * bins backproject with APIRL_BP_SCATTER only (the gather takes one LOR per bin);
-   * a gather-mode projector then fails those calls with that message. */
+   * numSamples x numAxialSamples LORs (apirl_plan_set_oversampling), in either
+   * backprojection mode (the gather takes a bin's LORs in turn). */
   HipSiddonProjector(int gpuId, int backprojectionMode, int numSamples, int numAxialSamples);
   virtual ~HipSiddonProjector();
 

@@ -189,6 +189,25 @@ def test_batch_with_mmr_job_equals_alone(torch, ctx):
         same_bits(res[k], ref[k])
 
 
+def test_batch_with_oversampled_job_equals_alone(torch, ctx):
+    """A job with 2 x 2 LORs per bin (apirl_plan_set_oversampling; it runs with the
+    single-job kernels, its gather over the plans of one LOR per bin) beside a plain 3D
+    job: each equals, bit for bit, the job alone with the gather backprojector."""
+    log("case: batch with an oversampled job beside a plain 3D job")
+    g = GEOMS["3d_tiny"]
+    os_plan = make_plan(ctx, g)
+    os_plan.set_oversampling(2, 2)
+    plans = {"os": os_plan, "3d_tiny": make_plan(ctx, g)}
+    jobs = [("os", make_job(torch, os_plan, g, 41, 3, 2)),
+            ("3d_tiny", make_job(torch, plans["3d_tiny"], g, 42, 2, 2))]
+    ref = [alone(plans[n], j) for n, j in jobs]
+    torch.cuda.synchronize()
+    assert np.nanmax(ref[0]) > 0
+    res = run_batch(torch, ctx, plans, jobs)
+    for k in range(len(jobs)):
+        same_bits(res[k], ref[k])
+
+
 def c3_specs():
     """One job of every geometry of the batched-jobs workload (2D 128x128 / 192x192,
     2D 256x256 / 344x252, 3D 128x128x47 / 192x192x233, 3D 256x256x81 /

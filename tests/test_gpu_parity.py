@@ -360,8 +360,9 @@ def test_mid3d_osem_gather_deterministic_and_close_to_reference(torch, ctx, gold
 def test_oversampled_projector_vs_reference(torch, ctx, golden):
     """Three radial x two axial LORs per bin (apirl_plan_set_oversampling): the
     projection is bit-exact against the reference's SiddonProjector(3, 2), the scatter
-    backprojection and DivideAndBackproject within the per-voxel bar; the gather
-    refuses; one LOR per bin restores the plain projector bit for bit."""
+    and gather backprojections and DivideAndBackproject within the per-voxel bar (the
+    gather run-to-run bit-identical: the six LORs of a bin gathered in turn); one LOR
+    per bin restores the plain projector bit for bit."""
     log("case: oversampled projector 3 x 2, mid3d")
     f = golden("oversampled.npz")
     g = json.loads(str(f["geom"]))
@@ -380,16 +381,27 @@ def test_oversampled_projector_vs_reference(torch, ctx, golden):
     assert_bitexact(host(proj).ravel(), f["proj"])
     assert_voxel_close(host(back), f["back"])
     assert_voxel_close(host(div), f["divback"])
-    with pytest.raises(hip.ApirlError, match="one LOR per bin"):
-        p.backproject(meas, mode=hip.BP_GATHER)
+    back_g = host(p.backproject(meas, mode=hip.BP_GATHER))
+    assert np.array_equal(back_g.view(np.int32),
+                          host(p.backproject(meas, mode=hip.BP_GATHER)).view(np.int32))
+    assert_voxel_close(back_g, f["back"])
+    assert_voxel_close(host(p.divide_backproject(dmeas, dest, mode=hip.BP_GATHER)),
+                       f["divback"])
+    with pytest.raises(hip.ApirlError, match="reference precision"):
+        p.backproject(meas, mode=hip.BP_GATHER, precision=hip.PREC_KAHAN)
     with pytest.raises(hip.ApirlError, match="reference precision"):
         p.project(dev(torch, m["img"]), precision=hip.PREC_KAHAN)
-    # OSEM with the oversampled projector and the scatter backprojector runs end to end
+    # OSEM with the oversampled projector runs end to end with either backprojector;
+    # the two differ only in the order of the float additions per voxel
     init = torch.ones(p.image_shape(), dtype=torch.float32, device="cuda")
     img, lik = p.reconstruct(meas, init, 1, subsets=7, bp_mode=hip.BP_SCATTER)
+    img_g, lik_g = p.reconstruct(meas, init, 1, subsets=7, bp_mode=hip.BP_GATHER)
     torch.cuda.synchronize()
-    out = host(img)
+    out, out_g = host(img), host(img_g)
     assert np.isfinite(lik).all() and np.nanmax(out) > 0
+    assert np.array_equal(np.isnan(out), np.isnan(out_g))
+    fin = np.isfinite(out)
+    assert np.linalg.norm(out_g[fin] - out[fin]) <= 1e-5 * np.linalg.norm(out[fin])
     p.set_oversampling(1, 1)
     plain = p.project(dev(torch, m["img"]))
     torch.cuda.synchronize()
