@@ -328,6 +328,8 @@ struct BatchHeader {
   long total[8];                // virtual blocks of each phase (BatchPhase)
   int count[8];                 // jobs in each phase's table
   unsigned long long next[8];   // work counters (uploaded as 0 with every round's table)
+  int grab;                     // virtual blocks a workgroup takes per counter draw (>= 1)
+  int pad;
 };
 enum BatchPhase { PH_CLEAR = 0, PH_PROJ2, PH_PROJ3, PH_ZSETUP, PH_GATHER2, PH_GATHER3, PH_UPDATE };
 

@@ -67,6 +67,8 @@ struct apirl_batch {
   int cap = 0, max_nsino = 0;
   int grid = 2048;
   bool use_graph = true;
+  int grab = 2;                // virtual blocks per counter draw (apirl_batch_set_schedule)
+  int order_mode = APIRL_BATCH_ORDER_SUBMISSION;
   std::vector<BJob*> jobs;         // by id
   std::vector<int> order;          // active job ids in admission order
   // job tables: one device copy, two pinned staging copies
@@ -273,6 +275,17 @@ apirl_status apirl_batch_set_launch(apirl_batch* b, int grid, int use_graph) {
   return APIRL_OK;
 }
 
+apirl_status apirl_batch_set_schedule(apirl_batch* b, int grab, int order) {
+  if (!b) return APIRL_ERR_INVALID;
+  if (grab < 1 || grab > 64) return bfail(b, APIRL_ERR_INVALID, "grab must be in 1..64");
+  if (order != APIRL_BATCH_ORDER_SUBMISSION && order != APIRL_BATCH_ORDER_LARGEST_FIRST)
+    return bfail(b, APIRL_ERR_INVALID, "unknown job order");
+  // both are read from the round's table, so a captured round needs no new capture
+  b->grab = grab;
+  b->order_mode = order;
+  return APIRL_OK;
+}
+
 apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* job_id) {
   if (!b || !d || !job_id || !d->plan || !d->d_meas || !d->d_init || !d->d_out ||
       d->iterations < 1 || d->subsets < 1)
@@ -396,6 +409,7 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
     return bfail(b, APIRL_ERR_HIP, "table staging wait failed");
   BatchHeader* hdr = (BatchHeader*)h;
   std::memset(hdr, 0, sizeof(BatchHeader));
+  hdr->grab = b->grab;
   ClearJob* cl = (ClearJob*)(h + b->off_clear);
   ProjJob* p2 = (ProjJob*)(h + b->off_p2);
   ProjJob* p3 = (ProjJob*)(h + b->off_p3);
@@ -405,7 +419,21 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
   UpdJob* up = (UpdJob*)(h + b->off_upd);
   std::vector<BJob*> finishing;
   std::vector<std::pair<BJob*, int>> direct;  // large jobs and their subset this round
-  for (int id : b->order) {
+  // The jobs' virtual blocks are laid out, and so drawn, in this order: submission,
+  // or the round's largest sub-iteration first (rays traced plus voxels) so that the
+  // small jobs fill the end of each phase.  Results do not depend on it.
+  std::vector<int> round_order(b->order);
+  if (b->order_mode == APIRL_BATCH_ORDER_LARGEST_FIRST) {
+    auto work = [&](int id) {
+      const BJob* j = b->jobs[id];
+      const Geo& g = j->plan->geo;
+      const double nrays = (double)g.nsino * subset_num_angles(g.nang, j->S, j->s) * g.nr;
+      return nrays * (g.nx + g.ny + (j->is2d ? 0 : g.nz)) + (double)j->nvox;
+    };
+    std::stable_sort(round_order.begin(), round_order.end(),
+                     [&](int x, int y) { return work(x) > work(y); });
+  }
+  for (int id : round_order) {
     BJob* j = b->jobs[id];
     apirl::ActiveForm form_scope(j->plan, j->form);
     const apirl_plan* p = j->plan;

@@ -786,10 +786,11 @@ __global__ __launch_bounds__(kGBlock) void zsetup_group_kernel(BatchHeader* __re
                                                                const ZsJob* __restrict__ jobs) {
   const int n = hdr->count[PH_ZSETUP];
   const long total = hdr->total[PH_ZSETUP];
+  const int grab = grab_size(hdr);
   for (;;) {
-    const long v0 = grab_vb(&hdr->next[PH_ZSETUP]);
+    const long v0 = grab_vb(&hdr->next[PH_ZSETUP], grab);
     if (v0 >= total) break;
-    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+    for (long vb = v0; vb < v0 + grab && vb < total; vb++) {
       const ZsJob& J = jobs[find_job(jobs, n, vb)];
       const long idx = (vb - J.first) * kGBlock + threadIdx.x;
       if (idx < J.nrays)
@@ -809,11 +810,12 @@ __global__ __launch_bounds__(kGBlock) void gather_group_kernel(BatchHeader* __re
   const int ph = DIMS == 2 ? PH_GATHER2 : PH_GATHER3;
   const int n = hdr->count[ph];
   const long total = hdr->total[ph];
+  const int grab = grab_size(hdr);
   int loaded = -1;
   for (;;) {
-    const long v0 = grab_vb(&hdr->next[ph]);
+    const long v0 = grab_vb(&hdr->next[ph], grab);
     if (v0 >= total) break;
-    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+    for (long vb = v0; vb < v0 + grab && vb < total; vb++) {
       const int jj = find_job(jobs, n, vb);
       const GJob& J = jobs[jj];
       const int ns = J.A.g.nsino;

@@ -4,14 +4,17 @@
 #ifndef APIRL_GROUP_COMMON_H
 #define APIRL_GROUP_COMMON_H
 
-// Virtual blocks are handed out by a per-phase counter, kGrab at a time, so a
-// workgroup that drew cheap tiles takes more (gather tiles differ ~10x in cost).
-// Returns the first virtual block of the grab (>= total: no work left).
-constexpr int kGrab = 2;
-__device__ __forceinline__ long grab_vb(unsigned long long* ctr) {
+// Virtual blocks are handed out by a per-phase counter, `grab` at a time (the round
+// header's, tuned per batch; 1 if unset), so a workgroup that drew cheap tiles takes
+// more (gather tiles differ ~10x in cost).  Returns the first virtual block of the
+// grab (>= total: no work left).
+__device__ __forceinline__ int grab_size(const BatchHeader* hdr) {
+  return hdr->grab > 0 ? hdr->grab : 1;
+}
+__device__ __forceinline__ long grab_vb(unsigned long long* ctr, int grab) {
   __shared__ long s_vb;
   __syncthreads();  // every lane has read the previous grab
-  if (threadIdx.x == 0) s_vb = (long)atomicAdd(ctr, (unsigned long long)kGrab);
+  if (threadIdx.x == 0) s_vb = (long)atomicAdd(ctr, (unsigned long long)grab);
   __syncthreads();
   return s_vb;
 }

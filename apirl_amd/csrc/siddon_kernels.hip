@@ -809,10 +809,11 @@ __global__ __launch_bounds__(kBlock) void clear_group_kernel(BatchHeader* __rest
                                                              const ClearJob* __restrict__ jobs) {
   const int n = hdr->count[PH_CLEAR];
   const long total = hdr->total[PH_CLEAR];
+  const int grab = grab_size(hdr);
   for (;;) {
-    const long v0 = grab_vb(&hdr->next[PH_CLEAR]);
+    const long v0 = grab_vb(&hdr->next[PH_CLEAR], grab);
     if (v0 >= total) break;
-    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+    for (long vb = v0; vb < v0 + grab && vb < total; vb++) {
       const ClearJob& J = jobs[find_job(jobs, n, vb)];
       const long i = (vb - J.first) * kBlock + threadIdx.x;
       if (i < J.n) J.p[i] = 0.f;
@@ -826,10 +827,11 @@ __global__ __launch_bounds__(kBlock) void project_group_kernel(BatchHeader* __re
   const int ph = DIMS == 2 ? PH_PROJ2 : PH_PROJ3;
   const int n = hdr->count[ph];
   const long total = hdr->total[ph];
+  const int grab = grab_size(hdr);
   for (;;) {
-    const long v0 = grab_vb(&hdr->next[ph]);
+    const long v0 = grab_vb(&hdr->next[ph], grab);
     if (v0 >= total) break;
-    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+    for (long vb = v0; vb < v0 + grab && vb < total; vb++) {
       const ProjJob& J = jobs[find_job(jobs, n, vb)];
       const long lidx = (vb - J.first) * kBlock + threadIdx.x;
       (void)project_ray<APIRL_PREC_REFERENCE, true, DIMS>(J.g, J.rays, J.zab, J.img, nullptr,
@@ -843,10 +845,11 @@ __global__ __launch_bounds__(kBlock) void update_group_kernel(BatchHeader* __res
                                                               const UpdJob* __restrict__ jobs) {
   const int n = hdr->count[PH_UPDATE];
   const long total = hdr->total[PH_UPDATE];
+  const int grab = grab_size(hdr);
   for (;;) {
-    const long v0 = grab_vb(&hdr->next[PH_UPDATE]);
+    const long v0 = grab_vb(&hdr->next[PH_UPDATE], grab);
     if (v0 >= total) break;
-    for (long vb = v0; vb < v0 + kGrab && vb < total; vb++) {
+    for (long vb = v0; vb < v0 + grab && vb < total; vb++) {
       const UpdJob& J = jobs[find_job(jobs, n, vb)];
       const long i = (vb - J.first) * kBlock + threadIdx.x;
       if (i < J.n) {

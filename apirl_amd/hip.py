@@ -102,6 +102,7 @@ SIGNATURES = {
     "apirl_batch_last_error": (ctypes.c_char_p, [_P]),
     "apirl_batch_job_bytes": (_c_size, [_P, _c_int]),
     "apirl_batch_set_launch": (_c_int, [_P, _c_int, _c_int]),
+    "apirl_batch_set_schedule": (_c_int, [_P, _c_int, _c_int]),
     "apirl_batch_submit": (_c_int, [_P, ctypes.POINTER(JobDesc), ctypes.POINTER(_c_int)]),
     "apirl_batch_step": (_c_int, [_P, ctypes.POINTER(_c_int)]),
     "apirl_batch_job_done": (_c_int, [_P, _c_int, ctypes.POINTER(_c_int)]),
@@ -499,6 +500,14 @@ class Batch:
     def set_launch(self, grid=0, use_graph=True):
         self._check(self.lib.apirl_batch_set_launch(self.h, grid, 1 if use_graph else 0),
                     "apirl_batch_set_launch")
+
+    ORDER_SUBMISSION, ORDER_LARGEST_FIRST = 0, 1
+
+    def set_schedule(self, grab=2, order=0):
+        """Virtual blocks per work-counter draw (1..64) and the jobs' order in a round
+        (ORDER_SUBMISSION or ORDER_LARGEST_FIRST); results do not depend on either."""
+        self._check(self.lib.apirl_batch_set_schedule(self.h, int(grab), int(order)),
+                    "apirl_batch_set_schedule")
 
     def submit(self, plan, meas, init, out, iterations, subsets=1, osem_rules=None,
                additive=None, mult=None, additive_own_angles=False):

@@ -3,9 +3,11 @@
 * run_sequential: the plain single-job path, one apirl_em_reconstruct per job.
 * run_batched: all jobs submitted to one batch (up to its capacity, later ones as
   slots free up), rounds stepped until every job is done.
-* autotune: picks the batch's persistent grid (workgroups per grouped launch) and
-  graph replay on or off by timing real rounds of the batch it tunes -- results do
-  not depend on either, so the rounds spent tuning are not wasted.
+* autotune: picks the batch's persistent grid (workgroups per grouped launch), graph
+  replay on or off, the virtual blocks (256 rays / voxels, or one gather tile) a
+  workgroup draws at a time and the jobs' order in a round, by timing real rounds of
+  the batch it tunes -- results do not depend on any of them, so the rounds spent
+  tuning are not wasted.
 * serve: open-loop Poisson arrivals; a job is admitted when the batch has a slot and
   its device state (apirl_batch_job_bytes) fits the HBM budget, joins the next
   round, and leaves when done; latency = completion - arrival."""
@@ -30,27 +32,45 @@ def run_sequential(ctx, plans, jobs, torch):
     return time.perf_counter() - t0
 
 
-def autotune(batch, grids=(512, 1024, 2048, 4096), rounds=2):
-    """Times `rounds` rounds per candidate on the live batch; returns the choice."""
-    best = None
+def autotune(batch, grids=(512, 1024, 2048, 4096), grabs=(1, 2, 4, 8),
+             orders=(hip.Batch.ORDER_SUBMISSION, hip.Batch.ORDER_LARGEST_FIRST), rounds=2):
+    """Times `rounds` rounds per candidate on the live batch, one knob after the other
+    (grid and graph replay, then the grab size, then the job order), each keeping the
+    best so far; returns the choice."""
+    state = {"grid": grids[0], "graph": True, "grab": 2, "order": orders[0]}
+    best = [None]
+    done = [False]
+
+    def trial(**kw):
+        cand = dict(state, **kw)
+        batch.set_launch(cand["grid"], cand["graph"])
+        batch.set_schedule(cand["grab"], cand["order"])
+        batch.synchronize()
+        t0 = time.perf_counter()
+        active = 1
+        for _ in range(rounds):
+            active = batch.step()
+        batch.synchronize()
+        dt = (time.perf_counter() - t0) / rounds
+        if best[0] is None or dt < best[0]:
+            best[0] = dt
+            state.update(cand)
+        done[0] = active == 0
+
     for use_graph in (True, False):
         for grid in grids:
-            batch.set_launch(grid, use_graph)
-            batch.synchronize()
-            t0 = time.perf_counter()
-            active = 1
-            for _ in range(rounds):
-                active = batch.step()
-            batch.synchronize()
-            dt = (time.perf_counter() - t0) / rounds
-            if best is None or dt < best[0]:
-                best = (dt, grid, use_graph)
-            if active == 0:
-                break
-        if active == 0:
-            break
-    batch.set_launch(best[1], best[2])
-    return {"grid": best[1], "graph": best[2], "round_s": best[0]}
+            if not done[0]:
+                trial(grid=grid, graph=use_graph)
+    for grab in grabs:
+        if not done[0]:
+            trial(grab=grab)
+    for order in orders:
+        if not done[0]:
+            trial(order=order)
+    batch.set_launch(state["grid"], state["graph"])
+    batch.set_schedule(state["grab"], state["order"])
+    return {"grid": state["grid"], "graph": state["graph"], "grab": state["grab"],
+            "order": state["order"], "round_s": best[0]}
 
 
 def run_batched(ctx, plans, jobs, torch, max_jobs=64, tune=True):

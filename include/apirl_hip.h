@@ -321,6 +321,14 @@ const char* apirl_batch_last_error(const apirl_batch* batch);
 size_t apirl_batch_job_bytes(const apirl_plan* plan, int subsets);
 /* Workgroups per grouped launch (0 keeps the current value) and graph replay on/off. */
 apirl_status apirl_batch_set_launch(apirl_batch* batch, int grid, int use_graph);
+/* Scheduling of the grouped launches (no reference counterpart: the batch engine is
+ * this library's own): `grab` virtual blocks (256 rays, 256 voxels or one gather
+ * tile each) per draw from a phase's work counter (1..64, default 2), and the order
+ * in which the jobs' blocks are laid out in a round.  Results do not depend on
+ * either; serving.autotune picks them by timing live rounds. */
+#define APIRL_BATCH_ORDER_SUBMISSION 0
+#define APIRL_BATCH_ORDER_LARGEST_FIRST 1
+apirl_status apirl_batch_set_schedule(apirl_batch* batch, int grab, int order);
 apirl_status apirl_batch_submit(apirl_batch* batch, const apirl_job_desc* job, int* job_id);
 /* Enqueues one round; *active = jobs still to be scheduled after it. */
 apirl_status apirl_batch_step(apirl_batch* batch, int* active);

@@ -68,9 +68,11 @@ def mix(torch, ctx):
     return plans, jobs, ref
 
 
-def run_batch(torch, ctx, plans, jobs, late=(), use_graph=True, grid=0):
+def run_batch(torch, ctx, plans, jobs, late=(), use_graph=True, grid=0, schedule=None):
     b = hip.Batch(ctx, max_jobs=16, max_sinograms=1024)
     b.set_launch(grid, use_graph)
+    if schedule is not None:
+        b.set_schedule(*schedule)
     outs, ids = {}, {}
     for k, (name, j) in enumerate(jobs):
         if k in late:
@@ -120,6 +122,18 @@ def test_batch_graph_and_direct_launches_agree(torch, ctx, mix):
     log("case: direct launches, small persistent grid")
     plans, jobs, ref = mix
     res = run_batch(torch, ctx, plans, jobs, use_graph=False, grid=96)
+    for k in range(len(jobs)):
+        same_bits(res[k], ref[k])
+
+
+@pytest.mark.parametrize("schedule", [(1, hip.Batch.ORDER_LARGEST_FIRST),
+                                      (8, hip.Batch.ORDER_SUBMISSION)])
+def test_batch_schedules_agree(torch, ctx, mix, schedule):
+    """The autotuner's scheduling knobs (virtual blocks per draw, the jobs' order in
+    a round) move work between workgroups only: every image equals the job alone."""
+    log(f"case: batch schedule grab={schedule[0]} order={schedule[1]}")
+    plans, jobs, ref = mix
+    res = run_batch(torch, ctx, plans, jobs, schedule=schedule)
     for k in range(len(jobs)):
         same_bits(res[k], ref[k])
 
