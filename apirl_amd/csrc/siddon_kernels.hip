@@ -193,10 +193,12 @@ __device__ __forceinline__ void walk3d_batch_inside(const Geo& g, Ray& ray,
 #pragma unroll
   for (int u = 0; u < kUnroll; u++) {
     const int ci = r.i, cj = r.j, ck = r.k;
-    const bool bx = (r.ax <= r.ay) && (r.ax <= r.az);
-    const bool by = !bx && (r.ay <= r.az);
-    const bool bz = !bx && !by;
-    const float an = bx ? r.ax : (by ? r.ay : r.az);
+    // the step's axis, from three comparisons (the tested form's rule: x, then y, then z)
+    const bool le_xy = r.ax <= r.ay, le_xz = r.ax <= r.az, le_yz = r.ay <= r.az;
+    const bool bx = le_xy & le_xz;
+    const bool by = !bx & le_yz;
+    const bool bz = !bx & !le_yz;
+    const float an = bx ? r.ax : (le_yz ? r.ay : r.az);
     wv[u] = (an - r.ac) * r.L;
     r.ac = an;
     const float nax = r.ax + r.axu, nay = r.ay + r.ayu, naz = r.az + r.azu;
@@ -246,8 +248,10 @@ template <bool TR>
 __device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
                                         const float* __restrict__ imgT, float acc = 0.f) {
   const int nxy = g.nx * g.ny;
-  for (int m0 = 0; m0 < ray.n; m0 += kUnroll)
+  int m0 = 0;
+  for (; m0 + kUnroll <= ray.n; m0 += kUnroll)
     walk3d_batch_inside<TR>(g, ray, img, imgT, nxy, m0, acc);
+  if (m0 < ray.n) walk3d_batch<TR, true>(g, ray, img, imgT, nxy, m0, acc);
   return acc;
 }
 
