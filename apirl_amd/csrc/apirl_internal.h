@@ -232,9 +232,17 @@ hipError_t launch_min_nz_max(const float* x, size_t n, float* d_out2, hipStream_
 hipError_t launch_count_intersections(const apirl_plan* p, int subset, int nsubsets,
                                       unsigned long long* d_out, hipStream_t st);
 
+// The EM update a gather can apply to its voxels instead of storing their
+// backprojection (GatherArgs::upd_sens): img is then the image being updated.
+struct EmFuse {
+  const float* sens;
+  float thr;
+  int strict;
+};
 hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, float* img,
                                      int subset, int nsubsets, int kind, int precision,
-                                     void* scratch, size_t scratch_bytes, hipStream_t st);
+                                     void* scratch, size_t scratch_bytes, hipStream_t st,
+                                     const EmFuse* fuse = nullptr);
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 // The 3D gather's 32-bit record and table offsets hold for this subset (fewer than
 // 2^31 subset bins; angle x radial bins, sinograms and crossing-table slots below 2^24).
@@ -322,6 +330,12 @@ struct GatherArgs {
   // takes tile tile_map[b] = tx + ntx (ty + nty tz), or nothing when it is -1.
   const int* tile_map;
   int tile_slots;
+  // EM update fused into the gather (null: img accumulates the backprojection): img is
+  // then the image x, and each voxel's backprojection b, summed from zero, updates it
+  // to (s > thr or, not strict, s >= thr) ? (x b) / s : 0 with s = upd_sens[v]
+  const float* upd_sens;
+  float upd_thr;
+  int upd_strict;
 };
 
 struct BatchHeader {

@@ -570,8 +570,20 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
     fr.lik_partials = nullptr;
     fr.lik_positive_only = 0;
     if ((e = launch_project(j->plan, j->x, nullptr, s, j->S, APIRL_PREC_REFERENCE, &fr, nullptr,
-                            b->stream)) != hipSuccess ||
-        (e = hipMemsetAsync(j->bp, 0, j->nvox * sizeof(float), b->stream)) != hipSuccess ||
+                            b->stream)) != hipSuccess)
+      return bfail(b, APIRL_ERR_HIP, std::string("large-job launch failed: ") +
+                                         hipGetErrorString(e));
+    if (j->plan->geo.ns * j->plan->geo.nas == 1) {
+      // the update fused into the gather, as apirl_em_reconstruct runs it
+      const EmFuse fz{j->sens + (size_t)s * j->nvox, j->thr[s], j->strict};
+      if ((e = launch_backproject_gather(j->plan, j->ratio, j->x, s, j->S, BP_PLAIN,
+                                         APIRL_PREC_REFERENCE, j->zscr, j->zscr_bytes, b->stream,
+                                         &fz)) != hipSuccess)
+        return bfail(b, APIRL_ERR_HIP, std::string("large-job launch failed: ") +
+                                           hipGetErrorString(e));
+      continue;
+    }
+    if ((e = hipMemsetAsync(j->bp, 0, j->nvox * sizeof(float), b->stream)) != hipSuccess ||
         (e = backproject_job(j, s, b->stream)) != hipSuccess ||
         (e = launch_em_update(j->x, j->bp, j->sens + (size_t)s * j->nvox, j->nvox, j->thr[s],
                               j->strict, b->stream)) != hipSuccess)

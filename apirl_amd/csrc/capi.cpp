@@ -146,9 +146,12 @@ bool os_precision_ok(const apirl_plan* p, int precision) {
   return p->geo.ns * p->geo.nas == 1 || (precision == APIRL_PREC_REFERENCE && p->geo.walk24);
 }
 
+// fuse (gather form, one LOR per bin): img is the EM image, updated by each voxel's
+// backprojection instead of accumulating it
 apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est, float* img,
                              int subset, int nsubsets, int kind, int mode,
-                             int precision = APIRL_PREC_REFERENCE) {
+                             int precision = APIRL_PREC_REFERENCE,
+                             const apirl::EmFuse* fuse = nullptr) {
   apirl_ctx* c = p->ctx;
   if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_SETUP)
     return fail(c, APIRL_ERR_INVALID, "invalid precision");
@@ -175,6 +178,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     return APIRL_OK;
   }
   if (mode != APIRL_BP_GATHER) return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
+  if (fuse && (mode != APIRL_BP_GATHER || p->geo.ns * p->geo.nas > 1))
+    return fail(c, APIRL_ERR_INVALID, "the fused update needs the gather of one LOR per bin");
   if (p->geo.ns * p->geo.nas > 1) {
     // every LOR of the oversampled bins (SiddonProjector.cpp:491-548): the gathers of
     // the plans of one LOR per bin, added in turn into the image
@@ -230,7 +235,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
   if (!ensure_tile_map(p))
     return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
-                                                p->d_gscratch, p->gscratch_bytes, c->stream));
+                                                p->d_gscratch, p->gscratch_bytes, c->stream,
+                                                fuse));
   return APIRL_OK;
 }
 
@@ -555,9 +561,15 @@ bool ensure_os_parts(apirl_plan* p) {
         free_os_parts(p);
         return false;
       }
+      // a part only gathers: the forward projector's transposed image is not needed
+      if (h->d_imgT) {
+        (void)hipFree(h->d_imgT);
+        h->d_imgT = nullptr;
+      }
       h->gather_zvox = p->gather_zvox;
       h->gather_zlanes = p->gather_zlanes;
       h->gather_xcd = p->gather_xcd;
+      (void)ensure_tile_map(h);
       p->os_parts.push_back(h);
     }
   return true;
@@ -705,6 +717,13 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
   DeviceGuard g(p->ctx->device);
   if (!ensure_tile_map(p))
     return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
+  for (apirl_plan* h : p->os_parts) {  // the oversampled plan's gathers follow it
+    h->gather_zlanes = p->gather_zlanes;
+    h->gather_zvox = p->gather_zvox;
+    h->gather_xcd = p->gather_xcd;
+    if (!ensure_tile_map(h))
+      return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
+  }
   return APIRL_OK;
 }
 
@@ -733,6 +752,18 @@ apirl_status apirl_backproject(apirl_plan* p, const float* d_sino, float* d_img,
   DeviceGuard g(p->ctx->device);
   return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN, mode,
                          precision);
+}
+
+apirl_status apirl_backproject_update(apirl_plan* p, const float* d_sino, float* d_img,
+                                      const float* d_sens, float thr, int strict, int subset,
+                                      int nsubsets, int precision) {
+  if (!p || !d_img || !d_sino || !d_sens) return APIRL_ERR_INVALID;
+  if (!check_subset(p, subset, nsubsets))
+    return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
+  DeviceGuard g(p->ctx->device);
+  const apirl::EmFuse fz{d_sens, thr, strict ? 1 : 0};
+  return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN,
+                         APIRL_BP_GATHER, precision, &fz);
 }
 
 apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const float* d_est,
@@ -958,6 +989,9 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
   // MLEM: 2D updates with s > thr (Mlem2d.cpp:175), 3D with s >= thr
   // (MlemSinogram3d.cpp:231); OSEM: threshold 0 with >= (OsemSinogram3d.cpp:28,221).
   const int strict = (!opt->osem_rules && is2d) ? 1 : 0;
+  // With the gather (one LOR per bin) the update is fused into the backprojection: the
+  // same float backprojection per voxel, without storing it (bit-identical images).
+  const bool fuse_update = opt->bp_mode == APIRL_BP_GATHER && p->geo.ns * p->geo.nas == 1;
   // Likelihood rule (2D: est > 0) and the additive sinogram's angle per subset bin:
   // the reference adds angle k of the full additive sinogram to subset bin k
   // (OsemSinogram3d.cpp:172-173, Sinogram3D.cpp:857-873).
@@ -990,6 +1024,14 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
                                     d_lik + (size_t)t * S + s, s, S, ratio_flags,
                                     opt->precision));
       mark(q + 1);
+      if (fuse_update) {
+        // the gather updates each voxel of the image from its backprojection
+        const apirl::EmFuse fz{d_sens + (size_t)s * nvox, thr[s], strict};
+        APIRL_TRY(backproject_any(p, d_ratio, nullptr, d_img, s, S, apirl::BP_PLAIN,
+                                  APIRL_BP_GATHER, bp_prec, &fz));
+        mark(q + 2);
+        continue;
+      }
       if (hipMemsetAsync(d_bp, 0, nvox * sizeof(float), c->stream) != hipSuccess) {
         cleanup();
         return fail(c, APIRL_ERR_HIP, "memset failed");

@@ -306,6 +306,19 @@ __device__ __forceinline__ bool in_parity_range(int rr, int lo0, int hi0, int lo
   return (rr & 1) ? (rr >= lo1 && rr <= hi1) : (rr >= lo0 && rr <= hi0);
 }
 
+// A voxel's backprojection b: stored, or with the fused EM update
+// (MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237, Mlem2d.cpp:171-185) applied
+// to the image voxel it belongs to, as em_update_kernel computes it.
+__device__ __forceinline__ void store_voxel(const GatherArgs& A, long v, float b) {
+  if (!A.upd_sens) {
+    A.img[v] = b;
+    return;
+  }
+  const float sv = A.upd_sens[v];
+  const bool ok = A.upd_strict ? (sv > A.upd_thr) : (sv >= A.upd_thr);
+  A.img[v] = ok ? A.img[v] * b / sv : 0.f;
+}
+
 // Gather of one voxel (i, j, k): adds its terms to A.img[v] in the fixed order
 // (angle, radial bin, segment, sinogram).  s_zab is the axial table in LDS (3D).
 template <int DIMS, int PREC>
@@ -323,7 +336,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
   // Accumulator: float in the reference order (REFERENCE), float with a Kahan
   // compensation term (KAHAN), or double (F64_ALPHA).
   Acc<PREC> acc;
-  acc.init(A.img[v]);
+  acc.init(A.upd_sens ? 0.f : A.img[v]);
   for (int ka = 0; ka < A.nang_sub; ka++) {
     const int a = A.subset + A.nsubsets * ka;
     int lo0, hi0, lo1, hi1;
@@ -519,7 +532,7 @@ __device__ __forceinline__ void gather_voxel(const GatherArgs& A, const float2* 
       }
     }
   }
-  A.img[v] = acc.value();
+  store_voxel(A, v, acc.value());
 }
 
 // One 32-byte record as two 16-byte loads.
@@ -580,7 +593,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
     const float zk0 = g.offz + k * g.dz, zk1 = zk0 + g.dz;
     zlo_t[q] = zk0 - kZMargin;
     zhi_t[q] = zk1 + kZMargin;
-    acc[q].init(q < nk ? A.img[v0 + q * nxy] : 0.f);
+    acc[q].init(q < nk && !A.upd_sens ? A.img[v0 + q * nxy] : 0.f);
   }
   const float zlo_all = zlo_t[0];
   float zhi_all = zhi_t[0];
@@ -700,7 +713,7 @@ __device__ __forceinline__ void gather_column(const GatherArgs& A, const float2*
   }
 #pragma unroll
   for (int q = 0; q < ZV; q++)
-    if (q < nk) A.img[v0 + q * nxy] = acc[q].value();
+    if (q < nk) store_voxel(A, v0 + q * nxy, acc[q].value());
 }
 
 // Eight waves per SIMD for up to three voxels per lane: the kernel waits mostly on
@@ -903,7 +916,8 @@ void launch_gather2(const GatherArgs& A, hipStream_t st) {
 
 hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, float* img,
                                      int subset, int nsubsets, int kind, int precision,
-                                     void* scratch, size_t scratch_bytes, hipStream_t st) {
+                                     void* scratch, size_t scratch_bytes, hipStream_t st,
+                                     const EmFuse* fuse) {
   if (kind != BP_PLAIN && kind != BP_ONES && kind != BP_MULT) return hipErrorNotSupported;
   if (precision < APIRL_PREC_REFERENCE || precision > APIRL_PREC_F64_SETUP)
     return hipErrorInvalidValue;
@@ -920,6 +934,11 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
   gather_args_for(p, subset, nsubsets, kind, sino, img, g.dims == 3 ? scratch : nullptr, A, zr,
                   nzp);
   A.zlanes = g.dims == 3 ? p->gather_zlanes : 1;
+  if (fuse) {
+    A.upd_sens = fuse->sens;
+    A.upd_thr = fuse->thr;
+    A.upd_strict = fuse->strict;
+  }
   if (g.dims == 3) {
     const long npair = (long)nang_sub * g.nr;
     hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair)), dim3(kGBlock), 0, st, nzp, npair);
@@ -996,6 +1015,9 @@ void gather_args_for(const apirl_plan* p, int subset, int nsubsets, int kind, co
   }
   A.zr = zr;
   A.nz_pair = nz_pair;
+  A.upd_sens = nullptr;
+  A.upd_thr = 0.f;
+  A.upd_strict = 0;
 }
 
 hipError_t launch_group_gather(BatchHeader* hdr, const ZsJob* zs, const GJob* g2,

@@ -84,6 +84,8 @@ SIGNATURES = {
     "apirl_plan_set_tuning": (_c_int, [_P, ctypes.POINTER(Tuning)]),
     "apirl_project": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int]),
     "apirl_backproject": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
+    "apirl_backproject_update": (_c_int, [_P, _P, _P, _P, _c_float, _c_int, _c_int, _c_int,
+                                          _c_int]),
     "apirl_divide_backproject": (_c_int, [_P, _P, _P, _P, _c_int, _c_int, _c_int]),
     "apirl_project_ratio": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_project_ratio_noalloc": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int,
@@ -358,6 +360,17 @@ class Plan:
                                                       nsubsets, mode, precision),
                        "apirl_backproject")
         return out
+
+    def backproject_update(self, sino, x, sens, thr=0.0, strict=0, subset=0, nsubsets=1,
+                           precision=PREC_REFERENCE):
+        """x := cond ? x * (A^T sino) / sens : 0 in one gather (apirl_backproject_update)."""
+        self._check_img(x)
+        self._check_img(sens)
+        self._check_sino(sino, subset, nsubsets)
+        self.ctx.check(self.ctx.lib.apirl_backproject_update(
+            self.h, _dptr(sino), _dptr(x), _dptr(sens), float(thr), int(strict), subset,
+            nsubsets, precision), "apirl_backproject_update")
+        return x
 
     def divide_backproject(self, meas, est, out=None, subset=0, nsubsets=1, mode=BP_SCATTER):
         torch = _torch()

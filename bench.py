@@ -428,10 +428,13 @@ def main():
 
     ev = {"fwd": [], "bwd": [], "upd": []}
 
+    fused = args.bp == "gather"
+
     def iteration(record):
         for s in range(SUBSETS):
             r = ratio[: n_ray[s]].view(plan.subset_shape(s, SUBSETS))
-            bp.zero_()
+            if not fused:
+                bp.zero_()
             if record:
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
@@ -443,10 +446,15 @@ def main():
             plan.project_ratio(x, meas, subset=s, nsubsets=SUBSETS, ratio=r, lik=lik[s:s + 1])
             if record:
                 e1.record()
-            plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=bp_mode)
+            if fused:
+                # the gather backprojection with the update fused in (OSEM's threshold 0, >=)
+                plan.backproject_update(r, x, sens[s], 0.0, 0, subset=s, nsubsets=SUBSETS)
+            else:
+                plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=bp_mode)
             if record:
                 e2.record()
-            hip.em_update(ctx, x, bp, sens[s], 0.0, 0)
+            if not fused:
+                hip.em_update(ctx, x, bp, sens[s], 0.0, 0)
             if record:
                 e3.record()
                 ev["fwd"].append((e0, e1, s))
@@ -479,9 +487,10 @@ def main():
     nray_it = float(sum(n_ray))
     # algorithmic bytes per launch (averaged over the 21 subsets):
     #   forward fused: 4 N_int (voxel reads) + 4 N_ray (meas read) + 4 N_ray (ratio write)
-    #   backprojection: 4 N_int (voxel read-modify-writes counted once) + 4 N_ray + 4 N_vox
+    #   backprojection: 4 N_int (voxel read-modify-writes counted once) + 4 N_ray + 4 N_vox;
+    #   with the update fused in (gather), 12 N_vox: image and sensitivity read, image write
     fwd_bytes = (4 * nint_it + 8 * nray_it) / SUBSETS
-    bwd_bytes = (4 * nint_it + 4 * nray_it) / SUBSETS + 4 * nvox
+    bwd_bytes = (4 * nint_it + 4 * nray_it) / SUBSETS + (12 if fused else 4) * nvox
     dom = "bwd" if bwd_ms >= fwd_ms else "fwd"
     dom_bytes, dom_ms = (bwd_bytes, bwd_ms) if dom == "bwd" else (fwd_bytes, fwd_ms)
     dom_entry = "forward_fused" if dom == "fwd" else f"backproject_{args.bp}"
@@ -516,7 +525,8 @@ def main():
         "likelihood_last_iteration": [float(v) for v in lik_last[[0, -1]]],
         "ray_voxel_gint_per_s": gint,
         "intersections_per_iteration": nint_it,
-        "kernel_ms": {"forward_fused": fwd_ms, "backproject": bwd_ms, "update": upd_ms},
+        "kernel_ms": ({"forward_fused": fwd_ms, "backproject_update": bwd_ms} if fused else
+                      {"forward_fused": fwd_ms, "backproject": bwd_ms, "update": upd_ms}),
         "roofline": {"bound": "hbm", "kernel": dom_entry,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_entry),

@@ -193,6 +193,14 @@ apirl_status apirl_project(apirl_plan* plan, const float* d_img, float* d_sino, 
  * crossings and sums in double; the scatter form takes APIRL_PREC_REFERENCE only. */
 apirl_status apirl_backproject(apirl_plan* plan, const float* d_sino, float* d_img, int subset,
                                int nsubsets, int mode, int precision);
+/* The gather backprojection fused with the EM update that consumes it (Backproject,
+ * then MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237 or Mlem2d.cpp:171-185):
+ * d_img[v] := cond ? d_img[v] * b[v] / d_sens[v] : 0, b = A^T d_sino, cond = d_sens[v]
+ * > thr (strict) or >= thr.  Bit for bit apirl_backproject (APIRL_BP_GATHER) into a
+ * zeroed image followed by apirl_em_update, without storing b.  One LOR per bin. */
+apirl_status apirl_backproject_update(apirl_plan* plan, const float* d_sino, float* d_img,
+                                      const float* d_sens, float thr, int strict, int subset,
+                                      int nsubsets, int precision);
 /* DivideAndBackproject: d_img += A^T r with r the reference's ratio rule per segment
  * (3D SiddonProjector.cpp:571-704; 2D :90-144).  APIRL_BP_GATHER evaluates the rule
  * per bin, then gathers (deterministic; w (m / e) instead of the reference's
