@@ -152,3 +152,39 @@ def test_backprojection_modes_vs_f64_oracle_c2(torch, ctx):
     assert errs["f64"][0] <= 1e-7
     assert errs["f64_setup"][0] <= 2e-4
     assert errs["f64_setup"][0] * 100 <= errs["reference"][0]
+
+
+def test_osem_image_modes_vs_f64_oracle_c2(torch, ctx):
+    """The image after one OSEM iteration (21 subsets) of the full 256x256x81 /
+    344x252x837 workload against the FP64 oracle's OSEM (precision_study.Fp64Osem: the
+    oracle's double projector and backprojector over sinogram chunks in threads, checked
+    against orc_osem3d_f64), MLEM threshold rule in both: the FP64 alpha setup mode and
+    the all-double mode within 1e-4 relative L2, the reference arithmetic's error
+    printed."""
+    import os
+    import time
+
+    import precision_study as PS
+    log("case: 256x256x81 OSEM image, 1 iteration x 21 subsets, vs the FP64 oracle")
+    g = C2
+    p = plan3d(ctx, g)
+    img = phantoms.cylinder_spheres3d(g["nx"], g["nz"], g["dx"], g["dz"])
+    truth = host(p.project(dev(torch, img)))
+    meas = phantoms.poisson_counts(truth, 1e9, np.random.default_rng(0))
+    del truth
+    init = np.ones((g["nz"], g["ny"], g["nx"]), np.float32)
+    md, x0 = dev(torch, meas), dev(torch, init)
+    out = {}
+    for k in ("reference", "f64_setup", "f64"):
+        a, _ = p.reconstruct(md, x0, 1, subsets=21, bp_mode=hip.BP_GATHER, precision=MODES[k],
+                             osem_rules=0)
+        out[k] = host(a)
+    t0 = time.time()
+    ref = PS.Fp64Osem(g, min(16, os.cpu_count() or 1)).osem(meas, init, 1)
+    log(f"  FP64 oracle OSEM: {time.time() - t0:.0f} s")
+    errs = {k: errors(v, ref) for k, v in out.items()}
+    for k, (rl2, mx) in errs.items():
+        log(f"  OSEM image {k:10s}: rel L2 {rl2:.3e}, max abs {mx:.3e}")
+    assert errs["f64"][0] <= 1e-5
+    assert errs["f64_setup"][0] <= 1e-4
+    assert errs["f64_setup"][0] * 10 <= errs["reference"][0]
