@@ -190,7 +190,8 @@ apirl_status apirl_project(apirl_plan* plan, const float* d_img, float* d_sino, 
  * Replaces SiddonProjector::Backproject (2D :34-87, 3D :444-568).  precision applies
  * to the gather form: APIRL_PREC_REFERENCE sums the reference's float weights in
  * float, APIRL_PREC_KAHAN with a compensation term, APIRL_PREC_F64 computes the
- * crossings and sums in double; the scatter form takes APIRL_PREC_REFERENCE only. */
+ * crossings and sums in double; the scatter form takes APIRL_PREC_REFERENCE only and
+ * allocates its 16 per-XCD image copies on its first call on a plan. */
 apirl_status apirl_backproject(apirl_plan* plan, const float* d_sino, float* d_img, int subset,
                                int nsubsets, int mode, int precision);
 /* The gather backprojection fused with the EM update that consumes it (Backproject,
