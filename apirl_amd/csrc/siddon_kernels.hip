@@ -131,9 +131,10 @@ __device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, 
   return w;
 }
 
-// kUnroll steps of walk3d (below), from step m0.  TAIL: the batch may run past the
-// ray's last step, whose terms are then not added.
-template <bool TR, bool TAIL>
+// kUnroll steps of walk3d (below), from step m0, with the reference's read test on
+// every step; the batch may run past the ray's last step, whose terms are then not
+// added.
+template <bool TR>
 __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float* __restrict__ img,
                                              const float* __restrict__ imgT, int nxy, int m0,
                                              float& acc) {
@@ -157,7 +158,7 @@ __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float
     ray.k += bz ? ray.kinc : 0;
     const int li = __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
     const bool ok = (ci < g.nx) & (cj < g.ny) & (ck < g.nz) & (li >= 0) &
-                    (!TAIL | (m0 + u < ray.n));
+                    (m0 + u < ray.n);
     int idx = li;
     const float* base = img;
     if (TR && ((ci >= 0) & (cj >= 0))) {
@@ -229,7 +230,7 @@ __device__ __forceinline__ void walk3d_batch_inside(const Geo& g, Ray& ray,
     for (int u = 0; u < kUnroll; u++) acc = acc + wv[u] * pv[u];
     ray = r;
   } else {
-    walk3d_batch<TR, true>(g, ray, img, imgT, nxy, m0, acc);
+    walk3d_batch<TR>(g, ray, img, imgT, nxy, m0, acc);
   }
 }
 
@@ -251,7 +252,7 @@ __device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __r
   int m0 = 0;
   for (; m0 + kUnroll <= ray.n; m0 += kUnroll)
     walk3d_batch_inside<TR>(g, ray, img, imgT, nxy, m0, acc);
-  if (m0 < ray.n) walk3d_batch<TR, true>(g, ray, img, imgT, nxy, m0, acc);
+  if (m0 < ray.n) walk3d_batch<TR>(g, ray, img, imgT, nxy, m0, acc);
   return acc;
 }
 
