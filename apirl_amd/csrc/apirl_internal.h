@@ -80,6 +80,11 @@ struct LorForm {
   int nrl = 0;
   bool built = false;
   bool gather = false;          // the gather's tables exist for this form
+  // APIRL_PREC_F64_SETUP tables of the gather's column kernel, built on its first
+  // use: the rays with the double setup's entry voxel, list length, alpha range and
+  // first crossings (rounded to float), and their entry-crossing table
+  RayXY* rays_f64s = nullptr;
+  float* xytab_f64s = nullptr;
 };
 
 }  // namespace apirl
@@ -223,6 +228,10 @@ hipError_t noalloc_scratch_address(double** out);
 // The one-LOR-per-bin plans of an oversampled plan (apirl_plan::os_parts); false with
 // the context's error set when they cannot be built.
 bool ensure_os_parts(apirl_plan* p);
+// The rays of an APIRL_PREC_F64_SETUP gather: each ray's transaxial setup carried out
+// in double from its float LOR (setup_xy_f64), rounded to float as the projector's
+// F64_SETUP loop starts from it.
+void build_f64s_rays(const Geo& g, const std::vector<RayXY>& rays, std::vector<RayXY>& out);
 void free_os_parts(apirl_plan* p);
 // Likelihood sum y log(x) - x over x > 0 (positive_only) or x != 0, deterministic:
 // block partials into `partials`, then a fixed-order sum into *out.

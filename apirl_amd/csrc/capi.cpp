@@ -59,7 +59,15 @@ bool check_subset(const apirl_plan* p, int subset, int nsubsets) {
   return nsubsets >= 1 && subset >= 0 && subset < nsubsets && nsubsets <= p->geo.nang;
 }
 
+void free_f64s(apirl::LorForm& f) {
+  if (f.rays_f64s) (void)hipFree(f.rays_f64s);
+  if (f.xytab_f64s) (void)hipFree(f.xytab_f64s);
+  f.rays_f64s = nullptr;
+  f.xytab_f64s = nullptr;
+}
+
 void free_form(apirl::LorForm& f) {
+  free_f64s(f);
   if (f.rays) (void)hipFree(f.rays);
   if (f.xytab) (void)hipFree(f.xytab);
   if (f.angcs_odd) (void)hipFree(f.angcs_odd);
@@ -137,6 +145,37 @@ bool ensure_tile_map(apirl_plan* p) {
   p->d_tilemap[zv] = d;
   p->tilemap_slots[zv] = (int)map.size();
   return true;
+}
+
+// The APIRL_PREC_F64_SETUP tables of the active form's gather (LorForm::rays_f64s,
+// xytab_f64s), built from its rays on first use: false when they cannot be, and the
+// F64_SETUP gather then keeps its one-voxel kernel.
+bool ensure_f64s_tables(apirl_plan* p) {
+  apirl::LorForm& f = p->forms[p->scanner];
+  if (f.rays_f64s && f.xytab_f64s) return true;
+  if (!f.rays || p->geo.dims != 3 || p->geo.ns * p->geo.nas != 1) return false;
+  std::vector<apirl::RayXY> rays((size_t)p->geo.nang * p->geo.nr);
+  if (hipMemcpy(rays.data(), f.rays, rays.size() * sizeof(apirl::RayXY),
+                hipMemcpyDeviceToHost) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  std::vector<apirl::RayXY> r64;
+  apirl::build_f64s_rays(p->geo, rays, r64);
+  std::vector<float> xy;
+  apirl::build_xy_crossings(p->img, r64, xy);
+  const bool ok =
+      hipMalloc(&f.rays_f64s, r64.size() * sizeof(apirl::RayXY)) == hipSuccess &&
+      hipMemcpy(f.rays_f64s, r64.data(), r64.size() * sizeof(apirl::RayXY),
+                hipMemcpyHostToDevice) == hipSuccess &&
+      hipMalloc(&f.xytab_f64s, xy.size() * sizeof(float)) == hipSuccess &&
+      hipMemcpy(f.xytab_f64s, xy.data(), xy.size() * sizeof(float), hipMemcpyHostToDevice) ==
+          hipSuccess;
+  if (!ok) {
+    (void)hipGetLastError();
+    free_f64s(f);
+  }
+  return ok;
 }
 
 // Backprojection of one subset in either form.
@@ -234,6 +273,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
   }
   if (!ensure_tile_map(p))
     return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
+  if (precision == APIRL_PREC_F64_SETUP && p->geo.dims == 3 && p->gather_zlanes <= 1)
+    (void)ensure_f64s_tables(p);  // without them the one-voxel F64_SETUP kernel runs
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream,
                                                 fuse));
@@ -662,6 +703,7 @@ apirl_status apirl_plan_set_oversampling(apirl_plan* p, int ns, int nas) {
   APIRL_HIP(c, hipStreamSynchronize(c->stream));
   apirl::free_os_parts(p);
   apirl::LorForm& f = p->forms[APIRL_SCANNER_CYLINDRICAL];
+  free_f64s(f);  // built from the rays replaced here
   (void)hipFree(f.rays);
   (void)hipFree(p->d_zab);
   f.rays = d_rays;

@@ -49,7 +49,7 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
                                            ZRay* __restrict__ out, int* __restrict__ nz_pair,
                                            float* __restrict__ ztab,
                                            const int* __restrict__ zsoff, int zblock,
-                                           long idx) {
+                                           long idx, int f64s = 0) {
   const int rr = (int)(idx % g.nr);
   const long t = idx / g.nr;
   const int ka = (int)(t % nang_sub);
@@ -61,16 +61,30 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   const float z1 = za + vz * R.amin;
   const float z2 = za + vz * R.amax;
   ZRay z;
-  z.k0 = (int)floorf((z1 - g.offz) / g.dz);
-  z.k1 = (short)(int)floorf((z2 - g.offz) / g.dz);
-  z.azu = fabsf(g.dz / vz);
   const int kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
   z.kinc = (short)kinc;
-  if (vz > 0) z.az0 = (g.offz + (z.k0 + kinc) * g.dz - za) / vz;
-  else if (vz < 0) z.az0 = (g.offz + z.k0 * g.dz - za) / vz;
-  else z.az0 = FLT_MAX;
-  const float ez = (za + vz) - za;
-  z.L = sqrtf(R.lxy + ez * ez);
+  if (f64s) {
+    // APIRL_PREC_F64_SETUP: the axial setup in double from the float LOR, rounded to
+    // float, as the projector's F64_SETUP loop starts from it
+    SetupXY64 xy;
+    setup_xy_f64(g, R.p0x, R.p0y, R.vx, R.vy, xy);
+    SetupZ64 zs;
+    setup_z_f64(g, xy, za, R.vx, R.vy, (double)vz, zs);
+    z.k0 = zs.k0;
+    z.k1 = (short)zs.k1;
+    z.az0 = (float)zs.az;
+    z.azu = (float)zs.azu;
+    z.L = (float)zs.L;
+  } else {
+    z.k0 = (int)floorf((z1 - g.offz) / g.dz);
+    z.k1 = (short)(int)floorf((z2 - g.offz) / g.dz);
+    z.azu = fabsf(g.dz / vz);
+    if (vz > 0) z.az0 = (g.offz + (z.k0 + kinc) * g.dz - za) / vz;
+    else if (vz < 0) z.az0 = (g.offz + z.k0 * g.dz - za) / vz;
+    else z.az0 = FLT_MAX;
+    const float ez = (za + vz) - za;
+    z.L = sqrtf(R.lxy + ez * ez);
+  }
   float b = 1.f;
   if (kind == BP_PLAIN) b = sino[idx];
   else if (kind == BP_MULT) b = sino[((long)s * g.nang + a) * g.nr + rr];
@@ -118,11 +132,11 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
                                                          int* __restrict__ nz_pair,
                                                          float* __restrict__ ztab,
                                                          const int* __restrict__ zsoff,
-                                                         int zblock) {
+                                                         int zblock, int f64s) {
   const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (idx < nrays)
     zsetup_one(g, rays, zab, sino, kind, nang_sub, subset, nsubsets, out, nz_pair, ztab, zsoff,
-               zblock, idx);
+               zblock, idx, f64s);
 }
 
 __global__ __launch_bounds__(kGBlock) void clear_int_kernel(int* __restrict__ p, long n) {
@@ -946,15 +960,26 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     A.upd_strict = fuse->strict;
   }
   if (g.dims == 3) {
+    // APIRL_PREC_F64_SETUP with its tables (the rays' double transaxial setup, their
+    // entry crossings, and zsetup's double axial setup): the column kernel, whose
+    // terms are then those of the one-voxel F64_SETUP gather, summed as KAHAN
+    const LorForm& F = p->forms[p->scanner];
+    const bool f64s = precision == APIRL_PREC_F64_SETUP && A.zlanes <= 1 && p->gather_zvox >= 1 &&
+                      F.rays_f64s && F.xytab_f64s;
+    if (f64s) {
+      A.rays = F.rays_f64s;
+      A.xytab = F.xytab_f64s;
+    }
     const long npair = (long)nang_sub * g.nr;
     hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair)), dim3(kGBlock), 0, st, nzp, npair);
     hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
                        p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp, A.ztab,
-                       A.zsoff, A.zblock);
+                       A.zsoff, A.zblock, f64s ? 1 : 0);
     const int zv = p->gather_zvox;
     if (precision == APIRL_PREC_REFERENCE)
       launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, zv, st);
-    else if (precision == APIRL_PREC_KAHAN) launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, zv, st);
+    else if (precision == APIRL_PREC_KAHAN || f64s)
+      launch_gather3<APIRL_PREC_KAHAN>(A, A.zlanes, zv, st);
     else if (precision == APIRL_PREC_F64_SETUP)
       launch_gather3<APIRL_PREC_F64_SETUP>(A, A.zlanes, zv, st);
     else launch_gather3<APIRL_PREC_F64_ALPHA>(A, A.zlanes, zv, st);

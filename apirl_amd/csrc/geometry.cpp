@@ -8,6 +8,7 @@
 #include <cstdlib>
 
 #include "apirl_internal.h"
+#include "siddon_f64.h"
 
 namespace apirl {
 
@@ -388,6 +389,23 @@ void build_xy_crossings(const apirl_image_desc& img, const std::vector<RayXY>& r
         a = a + t.ayu;
       }
     }
+  }
+}
+
+void build_f64s_rays(const Geo& g, const std::vector<RayXY>& rays, std::vector<RayXY>& out) {
+  out = rays;
+  for (RayXY& o : out) {
+    SetupXY64 xy;
+    setup_xy_f64(g, o.p0x, o.p0y, o.vx, o.vy, xy);
+    o.amin = (float)xy.amin;
+    o.amax = (float)xy.amax;
+    o.ax = (float)xy.ax;
+    o.ay = (float)xy.ay;
+    o.axu = (float)xy.axu;
+    o.ayu = (float)xy.ayu;
+    o.i0 = xy.i0;
+    o.j0 = xy.j0;
+    o.nxy = std::abs(xy.i1 - xy.i0) + std::abs(xy.j1 - xy.j0);
   }
 }
 

@@ -2,8 +2,9 @@
 // recon/src/Siddon.cpp:13-286 (alpha range of the square FOV with its [0,1] clamps,
 // entry / exit voxels by floor(), list length |di|+|dj|+|dk|, first crossings and
 // increments) evaluated in double from the float LOR (P1 = p0, P2 = p0 + v, both as
-// the reference computes them in float).  Used by the APIRL_PREC_F64_ALPHA mode of
-// the projector and of the gather backprojector.
+// the reference computes them in float).  Used by the APIRL_PREC_F64 and
+// APIRL_PREC_F64_SETUP modes of the projector and of the gather backprojector, and on
+// the host for the F64_SETUP tables of the gather's column kernel.
 #ifndef APIRL_SIDDON_F64_H
 #define APIRL_SIDDON_F64_H
 
@@ -24,7 +25,7 @@ struct SetupZ64 {
   int k0, k1, kinc;
 };
 
-__device__ __forceinline__ void setup_xy_f64(const Geo& g, double p0x, double p0y, double vx,
+__host__ __device__ inline void setup_xy_f64(const Geo& g, double p0x, double p0y, double vx,
                                              double vy, SetupXY64& s) {
   const double dx = g.dx, dy = g.dy;
   const double R = (dx * g.nx) / 2;
@@ -65,7 +66,7 @@ __device__ __forceinline__ void setup_xy_f64(const Geo& g, double p0x, double p0
   else s.ay = DBL_MAX;
 }
 
-__device__ __forceinline__ void setup_z_f64(const Geo& g, const SetupXY64& xy, double p0z,
+__host__ __device__ inline void setup_z_f64(const Geo& g, const SetupXY64& xy, double p0z,
                                             double vx, double vy, double vz, SetupZ64& s) {
   const double dz = g.dz;
   const double offz = dz / 2;

@@ -188,3 +188,34 @@ def test_osem_image_modes_vs_f64_oracle_c2(torch, ctx):
     assert errs["f64"][0] <= 1e-5
     assert errs["f64_setup"][0] <= 1e-4
     assert errs["f64_setup"][0] * 10 <= errs["reference"][0]
+
+
+@pytest.mark.parametrize("geom", ["mid", "c2"])
+def test_f64_setup_column_kernel_equals_one_voxel_kernel(torch, ctx, geom):
+    """The F64_SETUP gather in the column kernel (three voxels per lane, its tables
+    built from the double setup: rays, entry crossings, zsetup's axial setup) gives
+    every voxel the terms of the one-voxel F64_SETUP gather, which sets each ray up in
+    double per voxel: bit-identical backprojections (also with the update fused in)."""
+    g = {"mid": MID, "c2": C2}[geom]
+    log(f"case: F64_SETUP column kernel vs one-voxel kernel, {geom}")
+    p = plan3d(ctx, g)
+    img = phantoms.cylinder_spheres3d(g["nx"], g["nz"], g["dx"], g["dz"],
+                                      **({} if geom == "c2" else dict(radius=60.0,
+                                                                      ring_radius=30.0)))
+    S, s = 21, 2
+    sino = p.project(dev(torch, img), subset=s, nsubsets=S)
+    out = {}
+    for zl in (2, 1):  # zlanes 2: the one-voxel kernel; 1: the column kernel
+        p.set_tuning(gather_zlanes=zl)
+        out[zl] = host(p.backproject(sino, subset=s, nsubsets=S, mode=hip.BP_GATHER,
+                                     precision=hip.PREC_F64_SETUP))
+    assert np.abs(out[1]).max() > 0
+    assert np.array_equal(out[1].view(np.int32), out[2].view(np.int32))
+    # the fused update on the same backprojection
+    sens = p.backproject(torch.ones_like(sino), subset=s, nsubsets=S, mode=hip.BP_GATHER)
+    x0 = dev(torch, np.random.default_rng(3).uniform(0.5, 2.0, img.shape))
+    ref = hip.em_update(ctx, x0.clone(), dev(torch, out[2]), sens, 0.0, 0)
+    got = p.backproject_update(sino, x0.clone(), sens, 0.0, 0, subset=s, nsubsets=S,
+                               precision=hip.PREC_F64_SETUP)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(got).view(np.int32), host(ref).view(np.int32))
