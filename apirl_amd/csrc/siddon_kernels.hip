@@ -131,13 +131,37 @@ __device__ __forceinline__ float step_bf(Ray& r, RayIdx& v, int& li, int jstep, 
   return w;
 }
 
-// kUnroll steps of walk3d (below), from step m0, with the reference's read test on
-// every step; the batch may run past the ray's last step, whose terms are then not
-// added.
-template <bool TR>
+// A ray's sum in traversal order: the reference's float additions, or (KAHAN) the
+// same terms with a Kahan compensation term (APIRL_PREC_KAHAN, APIRL_PREC_F64_SETUP).
+template <bool KAHAN>
+struct RaySum {
+  float s = 0.f, c = 0.f;
+  __device__ __forceinline__ void add(float term) {
+    if (KAHAN) {
+      const float y = term - c;
+      const float t = s + y;
+      c = (t - s) - y;
+      s = t;
+    } else {
+      s = s + term;
+    }
+  }
+  __device__ __forceinline__ void add_if(bool ok, float term) {
+    RaySum n = *this;
+    n.add(term);
+    s = ok ? n.s : s;
+    c = ok ? n.c : c;
+  }
+};
+
+// kUnroll steps of walk3d (below), from step m0, with the read test on every step;
+// the batch may run past the ray's last step, whose terms are then not added.
+// STRICT: a voxel is read when it lies inside the image (the F64_SETUP loop's test);
+// otherwise the reference's test (upper bounds and li >= 0).
+template <bool TR, bool KAHAN, bool STRICT>
 __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float* __restrict__ img,
                                              const float* __restrict__ imgT, int nxy, int m0,
-                                             float& acc) {
+                                             RaySum<KAHAN>& acc) {
   float wv[kUnroll], pv[kUnroll];
   bool tv[kUnroll];
 #pragma unroll
@@ -157,8 +181,10 @@ __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float
     ray.j += by ? ray.jinc : 0;
     ray.k += bz ? ray.kinc : 0;
     const int li = __mul24(ck, nxy) + __mul24(cj, g.nx) + ci;
-    const bool ok = (ci < g.nx) & (cj < g.ny) & (ck < g.nz) & (li >= 0) &
-                    (m0 + u < ray.n);
+    const bool ok = STRICT ? ((unsigned)ci < (unsigned)g.nx) & ((unsigned)cj < (unsigned)g.ny) &
+                                 ((unsigned)ck < (unsigned)g.nz) & (m0 + u < ray.n)
+                           : (ci < g.nx) & (cj < g.ny) & (ck < g.nz) & (li >= 0) &
+                                 (m0 + u < ray.n);
     int idx = li;
     const float* base = img;
     if (TR && ((ci >= 0) & (cj >= 0))) {
@@ -170,23 +196,20 @@ __device__ __forceinline__ void walk3d_batch(const Geo& g, Ray& ray, const float
     pv[u] = base[(unsigned)(ok ? idx : 0)];
   }
 #pragma unroll
-  for (int u = 0; u < kUnroll; u++) {
-    const float nacc = acc + wv[u] * pv[u];
-    acc = tv[u] ? nacc : acc;
-  }
+  for (int u = 0; u < kUnroll; u++) acc.add_if(tv[u], wv[u] * pv[u]);
 }
 
 // kUnroll steps of walk3d from step m0 for a batch that stays inside the image.  A
 // ray's voxel index moves monotonically along each axis, so when the first and the
 // last voxel of a batch lie inside the image every voxel between them does: the
-// reference's bounds tests all pass and each term reads the in-image voxel, so they
-// and the out-of-image index are skipped.  A lane whose batch reaches outside the
-// image, or past the ray's last step, takes walk3d_batch from the same state.
-template <bool TR>
+// read tests all pass and each term reads the in-image voxel, so they and the
+// out-of-image index are skipped.  A lane whose batch reaches outside the image, or
+// past the ray's last step, takes walk3d_batch from the same state.
+template <bool TR, bool KAHAN, bool STRICT>
 __device__ __forceinline__ void walk3d_batch_inside(const Geo& g, Ray& ray,
                                                     const float* __restrict__ img,
                                                     const float* __restrict__ imgT, int nxy,
-                                                    int m0, float& acc) {
+                                                    int m0, RaySum<KAHAN>& acc) {
   Ray r = ray;
   float wv[kUnroll];
   int iv[kUnroll];
@@ -227,33 +250,32 @@ __device__ __forceinline__ void walk3d_batch_inside(const Geo& g, Ray& ray,
 #pragma unroll
     for (int u = 0; u < kUnroll; u++) pv[u] = base[(unsigned)iv[u]];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) acc = acc + wv[u] * pv[u];
+    for (int u = 0; u < kUnroll; u++) acc.add(wv[u] * pv[u]);
     ray = r;
   } else {
-    walk3d_batch<TR>(g, ray, img, imgT, nxy, m0, acc);
+    walk3d_batch<TR, KAHAN, STRICT>(g, ray, img, imgT, nxy, m0, acc);
   }
 }
 
-// 3D traversal of one ray in the reference's arithmetic, kUnroll steps per batch of
-// voxel loads (addresses never depend on loaded data), terms added in traversal
-// order.  The reference reads voxel (ci, cj, ck) when ci < nx, cj < ny, ck < nz and
-// the linear index li = ck nx ny + cj nx + ci is not negative (SiddonProjector.cpp:
-// 417-423; a negative ci or cj then reads a neighbouring voxel of the array, as
-// there).  li <= nx ny nz - 1 follows from the three upper bounds.  TR: the rays are
-// offset along y, so in-image voxels are read from the transposed copy (y fastest),
-// where the wavefront's 64 parallel rays touch neighbouring addresses; out-of-image
-// reads keep the reference's index into the image.  Indices use 24-bit multiplies
-// (the plan keeps nx ny below 2^23).  Batches inside the image skip the bounds and
-// step-count tests; the others, and the last, partial batch, take them.
-template <bool TR>
-__device__ __forceinline__ float walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
-                                        const float* __restrict__ imgT, float acc = 0.f) {
+// 3D traversal of one ray, kUnroll steps per batch of voxel loads (addresses never
+// depend on loaded data), terms added in traversal order.  The reference reads voxel
+// (ci, cj, ck) when ci < nx, cj < ny, ck < nz and the linear index li = ck nx ny +
+// cj nx + ci is not negative (SiddonProjector.cpp:417-423; a negative ci or cj then
+// reads a neighbouring voxel of the array, as there).  li <= nx ny nz - 1 follows
+// from the three upper bounds.  TR: the rays are offset along y, so in-image voxels
+// are read from the transposed copy (y fastest), where the wavefront's 64 parallel
+// rays touch neighbouring addresses; out-of-image reads keep the reference's index
+// into the image.  Indices use 24-bit multiplies (the plan keeps nx ny below 2^23).
+// Batches inside the image skip the bounds and step-count tests; the others, and the
+// last, partial batch, take them.  KAHAN / STRICT: see RaySum and walk3d_batch.
+template <bool TR, bool KAHAN = false, bool STRICT = false>
+__device__ __forceinline__ void walk3d(const Geo& g, Ray& ray, const float* __restrict__ img,
+                                       const float* __restrict__ imgT, RaySum<KAHAN>& acc) {
   const int nxy = g.nx * g.ny;
   int m0 = 0;
   for (; m0 + kUnroll <= ray.n; m0 += kUnroll)
-    walk3d_batch_inside<TR>(g, ray, img, imgT, nxy, m0, acc);
-  if (m0 < ray.n) walk3d_batch<TR>(g, ray, img, imgT, nxy, m0, acc);
-  return acc;
+    walk3d_batch_inside<TR, KAHAN, STRICT>(g, ray, img, imgT, nxy, m0, acc);
+  if (m0 < ray.n) walk3d_batch<TR, KAHAN, STRICT>(g, ray, img, imgT, nxy, m0, acc);
 }
 
 // Decomposes a ray index of a subset sinogram into (sinogram, angle, r).
@@ -298,6 +320,7 @@ __device__ __forceinline__ double project_ray(
       // oversampled bin: its LORs (o, p) in the reference's order, every term added to
       // the one bin total (SiddonProjector.cpp:366-425)
       valid = false;
+      RaySum<false> sum;
       for (int o = 0; o < g.ns; o++) {
         const RayXY to = rays[rbase + o];
         for (int q = 0; q < g.nas; q++) {
@@ -305,14 +328,53 @@ __device__ __forceinline__ double project_ray(
           Ray ro;
           if (!setup_ray<DIMS>(g, to, z.x, z.y, ro)) continue;
           valid = true;
-          if (imgT != nullptr && (to.flags & RAY_XMAJOR)) acc = walk3d<true>(g, ro, img, imgT, acc);
-          else acc = walk3d<false>(g, ro, img, imgT, acc);
+          if (imgT != nullptr && (to.flags & RAY_XMAJOR)) walk3d<true>(g, ro, img, imgT, sum);
+          else walk3d<false>(g, ro, img, imgT, sum);
         }
       }
+      acc = sum.s;
     } else if (valid) {
+      const bool tr3 = imgT != nullptr && (t.flags & RAY_XMAJOR);
       if (PREC == APIRL_PREC_REFERENCE && DIMS == 3 && g.walk24) {
-        if (imgT != nullptr && (t.flags & RAY_XMAJOR)) acc = walk3d<true>(g, ray, img, imgT);
-        else acc = walk3d<false>(g, ray, img, imgT);
+        RaySum<false> sum;
+        if (tr3) walk3d<true>(g, ray, img, imgT, sum);
+        else walk3d<false>(g, ray, img, imgT, sum);
+        acc = sum.s;
+      } else if (PREC == APIRL_PREC_KAHAN && DIMS == 3 && g.walk24) {
+        // the reference's traversal with a Kahan-compensated sum
+        RaySum<true> sum;
+        if (tr3) walk3d<true, true>(g, ray, img, imgT, sum);
+        else walk3d<false, true>(g, ray, img, imgT, sum);
+        acc = sum.s;
+      } else if (PREC == APIRL_PREC_F64_SETUP && DIMS == 3 && g.walk24) {
+        // F64_SETUP: the setup in double from the float LOR (siddon_f64.h), rounded to
+        // float; then the reference's float loop (Siddon.cpp:310-340: repeated
+        // additions, the x / y / z tie order) over in-image voxels with a
+        // Kahan-compensated float sum
+        SetupXY64 xy;
+        SetupZ64 zs;
+        setup_xy_f64(g, t.p0x, t.p0y, t.vx, t.vy, xy);
+        setup_z_f64(g, xy, za, t.vx, t.vy, (double)(zb - za), zs);
+        Ray r64;
+        r64.ax = (float)xy.ax;
+        r64.ay = (float)xy.ay;
+        r64.az = (float)zs.az;
+        r64.axu = (float)xy.axu;
+        r64.ayu = (float)xy.ayu;
+        r64.azu = (float)zs.azu;
+        r64.ac = (float)xy.amin;
+        r64.L = (float)zs.L;
+        r64.i = xy.i0;
+        r64.j = xy.j0;
+        r64.k = zs.k0;
+        r64.iinc = xy.iinc;
+        r64.jinc = xy.jinc;
+        r64.kinc = zs.kinc;
+        r64.n = abs(xy.i1 - xy.i0) + abs(xy.j1 - xy.j0) + abs(zs.k1 - zs.k0);
+        RaySum<true> sum;
+        if (tr3) walk3d<true, true, true>(g, r64, img, imgT, sum);
+        else walk3d<false, true, true>(g, r64, img, imgT, sum);
+        acc = sum.s;
       } else if (PREC == APIRL_PREC_REFERENCE) {
         // Steps are taken kUnroll at a time: the voxel loads of a batch are issued
         // together (addresses never depend on loaded data; a step outside the image
