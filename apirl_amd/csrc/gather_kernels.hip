@@ -100,7 +100,9 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   // with alpha_after.
   const int nzc = abs((int)z.k1 - z.k0);
   int nw = 0;
-  if (ztab && kinc != 0) {
+  // a zero bin adds nothing (the gathers skip it once they have its record), so its
+  // crossings are not written
+  if (ztab && kinc != 0 && b != 0.f) {
     const int base = zsoff[s], slot = zsoff[s + 1] - base, kmin = zsoff[g.nsino + 1 + s];
     // entry e of the sinogram's slot lives at ((base + e) * nang_sub + ka) * nr + rr:
     // the rays of one (sinogram, angle) are adjacent, as the gather's lanes read them
