@@ -61,11 +61,15 @@ struct Geo {
 // A run of consecutive sinograms with increasing mean axial coordinate (one segment
 // of the reference's enumeration): zmean(first + l) ~ z0 + l h.  The gather
 // backprojector starts its sinogram scan at the index this line predicts for a ray
-// of typical axial slope dlo (the middle sinogram's zb - za); dhi holds 1 / h.
+// of typical axial slope dlo (the middle sinogram's zb - za); dhi holds 1 / h.  The
+// tile backprojector bounds the run's axial slopes zb - za by [dmin, dmax] and the
+// deviation of zmean from the line by devz.
 struct GatherSeg {
   int first, count;
   float z0, h, dlo, dhi, margin;
   int pad;
+  float dmin, dmax, devz;
+  int pad2;
 };
 
 // One geometry of a plan's LOR endpoints (APIRL_SCANNER_*) with the tables the
@@ -129,6 +133,9 @@ struct apirl_plan {
                                    // Z slices x 64/Z columns per wavefront)
   int gather_zvox = 3;             // 3D gather voxels per lane along z (1..4, tiles only)
   int gather_xcd = APIRL_GATHER_XCD_WEDGES;  // column kernel's tile order
+  int tile_zs = 0;                 // tile backprojector's slab depth (0: about 6 slices)
+  int* d_tile_order = nullptr;     // its item order (built for tile_order_n items)
+  int tile_order_n = 0;
   // its tile maps (APIRL_GATHER_XCD_WEDGES), one per voxels-per-lane depth 1..4, built
   // on first use and kept until the plan ends (captured launches may hold them)
   int* d_tilemap[5] = {};
@@ -252,6 +259,17 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                                      int subset, int nsubsets, int kind, int precision,
                                      void* scratch, size_t scratch_bytes, hipStream_t st,
                                      const EmFuse* fuse = nullptr);
+// Tile-walk backprojector (APIRL_BP_TILE): same scratch as the gather; tile_misfit says
+// why a plan cannot take it (null when it can).
+hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
+                                   int subset, int nsubsets, int kind, void* scratch,
+                                   size_t scratch_bytes, hipStream_t st,
+                                   const EmFuse* fuse = nullptr);
+const char* tile_misfit(const apirl_plan* p);
+size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
+int tile_items(const apirl_plan* p);
+// item order of the tile backprojector: the tiles nearest the image centre first
+void build_tile_order(const apirl_plan* p, std::vector<int>& order);
 size_t gather_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
 // The 3D gather's 32-bit record and table offsets hold for this subset (fewer than
 // 2^31 subset bins; angle x radial bins, sinograms and crossing-table slots below 2^24).

@@ -178,6 +178,31 @@ bool ensure_f64s_tables(apirl_plan* p) {
   return ok;
 }
 
+// The tile backprojector's item order for the plan's current slab depth, built on
+// first use (and again when the slab depth changes the item count).
+bool ensure_tile_order(apirl_plan* p) {
+  const int n = apirl::tile_items(p);
+  if (p->d_tile_order && p->tile_order_n == n) return true;
+  std::vector<int> order;
+  apirl::build_tile_order(p, order);
+  if (p->d_tile_order) {
+    if (hipStreamSynchronize(p->ctx->stream) != hipSuccess) return false;
+    (void)hipFree(p->d_tile_order);
+  }
+  p->d_tile_order = nullptr;
+  p->tile_order_n = 0;
+  if (hipMalloc(&p->d_tile_order, order.size() * sizeof(int)) != hipSuccess ||
+      hipMemcpy(p->d_tile_order, order.data(), order.size() * sizeof(int),
+                hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    if (p->d_tile_order) (void)hipFree(p->d_tile_order);
+    p->d_tile_order = nullptr;
+    return false;
+  }
+  p->tile_order_n = n;
+  return true;
+}
+
 // Backprojection of one subset in either form.
 // Oversampled bins are traversed in the reference's arithmetic only (and need the
 // 24-bit index walk).
@@ -216,9 +241,16 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
                                            c->stream));
     return APIRL_OK;
   }
-  if (mode != APIRL_BP_GATHER) return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
-  if (fuse && (mode != APIRL_BP_GATHER || p->geo.ns * p->geo.nas > 1))
-    return fail(c, APIRL_ERR_INVALID, "the fused update needs the gather of one LOR per bin");
+  if (mode != APIRL_BP_GATHER && mode != APIRL_BP_TILE)
+    return fail(c, APIRL_ERR_INVALID, "invalid backprojection mode");
+  if (fuse && p->geo.ns * p->geo.nas > 1)
+    return fail(c, APIRL_ERR_INVALID,
+                "the fused update needs the gather or tile form of one LOR per bin");
+  if (mode == APIRL_BP_TILE) {
+    if (precision != APIRL_PREC_REFERENCE)
+      return fail(c, APIRL_ERR_UNSUPPORTED, "the tile backprojector has the reference precision only");
+    if (const char* why = apirl::tile_misfit(p)) return fail(c, APIRL_ERR_UNSUPPORTED, why);
+  }
   if (p->geo.ns * p->geo.nas > 1) {
     // every LOR of the oversampled bins (SiddonProjector.cpp:491-548): the gathers of
     // the plans of one LOR per bin, added in turn into the image
@@ -239,7 +271,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
   if (const char* why = apirl::gather_misfit(p, subset, nsubsets))
     return fail(c, APIRL_ERR_UNSUPPORTED,
                 std::string("gather: ") + why + " (use the scatter form)");
-  const size_t need = apirl::gather_scratch_bytes(p, subset, nsubsets);
+  const size_t need = mode == APIRL_BP_TILE ? apirl::tile_scratch_bytes(p, subset, nsubsets)
+                                            : apirl::gather_scratch_bytes(p, subset, nsubsets);
   if (need > p->gscratch_bytes) {
     if (p->d_gscratch) {
       APIRL_HIP(c, hipStreamSynchronize(c->stream));
@@ -275,6 +308,14 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
     return fail(c, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   if (precision == APIRL_PREC_F64_SETUP && p->geo.dims == 3 && p->gather_zlanes <= 1)
     (void)ensure_f64s_tables(p);  // without them the one-voxel F64_SETUP kernel runs
+  if (mode == APIRL_BP_TILE) {
+    if (!ensure_tile_order(p))
+      return fail(c, APIRL_ERR_NOMEM, "cannot allocate the tile backprojector's item order");
+    APIRL_HIP(c, apirl::launch_backproject_tile(p, sino, img, subset, nsubsets, kind,
+                                                p->d_gscratch, p->gscratch_bytes, c->stream,
+                                                fuse));
+    return APIRL_OK;
+  }
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream,
                                                 fuse));
@@ -634,6 +675,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   if (p->d_divbins) (void)hipFree(p->d_divbins);
   for (int* m : p->d_tilemap)
     if (m) (void)hipFree(m);
+  if (p->d_tile_order) (void)hipFree(p->d_tile_order);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
   delete p;
   return APIRL_OK;
@@ -736,6 +778,7 @@ apirl_status apirl_plan_get_tuning(const apirl_plan* p, apirl_tuning* out) {
   out->gather_zlanes = p->gather_zlanes;
   out->gather_zvox = p->gather_zvox;
   out->gather_xcd = p->gather_xcd;
+  out->tile_zs = p->tile_zs;
   return APIRL_OK;
 }
 
@@ -756,6 +799,11 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
       return fail(p->ctx, APIRL_ERR_INVALID, "unknown gather_xcd");
     p->gather_xcd = t->gather_xcd;
   }
+  if (t->tile_zs) {
+    if (t->tile_zs < 1 || t->tile_zs > 256)
+      return fail(p->ctx, APIRL_ERR_INVALID, "tile_zs must be in 1..256");
+    p->tile_zs = t->tile_zs;
+  }
   DeviceGuard g(p->ctx->device);
   if (!ensure_tile_map(p))
     return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
@@ -763,6 +811,7 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     h->gather_zlanes = p->gather_zlanes;
     h->gather_zvox = p->gather_zvox;
     h->gather_xcd = p->gather_xcd;
+    h->tile_zs = p->tile_zs;
     if (!ensure_tile_map(h))
       return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   }
@@ -789,7 +838,7 @@ apirl_status apirl_backproject(apirl_plan* p, const float* d_sino, float* d_img,
   if (!p || !d_img || !d_sino) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
-  if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
+  if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER && mode != APIRL_BP_TILE)
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
   DeviceGuard g(p->ctx->device);
   return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN, mode,
@@ -798,14 +847,16 @@ apirl_status apirl_backproject(apirl_plan* p, const float* d_sino, float* d_img,
 
 apirl_status apirl_backproject_update(apirl_plan* p, const float* d_sino, float* d_img,
                                       const float* d_sens, float thr, int strict, int subset,
-                                      int nsubsets, int precision) {
+                                      int nsubsets, int mode, int precision) {
   if (!p || !d_img || !d_sino || !d_sens) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
   DeviceGuard g(p->ctx->device);
+  if (mode != APIRL_BP_GATHER && mode != APIRL_BP_TILE)
+    return fail(p->ctx, APIRL_ERR_INVALID, "the fused update takes the gather or tile form");
   const apirl::EmFuse fz{d_sens, thr, strict ? 1 : 0};
-  return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN,
-                         APIRL_BP_GATHER, precision, &fz);
+  return backproject_any(p, d_sino, nullptr, d_img, subset, nsubsets, apirl::BP_PLAIN, mode,
+                         precision, &fz);
 }
 
 apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const float* d_est,
@@ -813,7 +864,7 @@ apirl_status apirl_divide_backproject(apirl_plan* p, const float* d_meas, const 
   if (!p || !d_meas || !d_est || !d_img) return APIRL_ERR_INVALID;
   if (!check_subset(p, subset, nsubsets))
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid subset");
-  if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER)
+  if (mode != APIRL_BP_SCATTER && mode != APIRL_BP_GATHER && mode != APIRL_BP_TILE)
     return fail(p->ctx, APIRL_ERR_INVALID, "invalid backprojection mode");
   DeviceGuard g(p->ctx->device);
   return backproject_any(p, d_meas, d_est, d_img, subset, nsubsets, apirl::BP_DIVIDE, mode);
@@ -1033,7 +1084,8 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
   const int strict = (!opt->osem_rules && is2d) ? 1 : 0;
   // With the gather (one LOR per bin) the update is fused into the backprojection: the
   // same float backprojection per voxel, without storing it (bit-identical images).
-  const bool fuse_update = opt->bp_mode == APIRL_BP_GATHER && p->geo.ns * p->geo.nas == 1;
+  const bool fuse_update = (opt->bp_mode == APIRL_BP_GATHER || opt->bp_mode == APIRL_BP_TILE) &&
+                           p->geo.ns * p->geo.nas == 1;
   // Likelihood rule (2D: est > 0) and the additive sinogram's angle per subset bin:
   // the reference adds angle k of the full additive sinogram to subset bin k
   // (OsemSinogram3d.cpp:172-173, Sinogram3D.cpp:857-873).
@@ -1070,7 +1122,7 @@ apirl_status apirl_em_reconstruct_timed(apirl_plan* p, const apirl_em_options* o
         // the gather updates each voxel of the image from its backprojection
         const apirl::EmFuse fz{d_sens + (size_t)s * nvox, thr[s], strict};
         APIRL_TRY(backproject_any(p, d_ratio, nullptr, d_img, s, S, apirl::BP_PLAIN,
-                                  APIRL_BP_GATHER, bp_prec, &fz));
+                                  opt->bp_mode, bp_prec, &fz));
         mark(q + 2);
         continue;
       }

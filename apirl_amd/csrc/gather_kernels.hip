@@ -17,6 +17,8 @@
 // Terms are summed in a fixed order (angle, radial bin, segment, sinogram), so the
 // result does not depend on scheduling: repeated runs are bit-identical.  No atomics.
 #include <cfloat>
+#include <algorithm>
+#include <climits>
 #include <cmath>
 #include <vector>
 
@@ -49,7 +51,7 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
                                            ZRay* __restrict__ out, int* __restrict__ nz_pair,
                                            float* __restrict__ ztab,
                                            const int* __restrict__ zsoff, int zblock,
-                                           long idx, int f64s = 0) {
+                                           long idx, int f64s = 0, int tmajor = 0) {
   const int rr = (int)(idx % g.nr);
   const long t = idx / g.nr;
   const int ka = (int)(t % nang_sub);
@@ -119,7 +121,11 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   }
   z.zcount = nw;
   z.pad = 0;
-  out[idx] = z;  // (sinogram, angle, radial bin) order
+  // (sinogram, angle, radial bin) order; the tile backprojector's lanes read the
+  // sinograms of one (angle, radial bin) together, and its records are (angle, radial
+  // bin, sinogram)
+  if (tmajor) out[((long)ka * g.nr + rr) * g.nsino + s] = z;
+  else out[idx] = z;
   if (b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
     if (*f == 0) atomicOr(f, 1);
@@ -134,11 +140,11 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
                                                          int* __restrict__ nz_pair,
                                                          float* __restrict__ ztab,
                                                          const int* __restrict__ zsoff,
-                                                         int zblock, int f64s) {
+                                                         int zblock, int f64s, int tmajor) {
   const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (idx < nrays)
     zsetup_one(g, rays, zab, sino, kind, nang_sub, subset, nsubsets, out, nz_pair, ztab, zsoff,
-               zblock, idx, f64s);
+               zblock, idx, f64s, tmajor);
 }
 
 __global__ __launch_bounds__(kGBlock) void clear_int_kernel(int* __restrict__ p, long n) {
@@ -934,6 +940,396 @@ void launch_gather2(const GatherArgs& A, hipStream_t st) {
   hipLaunchKernelGGL((gather_kernel<2, PREC>), grid, dim3(kGBlock), 0, st, A);
 }
 
+
+// ---- Tile-walk backprojector (3D, APIRL_PREC_REFERENCE) -----------------------------
+//
+// A wavefront owns an item: kTW x kTW voxel columns (a tile) times a slab of slices,
+// summed in its own LDS.  For every ray (angle, radial bin) of the subset that crosses
+// the tile, the wavefront finds once where the rays of all sinograms enter the tile
+// (they share the transaxial path) and its lanes take the sinograms whose rays can
+// reach the slab there, one ray per lane.  Each lane resumes the reference's Siddon
+// loop (Siddon.cpp:310-340) at the tile's entry: the x / y crossings are the loop's
+// accumulated float values from the plan's table, its z crossings come from the
+// first one by the loop's own additions.  It walks to the tile's exit, adding
+// (alpha - alpha_c) L b (SiddonProjector.cpp:546) to the voxels it visits inside the
+// slab, while its step count stays below the list length |di| + |dj| + |dk|
+// (:220, the dropped exit voxel), and only there: the voxels, weights and terms are
+// the reference's floats, bit for bit.  Each voxel's terms are summed in double in
+// the wavefront's order, which depends on the data only (one wavefront per item), so
+// repeated runs are bit-identical, and rounded to float once.
+
+constexpr int kTW = 8;      // item tile width in columns (kTW * kTW = one lane per column)
+constexpr int kTWaves = 4;  // items (wavefronts) per workgroup
+
+struct TileEntry {
+  int i, j;      // column and row of the ray's first voxel in the tile
+  int mxy;       // x and y crossings the loop has taken there
+  float ce;      // alpha_c there: the crossing just taken, or alpha_min (first voxel)
+  float Xn, Yn;  // the next x and y crossings (the loop's accumulated values)
+  float aout;    // alpha near the tile's exit (sinogram candidates only)
+};
+
+// A candidate ray of a tile, found once per subset by tile_rays_kernel for every
+// slab of the tile: its entry and the transaxial constants its lanes walk with.
+struct alignas(16) TileRay {
+  TileEntry E;
+  int pair;      // ka * nr + r, or -1: no ray (or one that misses the tile)
+  float axu, ayu;
+  int nxy;
+  int flags;
+};
+static_assert(sizeof(TileRay) == 48, "TileRay is three 16-byte loads");
+
+// Crossings p after which index i0 + inc p enters [lo, hi] (pin) and first lies
+// beyond it (pout); false when it never lies in it.
+__device__ __forceinline__ bool axis_span(int i0, int inc, int lo, int hi, int& pin, int& pout) {
+  if (inc == 0) {
+    pin = 0;
+    pout = INT_MAX;
+    return i0 >= lo && i0 <= hi;
+  }
+  if (inc > 0) {
+    pin = lo - i0 > 0 ? lo - i0 : 0;
+    pout = hi - i0 + 1;
+    return i0 <= hi;
+  }
+  pin = i0 - hi > 0 ? i0 - hi : 0;
+  pout = i0 - lo + 1;
+  return i0 >= lo;
+}
+
+// Crossing p >= 1 along one axis: the table's value for index c = i0 + inc p, or, for
+// the one index past the image that an item at its edge asks for, the loop's next
+// addition to the previous crossing.
+__device__ __forceinline__ float axis_cross(const float* tab, int n, int i0, int inc, float a0,
+                                            float au, int p) {
+  const int c = i0 + inc * p;
+  if (c >= 0 && c < n) return tab[c];
+  return p == 1 ? a0 : tab[c - inc] + au;
+}
+
+// Number of crossings p >= 1 whose value is <= v (strict: < v), given that crossing
+// plo (when plo >= 1) is and that the answer is taken as phi once it reaches phi:
+// the linear estimate, corrected against the exact values.
+__device__ __forceinline__ int axis_count(const float* tab, int n, int i0, int inc, float a0,
+                                          float au, int plo, int phi, float v, bool strict) {
+  const float e = (v - a0) / au + 1.f;
+  int p = !(e > (float)plo) ? plo : (e < (float)phi ? (int)e : phi);
+  while (p > plo) {
+    const float x = axis_cross(tab, n, i0, inc, a0, au, p);
+    if (strict ? x < v : x <= v) break;
+    p--;
+  }
+  while (p < phi) {
+    const float x = axis_cross(tab, n, i0, inc, a0, au, p + 1);
+    if (!(strict ? x < v : x <= v)) break;
+    p++;
+  }
+  return p;
+}
+
+// Where ray t enters the tile [ti0, ti1] x [tj0, tj1]: the loop's state after the
+// later of the crossings that bring i and j into range (an x crossing at the same
+// alpha as a y crossing is taken first, Siddon.cpp:318-331); false when the ray's
+// path misses the tile.
+__device__ __forceinline__ bool tile_entry(const Geo& g, const RayXY& t, const float* xy, int ti0,
+                                           int ti1, int tj0, int tj1, TileEntry& E) {
+  const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
+  int pin, pout, qin, qout;
+  if (!axis_span(t.i0, iinc, ti0, ti1, pin, pout) || !axis_span(t.j0, jinc, tj0, tj1, qin, qout))
+    return false;
+  const float* yt = xy + g.nx;
+  int p = 0, q = 0;
+  float ce = t.amin;
+  if (pin > 0 || qin > 0) {
+    const float Xin = pin > 0 ? axis_cross(xy, g.nx, t.i0, iinc, t.ax, t.axu, pin) : -FLT_MAX;
+    const float Yin = qin > 0 ? axis_cross(yt, g.ny, t.j0, jinc, t.ay, t.ayu, qin) : -FLT_MAX;
+    if (Xin <= Yin) {  // entered by y crossing qin; x crossings up to its alpha are taken
+      q = qin;
+      ce = Yin;
+      if (iinc != 0) {
+        p = axis_count(xy, g.nx, t.i0, iinc, t.ax, t.axu, pin, pout, Yin, false);
+        if (p >= pout) return false;
+      }
+    } else {  // entered by x crossing pin; y crossings below its alpha are taken
+      p = pin;
+      ce = Xin;
+      if (jinc != 0) {
+        q = axis_count(yt, g.ny, t.j0, jinc, t.ay, t.ayu, qin, qout, Xin, true);
+        if (q >= qout) return false;
+      }
+    }
+  }
+  E.i = t.i0 + iinc * p;
+  E.j = t.j0 + jinc * q;
+  E.mxy = p + q;
+  E.ce = ce;
+  E.Xn = (iinc == 0 || p == 0) ? t.ax : xy[E.i] + t.axu;
+  E.Yn = (jinc == 0 || q == 0) ? t.ay : yt[E.j] + t.ayu;
+  const float xo = iinc == 0 ? FLT_MAX : t.ax + (float)(pout - 1) * t.axu;
+  const float yo = jinc == 0 ? FLT_MAX : t.ay + (float)(qout - 1) * t.ayu;
+  E.aout = fminf(t.amax, fminf(xo, yo));
+  return true;
+}
+
+// Radial bins of angle a whose LORs can cross a tile centred at (xc, yc) with
+// half-widths (hx, hy): candidate_bins for a rectangle.
+__device__ __forceinline__ void tile_bins(const GatherArgs& A, int a, float xc, float yc, float hx,
+                                          float hy, float inv_dr, int& lo0, int& hi0, int& lo1,
+                                          int& hi1) {
+  const Geo& g = A.g;
+  const float2 cs = A.angcs[a];
+  const float sc = xc * cs.x + yc * cs.y;
+  const float hw = hx * fabsf(cs.x) + hy * fabsf(cs.y) + g.rmargin;
+  if (!grid_lut<kGridAny>(A)) {
+    int jlo = (int)ceilf((sc - hw - g.r0) * inv_dr);
+    int jhi = (int)floorf((sc + hw - g.r0) * inv_dr);
+    lo0 = lo1 = jlo < 0 ? 0 : jlo;
+    hi0 = hi1 = jhi > g.nr - 1 ? g.nr - 1 : jhi;
+    return;
+  }
+  lut_bins(A, sc, hw, lo0, hi0);
+  const float2 co = A.angcs_odd[a];
+  const float so = xc * co.x + yc * co.y;
+  const float hwo = hx * fabsf(co.x) + hy * fabsf(co.y) + g.rmargin;
+  lut_bins(A, so, hwo, lo1, hi1);
+}
+
+// One lane: the ray of record z from the tile's entry E to its exit, its terms added
+// to the item's sums acc[column * zs + slice - kz0] for slices kz0..kz1.  One loop
+// iteration is one step of the reference's loop, its branch taken by selects.
+__device__ __forceinline__ void tile_lane(const TileRay& R, const ZRay& z, int ti0, int ti1,
+                                          int tj0, int tj1, int kz0, int kz1, int zs, double* acc) {
+  const TileEntry& E = R.E;
+  const int iinc = ray_iinc(R.flags), jinc = ray_jinc(R.flags);
+  const float axu = R.axu, ayu = R.ayu;
+  const int nxy = R.nxy;
+  const int kinc = z.kinc;
+  const float azu = z.azu, L = z.L, b = z.b;
+  const int N = nxy + abs((int)z.k1 - z.k0);
+  int m = E.mxy, k = z.k0;
+  float Zn = z.az0;
+  // z crossings taken before the entry: those below its alpha (a z crossing at the
+  // alpha of an x or y crossing is taken after it, Siddon.cpp:318-339), by the
+  // loop's own additions, four at a time
+  if (m > 0 && kinc != 0) {
+    while (m < N) {
+      const float z1 = Zn + azu, z2 = z1 + azu, z3 = z2 + azu;
+      if (z3 < E.ce) {
+        Zn = z3 + azu;
+        k += 4 * kinc;
+        m += 4;
+        continue;
+      }
+      const int c = (Zn < E.ce ? 1 : 0) + (z1 < E.ce ? 1 : 0) + (z2 < E.ce ? 1 : 0);
+      Zn = c == 0 ? Zn : (c == 1 ? z1 : (c == 2 ? z2 : z3));
+      k += c * kinc;
+      m += c;
+      break;
+    }
+  }
+  // slice relative to the slab, its end in the direction of travel (a lane that
+  // reaches it adds nothing more), x / y steps left before the tile's exit, and the
+  // sum's index, all stepped along with the loop
+  const int nzs = kz1 - kz0 + 1;
+  int kl = k - kz0;
+  const int kend = kinc > 0 ? nzs : -1;
+  if (kinc == 0 ? (unsigned)kl >= (unsigned)nzs : (kinc > 0 ? kl >= nzs : kl < 0)) return;
+  int xleft = iinc > 0 ? ti1 - E.i : (iinc < 0 ? E.i - ti0 : INT_MAX);
+  int yleft = jinc > 0 ? tj1 - E.j : (jinc < 0 ? E.j - tj0 : INT_MAX);
+  const int dxo = iinc * zs, dyo = jinc * kTW * zs;
+  int idx = ((E.j - tj0) * kTW + (E.i - ti0)) * zs + kl;
+  float ac = E.ce, Xn = E.Xn, Yn = E.Yn;
+  for (int left = N - m; left > 0; left--) {
+    const float xy = fminf(Xn, Yn);
+    const bool zst = Zn < xy;  // Siddon.cpp:318-339: x, then y, then z on ties
+    const bool xst = !zst && Xn <= Yn;
+    const bool yst = !zst && !xst;
+    const float an = zst ? Zn : xy;
+    if ((unsigned)kl < (unsigned)nzs) {
+      const float term = (an - ac) * L * b;  // the reference's float term (geomFactor 1)
+      __hip_atomic_fetch_add(acc + idx, (double)term, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_WAVEFRONT);
+    }
+    ac = an;
+    Zn = zst ? Zn + azu : Zn;
+    Xn = xst ? Xn + axu : Xn;
+    Yn = yst ? Yn + ayu : Yn;
+    kl += zst ? kinc : 0;
+    idx += zst ? kinc : (xst ? dxo : dyo);
+    xleft -= xst ? 1 : 0;
+    yleft -= yst ? 1 : 0;
+    if ((xleft < 0) | (yleft < 0) | (kl == kend)) break;
+  }
+}
+
+// Orders a wavefront's LDS accesses before and after this point (lanes read what
+// other lanes of the wavefront wrote).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Bytes of one wavefront's LDS: the item's sums (double: LDS adds of doubles run
+// about 20 times faster than of floats here, and sum more exactly), then its
+// candidate list (16-bit sinogram indices), rounded to 16 bytes.
+__host__ __device__ inline int tile_wave_bytes(int zs, int nsino) {
+  return (kTW * kTW * zs * 8 + nsino * 2 + 15) / 16 * 16;
+}
+
+// Candidate ray c of angle ka for tile `tile` (slot (tile * nang_sub + ka) * maxc + c):
+// the c-th bin of the angle's candidate range, kept when its bins are not all zero,
+// its LOR is valid and its path enters the tile.  The slots of one (tile, angle) are
+// in bin order.
+__global__ __launch_bounds__(kGBlock) void tile_rays_kernel(GatherArgs A, int maxc, long nslots,
+                                                            TileRay* __restrict__ out) {
+  const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
+  if (idx >= nslots) return;
+  const Geo& g = A.g;
+  const int c = (int)(idx % maxc);
+  const long t2 = idx / maxc;
+  const int ka = (int)(t2 % A.nang_sub);
+  const int tile = (int)(t2 / A.nang_sub);
+  const int ntx = (g.nx + kTW - 1) / kTW;
+  const int ti0 = (tile % ntx) * kTW, tj0 = (tile / ntx) * kTW;
+  const int ti1 = (ti0 + kTW < g.nx ? ti0 + kTW : g.nx) - 1;
+  const int tj1 = (tj0 + kTW < g.ny ? tj0 + kTW : g.ny) - 1;
+  const float xt = -g.R + g.dx * 0.5f * (float)(ti0 + ti1 + 1);
+  const float yt = -g.R + g.dy * 0.5f * (float)(tj0 + tj1 + 1);
+  const float hx = 0.5f * g.dx * (float)(ti1 - ti0 + 1);
+  const float hy = 0.5f * g.dy * (float)(tj1 - tj0 + 1);
+  const int a = A.subset + A.nsubsets * ka;
+  int lo0, hi0, lo1, hi1;
+  tile_bins(A, a, xt, yt, hx, hy, 1.0f / g.dr, lo0, hi0, lo1, hi1);
+  const int jlo = lo0 < lo1 ? lo0 : lo1, jhi = hi0 > hi1 ? hi0 : hi1;
+  const int rr = jlo + c;
+  TileRay R;
+  R.pair = -1;
+  if (rr <= jhi && (!grid_lut<kGridAny>(A) || in_parity_range(rr, lo0, hi0, lo1, hi1))) {
+    const int pair = ka * g.nr + rr;
+    const RayXY t = A.rays[(long)a * g.nr + rr];
+    if (A.nz_pair[pair] != 0 && (t.flags & RAY_VALID) &&
+        tile_entry(g, t, A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny), ti0, ti1, tj0, tj1,
+                   R.E)) {
+      R.pair = pair;
+      R.axu = t.axu;
+      R.ayu = t.ayu;
+      R.nxy = t.nxy;
+      R.flags = t.flags;
+    }
+  }
+  out[idx] = R;
+}
+
+// Items (tile, slab) are taken from a queue in the order of order[] (the tiles
+// nearest the image centre first, whose rays are the most), counter at *next.  The
+// candidate rays of tile t are trays[t * nslot .. (t + 1) * nslot - 1].
+__global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, int nslab,
+                                                          const int* __restrict__ order,
+                                                          int nitems, int* __restrict__ next,
+                                                          const TileRay* __restrict__ trays,
+                                                          int nslot) {
+  extern __shared__ double tlds[];
+  const Geo& g = A.g;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  const int ntx = (g.nx + kTW - 1) / kTW;
+  double* acc = (double*)((char*)tlds + (size_t)w * tile_wave_bytes(zs, g.nsino));
+  unsigned short* list = (unsigned short*)(acc + kTW * kTW * zs);
+  // the axial table, shared by the workgroup's wavefronts
+  float2* s_zab = (float2*)((char*)tlds + (size_t)kTWaves * tile_wave_bytes(zs, g.nsino));
+  for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
+  __syncthreads();
+  const long nxy = (long)g.nx * g.ny;
+  const int il = lane % kTW, jl = lane / kTW;
+  for (;;) {
+    int q = 0;
+    if (lane == 0) q = atomicAdd(next, 1);
+    q = __builtin_amdgcn_readfirstlane(q);
+    if (q >= nitems) return;
+    const int item = order[q];
+    const int slab = item % nslab, tile = item / nslab;
+    const int ti0 = (tile % ntx) * kTW, tj0 = (tile / ntx) * kTW;
+    const int ti1 = (ti0 + kTW < g.nx ? ti0 + kTW : g.nx) - 1;
+    const int tj1 = (tj0 + kTW < g.ny ? tj0 + kTW : g.ny) - 1;
+    const int kz0 = slab * zs, kz1 = (kz0 + zs < g.nz ? kz0 + zs : g.nz) - 1;
+    const bool incol = ti0 + il <= ti1 && tj0 + jl <= tj1;
+    const long v0 = (long)(tj0 + jl) * g.nx + ti0 + il;
+    wave_lds_sync();  // the previous item's sums are read out
+    for (int kl = 0; kl <= kz1 - kz0; kl++)
+      acc[lane * zs + kl] = (incol && !A.upd_sens) ? (double)A.img[(kz0 + kl) * nxy + v0] : 0.0;
+    wave_lds_sync();
+    const float zlo = g.offz + kz0 * g.dz - kZMargin;
+    const float zhi = g.offz + (kz1 + 1) * g.dz + kZMargin;
+    const TileRay* tr = trays + (long)tile * nslot;
+    for (int c = 0; c < nslot; c++) {
+      const TileRay R = tr[c];
+      if (R.pair < 0) continue;
+      const TileEntry& E = R.E;
+      // sinograms whose ray can reach the slab inside the tile: per run, the
+      // indices whose mean z lies within the slab widened by the run's slopes over
+      // the tile's alpha range (z = zmean + (zb - za)(alpha - 1/2)) and its deviation
+      // from the line (one run per lane); of those, the sinograms whose ray's z over
+      // that alpha range meets the slab are listed, run by run
+      const float ulo = E.ce - 0.5f, uhi = E.aout - 0.5f;
+      int n = 0;
+      for (int sb = 0; sb < A.nseg; sb += kWave) {
+        int s0 = 0, cnt = 0;
+        if (sb + lane < A.nseg) {
+          const GatherSeg S = A.segs[sb + lane];
+          const float c1 = S.dmin * ulo, c2 = S.dmin * uhi, c3 = S.dmax * ulo, c4 = S.dmax * uhi;
+          const float dlo = fminf(fminf(c1, c2), fminf(c3, c4));
+          const float dhi = fmaxf(fmaxf(c1, c2), fmaxf(c3, c4));
+          const float lf = (zlo - dhi - S.devz - 0.01f - S.z0) * S.dhi - 1.f;
+          const float hf = (zhi - dlo + S.devz + 0.01f - S.z0) * S.dhi + 1.f;
+          const int l0 = lf > 0.f ? (int)lf : 0;
+          const int l1 = hf < (float)(S.count - 1) ? (int)hf : S.count - 1;
+          s0 = S.first + l0;
+          cnt = l1 >= l0 ? l1 - l0 + 1 : 0;
+        }
+        const int ns = A.nseg - sb < kWave ? A.nseg - sb : kWave;
+        for (int sg = 0; sg < ns; sg++) {
+          const int cs = __builtin_amdgcn_readlane(cnt, sg);
+          if (cs == 0) continue;
+          const int f = __builtin_amdgcn_readlane(s0, sg);
+          for (int l0 = 0; l0 < cs; l0 += kWave) {
+            const int s = f + l0 + lane;
+            bool keep = false;
+            if (l0 + lane < cs) {
+              const float2 zz = s_zab[s];
+              const float vz = zz.y - zz.x;
+              const float z1 = z_at(zz.x, vz, E.ce), z2 = z_at(zz.x, vz, E.aout);
+              keep = !(fmaxf(z1, z2) < zlo) && !(fminf(z1, z2) > zhi);
+            }
+            const unsigned long long bal = __ballot(keep);
+            if (keep)
+              list[n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))] =
+                  (unsigned short)s;
+            n += __popcll(bal);
+          }
+        }
+      }
+      wave_lds_sync();
+      const ZRay* zp = A.zr + (long)R.pair * g.nsino;
+      // each pass's records are loaded while the previous pass walks
+      ZRay zn;
+      if (lane < n) zn = load_zray(zp + list[lane]);
+      for (int base = 0; base < n; base += kWave) {
+        const ZRay z = zn;
+        if (base + kWave + lane < n) zn = load_zray(zp + list[base + kWave + lane]);
+        if (base + lane >= n || z.b == 0.f) continue;  // SiddonProjector.cpp:496
+        tile_lane(R, z, ti0, ti1, tj0, tj1, kz0, kz1, zs, acc);
+      }
+      wave_lds_sync();
+    }
+    if (incol)
+      for (int kl = 0; kl <= kz1 - kz0; kl++)
+        store_voxel(A, (kz0 + kl) * nxy + v0, (float)acc[lane * zs + kl]);
+  }
+}
+
 }  // namespace
 
 hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, float* img,
@@ -976,7 +1372,7 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair)), dim3(kGBlock), 0, st, nzp, npair);
     hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
                        p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp, A.ztab,
-                       A.zsoff, A.zblock, f64s ? 1 : 0);
+                       A.zsoff, A.zblock, f64s ? 1 : 0, 0);
     const int zv = p->gather_zvox;
     if (precision == APIRL_PREC_REFERENCE)
       launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, zv, st);
@@ -991,6 +1387,122 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
       launch_gather2<APIRL_PREC_KAHAN>(A, st);
     else launch_gather2<APIRL_PREC_F64_ALPHA>(A, st);
   }
+  return hipGetLastError();
+}
+
+// Slab depth of the tile backprojector's items: the plan's tuning, or about 6
+// slices (shallow items keep few lanes on one voxel per LDS add and leave room for
+// seven wavefronts per SIMD: 1.95 ms per 12-angle subset of the 256x256x81 /
+// 344x252x837 workload at 6 slices, 2.15 at 8, 2.4 at 12, 2.8 at 16).
+int tile_slab_depth(const apirl_plan* p) {
+  const int nz = p->geo.nz;
+  if (p->tile_zs > 0) return p->tile_zs < nz ? p->tile_zs : nz;
+  const int nslab = (nz + 5) / 6;
+  return (nz + nslab - 1) / nslab;
+}
+
+size_t tile_lds_bytes(const apirl_plan* p) {
+  return (size_t)kTWaves * tile_wave_bytes(tile_slab_depth(p), p->geo.nsino) +
+         (size_t)p->geo.nsino * sizeof(float2);
+}
+
+const char* tile_misfit(const apirl_plan* p) {
+  if (p->geo.dims != 3) return "the tile backprojector is 3D only";
+  if (p->geo.nsino >= 65536) return "the tile backprojector takes fewer than 65536 sinograms";
+  if (tile_lds_bytes(p) > 65536) return "the tile backprojector's items exceed 64 KiB of LDS";
+  return nullptr;
+}
+
+int tile_items(const apirl_plan* p) {
+  const int zs = tile_slab_depth(p);
+  return ((p->geo.nx + kTW - 1) / kTW) * ((p->geo.ny + kTW - 1) / kTW) *
+         ((p->geo.nz + zs - 1) / zs);
+}
+
+void build_tile_order(const apirl_plan* p, std::vector<int>& order) {
+  const int zs = tile_slab_depth(p), nslab = (p->geo.nz + zs - 1) / zs;
+  const int ntx = (p->geo.nx + kTW - 1) / kTW, nty = (p->geo.ny + kTW - 1) / kTW;
+  std::vector<std::pair<double, int>> tiles;
+  for (int ty = 0; ty < nty; ty++)
+    for (int tx = 0; tx < ntx; tx++) {
+      const double cx = (tx + 0.5) * kTW - 0.5 * p->geo.nx, cy = (ty + 0.5) * kTW - 0.5 * p->geo.ny;
+      tiles.push_back({cx * cx + cy * cy, ty * ntx + tx});
+    }
+  std::stable_sort(tiles.begin(), tiles.end(),
+                   [](const std::pair<double, int>& a, const std::pair<double, int>& b) {
+                     return a.first < b.first;
+                   });
+  order.clear();
+  for (const auto& t : tiles)
+    for (int s = 0; s < nslab; s++) order.push_back(t.second * nslab + s);
+}
+
+// Upper bound on the candidate bins of one (tile, angle): the bins within the tile's
+// projected half-width (plus the search margin, and one lookup-table step on each
+// side on a non-uniform grid) of its centre, for each parity.
+int tile_max_candidates(const apirl_plan* p) {
+  const Geo& g = p->geo;
+  double drmin = g.dr > 0 ? g.dr : 1.0;
+  for (size_t j = 1; j < p->r.size(); j++) drmin = std::min(drmin, (double)p->r[j] - p->r[j - 1]);
+  if (!(drmin > 0)) drmin = 1e-3;
+  const double hw = 0.5 * kTW * std::max(g.dx, g.dy) * 1.41421357 + g.rmargin + 1e-3;
+  const double h = p->rl_inv_h > 0 ? 1.0 / p->rl_inv_h : 0.0;
+  const int one = (int)std::ceil((2 * hw + 2 * h) / drmin) + 3;
+  const int c = p->d_rlut ? 2 * one : one;
+  return c < g.nr ? c : g.nr;
+}
+
+size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const size_t npair = (size_t)nang_sub * g.nr;
+  const size_t ntiles = (size_t)((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
+  return (size_t)g.nsino * npair * sizeof(ZRay) + ((npair + 1) * sizeof(int) + 255) / 256 * 256 +
+         ntiles * nang_sub * tile_max_candidates(p) * sizeof(TileRay);
+}
+
+hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
+                                   int subset, int nsubsets, int kind, void* scratch,
+                                   size_t scratch_bytes, hipStream_t st, const EmFuse* fuse) {
+  if (kind != BP_PLAIN && kind != BP_ONES && kind != BP_MULT) return hipErrorNotSupported;
+  if (tile_misfit(p)) return hipErrorNotSupported;
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  const long nrays = (long)g.nsino * nang_sub * g.nr;
+  if (nrays == 0 || (long)g.nx * g.ny * g.nz == 0) return hipSuccess;
+  if (scratch_bytes < tile_scratch_bytes(p, subset, nsubsets)) return hipErrorInvalidValue;
+  const int nitems = tile_items(p);
+  if (!p->d_tile_order || p->tile_order_n != nitems) return hipErrorInvalidValue;
+  GatherArgs A;
+  ZRay* zr = nullptr;
+  int* nzp = nullptr;
+  gather_args_for(p, subset, nsubsets, kind, sino, img, scratch, A, zr, nzp);
+  A.ztab = nullptr;  // the lanes take their z crossings by the loop's additions
+  if (fuse) {
+    A.upd_sens = fuse->sens;
+    A.upd_thr = fuse->thr;
+    A.upd_strict = fuse->strict;
+  }
+  // scratch: [ZRay x nrays][non-zero flag per (angle, r)][item counter][candidate rays
+  // of every (tile, angle)]
+  const long npair = (long)nang_sub * g.nr;
+  const int maxc = tile_max_candidates(p);
+  const int ntiles = ((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
+  const long nslots = (long)ntiles * nang_sub * maxc;
+  TileRay* trays = (TileRay*)((char*)scratch + (size_t)nrays * sizeof(ZRay) +
+                              ((npair + 1) * sizeof(int) + 255) / 256 * 256);
+  hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair + 1)), dim3(kGBlock), 0, st, nzp,
+                     npair + 1);
+  hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
+                     p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp,
+                     (float*)nullptr, A.zsoff, A.zblock, 0, 1);
+  hipLaunchKernelGGL(tile_rays_kernel, dim3(ggrid(nslots)), dim3(kGBlock), 0, st, A, maxc, nslots,
+                     trays);
+  const int zs = tile_slab_depth(p);
+  const int nslab = (g.nz + zs - 1) / zs;
+  hipLaunchKernelGGL(tile_bp_kernel, dim3((unsigned)((nitems + kTWaves - 1) / kTWaves)),
+                     dim3(kGBlock), tile_lds_bytes(p), st, A, zs, nslab, p->d_tile_order, nitems,
+                     nzp + npair, (const TileRay*)trays, nang_sub * maxc);
   return hipGetLastError();
 }
 

@@ -305,6 +305,9 @@ void build_gather_tables(const apirl_plan& p, std::vector<float2>& angcs,
     g.dlo = p.zab[mid].y - p.zab[mid].x;  // typical axial slope of the run
     g.dhi = (float)(1.0 / h);             // 1 / h for the start estimate
     g.margin = (float)(devz + std::max(std::fabs(dlo), std::fabs(dhi)));
+    g.dmin = (float)dlo;
+    g.dmax = (float)dhi;
+    g.devz = (float)devz;
     segs.push_back(g);
     first = s;
   }

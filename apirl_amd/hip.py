@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(HERE, "libapirl_hip.so")
 APIRL_OK = 0
 BP_SCATTER = 0
 BP_GATHER = 1
+BP_TILE = 2
 RATIO_LIK_POSITIVE = 1
 RATIO_ADDITIVE_LOCAL = 2
 PREC_REFERENCE = 0
@@ -47,7 +48,8 @@ class EmOptions(ctypes.Structure):
 
 
 class Tuning(ctypes.Structure):
-    _fields_ = [("gather_zlanes", _c_int), ("gather_zvox", _c_int), ("gather_xcd", _c_int)]
+    _fields_ = [("gather_zlanes", _c_int), ("gather_zvox", _c_int), ("gather_xcd", _c_int),
+                ("tile_zs", _c_int)]
 
 
 GATHER_XCD_ROUND_ROBIN = 1
@@ -85,7 +87,7 @@ SIGNATURES = {
     "apirl_project": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int]),
     "apirl_backproject": (_c_int, [_P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_backproject_update": (_c_int, [_P, _P, _P, _P, _c_float, _c_int, _c_int, _c_int,
-                                          _c_int]),
+                                          _c_int, _c_int]),
     "apirl_divide_backproject": (_c_int, [_P, _P, _P, _P, _c_int, _c_int, _c_int]),
     "apirl_project_ratio": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int, _c_int]),
     "apirl_project_ratio_noalloc": (_c_int, [_P, _P, _P, _P, _P, _P, _c_int, _c_int, _c_int,
@@ -311,8 +313,8 @@ class Plan:
                        "apirl_plan_set_oversampling")
 
     def set_tuning(self, **fields):
-        """Sets apirl_tuning fields (gather_zlanes, gather_zvox, gather_xcd); the
-        results do not depend on them."""
+        """Sets apirl_tuning fields (gather_zlanes, gather_zvox, gather_xcd, tile_zs);
+        gather results do not depend on them."""
         t = Tuning(**{k: int(v) for k, v in fields.items()})
         self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
                        "apirl_plan_set_tuning")
@@ -362,14 +364,15 @@ class Plan:
         return out
 
     def backproject_update(self, sino, x, sens, thr=0.0, strict=0, subset=0, nsubsets=1,
-                           precision=PREC_REFERENCE):
-        """x := cond ? x * (A^T sino) / sens : 0 in one gather (apirl_backproject_update)."""
+                           precision=PREC_REFERENCE, mode=BP_GATHER):
+        """x := cond ? x * (A^T sino) / sens : 0 in one gather or tile backprojection
+        (apirl_backproject_update)."""
         self._check_img(x)
         self._check_img(sens)
         self._check_sino(sino, subset, nsubsets)
         self.ctx.check(self.ctx.lib.apirl_backproject_update(
             self.h, _dptr(sino), _dptr(x), _dptr(sens), float(thr), int(strict), subset,
-            nsubsets, precision), "apirl_backproject_update")
+            nsubsets, mode, precision), "apirl_backproject_update")
         return x
 
     def divide_backproject(self, meas, est, out=None, subset=0, nsubsets=1, mode=BP_SCATTER):

@@ -35,9 +35,14 @@ typedef enum apirl_status {
 
 /* Backprojection forms. */
 enum {
-  APIRL_BP_SCATTER = 0, /* ray-driven, fp32 atomics: fastest, order not fixed */
-  APIRL_BP_GATHER = 1   /* voxel-driven, fixed order: deterministic and equal, bit for bit,
-                           to the reference's single-thread result */
+  APIRL_BP_SCATTER = 0, /* ray-driven, fp32 atomics into image copies: order not fixed */
+  APIRL_BP_GATHER = 1,  /* voxel-driven, fixed order: deterministic; in 2D equal, bit for
+                           bit, to the reference's single-thread result; in 3D the same
+                           float terms summed in (angle, bin, segment, sinogram) order */
+  APIRL_BP_TILE = 2     /* 3D, APIRL_PREC_REFERENCE: ray-driven walks clipped to tiles of
+                           8 x 8 columns by a slab of slices, summed in LDS by one
+                           wavefront per tile (no global atomics): deterministic, the
+                           same float terms in the wavefront's order */
 };
 
 /* Arithmetic of the projector and of the gather backprojector. */
@@ -125,12 +130,16 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
  *                  gathers V consecutive voxels of a column, sharing each candidate
  *                  ray's transaxial tests and sinogram search among them (float
  *                  precision modes; APIRL_PREC_F64_ALPHA always uses 1).
- * Results do not depend on either (the per-voxel summation order is fixed).
+ *   tile_zs:       slices per item of the tile backprojector (APIRL_BP_TILE; default
+ *                  about 6).
+ * Gather results do not depend on these (the per-voxel summation order is fixed);
+ * tile results are deterministic for a given tile_zs.
  * Zero in a field leaves it unchanged. */
 typedef struct apirl_tuning {
   int gather_zlanes;
   int gather_zvox;
   int gather_xcd;     /* workgroup order of the column kernel (below) */
+  int tile_zs;
 } apirl_tuning;
 /* gather_xcd: APIRL_GATHER_XCD_ROUND_ROBIN takes the 16 x 16 x zvox tiles in grid
  * order, which the hardware deals to the 8 XCDs in turn; APIRL_GATHER_XCD_WEDGES gives
@@ -194,14 +203,15 @@ apirl_status apirl_project(apirl_plan* plan, const float* d_img, float* d_sino, 
  * allocates its 16 per-XCD image copies on its first call on a plan. */
 apirl_status apirl_backproject(apirl_plan* plan, const float* d_sino, float* d_img, int subset,
                                int nsubsets, int mode, int precision);
-/* The gather backprojection fused with the EM update that consumes it (Backproject,
- * then MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237 or Mlem2d.cpp:171-185):
+/* The backprojection fused with the EM update that consumes it (Backproject, then
+ * MlemSinogram3d.cpp:228-242, OsemSinogram3d.cpp:217-237 or Mlem2d.cpp:171-185):
  * d_img[v] := cond ? d_img[v] * b[v] / d_sens[v] : 0, b = A^T d_sino, cond = d_sens[v]
- * > thr (strict) or >= thr.  Bit for bit apirl_backproject (APIRL_BP_GATHER) into a
- * zeroed image followed by apirl_em_update, without storing b.  One LOR per bin. */
+ * > thr (strict) or >= thr.  Bit for bit apirl_backproject (mode APIRL_BP_GATHER or
+ * APIRL_BP_TILE) into a zeroed image followed by apirl_em_update, without storing b.
+ * One LOR per bin. */
 apirl_status apirl_backproject_update(apirl_plan* plan, const float* d_sino, float* d_img,
                                       const float* d_sens, float thr, int strict, int subset,
-                                      int nsubsets, int precision);
+                                      int nsubsets, int mode, int precision);
 /* DivideAndBackproject: d_img += A^T r with r the reference's ratio rule per segment
  * (3D SiddonProjector.cpp:571-704; 2D :90-144).  APIRL_BP_GATHER evaluates the rule
  * per bin, then gathers (deterministic; w (m / e) instead of the reference's

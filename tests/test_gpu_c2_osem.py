@@ -20,7 +20,7 @@ from test_gpu_parity import ctx, dev, host, log, torch  # noqa: F401
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("mode", [hip.BP_GATHER, hip.BP_SCATTER])
+@pytest.mark.parametrize("mode", [hip.BP_GATHER, hip.BP_SCATTER, hip.BP_TILE])
 def test_c2_osem_one_iteration_vs_reference(torch, ctx, golden, mode):
     log(f"case: 256x256x81 OSEM 21 subsets x 1 vs the reference, mode {mode}")
     f = golden("c2_osem.npz")
