@@ -12,7 +12,7 @@ likelihood, backprojection, update) with every input resident in HBM.  The
 sensitivity images are computed once before timing, as the reference does before
 its loop (OsemSinogram3d.cpp:101-117).
 
-    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--bp scatter|gather]
+    python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--bp tile|gather|scatter]
                     [--no-cpu-baseline] [--calibrate-copy]
 
 Prints one JSON line on stdout; progress goes to stderr."""
@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--bp", choices=["scatter", "gather"], default="gather")
+    ap.add_argument("--bp", choices=["tile", "gather", "scatter"], default="tile")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a plain 1 GiB device copy first (for PMC byte calibration)")
@@ -414,7 +414,7 @@ def main():
     nvox = plan.nx * plan.ny * plan.nz
     progress("sensitivity images")
     sens = torch.empty((SUBSETS,) + plan.image_shape(), dtype=torch.float32, device=dev)
-    bp_mode = hip.BP_GATHER if args.bp == "gather" else hip.BP_SCATTER
+    bp_mode = {"tile": hip.BP_TILE, "gather": hip.BP_GATHER, "scatter": hip.BP_SCATTER}[args.bp]
     for s in range(SUBSETS):
         plan.sensitivity(out=sens[s], subset=s, nsubsets=SUBSETS, mode=bp_mode)
     n_int = [plan.intersections(s, SUBSETS) for s in range(SUBSETS)]
@@ -428,7 +428,7 @@ def main():
 
     ev = {"fwd": [], "bwd": [], "upd": []}
 
-    fused = args.bp == "gather"
+    fused = args.bp != "scatter"
 
     def iteration(record):
         for s in range(SUBSETS):
@@ -447,8 +447,9 @@ def main():
             if record:
                 e1.record()
             if fused:
-                # the gather backprojection with the update fused in (OSEM's threshold 0, >=)
-                plan.backproject_update(r, x, sens[s], 0.0, 0, subset=s, nsubsets=SUBSETS)
+                # the backprojection with the update fused in (OSEM's threshold 0, >=)
+                plan.backproject_update(r, x, sens[s], 0.0, 0, subset=s, nsubsets=SUBSETS,
+                                        mode=bp_mode)
             else:
                 plan.backproject(r, out=bp, subset=s, nsubsets=SUBSETS, mode=bp_mode)
             if record:
@@ -488,7 +489,8 @@ def main():
     # algorithmic bytes per launch (averaged over the 21 subsets):
     #   forward fused: 4 N_int (voxel reads) + 4 N_ray (meas read) + 4 N_ray (ratio write)
     #   backprojection: 4 N_int (voxel read-modify-writes counted once) + 4 N_ray + 4 N_vox;
-    #   with the update fused in (gather), 12 N_vox: image and sensitivity read, image write
+    #   with the update fused in (tile, gather), 12 N_vox: image and sensitivity read,
+    #   image write
     fwd_bytes = (4 * nint_it + 8 * nray_it) / SUBSETS
     bwd_bytes = (4 * nint_it + 4 * nray_it) / SUBSETS + (12 if fused else 4) * nvox
     dom = "bwd" if bwd_ms >= fwd_ms else "fwd"
