@@ -970,15 +970,26 @@ struct TileEntry {
 };
 
 // A candidate ray of a tile, found once per subset by tile_rays_kernel for every
-// slab of the tile: its entry and the transaxial constants its lanes walk with.
+// slab of the tile: its entry and the length of its transaxial path there.
 struct alignas(16) TileRay {
   TileEntry E;
   int pair;      // ka * nr + r, or -1: no ray (or one that misses the tile)
-  float axu, ayu;
-  int nxy;
-  int flags;
+  int ncol;      // columns on its path through the tile
+  int nxy;       // the ray's transaxial list length |di| + |dj|
+  int pad[2];
 };
 static_assert(sizeof(TileRay) == 48, "TileRay is three 16-byte loads");
+
+// The transaxial part of a candidate ray's walk through its tile, the same for every
+// sinogram: the columns it visits in order and, for each, the alpha at which the
+// loop leaves it by an x or y crossing (the smaller accumulated crossing there; x
+// before y on ties, Siddon.cpp:318-331).  At most 2 kTW - 1 columns.
+constexpr int kTPath = 2 * kTW;
+struct alignas(16) TilePath {
+  float ae[kTPath];
+  unsigned char col[kTPath];  // (j - tj0) kTW + (i - ti0)
+};
+static_assert(sizeof(TilePath) == 80, "TilePath is five 16-byte loads");
 
 // Crossings p after which index i0 + inc p enters [lo, hi] (pin) and first lies
 // beyond it (pout); false when it never lies in it.
@@ -1098,12 +1109,11 @@ __device__ __forceinline__ void tile_bins(const GatherArgs& A, int a, float xc, 
 // One lane: the ray of record z from the tile's entry E to its exit, its terms added
 // to the item's sums acc[column * zs + slice - kz0] for slices kz0..kz1.  One loop
 // iteration is one step of the reference's loop, its branch taken by selects.
-__device__ __forceinline__ void tile_lane(const TileRay& R, const ZRay& z, int ti0, int ti1,
-                                          int tj0, int tj1, int kz0, int kz1, int zs, double* acc) {
+__device__ __forceinline__ void tile_lane(const TileRay& R, const float* pae,
+                                          const unsigned char* pcol, const ZRay& z, int kz0,
+                                          int kz1, int zs, double* acc) {
   const TileEntry& E = R.E;
-  const int iinc = ray_iinc(R.flags), jinc = ray_jinc(R.flags);
-  const float axu = R.axu, ayu = R.ayu;
-  const int nxy = R.nxy;
+  const int nxy = R.nxy, ncol = R.ncol;
   const int kinc = z.kinc;
   const float azu = z.azu, L = z.L, b = z.b;
   const int N = nxy + abs((int)z.k1 - z.k0);
@@ -1128,38 +1138,32 @@ __device__ __forceinline__ void tile_lane(const TileRay& R, const ZRay& z, int t
       break;
     }
   }
-  // slice relative to the slab, its end in the direction of travel (a lane that
-  // reaches it adds nothing more), x / y steps left before the tile's exit, and the
-  // sum's index, all stepped along with the loop
+  // slice relative to the slab and its end in the direction of travel (a lane that
+  // reaches it adds nothing more); the column comes from the path, whose x / y steps
+  // are the same for every lane
   const int nzs = kz1 - kz0 + 1;
   int kl = k - kz0;
   const int kend = kinc > 0 ? nzs : -1;
   if (kinc == 0 ? (unsigned)kl >= (unsigned)nzs : (kinc > 0 ? kl >= nzs : kl < 0)) return;
-  int xleft = iinc > 0 ? ti1 - E.i : (iinc < 0 ? E.i - ti0 : INT_MAX);
-  int yleft = jinc > 0 ? tj1 - E.j : (jinc < 0 ? E.j - tj0 : INT_MAX);
-  const int dxo = iinc * zs, dyo = jinc * kTW * zs;
-  int idx = ((E.j - tj0) * kTW + (E.i - ti0)) * zs + kl;
-  float ac = E.ce, Xn = E.Xn, Yn = E.Yn;
+  int q = 0;
+  float ae = pae[0];
+  int cidx = pcol[0] * zs;
+  float ac = E.ce;
   for (int left = N - m; left > 0; left--) {
-    const float xy = fminf(Xn, Yn);
-    const bool zst = Zn < xy;  // Siddon.cpp:318-339: x, then y, then z on ties
-    const bool xst = !zst && Xn <= Yn;
-    const bool yst = !zst && !xst;
-    const float an = zst ? Zn : xy;
+    const bool zst = Zn < ae;  // Siddon.cpp:318-339: x, then y, then z on ties
+    const float an = zst ? Zn : ae;
     if ((unsigned)kl < (unsigned)nzs) {
       const float term = (an - ac) * L * b;  // the reference's float term (geomFactor 1)
-      __hip_atomic_fetch_add(acc + idx, (double)term, __ATOMIC_RELAXED,
+      __hip_atomic_fetch_add(acc + cidx + kl, (double)term, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_WAVEFRONT);
     }
     ac = an;
     Zn = zst ? Zn + azu : Zn;
-    Xn = xst ? Xn + axu : Xn;
-    Yn = yst ? Yn + ayu : Yn;
     kl += zst ? kinc : 0;
-    idx += zst ? kinc : (xst ? dxo : dyo);
-    xleft -= xst ? 1 : 0;
-    yleft -= yst ? 1 : 0;
-    if ((xleft < 0) | (yleft < 0) | (kl == kend)) break;
+    q += zst ? 0 : 1;
+    if ((q == ncol) | (kl == kend)) break;
+    ae = pae[q];
+    cidx = pcol[q] * zs;
   }
 }
 
@@ -1172,10 +1176,10 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Bytes of one wavefront's LDS: the item's sums (double: LDS adds of doubles run
-// about 20 times faster than of floats here, and sum more exactly), then its
-// candidate list (16-bit sinogram indices), rounded to 16 bytes.
+// about 20 times faster than of floats here, and sum more exactly), the current
+// ray's path, then its candidate list (16-bit sinogram indices), rounded to 16 bytes.
 __host__ __device__ inline int tile_wave_bytes(int zs, int nsino) {
-  return (kTW * kTW * zs * 8 + nsino * 2 + 15) / 16 * 16;
+  return (kTW * kTW * zs * 8 + (int)sizeof(TilePath) + nsino * 2 + 15) / 16 * 16;
 }
 
 // Candidate ray c of angle ka for tile `tile` (slot (tile * nang_sub + ka) * maxc + c):
@@ -1183,7 +1187,8 @@ __host__ __device__ inline int tile_wave_bytes(int zs, int nsino) {
 // its LOR is valid and its path enters the tile.  The slots of one (tile, angle) are
 // in bin order.
 __global__ __launch_bounds__(kGBlock) void tile_rays_kernel(GatherArgs A, int maxc, long nslots,
-                                                            TileRay* __restrict__ out) {
+                                                            TileRay* __restrict__ out,
+                                                            TilePath* __restrict__ paths) {
   const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (idx >= nslots) return;
   const Geo& g = A.g;
@@ -1213,30 +1218,86 @@ __global__ __launch_bounds__(kGBlock) void tile_rays_kernel(GatherArgs A, int ma
         tile_entry(g, t, A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny), ti0, ti1, tj0, tj1,
                    R.E)) {
       R.pair = pair;
-      R.axu = t.axu;
-      R.ayu = t.ayu;
       R.nxy = t.nxy;
-      R.flags = t.flags;
+      // the path: the loop's x / y steps from the entry until it leaves the tile
+      TilePath P;
+      const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
+      float Xn = R.E.Xn, Yn = R.E.Yn;
+      int i = R.E.i, j = R.E.j, q = 0;
+      while (q < kTPath) {
+        P.col[q] = (unsigned char)((j - tj0) * kTW + (i - ti0));
+        P.ae[q] = Xn < Yn ? Xn : Yn;
+        q++;
+        if (Xn <= Yn) {
+          i += iinc;
+          Xn = Xn + t.axu;
+        } else {
+          j += jinc;
+          Yn = Yn + t.ayu;
+        }
+        if (i < ti0 || i > ti1 || j < tj0 || j > tj1) break;
+      }
+      for (int r = q; r < kTPath; r++) {
+        P.col[r] = 0;
+        P.ae[r] = 0.f;
+      }
+      R.ncol = q;
+      paths[idx] = P;
     }
   }
   out[idx] = R;
 }
 
+// The candidate rays of each tile that enter it, moved to the front of its slots in
+// slot order (one wavefront per tile); count[tile] says how many.
+__global__ __launch_bounds__(kGBlock) void tile_compact_kernel(TileRay* __restrict__ trays,
+                                                               TilePath* __restrict__ paths,
+                                                               int nslot, int ntiles,
+                                                               int* __restrict__ count) {
+  const int tile = blockIdx.x * kTWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (tile >= ntiles) return;
+  TileRay* tr = trays + (long)tile * nslot;
+  TilePath* tp = paths + (long)tile * nslot;
+  int n = 0;
+  for (int c0 = 0; c0 < nslot; c0 += kWave) {
+    TileRay R;
+    TilePath P;
+    R.pair = -1;
+    if (c0 + lane < nslot) R = tr[c0 + lane];
+    const bool keep = R.pair >= 0;
+    if (keep) P = tp[c0 + lane];
+    const unsigned long long bal = __ballot(keep);
+    __builtin_amdgcn_wave_barrier();  // every lane has read its slot before any is written
+    if (keep) {
+      const int d = n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0));
+      tr[d] = R;
+      tp[d] = P;
+    }
+    n += __popcll(bal);
+  }
+  if (lane == 0) count[tile] = n;
+}
+
 // Items (tile, slab) are taken from a queue in the order of order[] (the tiles
 // nearest the image centre first, whose rays are the most), counter at *next.  The
-// candidate rays of tile t are trays[t * nslot .. (t + 1) * nslot - 1].
+// rays entering tile t are trays[t * nslot + c], c < tcount[t].
 __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, int nslab,
                                                           const int* __restrict__ order,
                                                           int nitems, int* __restrict__ next,
                                                           const TileRay* __restrict__ trays,
-                                                          int nslot) {
+                                                          const TilePath* __restrict__ tpaths,
+                                                          int nslot,
+                                                          const int* __restrict__ tcount) {
   extern __shared__ double tlds[];
   const Geo& g = A.g;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & (kWave - 1);
   const int ntx = (g.nx + kTW - 1) / kTW;
   double* acc = (double*)((char*)tlds + (size_t)w * tile_wave_bytes(zs, g.nsino));
-  unsigned short* list = (unsigned short*)(acc + kTW * kTW * zs);
+  TilePath* s_path = (TilePath*)(acc + kTW * kTW * zs);
+  unsigned short* list = (unsigned short*)(s_path + 1);
   // the axial table, shared by the workgroup's wavefronts
   float2* s_zab = (float2*)((char*)tlds + (size_t)kTWaves * tile_wave_bytes(zs, g.nsino));
   for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
@@ -1263,9 +1324,13 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
     const float zlo = g.offz + kz0 * g.dz - kZMargin;
     const float zhi = g.offz + (kz1 + 1) * g.dz + kZMargin;
     const TileRay* tr = trays + (long)tile * nslot;
-    for (int c = 0; c < nslot; c++) {
-      const TileRay R = tr[c];
-      if (R.pair < 0) continue;
+    const unsigned* tpw = (const unsigned*)(tpaths + (long)tile * nslot);
+    const int nray = tcount[tile];
+    TileRay Rn;  // the next ray, loaded while this one is processed
+    if (nray > 0) Rn = tr[0];
+    for (int c = 0; c < nray; c++) {
+      const TileRay R = Rn;
+      if (c + 1 < nray) Rn = tr[c + 1];
       const TileEntry& E = R.E;
       // sinograms whose ray can reach the slab inside the tile: per run, the
       // indices whose mean z lies within the slab widened by the run's slopes over
@@ -1311,6 +1376,9 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
           }
         }
       }
+      // the path, from global memory to LDS (20 words)
+      if (lane < (int)(sizeof(TilePath) / 4))
+        ((unsigned*)s_path)[lane] = tpw[(long)c * (sizeof(TilePath) / 4) + lane];
       wave_lds_sync();
       const ZRay* zp = A.zr + (long)R.pair * g.nsino;
       // each pass's records are loaded while the previous pass walks
@@ -1320,7 +1388,7 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
         const ZRay z = zn;
         if (base + kWave + lane < n) zn = load_zray(zp + list[base + kWave + lane]);
         if (base + lane >= n || z.b == 0.f) continue;  // SiddonProjector.cpp:496
-        tile_lane(R, z, ti0, ti1, tj0, tj1, kz0, kz1, zs, acc);
+        tile_lane(R, s_path->ae, s_path->col, z, kz0, kz1, zs, acc);
       }
       wave_lds_sync();
     }
@@ -1458,7 +1526,8 @@ size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
   const size_t npair = (size_t)nang_sub * g.nr;
   const size_t ntiles = (size_t)((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
   return (size_t)g.nsino * npair * sizeof(ZRay) + ((npair + 1) * sizeof(int) + 255) / 256 * 256 +
-         ntiles * nang_sub * tile_max_candidates(p) * sizeof(TileRay);
+         (ntiles * sizeof(int) + 255) / 256 * 256 +
+         ntiles * nang_sub * tile_max_candidates(p) * (sizeof(TileRay) + sizeof(TilePath));
 }
 
 hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
@@ -1483,26 +1552,31 @@ hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float
     A.upd_thr = fuse->thr;
     A.upd_strict = fuse->strict;
   }
-  // scratch: [ZRay x nrays][non-zero flag per (angle, r)][item counter][candidate rays
-  // of every (tile, angle)]
+  // scratch: [ZRay x nrays][non-zero flag per (angle, r)][item counter][ray count per
+  // tile][candidate rays of every (tile, angle)]
   const long npair = (long)nang_sub * g.nr;
   const int maxc = tile_max_candidates(p);
   const int ntiles = ((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
   const long nslots = (long)ntiles * nang_sub * maxc;
-  TileRay* trays = (TileRay*)((char*)scratch + (size_t)nrays * sizeof(ZRay) +
-                              ((npair + 1) * sizeof(int) + 255) / 256 * 256);
+  int* tcount = (int*)((char*)scratch + (size_t)nrays * sizeof(ZRay) +
+                       ((npair + 1) * sizeof(int) + 255) / 256 * 256);
+  TileRay* trays = (TileRay*)((char*)tcount + ((size_t)ntiles * sizeof(int) + 255) / 256 * 256);
+  TilePath* tpaths = (TilePath*)(trays + nslots);
   hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair + 1)), dim3(kGBlock), 0, st, nzp,
                      npair + 1);
   hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
                      p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp,
                      (float*)nullptr, A.zsoff, A.zblock, 0, 1);
   hipLaunchKernelGGL(tile_rays_kernel, dim3(ggrid(nslots)), dim3(kGBlock), 0, st, A, maxc, nslots,
-                     trays);
+                     trays, tpaths);
+  hipLaunchKernelGGL(tile_compact_kernel, dim3((ntiles + kTWaves - 1) / kTWaves), dim3(kGBlock), 0,
+                     st, trays, tpaths, nang_sub * maxc, ntiles, tcount);
   const int zs = tile_slab_depth(p);
   const int nslab = (g.nz + zs - 1) / zs;
   hipLaunchKernelGGL(tile_bp_kernel, dim3((unsigned)((nitems + kTWaves - 1) / kTWaves)),
                      dim3(kGBlock), tile_lds_bytes(p), st, A, zs, nslab, p->d_tile_order, nitems,
-                     nzp + npair, (const TileRay*)trays, nang_sub * maxc);
+                     nzp + npair, (const TileRay*)trays, (const TilePath*)tpaths, nang_sub * maxc,
+                     (const int*)tcount);
   return hipGetLastError();
 }
 
