@@ -1150,6 +1150,10 @@ __device__ __forceinline__ void tile_lane(const TileRay& R, const float* pae,
   int cidx = pcol[0] * zs;
   float ac = E.ce;
   for (int left = N - m; left > 0; left--) {
+    // the next column's entry, read before this step's add so that the wait for it
+    // does not include the add (entry q + 1 < kTPath always exists)
+    const float ae1 = pae[q + 1];
+    const int c1 = pcol[q + 1];
     const bool zst = Zn < ae;  // Siddon.cpp:318-339: x, then y, then z on ties
     const float an = zst ? Zn : ae;
     if ((unsigned)kl < (unsigned)nzs) {
@@ -1162,8 +1166,8 @@ __device__ __forceinline__ void tile_lane(const TileRay& R, const float* pae,
     kl += zst ? kinc : 0;
     q += zst ? 0 : 1;
     if ((q == ncol) | (kl == kend)) break;
-    ae = pae[q];
-    cidx = pcol[q] * zs;
+    ae = zst ? ae : ae1;
+    cidx = zst ? cidx : c1 * zs;
   }
 }
 
