@@ -220,6 +220,9 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
 hipError_t launch_backproject(const apirl_plan* p, const float* sino, const float* est,
                               float* img, int subset, int nsubsets, int kind, int mode,
                               hipStream_t st);
+// zeroes the scatter's n-float image copies (device-scope stores, as the reduction
+// clears them)
+hipError_t launch_zero_replicas(float* rep, size_t n, hipStream_t st);
 hipError_t launch_em_update(float* x, const float* b, const float* s, size_t n, float thr,
                             int strict, hipStream_t st);
 hipError_t launch_force_positive(float* x, size_t n, hipStream_t st);

@@ -231,7 +231,8 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
         p->d_replicas = nullptr;
         p->replicas_failed = true;
         (void)hipGetLastError();
-      } else if (hipMemsetAsync(p->d_replicas, 0, bytes, c->stream) != hipSuccess) {
+      } else if (apirl::launch_zero_replicas(p->d_replicas, bytes / sizeof(float), c->stream) !=
+                 hipSuccess) {
         (void)hipFree(p->d_replicas);
         p->d_replicas = nullptr;
         return fail(c, APIRL_ERR_HIP, "cannot clear the scatter backprojector's copies");

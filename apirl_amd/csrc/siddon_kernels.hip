@@ -809,6 +809,20 @@ __global__ __launch_bounds__(kBlock) void likelihood_kernel(const float* __restr
 // slice per workgroup: the transposed sums pass through LDS, so every copy is read
 // and cleared with coalesced accesses.  Summation order per voxel: the image-layout
 // copies by die, then the transposed ones by die.
+// The copies are read and cleared with device-scope accesses: every die's atomics and
+// the next launch's see the same values whichever die's L2 last held a line.
+__device__ __forceinline__ float rep_load(const float* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void rep_clear(float* p) {
+  __hip_atomic_store(p, 0.f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__global__ __launch_bounds__(kBlock) void zero_floats_kernel(float* __restrict__ p, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) rep_clear(p + i);
+}
+
 __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restrict__ img,
                                                                  float* __restrict__ rep, int nx,
                                                                  int ny) {
@@ -824,8 +838,8 @@ __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restri
       const long i = base + (long)x * ny + y;
 #pragma unroll
       for (int c = kNumXcd; c < 2 * kNumXcd; c++) {
-        acc += rep[(long)c * n + i];
-        rep[(long)c * n + i] = 0.f;
+        acc += rep_load(&rep[(long)c * n + i]);
+        rep_clear(&rep[(long)c * n + i]);
       }
     }
     tile[r][tx] = acc;
@@ -838,8 +852,8 @@ __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restri
       float acc = 0.f;
 #pragma unroll
       for (int c = 0; c < kNumXcd; c++) {
-        acc += rep[(long)c * n + i];
-        rep[(long)c * n + i] = 0.f;
+        acc += rep_load(&rep[(long)c * n + i]);
+        rep_clear(&rep[(long)c * n + i]);
       }
       img[i] += acc + tile[tx][r];
     }
@@ -1016,6 +1030,13 @@ hipError_t launch_count_intersections(const apirl_plan* p, int subset, int nsubs
   if (nrays == 0) return hipSuccess;
   hipLaunchKernelGGL(count_kernel, dim3(grid_for(nrays)), dim3(kBlock), 0, st, g, p->d_rays,
                      p->d_zab, nang_sub, subset, nsubsets, nrays, d_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_replicas(float* rep, size_t n, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(zero_floats_kernel, dim3(grid_for((long)n)), dim3(kBlock), 0, st, rep,
+                     (long)n);
   return hipGetLastError();
 }
 
