@@ -836,11 +836,13 @@ __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restri
     float acc = 0.f;
     if (x < nx && y < ny) {
       const long i = base + (long)x * ny + y;
+      float v[kNumXcd];
 #pragma unroll
-      for (int c = kNumXcd; c < 2 * kNumXcd; c++) {
-        acc += rep_load(&rep[(long)c * n + i]);
-        rep_clear(&rep[(long)c * n + i]);
-      }
+      for (int c = 0; c < kNumXcd; c++) v[c] = rep_load(&rep[(long)(kNumXcd + c) * n + i]);
+#pragma unroll
+      for (int c = 0; c < kNumXcd; c++) acc += v[c];
+#pragma unroll
+      for (int c = 0; c < kNumXcd; c++) rep_clear(&rep[(long)(kNumXcd + c) * n + i]);
     }
     tile[r][tx] = acc;
   }
@@ -849,12 +851,13 @@ __global__ __launch_bounds__(kBlock) void reduce_replicas_kernel(float* __restri
     const int y = y0 + r, x = x0 + tx;
     if (x < nx && y < ny) {
       const long i = base + (long)y * nx + x;
-      float acc = 0.f;
+      float acc = 0.f, v[kNumXcd];
 #pragma unroll
-      for (int c = 0; c < kNumXcd; c++) {
-        acc += rep_load(&rep[(long)c * n + i]);
-        rep_clear(&rep[(long)c * n + i]);
-      }
+      for (int c = 0; c < kNumXcd; c++) v[c] = rep_load(&rep[(long)c * n + i]);
+#pragma unroll
+      for (int c = 0; c < kNumXcd; c++) acc += v[c];
+#pragma unroll
+      for (int c = 0; c < kNumXcd; c++) rep_clear(&rep[(long)c * n + i]);
       img[i] += acc + tile[tx][r];
     }
   }
