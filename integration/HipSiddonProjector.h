@@ -33,8 +33,8 @@
 class HipSiddonProjector : virtual Projector {
  public:
   /* gpuId: device index; backprojectionMode: APIRL_BP_GATHER (deterministic, the
-   * default; Backproject and DivideAndBackproject alike) or APIRL_BP_SCATTER (fp32
-   * atomics). */
+   * default; Backproject and DivideAndBackproject alike), APIRL_BP_TILE (3D only,
+   * deterministic, the fastest) or APIRL_BP_SCATTER (fp32 atomics). */
   HipSiddonProjector(int gpuId = 0, int backprojectionMode = APIRL_BP_GATHER);
   /* SiddonProjector(nSamplesOnDetector, nAxialSamplesOnDetector)
    * (recon/src/SiddonProjector.cpp:28-32): 3D bins of a cylindrical scanner traced as
