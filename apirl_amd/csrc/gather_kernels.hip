@@ -1179,12 +1179,24 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// One in-flight ray of a wavefront, in LDS: its entry and path (the lanes walking
+// its sinograms read both).
+struct alignas(16) TileSlot {
+  TileRay R;
+  TilePath P;
+};
+static_assert(sizeof(TileSlot) == 128, "TileSlot is 32 words");
+constexpr int kTSlots = 8;    // rays in flight per wavefront
+constexpr int kTRing = 128;   // pending (ray, sinogram) entries per wavefront
+
 // Bytes of one wavefront's LDS: the item's sums (double: LDS adds of doubles run
-// about 20 times faster than of floats here, and sum more exactly), the current
-// ray's path, then its candidate list (16-bit sinogram indices), rounded to 16 bytes.
+// about 20 times faster than of floats here, and sum more exactly), the rays in
+// flight, and the ring of pending (ray, sinogram) entries.
 __host__ __device__ inline int tile_wave_bytes(int zs, int nsino) {
-  return (kTW * kTW * zs * 8 + (int)sizeof(TilePath) + nsino * 2 + 15) / 16 * 16;
+  (void)nsino;
+  return kTW * kTW * zs * 8 + kTSlots * (int)sizeof(TileSlot) + kTRing * 4;
 }
+
 
 // Candidate ray c of angle ka for tile `tile` (slot (tile * nang_sub + ka) * maxc + c):
 // the c-th bin of the angle's candidate range, kept when its bins are not all zero,
@@ -1286,7 +1298,9 @@ __global__ __launch_bounds__(kGBlock) void tile_compact_kernel(TileRay* __restri
 
 // Items (tile, slab) are taken from a queue in the order of order[] (the tiles
 // nearest the image centre first, whose rays are the most), counter at *next.  The
-// rays entering tile t are trays[t * nslot + c], c < tcount[t].
+// rays entering tile t are trays[t * nslot + c], c < tcount[t].  The (ray, sinogram)
+// pairs to walk go through a ring in LDS: a pass takes 64 of them as soon as there are,
+// so a pass carries the sinograms of several rays.
 __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, int nslab,
                                                           const int* __restrict__ order,
                                                           int nitems, int* __restrict__ next,
@@ -1300,8 +1314,8 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
   const int lane = threadIdx.x & (kWave - 1);
   const int ntx = (g.nx + kTW - 1) / kTW;
   double* acc = (double*)((char*)tlds + (size_t)w * tile_wave_bytes(zs, g.nsino));
-  TilePath* s_path = (TilePath*)(acc + kTW * kTW * zs);
-  unsigned short* list = (unsigned short*)(s_path + 1);
+  TileSlot* slots = (TileSlot*)(acc + kTW * kTW * zs);
+  unsigned* ring = (unsigned*)(slots + kTSlots);
   // the axial table, shared by the workgroup's wavefronts
   float2* s_zab = (float2*)((char*)tlds + (size_t)kTWaves * tile_wave_bytes(zs, g.nsino));
   for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
@@ -1328,21 +1342,45 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
     const float zlo = g.offz + kz0 * g.dz - kZMargin;
     const float zhi = g.offz + (kz1 + 1) * g.dz + kZMargin;
     const TileRay* tr = trays + (long)tile * nslot;
+    const unsigned* trw = (const unsigned*)tr;
     const unsigned* tpw = (const unsigned*)(tpaths + (long)tile * nslot);
     const int nray = tcount[tile];
-    TileRay Rn;  // the next ray, loaded while this one is processed
-    if (nray > 0) Rn = tr[0];
+    int head = 0, pend = 0;  // ring entries (ray << 16 | sinogram), head .. head + pend - 1
+    // one pass: lanes l < k walk entry head + l
+    auto run_pass = [&](int k) {
+      wave_lds_sync();
+      if (lane < k) {
+        const unsigned e = ring[(head + lane) & (kTRing - 1)];
+        const TileSlot& S = slots[(e >> 16) & (kTSlots - 1)];
+        const ZRay z = load_zray(A.zr + (long)S.R.pair * g.nsino + (e & 0xffffu));
+        if (z.b != 0.f)  // SiddonProjector.cpp:496
+          tile_lane(S.R, S.P.ae, S.P.col, z, kz0, kz1, zs, acc);
+      }
+      head = (head + k) & (kTRing - 1);
+      pend -= k;
+    };
     for (int c = 0; c < nray; c++) {
-      const TileRay R = Rn;
-      if (c + 1 < nray) Rn = tr[c + 1];
+      // the slot of ray c is free once no pending entry belongs to ray c - kTSlots
+      while (pend > 0) {
+        wave_lds_sync();
+        if ((int)(ring[head] >> 16) + kTSlots > c) break;
+        run_pass(pend < kWave ? pend : kWave);
+      }
+      wave_lds_sync();
+      if (lane < (int)(sizeof(TileRay) / 4))
+        ((unsigned*)&slots[c & (kTSlots - 1)].R)[lane] =
+            trw[(long)c * (sizeof(TileRay) / 4) + lane];
+      else if (lane < (int)(sizeof(TileSlot) / 4))
+        ((unsigned*)&slots[c & (kTSlots - 1)].R)[lane] =
+            tpw[(long)c * (sizeof(TilePath) / 4) + lane - (int)(sizeof(TileRay) / 4)];
+      const TileRay R = tr[c];
       const TileEntry& E = R.E;
       // sinograms whose ray can reach the slab inside the tile: per run, the
       // indices whose mean z lies within the slab widened by the run's slopes over
       // the tile's alpha range (z = zmean + (zb - za)(alpha - 1/2)) and its deviation
       // from the line (one run per lane); of those, the sinograms whose ray's z over
-      // that alpha range meets the slab are listed, run by run
+      // that alpha range meets the slab go to the ring, run by run
       const float ulo = E.ce - 0.5f, uhi = E.aout - 0.5f;
-      int n = 0;
       for (int sb = 0; sb < A.nseg; sb += kWave) {
         int s0 = 0, cnt = 0;
         if (sb + lane < A.nseg) {
@@ -1373,29 +1411,18 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
             }
             const unsigned long long bal = __ballot(keep);
             if (keep)
-              list[n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                 __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))] =
-                  (unsigned short)s;
-            n += __popcll(bal);
+              ring[(head + pend +
+                    __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))) &
+                   (kTRing - 1)] = ((unsigned)c << 16) | (unsigned)s;
+            pend += __popcll(bal);
+            if (pend >= kWave) run_pass(kWave);
           }
         }
       }
-      // the path, from global memory to LDS (20 words)
-      if (lane < (int)(sizeof(TilePath) / 4))
-        ((unsigned*)s_path)[lane] = tpw[(long)c * (sizeof(TilePath) / 4) + lane];
-      wave_lds_sync();
-      const ZRay* zp = A.zr + (long)R.pair * g.nsino;
-      // each pass's records are loaded while the previous pass walks
-      ZRay zn;
-      if (lane < n) zn = load_zray(zp + list[lane]);
-      for (int base = 0; base < n; base += kWave) {
-        const ZRay z = zn;
-        if (base + kWave + lane < n) zn = load_zray(zp + list[base + kWave + lane]);
-        if (base + lane >= n || z.b == 0.f) continue;  // SiddonProjector.cpp:496
-        tile_lane(R, s_path->ae, s_path->col, z, kz0, kz1, zs, acc);
-      }
-      wave_lds_sync();
     }
+    while (pend > 0) run_pass(pend < kWave ? pend : kWave);
+    wave_lds_sync();
     if (incol)
       for (int kl = 0; kl <= kz1 - kz0; kl++)
         store_voxel(A, (kz0 + kl) * nxy + v0, (float)acc[lane * zs + kl]);
@@ -1478,9 +1505,26 @@ size_t tile_lds_bytes(const apirl_plan* p) {
          (size_t)p->geo.nsino * sizeof(float2);
 }
 
+// Upper bound on the candidate bins of one (tile, angle): the bins within the tile's
+// projected half-width (plus the search margin, and one lookup-table step on each
+// side on a non-uniform grid) of its centre, for each parity.
+int tile_max_candidates(const apirl_plan* p) {
+  const Geo& g = p->geo;
+  double drmin = g.dr > 0 ? g.dr : 1.0;
+  for (size_t j = 1; j < p->r.size(); j++) drmin = std::min(drmin, (double)p->r[j] - p->r[j - 1]);
+  if (!(drmin > 0)) drmin = 1e-3;
+  const double hw = 0.5 * kTW * std::max(g.dx, g.dy) * 1.41421357 + g.rmargin + 1e-3;
+  const double h = p->rl_inv_h > 0 ? 1.0 / p->rl_inv_h : 0.0;
+  const int one = (int)std::ceil((2 * hw + 2 * h) / drmin) + 3;
+  const int c = p->d_rlut ? 2 * one : one;
+  return c < g.nr ? c : g.nr;
+}
+
 const char* tile_misfit(const apirl_plan* p) {
   if (p->geo.dims != 3) return "the tile backprojector is 3D only";
   if (p->geo.nsino >= 65536) return "the tile backprojector takes fewer than 65536 sinograms";
+  if ((long)p->geo.nang * tile_max_candidates(p) >= 65536)
+    return "the tile backprojector takes fewer than 65536 candidate rays per tile";
   if (tile_lds_bytes(p) > 65536) return "the tile backprojector's items exceed 64 KiB of LDS";
   return nullptr;
 }
@@ -1507,21 +1551,6 @@ void build_tile_order(const apirl_plan* p, std::vector<int>& order) {
   order.clear();
   for (const auto& t : tiles)
     for (int s = 0; s < nslab; s++) order.push_back(t.second * nslab + s);
-}
-
-// Upper bound on the candidate bins of one (tile, angle): the bins within the tile's
-// projected half-width (plus the search margin, and one lookup-table step on each
-// side on a non-uniform grid) of its centre, for each parity.
-int tile_max_candidates(const apirl_plan* p) {
-  const Geo& g = p->geo;
-  double drmin = g.dr > 0 ? g.dr : 1.0;
-  for (size_t j = 1; j < p->r.size(); j++) drmin = std::min(drmin, (double)p->r[j] - p->r[j - 1]);
-  if (!(drmin > 0)) drmin = 1e-3;
-  const double hw = 0.5 * kTW * std::max(g.dx, g.dy) * 1.41421357 + g.rmargin + 1e-3;
-  const double h = p->rl_inv_h > 0 ? 1.0 / p->rl_inv_h : 0.0;
-  const int one = (int)std::ceil((2 * hw + 2 * h) / drmin) + 3;
-  const int c = p->d_rlut ? 2 * one : one;
-  return c < g.nr ? c : g.nr;
 }
 
 size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
