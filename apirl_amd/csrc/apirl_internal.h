@@ -214,9 +214,14 @@ struct FusedRatio {
   // Sinogram3D::addBinToBin (Sinogram3D.cpp:857-873) reads it
   int additive_local = 0;
 };
+// 3D, reference precision, one LOR per bin (path_forward_ok): the path forward
+// projector, which writes project_partials(.., true) likelihood partials (one per
+// subset (angle, bin)) instead of one per 256 bins.
 hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, int subset,
                           int nsubsets, int precision, const FusedRatio* fused, int* nblocks_out,
                           hipStream_t st);
+bool path_forward_ok(const apirl_plan* p, int precision);
+int project_partials(const apirl_plan* p, int subset, int nsubsets, bool path);
 hipError_t launch_backproject(const apirl_plan* p, const float* sino, const float* est,
                               float* img, int subset, int nsubsets, int kind, int mode,
                               hipStream_t st);

@@ -888,8 +888,8 @@ apirl_status project_ratio_impl(apirl_plan* p, const float* d_img, const float* 
   if (!os_precision_ok(p, precision))
     return fail(c, APIRL_ERR_UNSUPPORTED, "oversampled LORs use the reference precision");
   DeviceGuard g(c->device);
-  const long nrays = (long)apirl_plan_subset_bins(p, subset, nsubsets);
-  const int nblocks = (int)((nrays + 255) / 256);
+  const int nblocks =
+      apirl::project_partials(p, subset, nsubsets, apirl::path_forward_ok(p, precision));
   apirl::FusedRatio fr;
   fr.meas = d_meas;
   fr.additive = d_additive;
@@ -900,7 +900,10 @@ apirl_status project_ratio_impl(apirl_plan* p, const float* d_img, const float* 
   fr.additive_local = (flags & APIRL_RATIO_ADDITIVE_LOCAL) ? 1 : 0;
   if (d_lik) {
     if (noalloc) {
-      if (nblocks > apirl::kNoallocPartials)
+      // the documented limit in bins (one partial per 256 bins at most, whichever
+      // projector kernel runs)
+      if (apirl_plan_subset_bins(p, subset, nsubsets) > (size_t)APIRL_NOALLOC_MAX_BINS ||
+          nblocks > apirl::kNoallocPartials)
         return fail(c, APIRL_ERR_INVALID,
                     "apirl_project_ratio_noalloc: the likelihood of more than "
                     "APIRL_NOALLOC_MAX_BINS bins per call needs apirl_project_ratio");

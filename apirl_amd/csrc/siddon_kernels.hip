@@ -947,7 +947,167 @@ __global__ __launch_bounds__(kBlock) void update_group_kernel(BatchHeader* __res
 
 inline unsigned grid_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
+
+// ---- Path forward projector (3D, APIRL_PREC_REFERENCE, one LOR per bin) ------------
+//
+// The rays of one (angle, radial bin) in every sinogram share their transaxial path:
+// the reference's loop takes its x and y steps in the same order whatever its z steps
+// (Siddon.cpp:318-339: a z step is taken only below both, and x before y on ties).
+// project_paths_kernel takes one (angle, bin) per workgroup, first writes that path to
+// LDS (the columns the loop visits and the alpha at which it leaves each, one thread
+// running the loop's x / y additions), then takes the sinograms one per lane: each
+// lane runs the reference's loop, taking a z step when its next z crossing is below
+// the column's exit alpha and the path's next column otherwise, and adds w * voxel in
+// the loop's order, so the bin is
+// the reference's float sum, bit for bit (as project_kernel's).  Voxels come from a
+// copy of the image with z fastest, where the lanes, at one column and neighbouring
+// slices, read neighbouring words.
+
+struct FwdStep {
+  float ae;     // alpha at which the loop leaves this column by an x or y step
+  short i, j;   // the column (may lie outside the image at the path's ends)
+};
+static_assert(sizeof(FwdStep) == 8, "FwdStep is 8 bytes");
+
+// Path length kept per (angle, bin): the transaxial list plus a few columns (a rounding
+// can trade a z step of the list for an x or y step).
+__host__ __device__ inline int fwd_path_len(const Geo& g) { return g.nx + g.ny + 4; }
+
+// out[(j nx + i) nz + k] = in[(k ny + j) nx + i] through a 32 (i) x 32 (k) LDS tile.
+__global__ __launch_bounds__(kBlock) void transpose_zfast_kernel(const float* __restrict__ in,
+                                                                 float* __restrict__ out, int nx,
+                                                                 int ny, int nz) {
+  __shared__ float tile[32][33];
+  const int i0 = blockIdx.x * 32, j = blockIdx.y, k0 = blockIdx.z * 32;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  const long nxy = (long)nx * ny;
+  for (int r = ty; r < 32; r += 8) {
+    const int k = k0 + r, i = i0 + tx;
+    if (k < nz && i < nx) tile[r][tx] = in[(long)k * nxy + (long)j * nx + i];
+  }
+  __syncthreads();
+  for (int r = ty; r < 32; r += 8) {
+    const int i = i0 + r, k = k0 + tx;
+    if (i < nx && k < nz) out[((long)j * nx + i) * nz + k] = tile[tx][r];
+  }
+}
+
+template <bool FUSED>
+__global__ __launch_bounds__(kBlock) void project_paths_kernel(
+    Geo g, const RayXY* __restrict__ rays, const float2* __restrict__ zab,
+    const float* __restrict__ img, const float* __restrict__ imgZ, float* __restrict__ sino,
+    int nang_sub, int subset, int nsubsets, FusedRatio fr, int qmax) {
+  extern __shared__ FwdStep sp[];
+  const int pair = blockIdx.x;
+  const int ka = pair / g.nr, rr = pair % g.nr;
+  const int a = subset + nsubsets * ka;
+  const RayXY t = rays[(long)a * g.nr + rr];
+  if (threadIdx.x == 0) {
+    const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
+    float Xn = t.ax, Yn = t.ay;
+    int i = t.i0, j = t.j0;
+    for (int q = 0; q < qmax; q++) {
+      FwdStep e;
+      e.ae = Xn < Yn ? Xn : Yn;
+      e.i = (short)i;
+      e.j = (short)j;
+      sp[q] = e;
+      if (Xn <= Yn) {
+        i += iinc;
+        Xn = Xn + t.axu;
+      } else {
+        j += jinc;
+        Yn = Yn + t.ayu;
+      }
+    }
+  }
+  __syncthreads();
+  const int nxy = g.nx * g.ny, npix = nxy * g.nz;
+  double lik = 0.0;
+  for (int s = threadIdx.x; s < g.nsino; s += kBlock) {
+    const float2 zz = zab[s];
+    Ray ray;
+    const bool valid = setup_ray<3>(g, t, zz.x, zz.y, ray);
+    float acc = 0.f;
+    if (valid) {
+      // Steps are taken kUnroll at a time: the batch's voxel loads are issued together,
+      // then added in the loop's order.
+      int q = 0, ci = sp[0].i, cj = sp[0].j, k = ray.k;
+      float ae = sp[0].ae, Zn = ray.az, ac = ray.ac;
+      for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
+        float wv[kUnroll], pv[kUnroll];
+        bool tv[kUnroll];
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const bool zst = Zn < ae;
+          const float an = zst ? Zn : ae;
+          wv[u] = (an - ac) * ray.L;
+          ac = an;
+          // the reference's read test (upper bounds and a non-negative linear index)
+          const int li = k * nxy + cj * g.nx + ci;
+          const bool ok = (ci < g.nx) & (cj < g.ny) & (k < g.nz) & (li >= 0) & (li < npix) &
+                          (m0 + u < ray.n);
+          const bool inside = (ci >= 0) & (cj >= 0);
+          tv[u] = ok;
+          pv[u] = ok ? (inside ? imgZ[(cj * g.nx + ci) * g.nz + k] : img[li]) : 0.f;
+          Zn = zst ? Zn + ray.azu : Zn;
+          k += zst ? ray.kinc : 0;
+          q += zst ? 0 : 1;
+          const FwdStep e = sp[q < qmax ? q : qmax - 1];
+          ae = e.ae;
+          ci = e.i;
+          cj = e.j;
+        }
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+          const float term = wv[u] * pv[u];
+          const float nacc = acc + term;
+          acc = tv[u] ? nacc : acc;
+        }
+      }
+    }
+    const long idx = ((long)s * nang_sub + ka) * g.nr + rr;
+    if (!FUSED) {
+      if (valid) sino[idx] = acc;
+    } else {
+      // fused EM epilogue, as project_ray's (Sinogram3D.cpp:584-607, 830-870)
+      const long full = ((long)s * g.nang + a) * g.nr + rr;
+      float est = valid ? acc : 0.f;
+      if (fr.additive)
+        est = est + fr.additive[fr.additive_local ? ((long)s * g.nang + ka) * g.nr + rr : full];
+      const float y = fr.meas[full];
+      const bool lik_term = fr.lik_positive_only ? (est > 0.f) : (est != 0.f);
+      if (lik_term) lik += (double)(y * logf(est) - est);
+      fr.ratio[idx] = (y != 0.f && est != 0.f) ? y / est : 0.f;
+    }
+  }
+  if (FUSED && fr.lik_partials) {
+    // deterministic block reduction of the likelihood partials (fixed tree)
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = lik;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) fr.lik_partials[blockIdx.x] = red[0];
+  }
+}
+
 }  // namespace
+
+bool path_forward_ok(const apirl_plan* p, int precision) {
+  const Geo& g = p->geo;
+  return g.dims == 3 && precision == APIRL_PREC_REFERENCE && g.ns * g.nas == 1 && g.walk24 &&
+         p->d_imgT != nullptr && (size_t)fwd_path_len(g) * sizeof(FwdStep) <= 48 * 1024;
+}
+
+int project_partials(const apirl_plan* p, int subset, int nsubsets, bool path) {
+  const Geo& g = p->geo;
+  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  if (path) return nang_sub * g.nr;
+  return (int)grid_for((long)g.nsino * nang_sub * g.nr);
+}
 
 hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, int subset,
                           int nsubsets, int precision, const FusedRatio* fused, int* nblocks_out,
@@ -960,6 +1120,25 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
   if (nrays == 0) return hipSuccess;
   FusedRatio fr = fused ? *fused : FusedRatio{};
   const bool f = fused != nullptr;
+  if (path_forward_ok(p, precision)) {
+    // the path forward projector: the image with z fastest (into the plan's
+    // transposed-image buffer), one workgroup per (angle, bin) of the subset
+    const int npair = nang_sub * g.nr, qmax = fwd_path_len(g);
+    if (nblocks_out) *nblocks_out = npair;
+    const dim3 tg((g.nx + 31) / 32, g.ny, (g.nz + 31) / 32);
+    hipLaunchKernelGGL(transpose_zfast_kernel, tg, dim3(kBlock), 0, st, img, p->d_imgT, g.nx,
+                       g.ny, g.nz);
+    const size_t lds = (size_t)qmax * sizeof(FwdStep);
+    if (f)
+      hipLaunchKernelGGL(project_paths_kernel<true>, dim3(npair), dim3(kBlock), lds, st, g,
+                         p->d_rays, p->d_zab, img, p->d_imgT, sino, nang_sub, subset, nsubsets, fr,
+                         qmax);
+    else
+      hipLaunchKernelGGL(project_paths_kernel<false>, dim3(npair), dim3(kBlock), lds, st, g,
+                         p->d_rays, p->d_zab, img, p->d_imgT, sino, nang_sub, subset, nsubsets, fr,
+                         qmax);
+    return hipGetLastError();
+  }
   const float* imgT = nullptr;
   if (p->d_imgT && precision == APIRL_PREC_REFERENCE) {
     const dim3 tg((g.nx + 31) / 32, (g.ny + 31) / 32, g.nz);
