@@ -965,9 +965,11 @@ inline unsigned grid_for(long n) { return (unsigned)((n + kBlock - 1) / kBlock);
 
 struct FwdStep {
   float ae;     // alpha at which the loop leaves this column by an x or y step
+  int base;     // (j nx + i) nz: the column's first voxel in the z-fastest copy
   short i, j;   // the column (may lie outside the image at the path's ends)
+  int run;      // columns inside the image from this one on, this one included
 };
-static_assert(sizeof(FwdStep) == 8, "FwdStep is 8 bytes");
+static_assert(sizeof(FwdStep) == 16, "FwdStep is 16 bytes");
 
 // Path length kept per (angle, bin): the transaxial list plus a few columns (a rounding
 // can trade a z step of the list for an x or y step).
@@ -1003,14 +1005,18 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
   const int a = subset + nsubsets * ka;
   const RayXY t = rays[(long)a * g.nr + rr];
   if (threadIdx.x == 0) {
+    // qmax + 1 entries: a fast batch that starts kUnroll columns before the list's end
+    // reads the entry after it
     const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
     float Xn = t.ax, Yn = t.ay;
     int i = t.i0, j = t.j0;
-    for (int q = 0; q < qmax; q++) {
+    for (int q = 0; q <= qmax; q++) {
       FwdStep e;
       e.ae = Xn < Yn ? Xn : Yn;
       e.i = (short)i;
       e.j = (short)j;
+      e.base = (j * g.nx + i) * g.nz;
+      e.run = 0;
       sp[q] = e;
       if (Xn <= Yn) {
         i += iinc;
@@ -1019,6 +1025,12 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
         j += jinc;
         Yn = Yn + t.ayu;
       }
+    }
+    int run = 0;
+    for (int q = qmax; q >= 0; q--) {
+      const bool in = sp[q].i >= 0 && sp[q].i < g.nx && sp[q].j >= 0 && sp[q].j < g.ny;
+      run = in ? run + 1 : 0;
+      sp[q].run = run;
     }
   }
   __syncthreads();
@@ -1032,31 +1044,53 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
     if (valid) {
       // Steps are taken kUnroll at a time: the batch's voxel loads are issued together,
       // then added in the loop's order.
-      int q = 0, ci = sp[0].i, cj = sp[0].j, k = ray.k;
-      float ae = sp[0].ae, Zn = ray.az, ac = ray.ac;
+      int q = 0, k = ray.k;
+      FwdStep e = sp[0];
+      float Zn = ray.az, ac = ray.ac;
       for (int m0 = 0; m0 < ray.n; m0 += kUnroll) {
         float wv[kUnroll], pv[kUnroll];
+        // a batch whose kUnroll steps all stay in the image (the path's next kUnroll
+        // columns inside; the slice inside and kUnroll - 1 z steps from the face it
+        // moves towards) needs no read test; the wavefront takes it when every lane can
+        const int kfar = k + (kUnroll - 1) * ray.kinc;
+        const bool fast = (m0 + kUnroll <= ray.n) & (e.run >= kUnroll) & (k >= 0) &
+                          (k < g.nz) & (kfar >= 0) & (kfar < g.nz);
+        if (__all(fast)) {
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) {
+            const bool zst = Zn < e.ae;
+            const float an = zst ? Zn : e.ae;
+            wv[u] = (an - ac) * ray.L;
+            ac = an;
+            pv[u] = imgZ[e.base + k];
+            Zn = zst ? Zn + ray.azu : Zn;
+            k += zst ? ray.kinc : 0;
+            q += zst ? 0 : 1;
+            e = sp[q];
+          }
+#pragma unroll
+          for (int u = 0; u < kUnroll; u++) acc = acc + wv[u] * pv[u];
+          continue;
+        }
         bool tv[kUnroll];
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
-          const bool zst = Zn < ae;
-          const float an = zst ? Zn : ae;
+          const bool zst = Zn < e.ae;
+          const float an = zst ? Zn : e.ae;
           wv[u] = (an - ac) * ray.L;
           ac = an;
           // the reference's read test (upper bounds and a non-negative linear index)
+          const int ci = e.i, cj = e.j;
           const int li = k * nxy + cj * g.nx + ci;
           const bool ok = (ci < g.nx) & (cj < g.ny) & (k < g.nz) & (li >= 0) & (li < npix) &
                           (m0 + u < ray.n);
           const bool inside = (ci >= 0) & (cj >= 0);
           tv[u] = ok;
-          pv[u] = ok ? (inside ? imgZ[(cj * g.nx + ci) * g.nz + k] : img[li]) : 0.f;
+          pv[u] = ok ? (inside ? imgZ[e.base + k] : img[li]) : 0.f;
           Zn = zst ? Zn + ray.azu : Zn;
           k += zst ? ray.kinc : 0;
           q += zst ? 0 : 1;
-          const FwdStep e = sp[q < qmax ? q : qmax - 1];
-          ae = e.ae;
-          ci = e.i;
-          cj = e.j;
+          e = sp[q < qmax ? q : qmax - 1];
         }
 #pragma unroll
         for (int u = 0; u < kUnroll; u++) {
@@ -1099,7 +1133,7 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
 bool path_forward_ok(const apirl_plan* p, int precision) {
   const Geo& g = p->geo;
   return g.dims == 3 && precision == APIRL_PREC_REFERENCE && g.ns * g.nas == 1 && g.walk24 &&
-         p->d_imgT != nullptr && (size_t)fwd_path_len(g) * sizeof(FwdStep) <= 48 * 1024;
+         p->d_imgT != nullptr && (size_t)(fwd_path_len(g) + 1) * sizeof(FwdStep) <= 48 * 1024;
 }
 
 int project_partials(const apirl_plan* p, int subset, int nsubsets, bool path) {
@@ -1128,7 +1162,7 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
     const dim3 tg((g.nx + 31) / 32, g.ny, (g.nz + 31) / 32);
     hipLaunchKernelGGL(transpose_zfast_kernel, tg, dim3(kBlock), 0, st, img, p->d_imgT, g.nx,
                        g.ny, g.nz);
-    const size_t lds = (size_t)qmax * sizeof(FwdStep);
+    const size_t lds = (size_t)(qmax + 1) * sizeof(FwdStep);  // 8.3 KB for 256 x 256
     if (f)
       hipLaunchKernelGGL(project_paths_kernel<true>, dim3(npair), dim3(kBlock), lds, st, g,
                          p->d_rays, p->d_zab, img, p->d_imgT, sino, nang_sub, subset, nsubsets, fr,
