@@ -136,6 +136,20 @@ struct apirl_plan {
   int tile_zs = 0;                 // tile backprojector's slab depth (0: about 6 slices)
   int* d_tile_order = nullptr;     // its item order (built for tile_order_n items)
   int tile_order_n = 0;
+  // its geometry per (LOR form, subset, subsets), kept from the first call on, within
+  // tile_geo_budget bytes in all (apirl_tuning.tile_cache_mb); `ready` once the launches
+  // that build it are enqueued on `stream`, whose event `ev` follows them
+  struct TileGeo {
+    int form, subset, nsubsets;
+    void* d;
+    size_t bytes;
+    bool ready;
+    hipStream_t stream;
+    hipEvent_t ev;
+  };
+  std::vector<TileGeo> tile_geo;
+  size_t tile_geo_used = 0;
+  size_t tile_geo_budget = (size_t)1024 << 20;
   // its tile maps (APIRL_GATHER_XCD_WEDGES), one per voxels-per-lane depth 1..4, built
   // on first use and kept until the plan ends (captured launches may hold them)
   int* d_tilemap[5] = {};
@@ -267,14 +281,20 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
                                      int subset, int nsubsets, int kind, int precision,
                                      void* scratch, size_t scratch_bytes, hipStream_t st,
                                      const EmFuse* fuse = nullptr);
-// Tile-walk backprojector (APIRL_BP_TILE): same scratch as the gather; tile_misfit says
-// why a plan cannot take it (null when it can).
+// Tile-walk backprojector (APIRL_BP_TILE); tile_misfit says why a plan cannot take it
+// (null when it can).  Its geometry for one subset (the candidate rays of every tile
+// and their paths; tile_geo_bytes) depends on the plan's LOR form alone: with `geo`
+// null it is built in the scratch on every call; otherwise it lives in `geo`, built
+// there when `geo_ready` is false and reused as it stands when true.  The scratch
+// (tile_scratch_bytes, with the geometry when `geo` is null) holds the call's axial
+// records with their bin values and its live-ray lists.
 hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
                                    int subset, int nsubsets, int kind, void* scratch,
-                                   size_t scratch_bytes, hipStream_t st,
-                                   const EmFuse* fuse = nullptr);
+                                   size_t scratch_bytes, hipStream_t st, const EmFuse* fuse,
+                                   void* geo, bool geo_ready);
 const char* tile_misfit(const apirl_plan* p);
-size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets);
+size_t tile_geo_bytes(const apirl_plan* p, int subset, int nsubsets);
+size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets, bool with_geo);
 int tile_items(const apirl_plan* p);
 // item order of the tile backprojector: the tiles nearest the image centre first
 void build_tile_order(const apirl_plan* p, std::vector<int>& order);

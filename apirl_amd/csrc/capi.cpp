@@ -203,6 +203,46 @@ bool ensure_tile_order(apirl_plan* p) {
   return true;
 }
 
+// The kept tile geometry of (the plan's LOR form, subset, subsets): found, or allocated
+// within the plan's budget (not ready until a call has built it).  Null when it does
+// not fit the budget or the stream is being captured (a capture would only record the
+// launches that build it): the call then builds it in its scratch.
+apirl_plan::TileGeo* tile_geo_for(apirl_plan* p, int subset, int nsubsets) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(p->ctx->stream, &cs) != hipSuccess ||
+      cs != hipStreamCaptureStatusNone) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  for (apirl_plan::TileGeo& e : p->tile_geo)
+    if (e.form == p->scanner && e.subset == subset && e.nsubsets == nsubsets) return &e;
+  const size_t bytes = apirl::tile_geo_bytes(p, subset, nsubsets);
+  if (bytes == 0 || p->tile_geo_used + bytes > p->tile_geo_budget) return nullptr;
+  apirl_plan::TileGeo e{p->scanner, subset, nsubsets, nullptr, bytes, false, nullptr, nullptr};
+  if (hipMalloc(&e.d, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    return nullptr;
+  }
+  if (hipEventCreateWithFlags(&e.ev, hipEventDisableTiming) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipFree(e.d);
+    return nullptr;
+  }
+  p->tile_geo_used += bytes;
+  p->tile_geo.push_back(e);
+  return &p->tile_geo.back();
+}
+
+void free_tile_geo(apirl_plan* p) {
+  for (apirl_plan::TileGeo& e : p->tile_geo) {
+    (void)hipEventSynchronize(e.ev);
+    (void)hipFree(e.d);
+    (void)hipEventDestroy(e.ev);
+  }
+  p->tile_geo.clear();
+  p->tile_geo_used = 0;
+}
+
 // Backprojection of one subset in either form.
 // Oversampled bins are traversed in the reference's arithmetic only (and need the
 // 24-bit index walk).
@@ -272,8 +312,10 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
   if (const char* why = apirl::gather_misfit(p, subset, nsubsets))
     return fail(c, APIRL_ERR_UNSUPPORTED,
                 std::string("gather: ") + why + " (use the scatter form)");
-  const size_t need = mode == APIRL_BP_TILE ? apirl::tile_scratch_bytes(p, subset, nsubsets)
-                                            : apirl::gather_scratch_bytes(p, subset, nsubsets);
+  apirl_plan::TileGeo* tg = mode == APIRL_BP_TILE ? tile_geo_for(p, subset, nsubsets) : nullptr;
+  const size_t need = mode == APIRL_BP_TILE
+                          ? apirl::tile_scratch_bytes(p, subset, nsubsets, tg == nullptr)
+                          : apirl::gather_scratch_bytes(p, subset, nsubsets);
   if (need > p->gscratch_bytes) {
     if (p->d_gscratch) {
       APIRL_HIP(c, hipStreamSynchronize(c->stream));
@@ -312,9 +354,16 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
   if (mode == APIRL_BP_TILE) {
     if (!ensure_tile_order(p))
       return fail(c, APIRL_ERR_NOMEM, "cannot allocate the tile backprojector's item order");
+    if (tg && tg->ready && tg->stream != c->stream)
+      APIRL_HIP(c, hipStreamWaitEvent(c->stream, tg->ev, 0));
     APIRL_HIP(c, apirl::launch_backproject_tile(p, sino, img, subset, nsubsets, kind,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream,
-                                                fuse));
+                                                fuse, tg ? tg->d : nullptr, tg && tg->ready));
+    if (tg && !tg->ready) {
+      APIRL_HIP(c, hipEventRecord(tg->ev, c->stream));
+      tg->ready = true;
+      tg->stream = c->stream;
+    }
     return APIRL_OK;
   }
   APIRL_HIP(c, apirl::launch_backproject_gather(p, sino, img, subset, nsubsets, kind, precision,
@@ -652,6 +701,8 @@ bool ensure_os_parts(apirl_plan* p) {
       h->gather_zvox = p->gather_zvox;
       h->gather_zlanes = p->gather_zlanes;
       h->gather_xcd = p->gather_xcd;
+      h->tile_zs = p->tile_zs;
+      h->tile_geo_budget = p->tile_geo_budget / ((size_t)ns * nas);  // the plan's budget
       (void)ensure_tile_map(h);
       p->os_parts.push_back(h);
     }
@@ -677,6 +728,7 @@ apirl_status apirl_plan_destroy(apirl_plan* p) {
   for (int* m : p->d_tilemap)
     if (m) (void)hipFree(m);
   if (p->d_tile_order) (void)hipFree(p->d_tile_order);
+  free_tile_geo(p);
   if (p->d_zsoff) (void)hipFree(p->d_zsoff);
   delete p;
   return APIRL_OK;
@@ -780,6 +832,7 @@ apirl_status apirl_plan_get_tuning(const apirl_plan* p, apirl_tuning* out) {
   out->gather_zvox = p->gather_zvox;
   out->gather_xcd = p->gather_xcd;
   out->tile_zs = p->tile_zs;
+  out->tile_cache_mb = (int)(p->tile_geo_budget >> 20);
   return APIRL_OK;
 }
 
@@ -806,6 +859,12 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     p->tile_zs = t->tile_zs;
   }
   DeviceGuard g(p->ctx->device);
+  if (t->tile_cache_mb) {
+    if (t->tile_cache_mb < -1)
+      return fail(p->ctx, APIRL_ERR_INVALID, "tile_cache_mb must be positive, or -1 for none");
+    p->tile_geo_budget = t->tile_cache_mb < 0 ? 0 : (size_t)t->tile_cache_mb << 20;
+    if (p->tile_geo_used > p->tile_geo_budget) free_tile_geo(p);
+  }
   if (!ensure_tile_map(p))
     return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   for (apirl_plan* h : p->os_parts) {  // the oversampled plan's gathers follow it
@@ -813,6 +872,11 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     h->gather_zvox = p->gather_zvox;
     h->gather_xcd = p->gather_xcd;
     h->tile_zs = p->tile_zs;
+    const size_t share = p->tile_geo_budget / p->os_parts.size();
+    if (h->tile_geo_budget != share) {
+      h->tile_geo_budget = share;
+      if (h->tile_geo_used > h->tile_geo_budget) free_tile_geo(h);
+    }
     if (!ensure_tile_map(h))
       return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   }

@@ -126,7 +126,7 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   // bin, sinogram)
   if (tmajor) out[((long)ka * g.nr + rr) * g.nsino + s] = z;
   else out[idx] = z;
-  if (b != 0.f && (R.flags & RAY_VALID)) {
+  if (nz_pair && b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
     if (*f == 0) atomicOr(f, 1);
   }
@@ -1230,7 +1230,7 @@ __global__ __launch_bounds__(kGBlock) void tile_rays_kernel(GatherArgs A, int ma
   if (rr <= jhi && (!grid_lut<kGridAny>(A) || in_parity_range(rr, lo0, hi0, lo1, hi1))) {
     const int pair = ka * g.nr + rr;
     const RayXY t = A.rays[(long)a * g.nr + rr];
-    if (A.nz_pair[pair] != 0 && (t.flags & RAY_VALID) &&
+    if ((t.flags & RAY_VALID) &&
         tile_entry(g, t, A.xytab + ((long)a * g.nr + rr) * (g.nx + g.ny), ti0, ti1, tj0, tj1,
                    R.E)) {
       R.pair = pair;
@@ -1296,18 +1296,47 @@ __global__ __launch_bounds__(kGBlock) void tile_compact_kernel(TileRay* __restri
   if (lane == 0) count[tile] = n;
 }
 
+// Per tile, the positions (in trays order) of its rays with a non-zero bin, live[tile *
+// nslot + c] for c < nlive[tile] (one wavefront per tile).
+__global__ __launch_bounds__(kGBlock) void tile_live_kernel(const TileRay* __restrict__ trays,
+                                                            const int* __restrict__ count,
+                                                            const int* __restrict__ nz_pair,
+                                                            int nslot, int ntiles,
+                                                            int* __restrict__ live,
+                                                            int* __restrict__ nlive) {
+  const int tile = blockIdx.x * kTWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (tile >= ntiles) return;
+  const TileRay* tr = trays + (long)tile * nslot;
+  int* lv = live + (long)tile * nslot;
+  const int nc = count[tile];
+  int n = 0;
+  for (int c0 = 0; c0 < nc; c0 += kWave) {
+    const int c = c0 + lane;
+    const bool keep = c < nc && nz_pair[tr[c].pair] != 0;
+    const unsigned long long bal = __ballot(keep);
+    if (keep)
+      lv[n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                       __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))] = c;
+    n += __popcll(bal);
+  }
+  if (lane == 0) nlive[tile] = n;
+}
+
 // Items (tile, slab) are taken from a queue in the order of order[] (the tiles
 // nearest the image centre first, whose rays are the most), counter at *next.  The
-// rays entering tile t are trays[t * nslot + c], c < tcount[t].  The (ray, sinogram)
-// pairs to walk go through a ring in LDS: a pass takes 64 of them as soon as there are,
-// so a pass carries the sinograms of several rays.
+// rays entering tile t are trays[t * nslot + c]; those with a non-zero bin are at the
+// positions live[t * nslot + c'], c' < nlive[t].  The (ray, sinogram) pairs to walk
+// go through a ring in LDS: a pass takes 64 of them as soon as there are, so a pass
+// carries the sinograms of several rays.
 __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, int nslab,
                                                           const int* __restrict__ order,
                                                           int nitems, int* __restrict__ next,
                                                           const TileRay* __restrict__ trays,
                                                           const TilePath* __restrict__ tpaths,
                                                           int nslot,
-                                                          const int* __restrict__ tcount) {
+                                                          const int* __restrict__ live,
+                                                          const int* __restrict__ nlive) {
   extern __shared__ double tlds[];
   const Geo& g = A.g;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1344,8 +1373,10 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
     const TileRay* tr = trays + (long)tile * nslot;
     const unsigned* trw = (const unsigned*)tr;
     const unsigned* tpw = (const unsigned*)(tpaths + (long)tile * nslot);
-    const int nray = tcount[tile];
+    const int* lv = live + (long)tile * nslot;
+    const int nray = nlive[tile];
     int head = 0, pend = 0;  // ring entries (ray << 16 | sinogram), head .. head + pend - 1
+    int lvc = 0;             // live positions c0 .. c0 + 63, one per lane
     // one pass: lanes l < k walk entry head + l
     auto run_pass = [&](int k) {
       wave_lds_sync();
@@ -1367,13 +1398,15 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
         run_pass(pend < kWave ? pend : kWave);
       }
       wave_lds_sync();
+      if ((c & (kWave - 1)) == 0) lvc = c + lane < nray ? lv[c + lane] : 0;
+      const int rc = __builtin_amdgcn_readlane(lvc, c & (kWave - 1));  // in the tile's list
       if (lane < (int)(sizeof(TileRay) / 4))
         ((unsigned*)&slots[c & (kTSlots - 1)].R)[lane] =
-            trw[(long)c * (sizeof(TileRay) / 4) + lane];
+            trw[(long)rc * (sizeof(TileRay) / 4) + lane];
       else if (lane < (int)(sizeof(TileSlot) / 4))
         ((unsigned*)&slots[c & (kTSlots - 1)].R)[lane] =
-            tpw[(long)c * (sizeof(TilePath) / 4) + lane - (int)(sizeof(TileRay) / 4)];
-      const TileRay R = tr[c];
+            tpw[(long)rc * (sizeof(TilePath) / 4) + lane - (int)(sizeof(TileRay) / 4)];
+      const TileRay R = tr[rc];
       const TileEntry& E = R.E;
       // sinograms whose ray can reach the slab inside the tile: per run, the
       // indices whose mean z lies within the slab widened by the run's slopes over
@@ -1553,63 +1586,110 @@ void build_tile_order(const apirl_plan* p, std::vector<int>& order) {
     for (int s = 0; s < nslab; s++) order.push_back(t.second * nslab + s);
 }
 
-size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets) {
+namespace {
+size_t align256(size_t n) { return (n + 255) / 256 * 256; }
+
+struct TileSizes {
+  int nang_sub, maxc, ntiles;
+  long nrays, npair, nslots;
+};
+
+TileSizes tile_sizes(const apirl_plan* p, int subset, int nsubsets) {
   const Geo& g = p->geo;
-  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
-  const size_t npair = (size_t)nang_sub * g.nr;
-  const size_t ntiles = (size_t)((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
-  return (size_t)g.nsino * npair * sizeof(ZRay) + ((npair + 1) * sizeof(int) + 255) / 256 * 256 +
-         (ntiles * sizeof(int) + 255) / 256 * 256 +
-         ntiles * nang_sub * tile_max_candidates(p) * (sizeof(TileRay) + sizeof(TilePath));
+  TileSizes z;
+  z.nang_sub = subset_num_angles(g.nang, nsubsets, subset);
+  z.npair = (long)z.nang_sub * g.nr;
+  z.nrays = (long)g.nsino * z.npair;
+  z.maxc = tile_max_candidates(p);
+  z.ntiles = ((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
+  z.nslots = (long)z.ntiles * z.nang_sub * z.maxc;
+  return z;
+}
+}  // namespace
+
+// geometry: [ray count per tile][candidate rays of every (tile, angle)][their paths]
+size_t tile_geo_bytes(const apirl_plan* p, int subset, int nsubsets) {
+  const TileSizes z = tile_sizes(p, subset, nsubsets);
+  return align256((size_t)z.ntiles * sizeof(int)) +
+         (size_t)z.nslots * (sizeof(TileRay) + sizeof(TilePath));
+}
+
+// scratch: [ZRay x nrays, (angle, radial bin, sinogram) order][non-zero flag per
+// (angle, r), item counter][live positions per slot][live count per tile][the
+// geometry, when it is not kept]
+size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets, bool with_geo) {
+  const TileSizes z = tile_sizes(p, subset, nsubsets);
+  return align256((size_t)z.nrays * sizeof(ZRay)) + align256((size_t)(z.npair + 1) * sizeof(int)) +
+         align256((size_t)z.nslots * sizeof(int)) + align256((size_t)z.ntiles * sizeof(int)) +
+         (with_geo ? tile_geo_bytes(p, subset, nsubsets) : 0);
 }
 
 hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
                                    int subset, int nsubsets, int kind, void* scratch,
-                                   size_t scratch_bytes, hipStream_t st, const EmFuse* fuse) {
+                                   size_t scratch_bytes, hipStream_t st, const EmFuse* fuse,
+                                   void* geo, bool geo_ready) {
   if (kind != BP_PLAIN && kind != BP_ONES && kind != BP_MULT) return hipErrorNotSupported;
   if (tile_misfit(p)) return hipErrorNotSupported;
   const Geo& g = p->geo;
-  const int nang_sub = subset_num_angles(g.nang, nsubsets, subset);
-  const long nrays = (long)g.nsino * nang_sub * g.nr;
-  if (nrays == 0 || (long)g.nx * g.ny * g.nz == 0) return hipSuccess;
-  if (scratch_bytes < tile_scratch_bytes(p, subset, nsubsets)) return hipErrorInvalidValue;
+  const TileSizes z = tile_sizes(p, subset, nsubsets);
+  if (z.nrays == 0 || (long)g.nx * g.ny * g.nz == 0) return hipSuccess;
+  if (scratch_bytes < tile_scratch_bytes(p, subset, nsubsets, geo == nullptr))
+    return hipErrorInvalidValue;
   const int nitems = tile_items(p);
   if (!p->d_tile_order || p->tile_order_n != nitems) return hipErrorInvalidValue;
+  char* sp = (char*)scratch;
+  ZRay* zr = (ZRay*)sp;
+  sp += align256((size_t)z.nrays * sizeof(ZRay));
+  int* nzp = (int*)sp;
+  sp += align256((size_t)(z.npair + 1) * sizeof(int));
+  int* live = (int*)sp;
+  sp += align256((size_t)z.nslots * sizeof(int));
+  int* nlive = (int*)sp;
+  sp += align256((size_t)z.ntiles * sizeof(int));
+  if (!geo) {
+    geo = sp;
+    geo_ready = false;
+  }
+  int* tcount = (int*)geo;
+  TileRay* trays = (TileRay*)((char*)geo + align256((size_t)z.ntiles * sizeof(int)));
+  TilePath* tpaths = (TilePath*)(trays + z.nslots);
   GatherArgs A;
-  ZRay* zr = nullptr;
-  int* nzp = nullptr;
-  gather_args_for(p, subset, nsubsets, kind, sino, img, scratch, A, zr, nzp);
+  ZRay* unused_zr = nullptr;
+  int* unused_nz = nullptr;
+  gather_args_for(p, subset, nsubsets, kind, sino, img, nullptr, A, unused_zr, unused_nz);
+  A.zr = zr;
+  A.nz_pair = nzp;
   A.ztab = nullptr;  // the lanes take their z crossings by the loop's additions
   if (fuse) {
     A.upd_sens = fuse->sens;
     A.upd_thr = fuse->thr;
     A.upd_strict = fuse->strict;
   }
-  // scratch: [ZRay x nrays][non-zero flag per (angle, r)][item counter][ray count per
-  // tile][candidate rays of every (tile, angle)]
-  const long npair = (long)nang_sub * g.nr;
-  const int maxc = tile_max_candidates(p);
-  const int ntiles = ((g.nx + kTW - 1) / kTW) * ((g.ny + kTW - 1) / kTW);
-  const long nslots = (long)ntiles * nang_sub * maxc;
-  int* tcount = (int*)((char*)scratch + (size_t)nrays * sizeof(ZRay) +
-                       ((npair + 1) * sizeof(int) + 255) / 256 * 256);
-  TileRay* trays = (TileRay*)((char*)tcount + ((size_t)ntiles * sizeof(int) + 255) / 256 * 256);
-  TilePath* tpaths = (TilePath*)(trays + nslots);
-  hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair + 1)), dim3(kGBlock), 0, st, nzp,
-                     npair + 1);
-  hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
-                     p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp,
+  // the call's axial records with their bin values, and the (angle, r) with a
+  // non-zero bin
+  hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(z.npair + 1)), dim3(kGBlock), 0, st, nzp,
+                     z.npair + 1);
+  hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(z.nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
+                     p->d_zab, sino, kind, z.nang_sub, subset, nsubsets, z.nrays, zr, nzp,
                      (float*)nullptr, A.zsoff, A.zblock, 0, 1);
-  hipLaunchKernelGGL(tile_rays_kernel, dim3(ggrid(nslots)), dim3(kGBlock), 0, st, A, maxc, nslots,
-                     trays, tpaths);
-  hipLaunchKernelGGL(tile_compact_kernel, dim3((ntiles + kTWaves - 1) / kTWaves), dim3(kGBlock), 0,
-                     st, trays, tpaths, nang_sub * maxc, ntiles, tcount);
+  if (!geo_ready) {
+    // the subset's geometry: the candidate rays of every (tile, angle) with their
+    // paths, compacted per tile
+    hipLaunchKernelGGL(tile_rays_kernel, dim3(ggrid(z.nslots)), dim3(kGBlock), 0, st, A, z.maxc,
+                       z.nslots, trays, tpaths);
+    hipLaunchKernelGGL(tile_compact_kernel, dim3((z.ntiles + kTWaves - 1) / kTWaves),
+                       dim3(kGBlock), 0, st, trays, tpaths, z.nang_sub * z.maxc, z.ntiles, tcount);
+  }
+  // each tile's rays with a non-zero bin in this call
+  hipLaunchKernelGGL(tile_live_kernel, dim3((z.ntiles + kTWaves - 1) / kTWaves), dim3(kGBlock), 0,
+                     st, (const TileRay*)trays, (const int*)tcount, (const int*)nzp,
+                     z.nang_sub * z.maxc, z.ntiles, live, nlive);
   const int zs = tile_slab_depth(p);
   const int nslab = (g.nz + zs - 1) / zs;
   hipLaunchKernelGGL(tile_bp_kernel, dim3((unsigned)((nitems + kTWaves - 1) / kTWaves)),
                      dim3(kGBlock), tile_lds_bytes(p), st, A, zs, nslab, p->d_tile_order, nitems,
-                     nzp + npair, (const TileRay*)trays, (const TilePath*)tpaths, nang_sub * maxc,
-                     (const int*)tcount);
+                     nzp + z.npair, (const TileRay*)trays, (const TilePath*)tpaths,
+                     z.nang_sub * z.maxc, (const int*)live, (const int*)nlive);
   return hipGetLastError();
 }
 

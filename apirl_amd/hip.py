@@ -49,7 +49,8 @@ class EmOptions(ctypes.Structure):
 
 class Tuning(ctypes.Structure):
     _fields_ = [("gather_zlanes", _c_int), ("gather_zvox", _c_int), ("gather_xcd", _c_int),
-                ("tile_zs", _c_int)]
+                ("tile_zs", _c_int),
+                ("tile_cache_mb", _c_int)]
 
 
 GATHER_XCD_ROUND_ROBIN = 1
@@ -313,8 +314,8 @@ class Plan:
                        "apirl_plan_set_oversampling")
 
     def set_tuning(self, **fields):
-        """Sets apirl_tuning fields (gather_zlanes, gather_zvox, gather_xcd, tile_zs);
-        gather results do not depend on them."""
+        """Sets apirl_tuning fields (gather_zlanes, gather_zvox, gather_xcd, tile_zs,
+        tile_cache_mb: -1 keeps no tile geometry); gather results do not depend on them."""
         t = Tuning(**{k: int(v) for k, v in fields.items()})
         self.ctx.check(self.ctx.lib.apirl_plan_set_tuning(self.h, ctypes.byref(t)),
                        "apirl_plan_set_tuning")

@@ -132,14 +132,21 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
  *                  precision modes; APIRL_PREC_F64_ALPHA always uses 1).
  *   tile_zs:       slices per item of the tile backprojector (APIRL_BP_TILE; default
  *                  about 6).
+ *   tile_cache_mb: device memory (MiB, default 1024) in which the plan keeps the tile
+ *                  backprojector's per-subset geometry (the candidate rays of every
+ *                  8 x 8-column tile and their paths: about 30 MB per 12-angle subset
+ *                  of the 256x256x81 / 344x252x837 workload) from its first call on;
+ *                  -1 keeps none and frees what is kept.  Subsets beyond the budget
+ *                  are rebuilt on every call.
  * Gather results do not depend on these (the per-voxel summation order is fixed);
- * tile results are deterministic for a given tile_zs.
+ * tile results are deterministic for a given tile_zs, kept geometry or not.
  * Zero in a field leaves it unchanged. */
 typedef struct apirl_tuning {
   int gather_zlanes;
   int gather_zvox;
   int gather_xcd;     /* workgroup order of the column kernel (below) */
   int tile_zs;
+  int tile_cache_mb;
 } apirl_tuning;
 /* gather_xcd: APIRL_GATHER_XCD_ROUND_ROBIN takes the 16 x 16 x zvox tiles in grid
  * order, which the hardware deals to the 8 XCDs in turn; APIRL_GATHER_XCD_WEDGES gives

@@ -134,6 +134,53 @@ def test_tile_fused_update_equals_backproject_then_update(torch, ctx, strict):
     assert np.array_equal(o.view(np.int32), r.view(np.int32))
 
 
+def test_tile_kept_geometry_equals_rebuilt(torch, ctx, golden):
+    """The geometry a plan keeps per subset (apirl_tuning.tile_cache_mb) gives the same
+    images, bit for bit, as geometry rebuilt on every call: subsets called in turn, twice,
+    with sinograms whose zero bins (whole rays and scattered bins) differ from call to
+    call, a multiplicative sensitivity and the fused update; each within the
+    backprojection bar of the gather."""
+    log("case: mid3d tile backprojection with kept vs rebuilt geometry")
+    f = golden("mid3d_project_backproject.npz")
+    g = json.loads(str(f["geom"]))
+    kept, rebuilt = plan3d(ctx, g), plan3d(ctx, g)
+    rebuilt.set_tuning(tile_cache_mb=-1)
+    assert rebuilt.get_tuning()["tile_cache_mb"] == 0
+    assert kept.get_tuning()["tile_cache_mb"] == 1024
+    S = 3
+    rng = np.random.default_rng(21)
+    for rep in range(2):
+        for s in range(S):
+            shape = kept.subset_shape(s, S)
+            sino = rng.uniform(0.5, 2.0, shape).astype(np.float32)
+            sino[:, :, rng.random(shape[2]) < 0.3] = 0.0
+            sino[rng.random(shape) < 0.2] = 0.0
+            st = dev(torch, sino)
+            a = kept.backproject(st, subset=s, nsubsets=S, mode=hip.BP_TILE)
+            b = rebuilt.backproject(st, subset=s, nsubsets=S, mode=hip.BP_TILE)
+            gth = kept.backproject(st, subset=s, nsubsets=S, mode=hip.BP_GATHER)
+            torch.cuda.synchronize()
+            assert_bitexact(host(a), host(b))
+            assert_voxel_close(host(a), host(gth))
+    mult = dev(torch, rng.uniform(0.5, 1.5, kept.subset_shape()).astype(np.float32))
+    for s in range(S):
+        a = kept.sensitivity(mult=mult, subset=s, nsubsets=S, mode=hip.BP_TILE)
+        b = rebuilt.sensitivity(mult=mult, subset=s, nsubsets=S, mode=hip.BP_TILE)
+        torch.cuda.synchronize()
+        assert_bitexact(host(a), host(b))
+    s = 1
+    sino = dev(torch, rng.uniform(0.0, 2.0, kept.subset_shape(s, S)).astype(np.float32))
+    sens = kept.sensitivity(subset=s, nsubsets=S, mode=hip.BP_TILE)
+    thr = hip.mlem_threshold(ctx, sens)
+    x0 = dev(torch, rng.uniform(0.5, 2.0, kept.image_shape()).astype(np.float32))
+    a = kept.backproject_update(sino, x0.clone(), sens, thr, 0, subset=s, nsubsets=S,
+                                mode=hip.BP_TILE)
+    b = rebuilt.backproject_update(sino, x0.clone(), sens, thr, 0, subset=s, nsubsets=S,
+                                   mode=hip.BP_TILE)
+    torch.cuda.synchronize()
+    assert_bitexact(host(a), host(b))
+
+
 def test_tile_osem_mid3d_vs_reference(torch, ctx, golden):
     log("case: mid3d OSEM 7x2 with the tile backprojector")
     f = golden("mid3d_osem.npz")
