@@ -41,6 +41,25 @@ __device__ __forceinline__ float z_at(float za, float vz, float alpha) { return 
 // (Siddon.cpp:150-182, 220, 244-255, 279-286) and the bin value, computed once per
 // backprojection by zsetup_kernel.
 
+// The axial part of one ray's setup in float (the reference's operations in its
+// order): entry and exit slices, first z crossing and its increment, and the length.
+__device__ __forceinline__ void zray_setup(const Geo& g, const RayXY& R, float za, float zb,
+                                           ZRay& z) {
+  const float vz = zb - za;
+  const float z1 = za + vz * R.amin;
+  const float z2 = za + vz * R.amax;
+  const int kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
+  z.kinc = (short)kinc;
+  z.k0 = (int)floorf((z1 - g.offz) / g.dz);
+  z.k1 = (short)(int)floorf((z2 - g.offz) / g.dz);
+  z.azu = fabsf(g.dz / vz);
+  if (vz > 0) z.az0 = (g.offz + (z.k0 + kinc) * g.dz - za) / vz;
+  else if (vz < 0) z.az0 = (g.offz + z.k0 * g.dz - za) / vz;
+  else z.az0 = FLT_MAX;
+  const float ez = (za + vz) - za;
+  z.L = sqrtf(R.lxy + ez * ez);
+}
+
 // nz_pair[ka * nr + r] != 0 iff some sinogram has a non-zero bin at (angle ka, r):
 // the gather skips (angle, r) pairs whose bins are all zero (zero bins add nothing,
 // SiddonProjector.cpp:496).
@@ -51,7 +70,7 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
                                            ZRay* __restrict__ out, int* __restrict__ nz_pair,
                                            float* __restrict__ ztab,
                                            const int* __restrict__ zsoff, int zblock,
-                                           long idx, int f64s = 0, int tmajor = 0) {
+                                           long idx, int f64s = 0) {
   const int rr = (int)(idx % g.nr);
   const long t = idx / g.nr;
   const int ka = (int)(t % nang_sub);
@@ -60,8 +79,6 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   const RayXY R = rays[(long)a * g.nr + rr];
   const float za = zab[s].x, zb = zab[s].y;
   const float vz = zb - za;
-  const float z1 = za + vz * R.amin;
-  const float z2 = za + vz * R.amax;
   ZRay z;
   const int kinc = vz > 0 ? 1 : (vz < 0 ? -1 : 0);
   z.kinc = (short)kinc;
@@ -78,14 +95,7 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
     z.azu = (float)zs.azu;
     z.L = (float)zs.L;
   } else {
-    z.k0 = (int)floorf((z1 - g.offz) / g.dz);
-    z.k1 = (short)(int)floorf((z2 - g.offz) / g.dz);
-    z.azu = fabsf(g.dz / vz);
-    if (vz > 0) z.az0 = (g.offz + (z.k0 + kinc) * g.dz - za) / vz;
-    else if (vz < 0) z.az0 = (g.offz + z.k0 * g.dz - za) / vz;
-    else z.az0 = FLT_MAX;
-    const float ez = (za + vz) - za;
-    z.L = sqrtf(R.lxy + ez * ez);
+    zray_setup(g, R, za, zb, z);
   }
   float b = 1.f;
   if (kind == BP_PLAIN) b = sino[idx];
@@ -121,12 +131,8 @@ __device__ __forceinline__ void zsetup_one(const Geo& g, const RayXY* __restrict
   }
   z.zcount = nw;
   z.pad = 0;
-  // (sinogram, angle, radial bin) order; the tile backprojector's lanes read the
-  // sinograms of one (angle, radial bin) together, and its records are (angle, radial
-  // bin, sinogram)
-  if (tmajor) out[((long)ka * g.nr + rr) * g.nsino + s] = z;
-  else out[idx] = z;
-  if (nz_pair && b != 0.f && (R.flags & RAY_VALID)) {
+  out[idx] = z;  // (sinogram, angle, radial bin) order
+  if (b != 0.f && (R.flags & RAY_VALID)) {
     int* f = &nz_pair[(long)ka * g.nr + rr];
     if (*f == 0) atomicOr(f, 1);
   }
@@ -140,11 +146,11 @@ __global__ __launch_bounds__(kGBlock) void zsetup_kernel(Geo g, const RayXY* __r
                                                          int* __restrict__ nz_pair,
                                                          float* __restrict__ ztab,
                                                          const int* __restrict__ zsoff,
-                                                         int zblock, int f64s, int tmajor) {
+                                                         int zblock, int f64s) {
   const long idx = (long)blockIdx.x * kGBlock + threadIdx.x;
   if (idx < nrays)
     zsetup_one(g, rays, zab, sino, kind, nang_sub, subset, nsubsets, out, nz_pair, ztab, zsoff,
-               zblock, idx, f64s, tmajor);
+               zblock, idx, f64s);
 }
 
 __global__ __launch_bounds__(kGBlock) void clear_int_kernel(int* __restrict__ p, long n) {
@@ -1296,6 +1302,49 @@ __global__ __launch_bounds__(kGBlock) void tile_compact_kernel(TileRay* __restri
   if (lane == 0) count[tile] = n;
 }
 
+// The tile backprojector's axial records, out[(ka nr + r) nsino + s] (zray_setup and
+// the bin value), through a 32 (r) x 32 (s) LDS tile of the bin values so that the
+// sinogram is read along r and the records are written along s; nz_pair[ka nr + r]
+// set where a valid LOR's bin is non-zero (one atomic per 32 sinograms).
+__global__ __launch_bounds__(kGBlock) void tile_zsetup_kernel(GatherArgs A,
+                                                              ZRay* __restrict__ out,
+                                                              int* __restrict__ nz_pair) {
+  __shared__ float tb[32][33];
+  const Geo& g = A.g;
+  const int r0 = blockIdx.x * 32, ka = blockIdx.y, s0 = blockIdx.z * 32;
+  const int a = A.subset + A.nsubsets * ka;
+  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
+  for (int q = ty; q < 32; q += 8) {
+    const int s = s0 + q, rr = r0 + tx;
+    float b = 1.f;
+    if (s < g.nsino && rr < g.nr) {
+      if (A.kind == BP_PLAIN) b = A.sino[((long)s * A.nang_sub + ka) * g.nr + rr];
+      else if (A.kind == BP_MULT) b = A.sino[((long)s * g.nang + a) * g.nr + rr];
+    }
+    tb[q][tx] = b;
+  }
+  __syncthreads();
+  const int s = s0 + tx;
+  const float2 zz = s < g.nsino ? A.zab[s] : make_float2(0.f, 0.f);
+  for (int q = ty; q < 32; q += 8) {
+    const int rr = r0 + q;  // uniform over each half wavefront
+    bool nz = false;
+    if (rr < g.nr && s < g.nsino) {
+      const RayXY R = A.rays[(long)a * g.nr + rr];
+      ZRay z;
+      zray_setup(g, R, zz.x, zz.y, z);
+      z.b = tb[tx][q];
+      z.zcount = 0;
+      z.pad = 0;
+      out[((long)ka * g.nr + rr) * g.nsino + s] = z;
+      nz = z.b != 0.f && (R.flags & RAY_VALID);
+    }
+    const unsigned long long bal = __ballot(nz);
+    const unsigned half = (unsigned)(bal >> (threadIdx.x & 32));
+    if (tx == 0 && half != 0u) atomicOr(&nz_pair[(long)ka * g.nr + rr], 1);
+  }
+}
+
 // Per tile, the positions (in trays order) of its rays with a non-zero bin, live[tile *
 // nslot + c] for c < nlive[tile] (one wavefront per tile).
 __global__ __launch_bounds__(kGBlock) void tile_live_kernel(const TileRay* __restrict__ trays,
@@ -1504,7 +1553,7 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
     hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(npair)), dim3(kGBlock), 0, st, nzp, npair);
     hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
                        p->d_zab, sino, kind, nang_sub, subset, nsubsets, nrays, zr, nzp, A.ztab,
-                       A.zsoff, A.zblock, f64s ? 1 : 0, 0);
+                       A.zsoff, A.zblock, f64s ? 1 : 0);
     const int zv = p->gather_zvox;
     if (precision == APIRL_PREC_REFERENCE)
       launch_gather3<APIRL_PREC_REFERENCE>(A, A.zlanes, zv, st);
@@ -1669,9 +1718,8 @@ hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float
   // non-zero bin
   hipLaunchKernelGGL(clear_int_kernel, dim3(ggrid(z.npair + 1)), dim3(kGBlock), 0, st, nzp,
                      z.npair + 1);
-  hipLaunchKernelGGL(zsetup_kernel, dim3(ggrid(z.nrays)), dim3(kGBlock), 0, st, g, p->d_rays,
-                     p->d_zab, sino, kind, z.nang_sub, subset, nsubsets, z.nrays, zr, nzp,
-                     (float*)nullptr, A.zsoff, A.zblock, 0, 1);
+  hipLaunchKernelGGL(tile_zsetup_kernel, dim3((g.nr + 31) / 32, z.nang_sub, (g.nsino + 31) / 32),
+                     dim3(kGBlock), 0, st, A, zr, nzp);
   if (!geo_ready) {
     // the subset's geometry: the candidate rays of every (tile, angle) with their
     // paths, compacted per tile
