@@ -1000,6 +1000,7 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
     const float* __restrict__ img, const float* __restrict__ imgZ, float* __restrict__ sino,
     int nang_sub, int subset, int nsubsets, FusedRatio fr, int qmax) {
   extern __shared__ FwdStep sp[];
+  __shared__ int s_qin[2];  // first and last list entries inside the image
   const int pair = blockIdx.x;
   const int ka = pair / g.nr, rr = pair % g.nr;
   const int a = subset + nsubsets * ka;
@@ -1007,17 +1008,18 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
   if (threadIdx.x == 0) {
     // qmax + 1 entries: a fast batch that starts kUnroll columns before the list's end
     // reads the entry after it
+    // the list's entries inside the image are one run: i and j each move one way, so
+    // each stays in range over one interval of entries
     const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
     float Xn = t.ax, Yn = t.ay;
-    int i = t.i0, j = t.j0;
+    int i = t.i0, j = t.j0, qa = qmax + 1, qb = -1;
     for (int q = 0; q <= qmax; q++) {
-      FwdStep e;
-      e.ae = Xn < Yn ? Xn : Yn;
-      e.i = (short)i;
-      e.j = (short)j;
-      e.base = (j * g.nx + i) * g.nz;
-      e.run = 0;
-      sp[q] = e;
+      sp[q].ae = Xn < Yn ? Xn : Yn;
+      sp[q].i = (short)i;
+      sp[q].j = (short)j;
+      const bool in = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+      qa = in && q < qa ? q : qa;
+      qb = in ? q : qb;
       if (Xn <= Yn) {
         i += iinc;
         Xn = Xn + t.axu;
@@ -1026,11 +1028,15 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
         Yn = Yn + t.ayu;
       }
     }
-    int run = 0;
-    for (int q = qmax; q >= 0; q--) {
-      const bool in = sp[q].i >= 0 && sp[q].i < g.nx && sp[q].j >= 0 && sp[q].j < g.ny;
-      run = in ? run + 1 : 0;
-      sp[q].run = run;
+    s_qin[0] = qa;
+    s_qin[1] = qb;
+  }
+  __syncthreads();
+  {
+    const int qa = s_qin[0], qb = s_qin[1];
+    for (int q = threadIdx.x; q <= qmax; q += kBlock) {
+      sp[q].base = (sp[q].j * g.nx + sp[q].i) * g.nz;
+      sp[q].run = q >= qa && q <= qb ? qb - q + 1 : 0;
     }
   }
   __syncthreads();
