@@ -974,6 +974,10 @@ static_assert(sizeof(FwdStep) == 16, "FwdStep is 16 bytes");
 // Path length kept per (angle, bin): the transaxial list plus a few columns (a rounding
 // can trade a z step of the list for an x or y step).
 __host__ __device__ inline int fwd_path_len(const Geo& g) { return g.nx + g.ny + 4; }
+// LDS of project_paths_kernel: the list (one entry more) and the two crossing chains
+inline size_t fwd_path_lds(const Geo& g) {
+  return (size_t)(fwd_path_len(g) + 1) * (sizeof(FwdStep) + 2 * sizeof(float));
+}
 
 // out[(j nx + i) nz + k] = in[(k ny + j) nx + i] through a 32 (i) x 32 (k) LDS tile.
 __global__ __launch_bounds__(kBlock) void transpose_zfast_kernel(const float* __restrict__ in,
@@ -1005,39 +1009,67 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
   const int ka = pair / g.nr, rr = pair % g.nr;
   const int a = subset + nsubsets * ka;
   const RayXY t = rays[(long)a * g.nr + rr];
+  // The list has qmax + 1 entries (a fast batch that starts kUnroll columns before the
+  // list's end reads the entry after it).  The loop's x crossings X[m] (X[0] = ax,
+  // X[m + 1] = X[m] + axu, in float as the loop adds them) and y crossings Y[k] are
+  // two chains, one lane each; the loop takes the x step at X[m] when X[m] <= the
+  // pending Y, so it takes X[m] at step m + #{k: Y[k] < X[m]} and Y[k] at step
+  // k + #{m: X[m] <= Y[k]} (a stable merge of the two sorted chains, x first on ties):
+  // each step's entry is written by the crossing the loop takes there.
+  float* xs = (float*)(sp + qmax + 1);
+  float* ys = xs + (qmax + 1);
+  if (threadIdx.x < 2) {
+    const bool isx = threadIdx.x == 0;
+    float v = isx ? t.ax : t.ay;
+    const float u = isx ? t.axu : t.ayu;
+    float* arr = isx ? xs : ys;
+    for (int m = 0; m <= qmax; m++) {
+      arr[m] = v;
+      v = v + u;
+    }
+  }
   if (threadIdx.x == 0) {
-    // qmax + 1 entries: a fast batch that starts kUnroll columns before the list's end
-    // reads the entry after it
-    // the list's entries inside the image are one run: i and j each move one way, so
-    // each stays in range over one interval of entries
+    s_qin[0] = qmax + 1;
+    s_qin[1] = -1;
+  }
+  __syncthreads();
+  {
     const int iinc = ray_iinc(t.flags), jinc = ray_jinc(t.flags);
-    float Xn = t.ax, Yn = t.ay;
-    int i = t.i0, j = t.j0, qa = qmax + 1, qb = -1;
-    for (int q = 0; q <= qmax; q++) {
-      sp[q].ae = Xn < Yn ? Xn : Yn;
-      sp[q].i = (short)i;
-      sp[q].j = (short)j;
-      const bool in = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-      qa = in && q < qa ? q : qa;
-      qb = in ? q : qb;
-      if (Xn <= Yn) {
-        i += iinc;
-        Xn = Xn + t.axu;
-      } else {
-        j += jinc;
-        Yn = Yn + t.ayu;
+    for (int c = threadIdx.x; c < 2 * (qmax + 1); c += kBlock) {
+      const bool isx = c <= qmax;
+      const int m = isx ? c : c - (qmax + 1);
+      const float v = isx ? xs[m] : ys[m];
+      const float* other = isx ? ys : xs;
+      int lo = 0, hi = qmax + 1;  // other[0 .. lo) come before v
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const float o = other[mid];
+        const bool before = isx ? (o < v) : (o <= v);
+        lo = before ? mid + 1 : lo;
+        hi = before ? hi : mid;
+      }
+      const int q = m + lo;
+      if (q <= qmax) {
+        const int mx = isx ? m : lo, my = isx ? lo : m;
+        const int i = t.i0 + mx * iinc, j = t.j0 + my * jinc;
+        sp[q].ae = v;
+        sp[q].i = (short)i;
+        sp[q].j = (short)j;
+        sp[q].base = (j * g.nx + i) * g.nz;
+        // the entries inside the image are one run: i and j each move one way, so each
+        // stays in range over one interval of entries
+        if ((unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny) {
+          atomicMin(&s_qin[0], q);
+          atomicMax(&s_qin[1], q);
+        }
       }
     }
-    s_qin[0] = qa;
-    s_qin[1] = qb;
   }
   __syncthreads();
   {
     const int qa = s_qin[0], qb = s_qin[1];
-    for (int q = threadIdx.x; q <= qmax; q += kBlock) {
-      sp[q].base = (sp[q].j * g.nx + sp[q].i) * g.nz;
+    for (int q = threadIdx.x; q <= qmax; q += kBlock)
       sp[q].run = q >= qa && q <= qb ? qb - q + 1 : 0;
-    }
   }
   __syncthreads();
   const int nxy = g.nx * g.ny, npix = nxy * g.nz;
@@ -1139,7 +1171,7 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
 bool path_forward_ok(const apirl_plan* p, int precision) {
   const Geo& g = p->geo;
   return g.dims == 3 && precision == APIRL_PREC_REFERENCE && g.ns * g.nas == 1 && g.walk24 &&
-         p->d_imgT != nullptr && (size_t)(fwd_path_len(g) + 1) * sizeof(FwdStep) <= 48 * 1024;
+         p->d_imgT != nullptr && fwd_path_lds(g) <= 48 * 1024;
 }
 
 int project_partials(const apirl_plan* p, int subset, int nsubsets, bool path) {
@@ -1168,7 +1200,7 @@ hipError_t launch_project(const apirl_plan* p, const float* img, float* sino, in
     const dim3 tg((g.nx + 31) / 32, g.ny, (g.nz + 31) / 32);
     hipLaunchKernelGGL(transpose_zfast_kernel, tg, dim3(kBlock), 0, st, img, p->d_imgT, g.nx,
                        g.ny, g.nz);
-    const size_t lds = (size_t)(qmax + 1) * sizeof(FwdStep);  // 8.3 KB for 256 x 256
+    const size_t lds = fwd_path_lds(g);  // 12.4 KB for 256 x 256
     if (f)
       hipLaunchKernelGGL(project_paths_kernel<true>, dim3(npair), dim3(kBlock), lds, st, g,
                          p->d_rays, p->d_zab, img, p->d_imgT, sino, nang_sub, subset, nsubsets, fr,
