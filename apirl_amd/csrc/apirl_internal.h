@@ -138,7 +138,9 @@ struct apirl_plan {
   int tile_order_n = 0;
   // its geometry per (LOR form, subset, subsets), kept from the first call on, within
   // tile_geo_budget bytes in all (apirl_tuning.tile_cache_mb); `ready` once the launches
-  // that build it are enqueued on `stream`, whose event `ev` follows them
+  // that build it are enqueued on `stream`, whose event `ev` follows them; `list` the
+  // sinogram lists of its (item, ray) pairs (lists: 0 not tried, 1 kept, -1 beyond the
+  // budget)
   struct TileGeo {
     int form, subset, nsubsets;
     void* d;
@@ -146,10 +148,13 @@ struct apirl_plan {
     bool ready;
     hipStream_t stream;
     hipEvent_t ev;
+    unsigned short* list;
+    size_t list_bytes;
+    int lists;
   };
   std::vector<TileGeo> tile_geo;
   size_t tile_geo_used = 0;
-  size_t tile_geo_budget = (size_t)1024 << 20;
+  size_t tile_geo_budget = (size_t)16384 << 20;
   // its tile maps (APIRL_GATHER_XCD_WEDGES), one per voxels-per-lane depth 1..4, built
   // on first use and kept until the plan ends (captured launches may hold them)
   int* d_tilemap[5] = {};
@@ -285,13 +290,21 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
 // (null when it can).  Its geometry for one subset (the candidate rays of every tile
 // and their paths; tile_geo_bytes) depends on the plan's LOR form alone: with `geo`
 // null it is built in the scratch on every call; otherwise it lives in `geo`, built
-// there when `geo_ready` is false and reused as it stands when true.  The scratch
+// there when `geo_ready` is false and reused as it stands when true (with `list`, the
+// sinogram lists launch_tile_lists kept for it, when not null).  The scratch
 // (tile_scratch_bytes, with the geometry when `geo` is null) holds the call's axial
 // records with their bin values and its live-ray lists.
 hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
                                    int subset, int nsubsets, int kind, void* scratch,
                                    size_t scratch_bytes, hipStream_t st, const EmFuse* fuse,
-                                   void* geo, bool geo_ready);
+                                   void* geo, bool geo_ready,
+                                   const unsigned short* list = nullptr);
+// The sinogram lists of every (item, candidate ray) of a kept geometry (built):
+// with list null, counts them (*total: the device address of their number, a long
+// long, valid once the stream reaches it); otherwise writes them to list (2 bytes
+// each).  launch_backproject_tile then reads them instead of listing.
+hipError_t launch_tile_lists(const apirl_plan* p, int subset, int nsubsets, void* geo,
+                             unsigned short* list, long long** total, hipStream_t st);
 const char* tile_misfit(const apirl_plan* p);
 size_t tile_geo_bytes(const apirl_plan* p, int subset, int nsubsets);
 size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets, bool with_geo);

@@ -218,7 +218,8 @@ apirl_plan::TileGeo* tile_geo_for(apirl_plan* p, int subset, int nsubsets) {
     if (e.form == p->scanner && e.subset == subset && e.nsubsets == nsubsets) return &e;
   const size_t bytes = apirl::tile_geo_bytes(p, subset, nsubsets);
   if (bytes == 0 || p->tile_geo_used + bytes > p->tile_geo_budget) return nullptr;
-  apirl_plan::TileGeo e{p->scanner, subset, nsubsets, nullptr, bytes, false, nullptr, nullptr};
+  apirl_plan::TileGeo e{p->scanner, subset, nsubsets, nullptr, bytes, false, nullptr, nullptr,
+                        nullptr, 0, 0};
   if (hipMalloc(&e.d, bytes) != hipSuccess) {
     (void)hipGetLastError();
     return nullptr;
@@ -237,6 +238,7 @@ void free_tile_geo(apirl_plan* p) {
   for (apirl_plan::TileGeo& e : p->tile_geo) {
     (void)hipEventSynchronize(e.ev);
     (void)hipFree(e.d);
+    if (e.list) (void)hipFree(e.list);
     (void)hipEventDestroy(e.ev);
   }
   p->tile_geo.clear();
@@ -358,11 +360,34 @@ apirl_status backproject_any(apirl_plan* p, const float* sino, const float* est,
       APIRL_HIP(c, hipStreamWaitEvent(c->stream, tg->ev, 0));
     APIRL_HIP(c, apirl::launch_backproject_tile(p, sino, img, subset, nsubsets, kind,
                                                 p->d_gscratch, p->gscratch_bytes, c->stream,
-                                                fuse, tg ? tg->d : nullptr, tg && tg->ready));
+                                                fuse, tg ? tg->d : nullptr, tg && tg->ready,
+                                                tg && tg->lists == 1 ? tg->list : nullptr));
     if (tg && !tg->ready) {
-      APIRL_HIP(c, hipEventRecord(tg->ev, c->stream));
       tg->ready = true;
       tg->stream = c->stream;
+      // the sinogram lists of its (item, ray) pairs: counted, then kept if they fit
+      long long* dtot = nullptr;
+      long long total = 0;
+      APIRL_HIP(c, apirl::launch_tile_lists(p, subset, nsubsets, tg->d, nullptr, &dtot,
+                                            c->stream));
+      APIRL_HIP(c, hipMemcpyAsync(&total, dtot, sizeof(total), hipMemcpyDeviceToHost,
+                                  c->stream));
+      APIRL_HIP(c, hipStreamSynchronize(c->stream));
+      const size_t lb = (size_t)(total > 0 ? total : 0) * sizeof(unsigned short);
+      tg->lists = -1;
+      if (lb > 0 && p->tile_geo_used + lb <= p->tile_geo_budget) {
+        if (hipMalloc(&tg->list, lb) == hipSuccess) {
+          p->tile_geo_used += lb;
+          tg->list_bytes = lb;
+          tg->lists = 1;
+          APIRL_HIP(c, apirl::launch_tile_lists(p, subset, nsubsets, tg->d, tg->list, nullptr,
+                                                c->stream));
+        } else {
+          (void)hipGetLastError();
+          tg->list = nullptr;
+        }
+      }
+      APIRL_HIP(c, hipEventRecord(tg->ev, c->stream));
     }
     return APIRL_OK;
   }
@@ -853,9 +878,11 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
       return fail(p->ctx, APIRL_ERR_INVALID, "unknown gather_xcd");
     p->gather_xcd = t->gather_xcd;
   }
+  bool zs_changed = false;
   if (t->tile_zs) {
     if (t->tile_zs < 1 || t->tile_zs > 256)
       return fail(p->ctx, APIRL_ERR_INVALID, "tile_zs must be in 1..256");
+    zs_changed = t->tile_zs != p->tile_zs;
     p->tile_zs = t->tile_zs;
   }
   DeviceGuard g(p->ctx->device);
@@ -865,6 +892,7 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     p->tile_geo_budget = t->tile_cache_mb < 0 ? 0 : (size_t)t->tile_cache_mb << 20;
     if (p->tile_geo_used > p->tile_geo_budget) free_tile_geo(p);
   }
+  if (zs_changed) free_tile_geo(p);  // the kept lists are per slab
   if (!ensure_tile_map(p))
     return fail(p->ctx, APIRL_ERR_NOMEM, "cannot allocate the gather's tile map");
   for (apirl_plan* h : p->os_parts) {  // the oversampled plan's gathers follow it
@@ -872,6 +900,7 @@ apirl_status apirl_plan_set_tuning(apirl_plan* p, const apirl_tuning* t) {
     h->gather_zvox = p->gather_zvox;
     h->gather_xcd = p->gather_xcd;
     h->tile_zs = p->tile_zs;
+    if (zs_changed) free_tile_geo(h);
     const size_t share = p->tile_geo_budget / p->os_parts.size();
     if (h->tile_geo_budget != share) {
       h->tile_geo_budget = share;

@@ -1372,12 +1372,136 @@ __global__ __launch_bounds__(kGBlock) void tile_live_kernel(const TileRay* __res
   if (lane == 0) nlive[tile] = n;
 }
 
+// The sinograms whose ray (tile entry E) can reach the slab [zlo, zhi] (widened)
+// inside the tile, in segment-run order: per run, the indices whose mean z lies
+// within the slab widened by the run's slopes over the tile's alpha range (z = zmean
+// + (zb - za)(alpha - 1/2)) and its deviation from the line (one run per lane); of
+// those, the sinograms whose ray's z over that alpha range meets the slab.  The
+// wavefront calls emit(keep, s) together for each group of up to 64 tested indices.
+template <class Emit>
+__device__ __forceinline__ void tile_list_ray(const GatherArgs& A, const TileEntry& E, float zlo,
+                                              float zhi, const float2* s_zab, int lane,
+                                              Emit&& emit) {
+  const float ulo = E.ce - 0.5f, uhi = E.aout - 0.5f;
+  for (int sb = 0; sb < A.nseg; sb += kWave) {
+    int s0 = 0, cnt = 0;
+    if (sb + lane < A.nseg) {
+      const GatherSeg S = A.segs[sb + lane];
+      const float c1 = S.dmin * ulo, c2 = S.dmin * uhi, c3 = S.dmax * ulo, c4 = S.dmax * uhi;
+      const float dlo = fminf(fminf(c1, c2), fminf(c3, c4));
+      const float dhi = fmaxf(fmaxf(c1, c2), fmaxf(c3, c4));
+      const float lf = (zlo - dhi - S.devz - 0.01f - S.z0) * S.dhi - 1.f;
+      const float hf = (zhi - dlo + S.devz + 0.01f - S.z0) * S.dhi + 1.f;
+      const int l0 = lf > 0.f ? (int)lf : 0;
+      const int l1 = hf < (float)(S.count - 1) ? (int)hf : S.count - 1;
+      s0 = S.first + l0;
+      cnt = l1 >= l0 ? l1 - l0 + 1 : 0;
+    }
+    const int ns = A.nseg - sb < kWave ? A.nseg - sb : kWave;
+    for (int sg = 0; sg < ns; sg++) {
+      const int cs = __builtin_amdgcn_readlane(cnt, sg);
+      if (cs == 0) continue;
+      const int f = __builtin_amdgcn_readlane(s0, sg);
+      for (int l0 = 0; l0 < cs; l0 += kWave) {
+        const int s = f + l0 + lane;
+        bool keep = false;
+        if (l0 + lane < cs) {
+          const float2 zz = s_zab[s];
+          const float vz = zz.y - zz.x;
+          const float z1 = z_at(zz.x, vz, E.ce), z2 = z_at(zz.x, vz, E.aout);
+          keep = !(fmaxf(z1, z2) < zlo) && !(fminf(z1, z2) > zhi);
+        }
+        emit(keep, s);
+      }
+    }
+  }
+}
+
+// The slab's z range widened by the listing margin.
+__device__ __forceinline__ void tile_slab_z(const Geo& g, int kz0, int kz1, float& zlo,
+                                            float& zhi) {
+  zlo = g.offz + kz0 * g.dz - kZMargin;
+  zhi = g.offz + (kz1 + 1) * g.dz + kZMargin;
+}
+
+// The listed sinograms of every (item, candidate ray) kept for a subset, one wavefront
+// per item, rays in the tile's order.  COUNT: loff[item * (nslot + 1) + c] = entries
+// of the item before ray c (c up to the tile's ray count) and tot[item] = its total.
+// Otherwise the sinograms (16 bits each) go to list[base[item] + ...].
+template <bool COUNT>
+__global__ __launch_bounds__(kGBlock) void tile_list_kernel(
+    GatherArgs A, int zs, int nslab, int nitems, const TileRay* __restrict__ trays,
+    const int* __restrict__ tcount, int nslot, int* __restrict__ loff, int* __restrict__ tot,
+    const long long* __restrict__ base, unsigned short* __restrict__ list) {
+  extern __shared__ float2 lzab[];
+  const Geo& g = A.g;
+  for (int q = threadIdx.x; q < g.nsino; q += kGBlock) lzab[q] = A.zab[q];
+  __syncthreads();
+  const int item = blockIdx.x * kTWaves + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & (kWave - 1);
+  if (item >= nitems) return;
+  const int slab = item % nslab, tile = item / nslab;
+  const int kz0 = slab * zs, kz1 = (kz0 + zs < g.nz ? kz0 + zs : g.nz) - 1;
+  float zlo, zhi;
+  tile_slab_z(g, kz0, kz1, zlo, zhi);
+  const TileRay* tr = trays + (long)tile * nslot;
+  const int nray = tcount[tile];
+  int n = 0;
+  unsigned short* out = COUNT ? nullptr : list + base[item];
+  for (int c = 0; c < nray; c++) {
+    if (COUNT && lane == 0) loff[(long)item * (nslot + 1) + c] = n;
+    const TileEntry E = tr[c].E;
+    tile_list_ray(A, E, zlo, zhi, lzab, lane, [&](bool keep, int s) {
+      const unsigned long long bal = __ballot(keep);
+      if (!COUNT && keep)
+        out[n + __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))] =
+            (unsigned short)s;
+      n += __popcll(bal);
+    });
+  }
+  if (COUNT && lane == 0) {
+    loff[(long)item * (nslot + 1) + nray] = n;
+    tot[item] = n;
+  }
+}
+
+// base[i] = sum of tot[0 .. i), for i = 0 .. n, in one workgroup.
+__global__ __launch_bounds__(kGBlock) void tile_scan_kernel(const int* __restrict__ tot, int n,
+                                                            long long* __restrict__ base) {
+  __shared__ long long part[kGBlock];
+  const int per = (n + kGBlock - 1) / kGBlock;
+  const int b = threadIdx.x * per, e = b + per < n ? b + per : n;
+  long long sum = 0;
+  for (int i = b; i < e; i++) sum += tot[i];
+  part[threadIdx.x] = sum;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    long long run = 0;
+    for (int t = 0; t < kGBlock; t++) {
+      const long long v = part[t];
+      part[t] = run;
+      run += v;
+    }
+    base[n] = run;
+  }
+  __syncthreads();
+  long long run = part[threadIdx.x];
+  for (int i = b; i < e; i++) {
+    base[i] = run;
+    run += tot[i];
+  }
+}
+
 // Items (tile, slab) are taken from a queue in the order of order[] (the tiles
 // nearest the image centre first, whose rays are the most), counter at *next.  The
 // rays entering tile t are trays[t * nslot + c]; those with a non-zero bin are at the
 // positions live[t * nslot + c'], c' < nlive[t].  The (ray, sinogram) pairs to walk
 // go through a ring in LDS: a pass takes 64 of them as soon as there are, so a pass
 // carries the sinograms of several rays.
+// LISTED: each (item, ray)'s sinograms are read from the lists tile_list_kernel keeps
+// (loff, base, list) instead of being listed here.
+template <bool LISTED>
 __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, int nslab,
                                                           const int* __restrict__ order,
                                                           int nitems, int* __restrict__ next,
@@ -1385,7 +1509,10 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
                                                           const TilePath* __restrict__ tpaths,
                                                           int nslot,
                                                           const int* __restrict__ live,
-                                                          const int* __restrict__ nlive) {
+                                                          const int* __restrict__ nlive,
+                                                          const int* __restrict__ loff,
+                                                          const long long* __restrict__ lbase,
+                                                          const unsigned short* __restrict__ list) {
   extern __shared__ double tlds[];
   const Geo& g = A.g;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1417,8 +1544,9 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
     for (int kl = 0; kl <= kz1 - kz0; kl++)
       acc[lane * zs + kl] = (incol && !A.upd_sens) ? (double)A.img[(kz0 + kl) * nxy + v0] : 0.0;
     wave_lds_sync();
-    const float zlo = g.offz + kz0 * g.dz - kZMargin;
-    const float zhi = g.offz + (kz1 + 1) * g.dz + kZMargin;
+    float zlo, zhi;
+    tile_slab_z(g, kz0, kz1, zlo, zhi);
+    const long long ibase = LISTED ? lbase[item] : 0;
     const TileRay* tr = trays + (long)tile * nslot;
     const unsigned* trw = (const unsigned*)tr;
     const unsigned* tpw = (const unsigned*)(tpaths + (long)tile * nslot);
@@ -1455,52 +1583,26 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
       else if (lane < (int)(sizeof(TileSlot) / 4))
         ((unsigned*)&slots[c & (kTSlots - 1)].R)[lane] =
             tpw[(long)rc * (sizeof(TilePath) / 4) + lane - (int)(sizeof(TileRay) / 4)];
-      const TileRay R = tr[rc];
-      const TileEntry& E = R.E;
-      // sinograms whose ray can reach the slab inside the tile: per run, the
-      // indices whose mean z lies within the slab widened by the run's slopes over
-      // the tile's alpha range (z = zmean + (zb - za)(alpha - 1/2)) and its deviation
-      // from the line (one run per lane); of those, the sinograms whose ray's z over
-      // that alpha range meets the slab go to the ring, run by run
-      const float ulo = E.ce - 0.5f, uhi = E.aout - 0.5f;
-      for (int sb = 0; sb < A.nseg; sb += kWave) {
-        int s0 = 0, cnt = 0;
-        if (sb + lane < A.nseg) {
-          const GatherSeg S = A.segs[sb + lane];
-          const float c1 = S.dmin * ulo, c2 = S.dmin * uhi, c3 = S.dmax * ulo, c4 = S.dmax * uhi;
-          const float dlo = fminf(fminf(c1, c2), fminf(c3, c4));
-          const float dhi = fmaxf(fmaxf(c1, c2), fmaxf(c3, c4));
-          const float lf = (zlo - dhi - S.devz - 0.01f - S.z0) * S.dhi - 1.f;
-          const float hf = (zhi - dlo + S.devz + 0.01f - S.z0) * S.dhi + 1.f;
-          const int l0 = lf > 0.f ? (int)lf : 0;
-          const int l1 = hf < (float)(S.count - 1) ? (int)hf : S.count - 1;
-          s0 = S.first + l0;
-          cnt = l1 >= l0 ? l1 - l0 + 1 : 0;
+      // the sinograms of ray c that reach the slab inside the tile go to the ring
+      auto emit = [&](bool keep, int s) {
+        const unsigned long long bal = __ballot(keep);
+        if (keep)
+          ring[(head + pend +
+                __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                          __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))) &
+               (kTRing - 1)] = ((unsigned)c << 16) | (unsigned)s;
+        pend += __popcll(bal);
+        if (pend >= kWave) run_pass(kWave);
+      };
+      if (LISTED) {
+        const int* lo = loff + (long)item * (nslot + 1) + rc;
+        const long long b0 = ibase + lo[0], b1 = ibase + lo[1];
+        for (long long l0 = b0; l0 < b1; l0 += kWave) {
+          const bool keep = l0 + lane < b1;
+          emit(keep, keep ? (int)list[l0 + lane] : 0);
         }
-        const int ns = A.nseg - sb < kWave ? A.nseg - sb : kWave;
-        for (int sg = 0; sg < ns; sg++) {
-          const int cs = __builtin_amdgcn_readlane(cnt, sg);
-          if (cs == 0) continue;
-          const int f = __builtin_amdgcn_readlane(s0, sg);
-          for (int l0 = 0; l0 < cs; l0 += kWave) {
-            const int s = f + l0 + lane;
-            bool keep = false;
-            if (l0 + lane < cs) {
-              const float2 zz = s_zab[s];
-              const float vz = zz.y - zz.x;
-              const float z1 = z_at(zz.x, vz, E.ce), z2 = z_at(zz.x, vz, E.aout);
-              keep = !(fmaxf(z1, z2) < zlo) && !(fminf(z1, z2) > zhi);
-            }
-            const unsigned long long bal = __ballot(keep);
-            if (keep)
-              ring[(head + pend +
-                    __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                              __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0))) &
-                   (kTRing - 1)] = ((unsigned)c << 16) | (unsigned)s;
-            pend += __popcll(bal);
-            if (pend >= kWave) run_pass(kWave);
-          }
-        }
+      } else {
+        tile_list_ray(A, tr[rc].E, zlo, zhi, s_zab, lane, emit);
       }
     }
     while (pend > 0) run_pass(pend < kWave ? pend : kWave);
@@ -1656,11 +1758,74 @@ TileSizes tile_sizes(const apirl_plan* p, int subset, int nsubsets) {
 }
 }  // namespace
 
+namespace {
 // geometry: [ray count per tile][candidate rays of every (tile, angle)][their paths]
+// [per item: entries before each ray, and the total][per item: first entry]
+struct TileGeoPtrs {
+  int* tcount;
+  TileRay* trays;
+  TilePath* tpaths;
+  int* loff;
+  long long* lbase;
+  int* tot;
+  size_t bytes;
+};
+
+TileGeoPtrs tile_geo_ptrs(const apirl_plan* p, const TileSizes& z, void* geo) {
+  const int nitems = tile_items(p);
+  const int nslot = z.nang_sub * z.maxc;
+  TileGeoPtrs q;
+  char* c = (char*)geo;
+  size_t o = 0;
+  q.tcount = (int*)(c + o);
+  o += align256((size_t)z.ntiles * sizeof(int));
+  q.trays = (TileRay*)(c + o);
+  q.tpaths = (TilePath*)(c + o + (size_t)z.nslots * sizeof(TileRay));
+  o += align256((size_t)z.nslots * (sizeof(TileRay) + sizeof(TilePath)));
+  q.loff = (int*)(c + o);
+  o += align256((size_t)nitems * (nslot + 1) * sizeof(int));
+  q.lbase = (long long*)(c + o);
+  o += align256((size_t)(nitems + 1) * sizeof(long long));
+  q.tot = (int*)(c + o);
+  o += align256((size_t)nitems * sizeof(int));
+  q.bytes = o;
+  return q;
+}
+}  // namespace
+
 size_t tile_geo_bytes(const apirl_plan* p, int subset, int nsubsets) {
+  return tile_geo_ptrs(p, tile_sizes(p, subset, nsubsets), nullptr).bytes;
+}
+
+hipError_t launch_tile_lists(const apirl_plan* p, int subset, int nsubsets, void* geo,
+                             unsigned short* list, long long** total, hipStream_t st) {
+  if (tile_misfit(p)) return hipErrorNotSupported;
+  const Geo& g = p->geo;
   const TileSizes z = tile_sizes(p, subset, nsubsets);
-  return align256((size_t)z.ntiles * sizeof(int)) +
-         (size_t)z.nslots * (sizeof(TileRay) + sizeof(TilePath));
+  const TileGeoPtrs q = tile_geo_ptrs(p, z, geo);
+  const int nitems = tile_items(p);
+  const int zs = tile_slab_depth(p), nslab = (g.nz + zs - 1) / zs;
+  const int nslot = z.nang_sub * z.maxc;
+  if (total) *total = q.lbase + nitems;
+  GatherArgs A;
+  ZRay* unused_zr = nullptr;
+  int* unused_nz = nullptr;
+  gather_args_for(p, subset, nsubsets, BP_ONES, nullptr, nullptr, nullptr, A, unused_zr,
+                  unused_nz);
+  const dim3 grid((nitems + kTWaves - 1) / kTWaves);
+  const size_t lds = (size_t)g.nsino * sizeof(float2);
+  if (!list) {
+    hipLaunchKernelGGL(tile_list_kernel<true>, grid, dim3(kGBlock), lds, st, A, zs, nslab, nitems,
+                       (const TileRay*)q.trays, (const int*)q.tcount, nslot, q.loff, q.tot,
+                       (const long long*)nullptr, (unsigned short*)nullptr);
+    hipLaunchKernelGGL(tile_scan_kernel, dim3(1), dim3(kGBlock), 0, st, (const int*)q.tot, nitems,
+                       q.lbase);
+  } else {
+    hipLaunchKernelGGL(tile_list_kernel<false>, grid, dim3(kGBlock), lds, st, A, zs, nslab,
+                       nitems, (const TileRay*)q.trays, (const int*)q.tcount, nslot,
+                       (int*)nullptr, (int*)nullptr, (const long long*)q.lbase, list);
+  }
+  return hipGetLastError();
 }
 
 // scratch: [ZRay x nrays, (angle, radial bin, sinogram) order][non-zero flag per
@@ -1676,7 +1841,7 @@ size_t tile_scratch_bytes(const apirl_plan* p, int subset, int nsubsets, bool wi
 hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float* img,
                                    int subset, int nsubsets, int kind, void* scratch,
                                    size_t scratch_bytes, hipStream_t st, const EmFuse* fuse,
-                                   void* geo, bool geo_ready) {
+                                   void* geo, bool geo_ready, const unsigned short* list) {
   if (kind != BP_PLAIN && kind != BP_ONES && kind != BP_MULT) return hipErrorNotSupported;
   if (tile_misfit(p)) return hipErrorNotSupported;
   const Geo& g = p->geo;
@@ -1698,10 +1863,12 @@ hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float
   if (!geo) {
     geo = sp;
     geo_ready = false;
+    list = nullptr;
   }
-  int* tcount = (int*)geo;
-  TileRay* trays = (TileRay*)((char*)geo + align256((size_t)z.ntiles * sizeof(int)));
-  TilePath* tpaths = (TilePath*)(trays + z.nslots);
+  const TileGeoPtrs q = tile_geo_ptrs(p, z, geo);
+  int* tcount = q.tcount;
+  TileRay* trays = q.trays;
+  TilePath* tpaths = q.tpaths;
   GatherArgs A;
   ZRay* unused_zr = nullptr;
   int* unused_nz = nullptr;
@@ -1734,10 +1901,18 @@ hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float
                      z.nang_sub * z.maxc, z.ntiles, live, nlive);
   const int zs = tile_slab_depth(p);
   const int nslab = (g.nz + zs - 1) / zs;
-  hipLaunchKernelGGL(tile_bp_kernel, dim3((unsigned)((nitems + kTWaves - 1) / kTWaves)),
-                     dim3(kGBlock), tile_lds_bytes(p), st, A, zs, nslab, p->d_tile_order, nitems,
-                     nzp + z.npair, (const TileRay*)trays, (const TilePath*)tpaths,
-                     z.nang_sub * z.maxc, (const int*)live, (const int*)nlive);
+  const dim3 grid((unsigned)((nitems + kTWaves - 1) / kTWaves));
+  if (list && geo_ready)
+    hipLaunchKernelGGL(tile_bp_kernel<true>, grid, dim3(kGBlock), tile_lds_bytes(p), st, A, zs,
+                       nslab, p->d_tile_order, nitems, nzp + z.npair, (const TileRay*)trays,
+                       (const TilePath*)tpaths, z.nang_sub * z.maxc, (const int*)live,
+                       (const int*)nlive, (const int*)q.loff, (const long long*)q.lbase, list);
+  else
+    hipLaunchKernelGGL(tile_bp_kernel<false>, grid, dim3(kGBlock), tile_lds_bytes(p), st, A, zs,
+                       nslab, p->d_tile_order, nitems, nzp + z.npair, (const TileRay*)trays,
+                       (const TilePath*)tpaths, z.nang_sub * z.maxc, (const int*)live,
+                       (const int*)nlive, (const int*)nullptr, (const long long*)nullptr,
+                       (const unsigned short*)nullptr);
   return hipGetLastError();
 }
 

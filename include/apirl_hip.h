@@ -132,12 +132,14 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
  *                  precision modes; APIRL_PREC_F64_ALPHA always uses 1).
  *   tile_zs:       slices per item of the tile backprojector (APIRL_BP_TILE; default
  *                  about 6).
- *   tile_cache_mb: device memory (MiB, default 1024) in which the plan keeps the tile
- *                  backprojector's per-subset geometry (the candidate rays of every
- *                  8 x 8-column tile and their paths: about 30 MB per 12-angle subset
- *                  of the 256x256x81 / 344x252x837 workload) from its first call on;
- *                  -1 keeps none and frees what is kept.  Subsets beyond the budget
- *                  are rebuilt on every call.
+ *   tile_cache_mb: device memory (MiB, default 16384) in which the plan keeps the
+ *                  tile backprojector's per-subset geometry from its first call on:
+ *                  the candidate rays of every 8 x 8-column tile with their paths, and
+ *                  for each (tile, slab, ray) the sinograms that reach the slab inside
+ *                  the tile (2 bytes each), which later calls read instead of listing
+ *                  them.  -1 keeps none and frees what is kept; a subset beyond the
+ *                  budget is rebuilt on every call.  The first call on a subset waits
+ *                  for the stream once (to size its lists).
  * Gather results do not depend on these (the per-voxel summation order is fixed);
  * tile results are deterministic for a given tile_zs, kept geometry or not.
  * Zero in a field leaves it unchanged. */

@@ -146,7 +146,7 @@ def test_tile_kept_geometry_equals_rebuilt(torch, ctx, golden):
     kept, rebuilt = plan3d(ctx, g), plan3d(ctx, g)
     rebuilt.set_tuning(tile_cache_mb=-1)
     assert rebuilt.get_tuning()["tile_cache_mb"] == 0
-    assert kept.get_tuning()["tile_cache_mb"] == 1024
+    assert kept.get_tuning()["tile_cache_mb"] == 16384
     S = 3
     rng = np.random.default_rng(21)
     for rep in range(2):
