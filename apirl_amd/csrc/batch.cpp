@@ -46,6 +46,7 @@ struct BJob {
   void* zscr = nullptr;      // gather axial records + non-zero flags (3D)
   size_t zscr_bytes = 0;
   bool large = false;        // runs its sub-iterations with the single-job kernels
+  bool tile = false;         //   and, large, the tile backprojector (else the gather)
   std::vector<std::pair<void*, size_t>> bufs;
 };
 
@@ -244,6 +245,15 @@ apirl_status apirl_batch_destroy(apirl_batch* b) {
 
 const char* apirl_batch_last_error(const apirl_batch* b) { return b ? b->err.c_str() : ""; }
 
+int apirl_batch_job_backprojector(const apirl_plan* p) {
+  if (!p) return -1;
+  const bool large = (p->geo.dims == 3 && apirl_plan_image_voxels(p) >= kLargeVoxels) ||
+                     p->geo.ns * p->geo.nas > 1;
+  if (large && p->geo.dims == 3 && p->geo.ns * p->geo.nas == 1 && !apirl::tile_misfit(p))
+    return APIRL_BP_TILE;
+  return APIRL_BP_GATHER;
+}
+
 size_t apirl_batch_job_bytes(const apirl_plan* p, int subsets) {
   if (!p || subsets < 1) return 0;
   const size_t nvox = apirl_plan_image_voxels(p);
@@ -348,6 +358,10 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   if (z_b) j->zscr = get(z_b);
   j->zscr_bytes = z_b;
   j->large = (!j->is2d && j->nvox >= kLargeVoxels) || os;
+  {
+    apirl::ActiveForm fs(p, j->form);
+    j->tile = apirl_batch_job_backprojector(p) == APIRL_BP_TILE;
+  }
   if (!j->sens || !j->ratio || !j->bp || (z_b && !j->zscr) ||
       hipEventCreateWithFlags(&j->done_ev, hipEventDisableTiming) != hipSuccess) {
     release_bufs(b, j);
@@ -373,8 +387,8 @@ apirl_status apirl_batch_submit(apirl_batch* b, const apirl_job_desc* d, int* jo
   j->thr.assign(j->S, 0.f);
   apirl::ActiveForm form_scope(p, j->form);
   for (int s = 0; s < j->S && st == APIRL_OK; s++) {
-    st = apirl_sensitivity(p, d->d_mult, j->sens + (size_t)s * j->nvox, s, j->S, APIRL_BP_GATHER,
-                           j->osem);
+    st = apirl_sensitivity(p, d->d_mult, j->sens + (size_t)s * j->nvox, s, j->S,
+                           j->tile ? APIRL_BP_TILE : APIRL_BP_GATHER, j->osem);
     if (st == APIRL_OK && !j->osem)
       st = apirl_mlem_threshold(c, j->sens + (size_t)s * j->nvox, j->nvox, &j->thr[s]);
   }
@@ -573,6 +587,20 @@ apirl_status apirl_batch_step(apirl_batch* b, int* active) {
                             b->stream)) != hipSuccess)
       return bfail(b, APIRL_ERR_HIP, std::string("large-job launch failed: ") +
                                          hipGetErrorString(e));
+    if (j->tile) {
+      // the tile backprojection with the update fused in, as apirl_em_reconstruct runs
+      // it with APIRL_BP_TILE (on the batch stream; the plan keeps its geometry)
+      apirl_ctx* c = b->ctx;
+      hipStream_t saved = c->stream;
+      c->stream = b->stream;
+      const apirl_status st = apirl_backproject_update(
+          j->plan, j->ratio, j->x, j->sens + (size_t)s * j->nvox, j->thr[s], j->strict, s, j->S,
+          APIRL_BP_TILE, APIRL_PREC_REFERENCE);
+      c->stream = saved;
+      if (st != APIRL_OK)
+        return bfail(b, st, std::string("large-job tile backprojection failed: ") + c->err);
+      continue;
+    }
     if (j->plan->geo.ns * j->plan->geo.nas == 1) {
       // the update fused into the gather, as apirl_em_reconstruct runs it
       const EmFuse fz{j->sens + (size_t)s * j->nvox, j->thr[s], j->strict};

@@ -106,6 +106,7 @@ SIGNATURES = {
     "apirl_batch_destroy": (_c_int, [_P]),
     "apirl_batch_last_error": (ctypes.c_char_p, [_P]),
     "apirl_batch_job_bytes": (_c_size, [_P, _c_int]),
+    "apirl_batch_job_backprojector": (_c_int, [_P]),
     "apirl_batch_set_launch": (_c_int, [_P, _c_int, _c_int]),
     "apirl_batch_set_schedule": (_c_int, [_P, _c_int, _c_int]),
     "apirl_batch_submit": (_c_int, [_P, ctypes.POINTER(JobDesc), ctypes.POINTER(_c_int)]),
@@ -513,6 +514,12 @@ class Batch:
     @staticmethod
     def job_bytes(plan, subsets):
         return plan.ctx.lib.apirl_batch_job_bytes(plan.h, subsets)
+
+    @staticmethod
+    def backprojector(plan):
+        """The backprojector (BP_TILE or BP_GATHER) a batch runs a job of this plan with:
+        its image equals plan.reconstruct(..., bp_mode=Batch.backprojector(plan))."""
+        return plan.ctx.lib.apirl_batch_job_backprojector(plan.h)
 
     def set_launch(self, grid=0, use_graph=True):
         self._check(self.lib.apirl_batch_set_launch(self.h, grid, 1 if use_graph else 0),

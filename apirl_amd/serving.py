@@ -1,6 +1,7 @@
 """Batched throughput and a serving loop over apirl_batch (continuous batching).
 
-* run_sequential: the plain single-job path, one apirl_em_reconstruct per job.
+* run_sequential: the plain single-job path, one apirl_em_reconstruct per job, each
+  with the backprojector the batch would run it with (Batch.backprojector).
 * run_batched: all jobs submitted to one batch (up to its capacity, later ones as
   slots free up), rounds stepped until every job is done.
 * autotune: picks the batch's persistent grid (workgroups per grouped launch), graph
@@ -25,9 +26,9 @@ def _sync(torch):
 def run_sequential(ctx, plans, jobs, torch):
     t0 = time.perf_counter()
     for spec, d in jobs:
-        plans[spec["kind"]].reconstruct(d["meas"], d["init"], spec["iterations"],
-                                        subsets=spec["subsets"], bp_mode=hip.BP_GATHER,
-                                        mult=d["mult"], out=d["out"])
+        p = plans[spec["kind"]]
+        p.reconstruct(d["meas"], d["init"], spec["iterations"], subsets=spec["subsets"],
+                      bp_mode=hip.Batch.backprojector(p), mult=d["mult"], out=d["out"])
     _sync(torch)
     return time.perf_counter() - t0
 

@@ -171,7 +171,8 @@ def batch_main(args):
             "value": out["mix"]["batched_jobs_per_s"], "unit": "jobs/s", "n_gpus": 1,
             "higher_is_better": True, "dtype": "f32", "data": "synthetic (seeded ellipsoids, "
             "Poisson 1e5-1e8 counts, attenuation sinograms on half the jobs)",
-            "config": {"workload": "batched jobs", "backprojector": "gather"},
+            "config": {"workload": "batched jobs",
+                       "backprojector": "tile for 3D jobs of 2^21 voxels or more, else gather"},
             "vs_single_job_path": out["mix"]["speedup"], "detail": out}
     print(json.dumps(line), flush=True)
 

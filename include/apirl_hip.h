@@ -347,6 +347,11 @@ const char* apirl_batch_last_error(const apirl_batch* batch);
 /* Device memory a job of this plan holds while admitted (sensitivities, ratio,
  * backprojection, gather records), for admission by HBM footprint. */
 size_t apirl_batch_job_bytes(const apirl_plan* plan, int subsets);
+/* The backprojector a batch runs a job of this plan with: APIRL_BP_TILE for 3D jobs of
+ * 2^21 voxels or more with one LOR per bin that the tile form takes (they run with the
+ * single-job kernels after each grouped round), else APIRL_BP_GATHER.  A batched job's
+ * image equals apirl_em_reconstruct's with that bp_mode, bit for bit. */
+int apirl_batch_job_backprojector(const apirl_plan* plan);
 /* Workgroups per grouped launch (0 keeps the current value) and graph replay on/off. */
 apirl_status apirl_batch_set_launch(apirl_batch* batch, int grid, int use_graph);
 /* Scheduling of the grouped launches (no reference counterpart: the batch engine is

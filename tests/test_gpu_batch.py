@@ -1,5 +1,7 @@
 """Batched jobs (apirl_batch_*): every job's image equals, bit for bit, the image the
-single-job driver (apirl_em_reconstruct, gather backprojector) gives it alone --
+single-job driver (apirl_em_reconstruct, with the backprojector the batch runs the
+job with: the tile form for 3D jobs of 2^21 voxels or more, else the gather) gives it
+alone --
 whether the jobs start together, join while others run (continuous batching), or
 run with the round graph replayed or launched directly.  NaN voxels (the top
 slice, never visited by any LOR) must coincide."""
@@ -42,7 +44,7 @@ def make_job(torch, plan, g, seed, iterations, subsets):
 
 def alone(plan, job):
     img, _ = plan.reconstruct(job["meas"], job["init"], job["iterations"],
-                              subsets=job["subsets"], bp_mode=hip.BP_GATHER)
+                              subsets=job["subsets"], bp_mode=hip.Batch.backprojector(plan))
     return host(img)
 
 
@@ -150,9 +152,9 @@ def test_serving_loop_results_equal_alone(torch, ctx, budget):
     plans, jobs = serving.materialise_all(ctx, specs, torch)
     ref = []
     for spec, d in jobs:
-        img, _ = plans[spec["kind"]].reconstruct(d["meas"], d["init"], spec["iterations"],
-                                                 subsets=spec["subsets"], bp_mode=hip.BP_GATHER,
-                                                 mult=d["mult"])
+        p = plans[spec["kind"]]
+        img, _ = p.reconstruct(d["meas"], d["init"], spec["iterations"], subsets=spec["subsets"],
+                               bp_mode=hip.Batch.backprojector(p), mult=d["mult"])
         ref.append(host(img))
     need = max(hip.Batch.job_bytes(plans[s["kind"]], s["subsets"]) for s in specs)
     lat, wall = serving.serve(ctx, plans, jobs, rate=200.0, torch=torch, max_jobs=8,
@@ -236,9 +238,9 @@ def c3_specs():
 
 
 def alone_spec(plans, spec, d):
-    img, _ = plans[spec["kind"]].reconstruct(d["meas"], d["init"], spec["iterations"],
-                                             subsets=spec["subsets"], bp_mode=hip.BP_GATHER,
-                                             mult=d["mult"])
+    p = plans[spec["kind"]]
+    img, _ = p.reconstruct(d["meas"], d["init"], spec["iterations"], subsets=spec["subsets"],
+                           bp_mode=hip.Batch.backprojector(p), mult=d["mult"])
     return host(img)
 
 

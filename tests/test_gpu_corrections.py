@@ -247,9 +247,9 @@ def test_batch_mmr_osem_and_corrections_equal_alone(torch, ctx, golden):
     add, mult = dev(torch, add_np), dev(torch, mult_np)
     ones_m = torch.ones(pm.image_shape(), dtype=torch.float32, device="cuda")
     ones_c = torch.ones(pc.image_shape(), dtype=torch.float32, device="cuda")
-    alone_m, _ = pm.reconstruct(mm_meas, ones_m, 2, subsets=3, bp_mode=hip.BP_GATHER)
+    alone_m, _ = pm.reconstruct(mm_meas, ones_m, 2, subsets=3, bp_mode=hip.Batch.backprojector(pm))
     assert np.isfinite(host(alone_m)).sum() > 0
-    alone_c, _ = pc.reconstruct(c_meas, ones_c, 2, subsets=7, bp_mode=hip.BP_GATHER,
+    alone_c, _ = pc.reconstruct(c_meas, ones_c, 2, subsets=7, bp_mode=hip.Batch.backprojector(pc),
                                 additive=add, mult=mult)
     b = hip.Batch(ctx, max_jobs=4, max_sinograms=1024)
     out_m = torch.empty_like(ones_m)
