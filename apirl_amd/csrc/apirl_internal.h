@@ -133,7 +133,7 @@ struct apirl_plan {
                                    // Z slices x 64/Z columns per wavefront)
   int gather_zvox = 3;             // 3D gather voxels per lane along z (1..4, tiles only)
   int gather_xcd = APIRL_GATHER_XCD_WEDGES;  // column kernel's tile order
-  int tile_zs = 0;                 // tile backprojector's slab depth (0: about 6 slices)
+  int tile_zs = 0;                 // tile backprojector's slab depth (0: about 5 slices)
   int* d_tile_order = nullptr;     // its item order (built for tile_order_n items)
   int tile_order_n = 0;
   // its geometry per (LOR form, subset, subsets), kept from the first call on, within

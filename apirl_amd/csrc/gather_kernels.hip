@@ -1673,14 +1673,14 @@ hipError_t launch_backproject_gather(const apirl_plan* p, const float* sino, flo
   return hipGetLastError();
 }
 
-// Slab depth of the tile backprojector's items: the plan's tuning, or about 6
+// Slab depth of the tile backprojector's items: the plan's tuning, or about 5
 // slices (shallow items keep few lanes on one voxel per LDS add and leave room for
-// seven wavefronts per SIMD: 1.95 ms per 12-angle subset of the 256x256x81 /
-// 344x252x837 workload at 6 slices, 2.15 at 8, 2.4 at 12, 2.8 at 16).
+// more wavefronts per SIMD; with the kept lists, 1.26 ms per 12-angle subset of the
+// 256x256x81 / 344x252x837 workload at 5 slices, 1.31 at 4 and 1.32 at 6).
 int tile_slab_depth(const apirl_plan* p) {
   const int nz = p->geo.nz;
   if (p->tile_zs > 0) return p->tile_zs < nz ? p->tile_zs : nz;
-  const int nslab = (nz + 5) / 6;
+  const int nslab = (nz + 4) / 5;
   return (nz + nslab - 1) / nslab;
 }
 

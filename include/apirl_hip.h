@@ -131,7 +131,7 @@ apirl_status apirl_plan_intersections(apirl_plan* plan, int subset, int nsubsets
  *                  ray's transaxial tests and sinogram search among them (float
  *                  precision modes; APIRL_PREC_F64_ALPHA always uses 1).
  *   tile_zs:       slices per item of the tile backprojector (APIRL_BP_TILE; default
- *                  about 6).
+ *                  about 5).
  *   tile_cache_mb: device memory (MiB, default 16384) in which the plan keeps the
  *                  tile backprojector's per-subset geometry from its first call on:
  *                  the candidate rays of every 8 x 8-column tile with their paths, and
