@@ -1521,9 +1521,11 @@ __global__ __launch_bounds__(kGBlock) void tile_bp_kernel(GatherArgs A, int zs, 
   double* acc = (double*)((char*)tlds + (size_t)w * tile_wave_bytes(zs, g.nsino));
   TileSlot* slots = (TileSlot*)(acc + kTW * kTW * zs);
   unsigned* ring = (unsigned*)(slots + kTSlots);
-  // the axial table, shared by the workgroup's wavefronts
+  // the axial table, shared by the workgroup's wavefronts, for the listing (the kept
+  // lists need none: their launch leaves it out of the LDS)
   float2* s_zab = (float2*)((char*)tlds + (size_t)kTWaves * tile_wave_bytes(zs, g.nsino));
-  for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
+  if (!LISTED)
+    for (int q = threadIdx.x; q < g.nsino; q += kGBlock) s_zab[q] = A.zab[q];
   __syncthreads();
   const long nxy = (long)g.nx * g.ny;
   const int il = lane % kTW, jl = lane / kTW;
@@ -1684,7 +1686,8 @@ int tile_slab_depth(const apirl_plan* p) {
   return (nz + nslab - 1) / nslab;
 }
 
-size_t tile_lds_bytes(const apirl_plan* p) {
+size_t tile_lds_bytes(const apirl_plan* p, bool listed) {
+  if (listed) return (size_t)kTWaves * tile_wave_bytes(tile_slab_depth(p), p->geo.nsino);
   return (size_t)kTWaves * tile_wave_bytes(tile_slab_depth(p), p->geo.nsino) +
          (size_t)p->geo.nsino * sizeof(float2);
 }
@@ -1709,7 +1712,7 @@ const char* tile_misfit(const apirl_plan* p) {
   if (p->geo.nsino >= 65536) return "the tile backprojector takes fewer than 65536 sinograms";
   if ((long)p->geo.nang * tile_max_candidates(p) >= 65536)
     return "the tile backprojector takes fewer than 65536 candidate rays per tile";
-  if (tile_lds_bytes(p) > 65536) return "the tile backprojector's items exceed 64 KiB of LDS";
+  if (tile_lds_bytes(p, false) > 65536) return "the tile backprojector's items exceed 64 KiB of LDS";
   return nullptr;
 }
 
@@ -1903,12 +1906,12 @@ hipError_t launch_backproject_tile(const apirl_plan* p, const float* sino, float
   const int nslab = (g.nz + zs - 1) / zs;
   const dim3 grid((unsigned)((nitems + kTWaves - 1) / kTWaves));
   if (list && geo_ready)
-    hipLaunchKernelGGL(tile_bp_kernel<true>, grid, dim3(kGBlock), tile_lds_bytes(p), st, A, zs,
+    hipLaunchKernelGGL(tile_bp_kernel<true>, grid, dim3(kGBlock), tile_lds_bytes(p, true), st, A, zs,
                        nslab, p->d_tile_order, nitems, nzp + z.npair, (const TileRay*)trays,
                        (const TilePath*)tpaths, z.nang_sub * z.maxc, (const int*)live,
                        (const int*)nlive, (const int*)q.loff, (const long long*)q.lbase, list);
   else
-    hipLaunchKernelGGL(tile_bp_kernel<false>, grid, dim3(kGBlock), tile_lds_bytes(p), st, A, zs,
+    hipLaunchKernelGGL(tile_bp_kernel<false>, grid, dim3(kGBlock), tile_lds_bytes(p, false), st, A, zs,
                        nslab, p->d_tile_order, nitems, nzp + z.npair, (const TileRay*)trays,
                        (const TilePath*)tpaths, z.nang_sub * z.maxc, (const int*)live,
                        (const int*)nlive, (const int*)nullptr, (const long long*)nullptr,
