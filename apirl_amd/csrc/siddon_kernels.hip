@@ -1100,7 +1100,7 @@ __global__ __launch_bounds__(kBlock) void project_paths_kernel(
             const float an = zst ? Zn : e.ae;
             wv[u] = (an - ac) * ray.L;
             ac = an;
-            pv[u] = imgZ[e.base + k];
+            pv[u] = imgZ[(unsigned)(e.base + k)];  // inside the image: non-negative
             Zn = zst ? Zn + ray.azu : Zn;
             k += zst ? ray.kinc : 0;
             q += zst ? 0 : 1;
